@@ -1,0 +1,744 @@
+// api.hip -- the C ABI of libdsht (include/dsht.h).
+//
+// Plan set-up replaces SpmlUtil_Init/wa_Initial (SpmlUtil_mod.f90:192-241); the host
+// entry points replace the SpmlUtil/SPMODEL functions the Fortran model calls (the
+// Fortran shim dennou-ogcm_amd/fortran/SpmlUtil_mod.f90 binds them); the *_d entry points
+// are the device-resident forms used by batched and resident-state callers.
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <unordered_map>
+
+#include "dsht_internal.hpp"
+
+namespace dsht {
+
+static thread_local char g_err[512];
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+// ---------------------------------------------------------------- workspaces
+// One scratch buffer per (plan, stream); a call holds its lock while it enqueues, so
+// calls on one stream are serialised by the stream and calls on different streams never
+// share scratch.
+struct WsLock {
+  Workspace* ws;
+  std::unique_lock<std::mutex> lk;
+};
+
+static int acquire_ws(dsht_plan* p, hipStream_t s, size_t bytes, WsLock& out) {
+  Workspace* ws;
+  {
+    std::lock_guard<std::mutex> g(p->ws_mu);
+    auto& slot = p->ws[(void*)s];
+    if (!slot) slot.reset(new Workspace());
+    ws = slot.get();
+  }
+  out.ws = ws;
+  out.lk = std::unique_lock<std::mutex>(ws->mu);
+  if (ws->bytes < bytes) {
+    if (ws->buf) {
+      DSHT_HIP(hipStreamSynchronize(s));
+      DSHT_HIP(hipFree(ws->buf));
+      ws->buf = nullptr;
+      ws->bytes = 0;
+    }
+    if (hipMalloc(&ws->buf, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(DSHT_ENOMEM, "workspace allocation of %zu bytes failed", bytes);
+    }
+    ws->bytes = bytes;
+  }
+  return 0;
+}
+
+// column chunk: keeps each Fourier scratch buffer near 48 MiB (MALL resident)
+static int chunk_cols(const dsht_plan* p, int ncols) {
+  if (p->chunk_cols > 0) return p->chunk_cols < ncols ? p->chunk_cols : ncols;
+  long per = (long)(p->N + 1) * p->J * (long)sizeof(double2);
+  long c = (48l << 20) / per;
+  c = c / 32 * 32;
+  if (c < 32) c = 32;
+  return c < ncols ? (int)c : ncols;
+}
+
+static size_t fourier_bytes(const dsht_plan* p, int ch) {
+  return (size_t)(p->N + 1) * p->J * ch * sizeof(double2);
+}
+
+static int check_plan(const dsht_plan* p) {
+  if (!p || p->N < 0) return fail(DSHT_ESTATE, "plan is NULL or destroyed");
+  return 0;
+}
+
+// forward family: grids -> spectra
+static int run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, long ldg, double* out0,
+                   double* out1, long sl, long sc, int ncols, const double* diag0, const double* diag1,
+                   double scale, const double* cw, hipStream_t s) {
+  if (ncols <= 0) return 0;
+  const int nin = (op == F_ALPHA || op == F_UV2VORDIV) ? 2 : 1;
+  const int ch = chunk_cols(p, ncols);
+  WsLock wl;
+  int rc = acquire_ws(p, s, nin * fourier_bytes(p, ch), wl);
+  if (rc) return rc;
+  double2* G0 = (double2*)wl.ws->buf;
+  double2* G1 = G0 + (size_t)(p->N + 1) * p->J * ch;
+  for (int c0 = 0; c0 < ncols; c0 += ch) {
+    int nc = ncols - c0 < ch ? ncols - c0 : ch;
+    if ((rc = launch_fft_fwd(p, in0 + (long)c0 * ldg, ldg, nc, G0, nc, s))) return rc;
+    if (nin > 1 && (rc = launch_fft_fwd(p, in1 + (long)c0 * ldg, ldg, nc, G1, nc, s))) return rc;
+    LegFwdArgs a;
+    a.G0 = G0;
+    a.G1 = nin > 1 ? G1 : nullptr;
+    a.W0 = out0 + (long)c0 * sc;
+    a.W1 = out1 ? out1 + (long)c0 * sc : nullptr;
+    a.sl = sl;
+    a.sc = sc;
+    a.ncols = nc;
+    a.gld = nc;
+    a.gmst = (long)p->J * nc;
+    a.tabP = p->d_tabFP;
+    a.tabH = p->d_tabFH;
+    a.offF = p->d_offF;
+    a.cw = cw;
+    a.diag0 = diag0;
+    a.diag1 = diag1;
+    a.scale = scale;
+    a.N = p->N;
+    a.J2 = p->J2;
+    a.J2pad = p->J2pad;
+    if ((rc = launch_leg_fwd(p, op, a, s))) return rc;
+  }
+  return 0;
+}
+
+// inverse family: spectra -> grids
+static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, long sl, long sc,
+                   double* out0, double* out1, long ldg, int ncols, const double* pro, const double* rs,
+                   hipStream_t s) {
+  if (ncols <= 0) return 0;
+  const int nout = (op == I_VORDIV2UV) ? 2 : 1;
+  const int ch = chunk_cols(p, ncols);
+  WsLock wl;
+  int rc = acquire_ws(p, s, nout * fourier_bytes(p, ch), wl);
+  if (rc) return rc;
+  double2* G0 = (double2*)wl.ws->buf;
+  double2* G1 = G0 + (size_t)(p->N + 1) * p->J * ch;
+  for (int c0 = 0; c0 < ncols; c0 += ch) {
+    int nc = ncols - c0 < ch ? ncols - c0 : ch;
+    LegInvArgs a;
+    a.W0 = in0 + (long)c0 * sc;
+    a.W1 = in1 ? in1 + (long)c0 * sc : nullptr;
+    a.sl = sl;
+    a.sc = sc;
+    a.G0 = G0;
+    a.G1 = nout > 1 ? G1 : nullptr;
+    a.ncols = nc;
+    a.gld = nc;
+    a.gmst = (long)p->J * nc;
+    a.tabP = p->d_tabIP;
+    a.tabH = p->d_tabIH;
+    a.offI = p->d_offI;
+    a.rs = rs;
+    a.pro = pro;
+    a.N = p->N;
+    a.J = p->J;
+    a.J2 = p->J2;
+    a.J2pad = p->J2pad;
+    if ((rc = launch_leg_inv(p, op, a, s))) return rc;
+    if ((rc = launch_fft_inv(p, G0, nc, nc, out0 + (long)c0 * ldg, ldg, s))) return rc;
+    if (nout > 1 && (rc = launch_fft_inv(p, G1, nc, nc, out1 + (long)c0 * ldg, ldg, s))) return rc;
+  }
+  return 0;
+}
+
+static int check_grid_ptr(const void* x, long ldg, const dsht_plan* p) {
+  if (!x) return fail(DSHT_EARG, "grid pointer is NULL");
+  if (((uintptr_t)x) % 16) return fail(DSHT_EARG, "grid pointers must be 16-byte aligned");
+  if (ldg < (long)p->I * p->J || (ldg & 1)) return fail(DSHT_EARG, "grid column pitch %ld must be even and >= I*J", ldg);
+  return 0;
+}
+
+static int check_spec(const void* w, long sl, long sc, int ncols, const dsht_plan* p) {
+  if (!w) return fail(DSHT_EARG, "spectral pointer is NULL");
+  if (sl <= 0 || sc <= 0) return fail(DSHT_EARG, "spectral strides must be positive");
+  bool lc = sc == 1 && sl >= ncols, cl = sl == 1 && sc >= p->L;
+  if (!(lc || cl || ncols == 1)) return fail(DSHT_EARG, "spectral strides (sl=%ld, sc=%ld) overlap", sl, sc);
+  return 0;
+}
+
+// ---------------------------------------------------------------- host contexts
+struct HostCtx {
+  hipStream_t s = nullptr;
+  double* buf[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t cap[4] = {0, 0, 0, 0};
+  double* dout = nullptr;  // small outputs
+  size_t dcap = 0;
+};
+
+struct HostReg {
+  std::mutex mu;
+  std::vector<HostCtx*> all;
+};
+
+static std::atomic<long> g_plan_serial{1};
+static std::mutex g_reg_mu;
+static std::unordered_map<long, HostReg*> g_reg;
+
+static thread_local std::unordered_map<long, HostCtx*> t_ctx;
+
+}  // namespace dsht
+
+using namespace dsht;
+
+struct PlanExtra {
+  long serial;
+};
+
+static std::mutex g_extra_mu;
+static std::unordered_map<const dsht_plan*, long> g_serial;
+
+static long plan_serial(const dsht_plan* p) {
+  std::lock_guard<std::mutex> g(g_extra_mu);
+  auto it = g_serial.find(p);
+  return it == g_serial.end() ? 0 : it->second;
+}
+
+static int host_ctx(dsht_plan* p, HostCtx** out) {
+  long id = plan_serial(p);
+  if (!id) return fail(DSHT_ESTATE, "plan is not registered");
+  auto it = t_ctx.find(id);
+  if (it != t_ctx.end()) {
+    *out = it->second;
+    return 0;
+  }
+  HostCtx* c = new HostCtx();
+  DSHT_HIP(hipSetDevice(p->device));
+  DSHT_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+  {
+    std::lock_guard<std::mutex> g(g_reg_mu);
+    HostReg*& r = g_reg[id];
+    if (!r) r = new HostReg();
+    std::lock_guard<std::mutex> g2(r->mu);
+    r->all.push_back(c);
+  }
+  t_ctx[id] = c;
+  *out = c;
+  return 0;
+}
+
+static int ctx_buf(HostCtx* c, int i, size_t n, double** out) {
+  if (c->cap[i] < n) {
+    if (c->buf[i]) DSHT_HIP(hipFree(c->buf[i]));
+    c->buf[i] = nullptr;
+    c->cap[i] = 0;
+    if (hipMalloc(&c->buf[i], sizeof(double) * (n ? n : 1)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(DSHT_ENOMEM, "host-API device buffer of %zu doubles failed", n);
+    }
+    c->cap[i] = n;
+  }
+  *out = c->buf[i];
+  return 0;
+}
+
+extern "C" {
+
+const char* dsht_last_error(void) { return dsht::g_err; }
+
+int dsht_create(int nmax, int im, int jm, int kmax, double radius, dsht_plan** plan) {
+  if (!plan) return fail(DSHT_EARG, "dsht_create: plan pointer is NULL");
+  *plan = nullptr;
+  if (nmax < 0 || im < 2 || jm < 2) return fail(DSHT_EARG, "dsht_create: need nmax >= 0, im >= 2, jm >= 2");
+  if (im % 2) return fail(DSHT_EARG, "dsht_create: im = %d must be even", im);
+  if (jm % 2) return fail(DSHT_EARG, "dsht_create: jm = %d must be even", jm);
+  if (im < 2 * nmax + 2) return fail(DSHT_EARG, "dsht_create: im = %d < 2*nmax+2 = %d", im, 2 * nmax + 2);
+  if (jm < nmax + 1) return fail(DSHT_EARG, "dsht_create: jm = %d < nmax+1 = %d", jm, nmax + 1);
+  if (!(radius > 0.0) || !std::isfinite(radius)) return fail(DSHT_EARG, "dsht_create: radius must be positive");
+  if (kmax < 0) return fail(DSHT_EARG, "dsht_create: kmax must be >= 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return fail(DSHT_EDEVICE, "dsht_create: no HIP device available");
+  }
+  dsht_plan* p = new dsht_plan();
+  p->N = nmax;
+  p->I = im;
+  p->J = jm;
+  p->K = kmax;
+  p->L = (nmax + 1) * (nmax + 1);
+  p->J2 = jm / 2;
+  p->J2pad = cdiv(p->J2, KCH) * KCH;
+  p->R = radius;
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    delete p;
+    return fail(DSHT_EDEVICE, "dsht_create: hipGetDevice failed");
+  }
+  int rc = fft_plan_init(p);
+  if (!rc) rc = build_tables(p);
+  if (rc) {
+    p->N = 0;
+    dsht_destroy(p);
+    return rc;
+  }
+  {
+    std::lock_guard<std::mutex> g(g_extra_mu);
+    g_serial[p] = g_plan_serial++;
+  }
+  *plan = p;
+  return 0;
+}
+
+int dsht_destroy(dsht_plan* p) {
+  if (!p) return 0;
+  long id = plan_serial(p);
+  if (id) {
+    HostReg* r = nullptr;
+    {
+      std::lock_guard<std::mutex> g(g_reg_mu);
+      auto it = g_reg.find(id);
+      if (it != g_reg.end()) {
+        r = it->second;
+        g_reg.erase(it);
+      }
+    }
+    if (r) {
+      for (HostCtx* c : r->all) {
+        if (c->s) (void)hipStreamSynchronize(c->s);
+        for (int i = 0; i < 4; i++)
+          if (c->buf[i]) (void)hipFree(c->buf[i]);
+        if (c->dout) (void)hipFree(c->dout);
+        if (c->s) (void)hipStreamDestroy(c->s);
+        c->s = nullptr;
+        for (int i = 0; i < 4; i++) c->buf[i] = nullptr;
+        c->dout = nullptr;
+        // the owning thread's map still holds the pointer under the dead serial id; it is
+        // never looked up again (serials are not reused) so the small struct is leaked.
+      }
+      delete r;
+    }
+    std::lock_guard<std::mutex> g(g_extra_mu);
+    g_serial.erase(p);
+  }
+  (void)hipDeviceSynchronize();
+  for (auto& kv : p->ws)
+    if (kv.second && kv.second->buf) (void)hipFree(kv.second->buf);
+  double* ds[] = {p->d_mu, p->d_wq, p->d_wl, p->d_cwv, p->d_cwd, p->d_rs_icos, p->d_rs_alpha, p->d_rs_uv,
+                  p->d_pro_psi, p->d_tabFP, p->d_tabFH, p->d_tabIP, p->d_tabIH};
+  for (double* d : ds)
+    if (d) (void)hipFree(d);
+  if (p->d_offF) (void)hipFree(p->d_offF);
+  if (p->d_offI) (void)hipFree(p->d_offI);
+  fft_plan_free(p);
+  mixed_free(p);
+  p->N = -1;
+  delete p;
+  return 0;
+}
+
+int dsht_get_dims(const dsht_plan* p, int* nmax, int* im, int* jm, int* lm) {
+  if (int rc = check_plan(p)) return rc;
+  if (nmax) *nmax = p->N;
+  if (im) *im = p->I;
+  if (jm) *jm = p->J;
+  if (lm) *lm = p->L;
+  return 0;
+}
+
+int dsht_set_precision(dsht_plan* p, int prec) {
+  if (int rc = check_plan(p)) return rc;
+  if (prec < DSHT_PREC_FP64 || prec > DSHT_PREC_FP16X3) return fail(DSHT_EARG, "unknown precision form %d", prec);
+  p->prec = prec;
+  return 0;
+}
+
+int dsht_get_precision(const dsht_plan* p) { return p ? p->prec : -1; }
+
+int dsht_grid_info(const dsht_plan* p, double* x_lon, double* y_lat, double* x_lon_weight,
+                   double* y_lat_weight, double* xy_lon, double* xy_lat) {
+  if (int rc = check_plan(p)) return rc;
+  for (int i = 0; i < p->I; i++) {
+    if (x_lon) x_lon[i] = p->lon[i];
+    if (x_lon_weight) x_lon_weight[i] = 2.0 * M_PI / p->I;
+  }
+  for (int j = 0; j < p->J; j++) {
+    if (y_lat) y_lat[j] = p->lat[j];
+    if (y_lat_weight) y_lat_weight[j] = p->wl[j];
+    for (int i = 0; i < p->I; i++) {
+      if (xy_lon) xy_lon[(long)j * p->I + i] = p->lon[i];
+      if (xy_lat) xy_lat[(long)j * p->I + i] = p->lat[j];
+    }
+  }
+  return 0;
+}
+
+int dsht_nm_l(const dsht_plan* p, int* n_of_l, int* m_of_l) {
+  if (int rc = check_plan(p)) return rc;
+  for (int n = 0; n <= p->N; n++)
+    for (int m = -n; m <= n; m++) {
+      int l = n * (n + 1) + m;
+      if (n_of_l) n_of_l[l] = n;
+      if (m_of_l) m_of_l[l] = m;
+    }
+  return 0;
+}
+
+int dsht_l_nm(const dsht_plan* p, int n, int m) {
+  if (!p || n < 0 || n > p->N || m < -n || m > n) return 0;
+  return n * (n + 1) + m + 1;
+}
+
+int dsht_rn(const dsht_plan* p, double* rn) {
+  if (int rc = check_plan(p)) return rc;
+  if (!rn) return fail(DSHT_EARG, "dsht_rn: NULL output");
+  for (int n = 0; n <= p->N; n++)
+    for (int m = -n; m <= n; m++) rn[n * (n + 1) + m] = -(double)n * (n + 1);
+  return 0;
+}
+
+// ---------------------------------------------------------------- device API
+static int kind_fwd_op(int kind) {
+  switch (kind) {
+    case DSHT_VAL: return F_VAL;
+    case DSHT_DIVLAMBDA: return F_DIVLAMBDA;
+    case DSHT_DIVMU: return F_DIVMU;
+  }
+  return -1;
+}
+
+static int dispatch_fwd(dsht_plan* p, int op, const double* in0, const double* in1, long ldg, double* out0,
+                        double* out1, long sl, long sc, int ncols, const double* diag0, const double* diag1,
+                        double scale, const double* cw, hipStream_t s) {
+  if (p->prec != DSHT_PREC_FP64)
+    return mixed_run_fwd(p, op, in0, in1, ldg, out0, out1, sl, sc, ncols, diag0, diag1, scale, cw, s);
+  return run_fwd(p, op, in0, in1, ldg, out0, out1, sl, sc, ncols, diag0, diag1, scale, cw, s);
+}
+
+static int dispatch_inv(dsht_plan* p, int op, const double* in0, const double* in1, long sl, long sc,
+                        double* out0, double* out1, long ldg, int ncols, const double* pro, const double* rs,
+                        hipStream_t s) {
+  if (p->prec != DSHT_PREC_FP64)
+    return mixed_run_inv(p, op, in0, in1, sl, sc, out0, out1, ldg, ncols, pro, rs, s);
+  return run_inv(p, op, in0, in1, sl, sc, out0, out1, ldg, ncols, pro, rs, s);
+}
+
+int dsht_fwd_d(dsht_plan* p, int kind, const double* xy, long ldg, double* w, long sl, long sc, int ncols,
+               const double* diag, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  int op = kind_fwd_op(kind);
+  if (op < 0) return fail(DSHT_EARG, "dsht_fwd_d: kind %d is not a forward kind", kind);
+  if (ncols < 0) return fail(DSHT_EARG, "ncols < 0");
+  if (ncols == 0) return 0;
+  if (int rc = check_grid_ptr(xy, ldg, p)) return rc;
+  if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
+  return dispatch_fwd(p, op, xy, nullptr, ldg, w, nullptr, sl, sc, ncols, diag, nullptr, 1.0,
+                      op == F_VAL ? p->d_cwv : p->d_cwd, (hipStream_t)stream);
+}
+
+int dsht_inv_d(dsht_plan* p, int kind, const double* w, long sl, long sc, double* xy, long ldg, int ncols,
+               const double* diag, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  int op;
+  const double* rs = nullptr;
+  switch (kind) {
+    case DSHT_VAL: op = I_VAL; break;
+    case DSHT_GRADLAMBDA: op = I_GRADLAMBDA; break;
+    case DSHT_GRADMU: op = I_GRADMU; break;
+    case DSHT_GRADLON: op = I_GRADLAMBDA; rs = p->d_rs_icos; break;
+    case DSHT_GRADLAT: op = I_GRADMU; rs = p->d_rs_icos; break;
+    default: return fail(DSHT_EARG, "dsht_inv_d: kind %d is not an inverse kind", kind);
+  }
+  if (ncols < 0) return fail(DSHT_EARG, "ncols < 0");
+  if (ncols == 0) return 0;
+  if (int rc = check_grid_ptr(xy, ldg, p)) return rc;
+  if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
+  return dispatch_inv(p, op, w, nullptr, sl, sc, xy, nullptr, ldg, ncols, diag, rs, (hipStream_t)stream);
+}
+
+int dsht_alpha_fwd_d(dsht_plan* p, const double* a, const double* b, long ldg, double* w, long sl, long sc,
+                     int ncols, const double* diag, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (ncols < 0) return fail(DSHT_EARG, "ncols < 0");
+  if (ncols == 0) return 0;
+  if (int rc = check_grid_ptr(a, ldg, p)) return rc;
+  if (int rc = check_grid_ptr(b, ldg, p)) return rc;
+  if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
+  return dispatch_fwd(p, F_ALPHA, a, b, ldg, w, nullptr, sl, sc, ncols, diag, nullptr, 1.0 / p->R, p->d_cwd,
+                      (hipStream_t)stream);
+}
+
+int dsht_alpha_inv_d(dsht_plan* p, const double* wa, const double* wb, long sl, long sc, double* xy, long ldg,
+                     int ncols, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (ncols < 0) return fail(DSHT_EARG, "ncols < 0");
+  if (ncols == 0) return 0;
+  if (int rc = check_grid_ptr(xy, ldg, p)) return rc;
+  if (int rc = check_spec(wa, sl, sc, ncols, p)) return rc;
+  if (int rc = check_spec(wb, sl, sc, ncols, p)) return rc;
+  return dispatch_inv(p, I_ALPHA, wa, wb, sl, sc, xy, nullptr, ldg, ncols, nullptr, p->d_rs_alpha,
+                      (hipStream_t)stream);
+}
+
+int dsht_uv2vordiv_d(dsht_plan* p, const double* ucos, const double* vcos, long ldg, double* vor, double* div,
+                     long sl, long sc, int ncols, const double* diag, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (ncols < 0) return fail(DSHT_EARG, "ncols < 0");
+  if (ncols == 0) return 0;
+  if (int rc = check_grid_ptr(ucos, ldg, p)) return rc;
+  if (int rc = check_grid_ptr(vcos, ldg, p)) return rc;
+  if (int rc = check_spec(vor, sl, sc, ncols, p)) return rc;
+  if (int rc = check_spec(div, sl, sc, ncols, p)) return rc;
+  return dispatch_fwd(p, F_UV2VORDIV, ucos, vcos, ldg, vor, div, sl, sc, ncols, diag, diag, 1.0 / p->R, p->d_cwd,
+                      (hipStream_t)stream);
+}
+
+int dsht_vordiv2uv_d(dsht_plan* p, const double* vor, const double* div, long sl, long sc, double* u, double* v,
+                     long ldg, int ncols, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (ncols < 0) return fail(DSHT_EARG, "ncols < 0");
+  if (ncols == 0) return 0;
+  if (int rc = check_grid_ptr(u, ldg, p)) return rc;
+  if (int rc = check_grid_ptr(v, ldg, p)) return rc;
+  if (int rc = check_spec(vor, sl, sc, ncols, p)) return rc;
+  if (int rc = check_spec(div, sl, sc, ncols, p)) return rc;
+  return dispatch_inv(p, I_VORDIV2UV, vor, div, sl, sc, u, v, ldg, ncols, p->d_pro_psi, p->d_rs_uv,
+                      (hipStream_t)stream);
+}
+
+int dsht_diag_d(dsht_plan* p, const double* d, const double* w, double* out, long sl, long sc, int ncols,
+                void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!d) return fail(DSHT_EARG, "dsht_diag_d: NULL diagonal");
+  if (ncols < 0) return fail(DSHT_EARG, "ncols < 0");
+  if (ncols == 0) return 0;
+  if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
+  if (int rc = check_spec(out, sl, sc, ncols, p)) return rc;
+  return launch_diag(d, w, out, sl, sc, p->L, ncols, (hipStream_t)stream);
+}
+
+int dsht_intlonlat_d(dsht_plan* p, const double* xy, long ldg, double* out, int ncols, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (ncols < 0 || !out) return fail(DSHT_EARG, "bad arguments");
+  if (ncols == 0) return 0;
+  if (int rc = check_grid_ptr(xy, ldg, p)) return rc;
+  return launch_lonlat_sum(p, xy, ldg, ncols, out, 2.0 * M_PI / p->I, (hipStream_t)stream);
+}
+
+int dsht_avrlonlat_d(dsht_plan* p, const double* xy, long ldg, double* out, int ncols, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (ncols < 0 || !out) return fail(DSHT_EARG, "bad arguments");
+  if (ncols == 0) return 0;
+  if (int rc = check_grid_ptr(xy, ldg, p)) return rc;
+  return launch_lonlat_sum(p, xy, ldg, ncols, out, 1.0 / (2.0 * p->I), (hipStream_t)stream);
+}
+
+int dsht_ya_intlon_d(dsht_plan* p, const double* xy, long ldg, double* out, int ncols, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (ncols < 0 || !out) return fail(DSHT_EARG, "bad arguments");
+  if (ncols == 0) return 0;
+  if (int rc = check_grid_ptr(xy, ldg, p)) return rc;
+  return launch_lon_sum(p, xy, ldg, ncols, out, 2.0 * M_PI / p->I, p->J, (hipStream_t)stream);
+}
+
+int dsht_interpolate_d(dsht_plan* p, const double* w, long sl, const double* lon, const double* lat, double* out,
+                       int npts, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (npts < 0 || !w || !lon || !lat || !out || sl <= 0) return fail(DSHT_EARG, "bad arguments");
+  return launch_interpolate(p, w, sl, lon, lat, out, npts, (hipStream_t)stream);
+}
+
+int dsht_coef_d(dsht_plan* p, int which, const double* params, int nparams, double* out, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (which < DSHT_COEF_LAPLA || which > DSHT_COEF_BEDIFF) return fail(DSHT_EARG, "unknown coefficient table %d", which);
+  if (!out || nparams < 0 || nparams > 8) return fail(DSHT_EARG, "bad arguments");
+  static const int need[] = {0, 0, 3, 3, 0, 2, 5};
+  if (nparams < need[which]) return fail(DSHT_EARG, "coefficient table %d needs %d parameters", which, need[which]);
+  return launch_coef(p, which, params, nparams, out, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------- host API
+// copy helpers
+static int h2d(HostCtx* c, double* d, const double* h, size_t n) {
+  DSHT_HIP(hipMemcpyAsync(d, h, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+  return 0;
+}
+static int d2h_sync(HostCtx* c, double* h, const double* d, size_t n) {
+  DSHT_HIP(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+  DSHT_HIP(hipStreamSynchronize(c->s));
+  return 0;
+}
+
+#define HOST_PROLOGUE(p, ncols)                                 \
+  if (int rc_ = check_plan(p)) return rc_;                      \
+  if ((ncols) < 0) return fail(DSHT_EARG, "ncols < 0");         \
+  if ((ncols) == 0) return 0;                                   \
+  HostCtx* c;                                                   \
+  if (int rc_ = host_ctx(p, &c)) return rc_;                    \
+  DSHT_HIP(hipSetDevice(p->device));                            \
+  const size_t IJ = (size_t)p->I * p->J, LL = (size_t)p->L;     \
+  (void)IJ;                                                     \
+  (void)LL;
+
+int dsht_w_xy(dsht_plan* p, int kind, const double* xy, double* w, int ncols) {
+  HOST_PROLOGUE(p, ncols);
+  if (!xy || !w) return fail(DSHT_EARG, "NULL array");
+  double *dg, *dw;
+  int rc;
+  if ((rc = ctx_buf(c, 0, IJ * ncols, &dg)) || (rc = ctx_buf(c, 1, LL * ncols, &dw))) return rc;
+  if ((rc = h2d(c, dg, xy, IJ * ncols))) return rc;
+  if ((rc = dsht_fwd_d(p, kind, dg, (long)IJ, dw, 1, (long)LL, ncols, nullptr, c->s))) return rc;
+  return d2h_sync(c, w, dw, LL * ncols);
+}
+
+int dsht_xy_w(dsht_plan* p, int kind, const double* w, double* xy, int ncols) {
+  HOST_PROLOGUE(p, ncols);
+  if (!xy || !w) return fail(DSHT_EARG, "NULL array");
+  double *dg, *dw;
+  int rc;
+  if ((rc = ctx_buf(c, 0, IJ * ncols, &dg)) || (rc = ctx_buf(c, 1, LL * ncols, &dw))) return rc;
+  if ((rc = h2d(c, dw, w, LL * ncols))) return rc;
+  if ((rc = dsht_inv_d(p, kind, dw, 1, (long)LL, dg, (long)IJ, ncols, nullptr, c->s))) return rc;
+  return d2h_sync(c, xy, dg, IJ * ncols);
+}
+
+int dsht_w_alpha_xy(dsht_plan* p, const double* a, const double* b, double* w, int ncols) {
+  HOST_PROLOGUE(p, ncols);
+  if (!a || !b || !w) return fail(DSHT_EARG, "NULL array");
+  double *da, *db, *dw;
+  int rc;
+  if ((rc = ctx_buf(c, 0, IJ * ncols, &da)) || (rc = ctx_buf(c, 2, IJ * ncols, &db)) ||
+      (rc = ctx_buf(c, 1, LL * ncols, &dw)))
+    return rc;
+  if ((rc = h2d(c, da, a, IJ * ncols)) || (rc = h2d(c, db, b, IJ * ncols))) return rc;
+  if ((rc = dsht_alpha_fwd_d(p, da, db, (long)IJ, dw, 1, (long)LL, ncols, nullptr, c->s))) return rc;
+  return d2h_sync(c, w, dw, LL * ncols);
+}
+
+int dsht_xy_alpha_w(dsht_plan* p, const double* wa, const double* wb, double* xy, int ncols) {
+  HOST_PROLOGUE(p, ncols);
+  if (!wa || !wb || !xy) return fail(DSHT_EARG, "NULL array");
+  double *dg, *dwa, *dwb;
+  int rc;
+  if ((rc = ctx_buf(c, 0, IJ * ncols, &dg)) || (rc = ctx_buf(c, 1, LL * ncols, &dwa)) ||
+      (rc = ctx_buf(c, 3, LL * ncols, &dwb)))
+    return rc;
+  if ((rc = h2d(c, dwa, wa, LL * ncols)) || (rc = h2d(c, dwb, wb, LL * ncols))) return rc;
+  if ((rc = dsht_alpha_inv_d(p, dwa, dwb, 1, (long)LL, dg, (long)IJ, ncols, c->s))) return rc;
+  return d2h_sync(c, xy, dg, IJ * ncols);
+}
+
+int dsht_uvcoslat2vordiv(dsht_plan* p, const double* ucos, const double* vcos, double* vor, double* div,
+                         int ncols) {
+  HOST_PROLOGUE(p, ncols);
+  if (!ucos || !vcos || !vor || !div) return fail(DSHT_EARG, "NULL array");
+  double *du, *dv, *dz, *dd;
+  int rc;
+  if ((rc = ctx_buf(c, 0, IJ * ncols, &du)) || (rc = ctx_buf(c, 2, IJ * ncols, &dv)) ||
+      (rc = ctx_buf(c, 1, LL * ncols, &dz)) || (rc = ctx_buf(c, 3, LL * ncols, &dd)))
+    return rc;
+  if ((rc = h2d(c, du, ucos, IJ * ncols)) || (rc = h2d(c, dv, vcos, IJ * ncols))) return rc;
+  if ((rc = dsht_uv2vordiv_d(p, du, dv, (long)IJ, dz, dd, 1, (long)LL, ncols, nullptr, c->s))) return rc;
+  DSHT_HIP(hipMemcpyAsync(vor, dz, sizeof(double) * LL * ncols, hipMemcpyDeviceToHost, c->s));
+  return d2h_sync(c, div, dd, LL * ncols);
+}
+
+int dsht_vordiv2uv(dsht_plan* p, const double* vor, const double* div, double* u, double* v, int ncols) {
+  HOST_PROLOGUE(p, ncols);
+  if (!u || !v || !vor || !div) return fail(DSHT_EARG, "NULL array");
+  double *du, *dv, *dz, *dd;
+  int rc;
+  if ((rc = ctx_buf(c, 0, IJ * ncols, &du)) || (rc = ctx_buf(c, 2, IJ * ncols, &dv)) ||
+      (rc = ctx_buf(c, 1, LL * ncols, &dz)) || (rc = ctx_buf(c, 3, LL * ncols, &dd)))
+    return rc;
+  if ((rc = h2d(c, dz, vor, LL * ncols)) || (rc = h2d(c, dd, div, LL * ncols))) return rc;
+  if ((rc = dsht_vordiv2uv_d(p, dz, dd, 1, (long)LL, du, dv, (long)IJ, ncols, c->s))) return rc;
+  DSHT_HIP(hipMemcpyAsync(u, du, sizeof(double) * IJ * ncols, hipMemcpyDeviceToHost, c->s));
+  return d2h_sync(c, v, dv, IJ * ncols);
+}
+
+int dsht_w_diag_w(dsht_plan* p, const double* d, const double* w, double* out, int ncols) {
+  HOST_PROLOGUE(p, ncols);
+  if (!d || !w || !out) return fail(DSHT_EARG, "NULL array");
+  double *dd, *dw;
+  int rc;
+  if ((rc = ctx_buf(c, 2, LL, &dd)) || (rc = ctx_buf(c, 1, LL * ncols, &dw))) return rc;
+  if ((rc = h2d(c, dd, d, LL)) || (rc = h2d(c, dw, w, LL * ncols))) return rc;
+  if ((rc = launch_diag(dd, dw, dw, 1, (long)LL, p->L, ncols, c->s))) return rc;
+  return d2h_sync(c, out, dw, LL * ncols);
+}
+
+int dsht_xy_diag_xy(dsht_plan* p, const double* d, const double* xy, double* out, int ncols) {
+  HOST_PROLOGUE(p, ncols);
+  if (!d || !xy || !out) return fail(DSHT_EARG, "NULL array");
+  double *dg, *dw, *dd;
+  int rc;
+  if ((rc = ctx_buf(c, 0, IJ * ncols, &dg)) || (rc = ctx_buf(c, 1, LL * ncols, &dw)) ||
+      (rc = ctx_buf(c, 2, LL, &dd)))
+    return rc;
+  if ((rc = h2d(c, dd, d, LL)) || (rc = h2d(c, dg, xy, IJ * ncols))) return rc;
+  if ((rc = dsht_fwd_d(p, DSHT_VAL, dg, (long)IJ, dw, 1, (long)LL, ncols, dd, c->s))) return rc;
+  if ((rc = dsht_inv_d(p, DSHT_VAL, dw, 1, (long)LL, dg, (long)IJ, ncols, nullptr, c->s))) return rc;
+  return d2h_sync(c, out, dg, IJ * ncols);
+}
+
+static int host_sum(dsht_plan* p, const double* xy, double* out, int ncols, int what) {
+  HOST_PROLOGUE(p, ncols);
+  if (!xy || !out) return fail(DSHT_EARG, "NULL array");
+  double *dg, *dout;
+  int rc;
+  size_t nout = what >= 2 ? (size_t)p->J * ncols : (size_t)ncols;
+  if ((rc = ctx_buf(c, 0, IJ * ncols, &dg)) || (rc = ctx_buf(c, 1, nout, &dout))) return rc;
+  if ((rc = h2d(c, dg, xy, IJ * ncols))) return rc;
+  switch (what) {
+    case 0: rc = dsht_intlonlat_d(p, dg, (long)IJ, dout, ncols, c->s); break;
+    case 1: rc = dsht_avrlonlat_d(p, dg, (long)IJ, dout, ncols, c->s); break;
+    case 2: rc = dsht_ya_intlon_d(p, dg, (long)IJ, dout, ncols, c->s); break;
+    default: rc = launch_lon_sum(p, dg, (long)IJ, ncols, dout, 1.0 / p->I, p->J, c->s); break;
+  }
+  if (rc) return rc;
+  return d2h_sync(c, out, dout, nout);
+}
+
+int dsht_intlonlat(dsht_plan* p, const double* xy, double* out, int ncols) { return host_sum(p, xy, out, ncols, 0); }
+int dsht_avrlonlat(dsht_plan* p, const double* xy, double* out, int ncols) { return host_sum(p, xy, out, ncols, 1); }
+int dsht_ya_intlon(dsht_plan* p, const double* xy, double* out, int ncols) { return host_sum(p, xy, out, ncols, 2); }
+int dsht_ya_avrlon(dsht_plan* p, const double* xy, double* out, int ncols) { return host_sum(p, xy, out, ncols, 3); }
+
+int dsht_interpolate(dsht_plan* p, const double* w, const double* lon, const double* lat, double* out, int npts) {
+  HOST_PROLOGUE(p, npts);
+  if (!w || !lon || !lat || !out) return fail(DSHT_EARG, "NULL array");
+  double *dw, *dl;
+  int rc;
+  if ((rc = ctx_buf(c, 1, LL, &dw)) || (rc = ctx_buf(c, 0, 3 * (size_t)npts, &dl))) return rc;
+  if ((rc = h2d(c, dw, w, LL)) || (rc = h2d(c, dl, lon, npts)) || (rc = h2d(c, dl + npts, lat, npts))) return rc;
+  if ((rc = launch_interpolate(p, dw, 1, dl, dl + npts, dl + 2 * npts, npts, c->s))) return rc;
+  return d2h_sync(c, out, dl + 2 * npts, npts);
+}
+
+int dsht_coef(const dsht_plan* pc, int which, const double* params, int nparams, double* out) {
+  dsht_plan* p = const_cast<dsht_plan*>(pc);
+  HOST_PROLOGUE(p, 1);
+  if (!out) return fail(DSHT_EARG, "NULL array");
+  double* dd;
+  int rc;
+  if ((rc = ctx_buf(c, 1, LL, &dd))) return rc;
+  if ((rc = dsht_coef_d(p, which, params, nparams, dd, c->s))) return rc;
+  return d2h_sync(c, out, dd, LL);
+}
+
+}  // extern "C"

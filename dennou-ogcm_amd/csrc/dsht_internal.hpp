@@ -1,0 +1,191 @@
+// dsht_internal.hpp -- shared definitions of libdsht (MI355X spherical-harmonic transforms).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dsht.h"
+
+namespace dsht {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------- geometry
+// K-chunk of the forward contraction (Gauss latitudes of one hemisphere) and of
+// the inverse contraction (total wavenumbers of one parity).
+constexpr int KCH = 16;
+constexpr int TILE = 16;  // MFMA output tile edge
+
+__host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+__host__ __device__ inline long cdivl(long a, long b) { return (a + b - 1) / b; }
+
+// number of total wavenumbers n in [m, N] with (n - m) of parity p
+__host__ __device__ inline int n_par(int N, int m, int p) {
+  int cnt = N - m + 1;
+  return p == 0 ? (cnt + 1) / 2 : cnt / 2;
+}
+
+// ---------------------------------------------------------------- ops
+enum FwdOp { F_VAL = 0, F_DIVLAMBDA = 1, F_DIVMU = 2, F_ALPHA = 3, F_UV2VORDIV = 4 };
+enum InvOp { I_VAL = 0, I_GRADLAMBDA = 1, I_GRADMU = 2, I_ALPHA = 3, I_VORDIV2UV = 4 };
+
+// Arguments of the forward Legendre contraction (Fourier -> spectral).
+struct LegFwdArgs {
+  const double2* G0;  // Fourier coefficients [m][j*gld + c] (complex), input 0
+  const double2* G1;  // input 1 (two-input ops)
+  double* W0;         // spectral output 0 (element (l, c) at l*sl + c*sc)
+  double* W1;         // spectral output 1
+  long sl, sc;
+  int ncols;          // columns in this call
+  long gld;           // Fourier row pitch (columns per latitude row in G)
+  long gmst;          // Fourier m-stride (J * gld)
+  const double* tabP; // forward-swizzled P table
+  const double* tabH; // forward-swizzled H table
+  const long* offF;   // per-m offsets into the forward tables
+  const double* cw;   // per-latitude weights of the north half (J2pad)
+  const double* diag0;  // per-l epilogue multipliers (nullable)
+  const double* diag1;
+  double scale;       // extra scalar factor (1/R for the H operator)
+  int N, J2, J2pad;
+};
+
+// Arguments of the inverse Legendre contraction (spectral -> Fourier).
+struct LegInvArgs {
+  const double* W0;  // spectral input 0
+  const double* W1;  // spectral input 1
+  long sl, sc;
+  double2* G0;       // Fourier output 0 [m][j*gld + c]
+  double2* G1;       // Fourier output 1
+  int ncols;
+  long gld, gmst;
+  const double* tabP;  // inverse-swizzled P table
+  const double* tabH;  // inverse-swizzled H table
+  const long* offI;    // per-m offsets into the inverse tables
+  const double* rs;    // per-latitude output row scale (J), nullable
+  const double* pro;   // per-l prologue multipliers applied to both inputs (nullable)
+  int N, J, J2, J2pad;
+};
+
+// ---------------------------------------------------------------- FFT plan
+struct FftPlan {
+  int M = 0;          // complex length I/2
+  int nrad = 0;
+  int rad[24] = {0};
+  double2* twM = nullptr;  // exp(-2 pi i k / M), k < M  (device)
+  double2* twI = nullptr;  // exp(-2 pi i m / I), m < M  (device)
+  float2* twM32 = nullptr;
+  float2* twI32 = nullptr;
+};
+
+struct FftArgs {
+  int I, M, N, J;
+  int ncols;
+  long ldg;       // grid column pitch (elements)
+  long gld, gmst; // Fourier layout: [m][j*gld + c]
+  int nrad;
+  int rad[24];
+  const double2* twM;
+  const double2* twI;
+  int rows_per_block;
+  const double* rowscale;  // optional per-latitude scale applied after the inverse (nullable)
+};
+
+// ---------------------------------------------------------------- workspace
+struct Workspace {
+  void* buf = nullptr;
+  size_t bytes = 0;
+  std::mutex mu;
+};
+
+struct HostCtx;  // per-thread host-API context (api.hip)
+
+// ---------------------------------------------------------------- plan
+}  // namespace dsht
+
+struct dsht_plan {
+  int N = 0, I = 0, J = 0, K = 0, L = 0, J2 = 0, J2pad = 0;
+  double R = 1.0;
+  int device = 0;
+  int prec = DSHT_PREC_FP64;
+  // host copies of the grid (computed on the device)
+  std::vector<double> mu, wq, wl, lat, lon;
+  // device grid vectors
+  double* d_mu = nullptr;
+  double* d_wq = nullptr;     // quadrature weights, sum 1 (J)
+  double* d_wl = nullptr;     // y_Lat_Weight, sum 2 (J)
+  double* d_cwv = nullptr;    // north-half weights wq (J2pad)
+  double* d_cwd = nullptr;    // north-half weights wq/(1-mu^2) (J2pad)
+  double* d_rs_icos = nullptr;    // 1/cos(phi)       (J)
+  double* d_rs_alpha = nullptr;   // 1/(R cos^2 phi)  (J)
+  double* d_rs_uv = nullptr;      // 1/(R cos phi)    (J)
+  double* d_pro_psi = nullptr;    // R^2 * LaplaInv = -R^2/(n(n+1)), 0 at n=0 (L)
+  // Legendre tables (FP64), swizzled into MFMA fragment order
+  double* d_tabFP = nullptr;
+  double* d_tabFH = nullptr;
+  double* d_tabIP = nullptr;
+  double* d_tabIH = nullptr;
+  long* d_offF = nullptr;
+  long* d_offI = nullptr;
+  std::vector<long> offF, offI;
+  long sizeF = 0, sizeI = 0;
+  dsht::FftPlan fft;
+  // workspaces keyed by stream
+  std::mutex ws_mu;
+  std::map<void*, std::unique_ptr<dsht::Workspace>> ws;
+  // mixed-precision tables (built lazily)
+  void* mixed = nullptr;
+  int chunk_cols = 0;  // column chunk of the device path (0 = auto)
+};
+
+namespace dsht {
+// error handling
+void set_error(const char* fmt, ...);
+int fail(int code, const char* fmt, ...);
+#define DSHT_HIP(call)                                                            \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::dsht::fail(DSHT_EDEVICE, "%s:%d %s: %s", __FILE__, __LINE__, #call, \
+                          hipGetErrorString(e_));                                 \
+  } while (0)
+
+// tables.hip
+int build_tables(dsht_plan* p);
+// fft.hip
+int fft_plan_init(dsht_plan* p);
+void fft_plan_free(dsht_plan* p);
+int launch_fft_fwd(const dsht_plan* p, const double* xy, long ldg, int ncols, double2* G, long gld,
+                   hipStream_t s);
+int launch_fft_inv(const dsht_plan* p, const double2* G, long gld, int ncols, double* xy, long ldg,
+                   hipStream_t s);
+// legendre_f64.hip
+int launch_leg_fwd(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t s);
+int launch_leg_inv(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t s);
+// ops.hip
+int launch_diag(const double* d, const double* w, double* out, long sl, long sc, int L, int ncols,
+                hipStream_t s);
+int launch_lonlat_sum(const dsht_plan* p, const double* xy, long ldg, int ncols, double* out,
+                      double factor, hipStream_t s);
+int launch_lon_sum(const dsht_plan* p, const double* xy, long ldg, int ncols, double* out,
+                   double factor, long out_ld, hipStream_t s);
+int launch_interpolate(const dsht_plan* p, const double* w, long sl, const double* lon,
+                       const double* lat, double* out, int npts, hipStream_t s);
+int launch_coef(const dsht_plan* p, int which, const double* params, int nparams, double* out,
+                hipStream_t s);
+int launch_transpose(const double* in, long ld_in, double* out, long ld_out, int rows, int cols,
+                     hipStream_t s);
+// legendre_mixed.hip: split-precision forms
+void mixed_free(dsht_plan* p);
+int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, long ldg, double* out0,
+                  double* out1, long sl, long sc, int ncols, const double* diag0, const double* diag1,
+                  double scale, const double* cw, hipStream_t s);
+int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, long sl, long sc,
+                  double* out0, double* out1, long ldg, int ncols, const double* pro, const double* rs,
+                  hipStream_t s);
+}  // namespace dsht

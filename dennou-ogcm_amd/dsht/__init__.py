@@ -1,0 +1,411 @@
+"""Python host wrapper of libdsht.so (the C ABI in include/dsht.h).
+
+Mirrors the horizontal SpmlUtil_mod API of dogcm (model/dogcm/src/util/math/
+SpmlUtil_mod.f90:71-135): the same operator names with the same argument meaning, so a
+test written against SpmlUtil reads the same here.  Arrays follow the Fortran layouts
+in C order:
+
+    grid      numpy (ncols, J, I) or (J, I)  == Fortran xya(0:I-1, J, ncols)
+    spectral  numpy (ncols, L)   or (L,)     == Fortran wa(L, ncols), SPMODEL l order
+
+The device-resident entry points (``*_d``) take torch CUDA tensors and run on the
+current torch stream.  There is no CPU fallback: if libdsht.so is missing or no GPU is
+present the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libdsht.so")
+
+# kinds
+VAL, DIVLAMBDA, DIVMU, GRADLAMBDA, GRADMU, GRADLON, GRADLAT = range(7)
+# precision forms
+PREC_FP64, PREC_BF16X3, PREC_FP16, PREC_BF16X6, PREC_FP16X3 = range(5)
+PREC_NAMES = {PREC_FP64: "fp64", PREC_BF16X3: "bf16x3", PREC_FP16: "fp16",
+              PREC_BF16X6: "bf16x6", PREC_FP16X3: "fp16x3"}
+# coefficient tables
+(COEF_LAPLA, COEF_LAPLAINV, COEF_HDIFF, COEF_HVISC, COEF_ADVFILTER, COEF_HELMHOLTZ,
+ COEF_BEDIFF) = range(7)
+
+# every entry point declared in include/dsht.h
+EXPORTS = [
+    "dsht_create", "dsht_destroy", "dsht_last_error", "dsht_get_dims", "dsht_set_precision",
+    "dsht_get_precision", "dsht_grid_info", "dsht_nm_l", "dsht_l_nm", "dsht_rn", "dsht_coef",
+    "dsht_w_xy", "dsht_xy_w", "dsht_w_alpha_xy", "dsht_xy_alpha_w", "dsht_uvcoslat2vordiv",
+    "dsht_vordiv2uv", "dsht_w_diag_w", "dsht_xy_diag_xy", "dsht_intlonlat", "dsht_avrlonlat",
+    "dsht_ya_intlon", "dsht_ya_avrlon", "dsht_interpolate", "dsht_fwd_d", "dsht_inv_d",
+    "dsht_alpha_fwd_d", "dsht_alpha_inv_d", "dsht_uv2vordiv_d", "dsht_vordiv2uv_d",
+    "dsht_diag_d", "dsht_intlonlat_d", "dsht_avrlonlat_d", "dsht_ya_intlon_d",
+    "dsht_interpolate_d", "dsht_coef_d",
+]
+
+
+class DshtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dsht error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Loads libdsht.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run python dennou-ogcm_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, d, lg = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    ip = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "dsht_create": ([i, i, i, i, d, pp], i),
+        "dsht_destroy": ([vp], i),
+        "dsht_last_error": ([], ctypes.c_char_p),
+        "dsht_get_dims": ([vp, ip, ip, ip, ip], i),
+        "dsht_set_precision": ([vp, i], i),
+        "dsht_get_precision": ([vp], i),
+        "dsht_grid_info": ([vp, vp, vp, vp, vp, vp, vp], i),
+        "dsht_nm_l": ([vp, vp, vp], i),
+        "dsht_l_nm": ([vp, i, i], i),
+        "dsht_rn": ([vp, vp], i),
+        "dsht_coef": ([vp, i, vp, i, vp], i),
+        "dsht_w_xy": ([vp, i, vp, vp, i], i),
+        "dsht_xy_w": ([vp, i, vp, vp, i], i),
+        "dsht_w_alpha_xy": ([vp, vp, vp, vp, i], i),
+        "dsht_xy_alpha_w": ([vp, vp, vp, vp, i], i),
+        "dsht_uvcoslat2vordiv": ([vp, vp, vp, vp, vp, i], i),
+        "dsht_vordiv2uv": ([vp, vp, vp, vp, vp, i], i),
+        "dsht_w_diag_w": ([vp, vp, vp, vp, i], i),
+        "dsht_xy_diag_xy": ([vp, vp, vp, vp, i], i),
+        "dsht_intlonlat": ([vp, vp, vp, i], i),
+        "dsht_avrlonlat": ([vp, vp, vp, i], i),
+        "dsht_ya_intlon": ([vp, vp, vp, i], i),
+        "dsht_ya_avrlon": ([vp, vp, vp, i], i),
+        "dsht_interpolate": ([vp, vp, vp, vp, vp, i], i),
+        "dsht_fwd_d": ([vp, i, vp, lg, vp, lg, lg, i, vp, vp], i),
+        "dsht_inv_d": ([vp, i, vp, lg, lg, vp, lg, i, vp, vp], i),
+        "dsht_alpha_fwd_d": ([vp, vp, vp, lg, vp, lg, lg, i, vp, vp], i),
+        "dsht_alpha_inv_d": ([vp, vp, vp, lg, lg, vp, lg, i, vp], i),
+        "dsht_uv2vordiv_d": ([vp, vp, vp, lg, vp, vp, lg, lg, i, vp, vp], i),
+        "dsht_vordiv2uv_d": ([vp, vp, vp, lg, lg, vp, vp, lg, i, vp], i),
+        "dsht_diag_d": ([vp, vp, vp, vp, lg, lg, i, vp], i),
+        "dsht_intlonlat_d": ([vp, vp, lg, vp, i, vp], i),
+        "dsht_avrlonlat_d": ([vp, vp, lg, vp, i, vp], i),
+        "dsht_ya_intlon_d": ([vp, vp, lg, vp, i, vp], i),
+        "dsht_interpolate_d": ([vp, vp, lg, vp, vp, vp, i, vp], i),
+        "dsht_coef_d": ([vp, i, vp, i, vp, vp], i),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise DshtError(rc, load().dsht_last_error().decode())
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _tptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    if stream is not None:
+        return ctypes.c_void_p(stream)
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Plan:
+    """One transform set-up: replaces SpmlUtil_Init(iMax, jMax, kMax, nMax, tMax, RPlanet)."""
+
+    def __init__(self, nmax, im, jm, kmax=1, radius=1.0):
+        L = load()
+        h = ctypes.c_void_p()
+        _check(L.dsht_create(int(nmax), int(im), int(jm), int(kmax), float(radius), ctypes.byref(h)))
+        self.h = h
+        self.nmax, self.im, self.jm, self.kmax, self.radius = nmax, im, jm, kmax, float(radius)
+        self.lm = (nmax + 1) ** 2
+        x = [np.zeros(im), np.zeros(jm), np.zeros(im), np.zeros(jm),
+             np.zeros((jm, im)), np.zeros((jm, im))]
+        _check(L.dsht_grid_info(h, *[_ptr(a) for a in x]))
+        (self.x_Lon, self.y_Lat, self.x_Lon_Weight, self.y_Lat_Weight,
+         self.xy_Lon, self.xy_Lat) = x
+        self.xy_CosLat = np.cos(self.xy_Lat)
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            load().dsht_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -------------------------------------------------- configuration / maps
+    def set_precision(self, prec):
+        _check(load().dsht_set_precision(self.h, int(prec)))
+
+    @property
+    def precision(self):
+        return load().dsht_get_precision(self.h)
+
+    def nm_l(self):
+        n = np.zeros(self.lm, dtype=np.int32)
+        m = np.zeros(self.lm, dtype=np.int32)
+        _check(load().dsht_nm_l(self.h, _ptr(n), _ptr(m)))
+        return n, m
+
+    def l_nm(self, n, m):
+        """1-based SPMODEL index (0 for an invalid pair)."""
+        return load().dsht_l_nm(self.h, int(n), int(m))
+
+    @property
+    def rn(self):
+        r = np.zeros(self.lm)
+        _check(load().dsht_rn(self.h, _ptr(r)))
+        return r
+
+    def coef(self, which, params=()):
+        p = _f64(np.asarray(params, dtype=np.float64).reshape(-1)) if len(params) else np.zeros(1)
+        out = np.zeros(self.lm)
+        _check(load().dsht_coef(self.h, int(which), _ptr(p), len(params), _ptr(out)))
+        return out
+
+    # -------------------------------------------------- host API helpers
+    def _grid_in(self, a):
+        a = _f64(a)
+        if a.size % (self.im * self.jm):
+            raise ValueError(f"grid array size {a.size} is not a multiple of I*J")
+        return a, a.size // (self.im * self.jm)
+
+    def _spec_in(self, a):
+        a = _f64(a)
+        if a.size % self.lm:
+            raise ValueError(f"spectral array size {a.size} is not a multiple of L")
+        return a, a.size // self.lm
+
+    def _fwd(self, xy, kind):
+        a, nc = self._grid_in(xy)
+        w = np.empty((nc, self.lm)) if a.ndim == 3 else np.empty(self.lm)
+        _check(load().dsht_w_xy(self.h, kind, _ptr(a), _ptr(w), nc))
+        return w
+
+    def _inv(self, w, kind):
+        a, nc = self._spec_in(w)
+        xy = np.empty((nc, self.jm, self.im)) if a.ndim == 2 else np.empty((self.jm, self.im))
+        _check(load().dsht_xy_w(self.h, kind, _ptr(a), _ptr(xy), nc))
+        return xy
+
+    # -------------------------------------------------- SpmlUtil / SPMODEL names
+    def w_xy(self, xy):
+        return self._fwd(xy, VAL)
+
+    wa_xya = w_xy
+
+    def xy_w(self, w):
+        return self._inv(w, VAL)
+
+    xya_wa = xy_w
+
+    def w_DivLambda_xy(self, xy):
+        return self._fwd(xy, DIVLAMBDA)
+
+    def w_DivMu_xy(self, xy):
+        return self._fwd(xy, DIVMU)
+
+    def xy_GradLambda_w(self, w):
+        return self._inv(w, GRADLAMBDA)
+
+    def xy_GradMu_w(self, w):
+        return self._inv(w, GRADMU)
+
+    def xy_GradLon_w(self, w):
+        return self._inv(w, GRADLON)
+
+    def xy_GradLat_w(self, w):
+        return self._inv(w, GRADLAT)
+
+    xya_GradLambda_wa = xy_GradLambda_w
+    xya_GradMu_wa = xy_GradMu_w
+    xya_GradLon_wa = xy_GradLon_w
+    xya_GradLat_wa = xy_GradLat_w
+
+    def w_AlphaOptr_xy(self, xy_A, xy_B):
+        a, nc = self._grid_in(xy_A)
+        b, nc2 = self._grid_in(xy_B)
+        if nc != nc2:
+            raise ValueError("A and B differ in size")
+        w = np.empty((nc, self.lm)) if a.ndim == 3 else np.empty(self.lm)
+        _check(load().dsht_w_alpha_xy(self.h, _ptr(a), _ptr(b), _ptr(w), nc))
+        return w
+
+    wa_AlphaOptr_xya = w_AlphaOptr_xy
+
+    def xy_AlphaOptr_w(self, w_A, w_B):
+        a, nc = self._spec_in(w_A)
+        b, nc2 = self._spec_in(w_B)
+        if nc != nc2:
+            raise ValueError("A and B differ in size")
+        xy = np.empty((nc, self.jm, self.im)) if a.ndim == 2 else np.empty((self.jm, self.im))
+        _check(load().dsht_xy_alpha_w(self.h, _ptr(a), _ptr(b), _ptr(xy), nc))
+        return xy
+
+    xya_AlphaOptr_wa = xy_AlphaOptr_w
+
+    def calc_UVCosLat2VorDiv(self, xy_UCosLat, xy_VCosLat):
+        u, nc = self._grid_in(xy_UCosLat)
+        v, _ = self._grid_in(xy_VCosLat)
+        shape = (nc, self.lm) if u.ndim == 3 else (self.lm,)
+        vor, div = np.empty(shape), np.empty(shape)
+        _check(load().dsht_uvcoslat2vordiv(self.h, _ptr(u), _ptr(v), _ptr(vor), _ptr(div), nc))
+        return vor, div
+
+    def calc_VorDiv2UV(self, w_Vor, w_Div):
+        z, nc = self._spec_in(w_Vor)
+        d, _ = self._spec_in(w_Div)
+        shape = (nc, self.jm, self.im) if z.ndim == 2 else (self.jm, self.im)
+        u, v = np.empty(shape), np.empty(shape)
+        _check(load().dsht_vordiv2uv(self.h, _ptr(z), _ptr(d), _ptr(u), _ptr(v), nc))
+        return u, v
+
+    def w_diag_w(self, d, w):
+        a, nc = self._spec_in(w)
+        dd = _f64(d)
+        out = np.empty_like(a)
+        _check(load().dsht_w_diag_w(self.h, _ptr(dd), _ptr(a), _ptr(out), nc))
+        return out
+
+    def w_Lapla_w(self, w):
+        return self.w_diag_w(self.coef(COEF_LAPLA), w)
+
+    def w_LaplaInv_w(self, w):
+        return self.w_diag_w(self.coef(COEF_LAPLAINV), w)
+
+    def xy_diag_xy(self, d, xy):
+        """xy_w(d * w_xy(xy)) in one call (the implicit-diffusion epilogue)."""
+        a, nc = self._grid_in(xy)
+        dd = _f64(d)
+        out = np.empty_like(a)
+        _check(load().dsht_xy_diag_xy(self.h, _ptr(dd), _ptr(a), _ptr(out), nc))
+        return out
+
+    def _sum(self, fn, xy, per_lat):
+        a, nc = self._grid_in(xy)
+        out = np.empty((nc, self.jm) if per_lat else nc)
+        _check(getattr(load(), fn)(self.h, _ptr(a), _ptr(out), nc))
+        if a.ndim == 2:
+            return out[0] if per_lat else float(out[0])
+        return out
+
+    def IntLonLat_xy(self, xy):
+        return self._sum("dsht_intlonlat", xy, False)
+
+    def AvrLonLat_xy(self, xy):
+        return self._sum("dsht_avrlonlat", xy, False)
+
+    def ya_IntLon_xya(self, xya):
+        return self._sum("dsht_ya_intlon", xya, True)
+
+    def ya_AvrLon_xya(self, xya):
+        return self._sum("dsht_ya_avrlon", xya, True)
+
+    def Interpolate_w(self, w, alon, alat):
+        ww = _f64(w)
+        lon = _f64(np.atleast_1d(alon))
+        lat = _f64(np.atleast_1d(alat))
+        out = np.empty(lon.size)
+        _check(load().dsht_interpolate(self.h, _ptr(ww), _ptr(lon), _ptr(lat), _ptr(out), lon.size))
+        return out if np.ndim(alon) else float(out[0])
+
+    # -------------------------------------------------- device-resident API (torch)
+    # grids: tensor (ncols, J, I) float64 CUDA; spectra: tensor (L, ncols) ("LC", fast) or
+    # (ncols, L) ("CL", Fortran order) float64 CUDA.
+    def _spec_strides(self, t, ncols, layout):
+        if layout == "LC":
+            return t.stride(0), 1
+        return 1, t.stride(0)
+
+    def fwd_d(self, kind, xy, w, layout="LC", diag=None, stream=None):
+        nc = xy.shape[0]
+        sl, sc = self._spec_strides(w, nc, layout)
+        _check(load().dsht_fwd_d(self.h, kind, _tptr(xy), xy.stride(0), _tptr(w), sl, sc, nc,
+                                 _tptr(diag) if diag is not None else None, _stream(stream)))
+        return w
+
+    def inv_d(self, kind, w, xy, layout="LC", diag=None, stream=None):
+        nc = xy.shape[0]
+        sl, sc = self._spec_strides(w, nc, layout)
+        _check(load().dsht_inv_d(self.h, kind, _tptr(w), sl, sc, _tptr(xy), xy.stride(0), nc,
+                                 _tptr(diag) if diag is not None else None, _stream(stream)))
+        return xy
+
+    def alpha_fwd_d(self, a, b, w, layout="LC", diag=None, stream=None):
+        nc = a.shape[0]
+        sl, sc = self._spec_strides(w, nc, layout)
+        _check(load().dsht_alpha_fwd_d(self.h, _tptr(a), _tptr(b), a.stride(0), _tptr(w), sl, sc, nc,
+                                       _tptr(diag) if diag is not None else None, _stream(stream)))
+        return w
+
+    def alpha_inv_d(self, wa, wb, xy, layout="LC", stream=None):
+        nc = xy.shape[0]
+        sl, sc = self._spec_strides(wa, nc, layout)
+        _check(load().dsht_alpha_inv_d(self.h, _tptr(wa), _tptr(wb), sl, sc, _tptr(xy), xy.stride(0), nc,
+                                       _stream(stream)))
+        return xy
+
+    def uv2vordiv_d(self, ucos, vcos, vor, div, layout="LC", diag=None, stream=None):
+        nc = ucos.shape[0]
+        sl, sc = self._spec_strides(vor, nc, layout)
+        _check(load().dsht_uv2vordiv_d(self.h, _tptr(ucos), _tptr(vcos), ucos.stride(0), _tptr(vor),
+                                       _tptr(div), sl, sc, nc,
+                                       _tptr(diag) if diag is not None else None, _stream(stream)))
+        return vor, div
+
+    def vordiv2uv_d(self, vor, div, u, v, layout="LC", stream=None):
+        nc = u.shape[0]
+        sl, sc = self._spec_strides(vor, nc, layout)
+        _check(load().dsht_vordiv2uv_d(self.h, _tptr(vor), _tptr(div), sl, sc, _tptr(u), _tptr(v),
+                                       u.stride(0), nc, _stream(stream)))
+        return u, v
+
+    def diag_d(self, d, w, out, layout="LC", stream=None):
+        nc = w.shape[1] if layout == "LC" else w.shape[0]
+        sl, sc = self._spec_strides(w, nc, layout)
+        _check(load().dsht_diag_d(self.h, _tptr(d), _tptr(w), _tptr(out), sl, sc, nc, _stream(stream)))
+        return out
+
+    def avrlonlat_d(self, xy, out, stream=None):
+        _check(load().dsht_avrlonlat_d(self.h, _tptr(xy), xy.stride(0), _tptr(out), xy.shape[0],
+                                       _stream(stream)))
+        return out
+
+    def intlonlat_d(self, xy, out, stream=None):
+        _check(load().dsht_intlonlat_d(self.h, _tptr(xy), xy.stride(0), _tptr(out), xy.shape[0],
+                                       _stream(stream)))
+        return out
+
+    def coef_d(self, which, params, out, stream=None):
+        p = np.asarray(params, dtype=np.float64).reshape(-1)
+        pp = _f64(p) if p.size else np.zeros(1)
+        _check(load().dsht_coef_d(self.h, int(which), _ptr(pp), int(p.size), _tptr(out), _stream(stream)))
+        return out

@@ -1,0 +1,162 @@
+/*
+ * dsht.h -- C ABI of the MI355X spherical-harmonic transform library (libdsht.so).
+ *
+ * Drop-in replacement for the horizontal half of dogcm's SpmlUtil_mod
+ * (model/dogcm/src/util/math/SpmlUtil_mod.f90:71-135) and the SPMODEL wa_module /
+ * ISPACK routines it cascades.  The Fortran module in
+ * dennou-ogcm_amd/fortran/SpmlUtil_mod.f90 binds these entry points with
+ * ISO_C_BINDING and keeps the reference's function-style API.
+ *
+ * Conventions
+ *   grid      xy(0:I-1,1:J) per column, longitude fastest; lambda_i = 2 pi i / I;
+ *             mu_j = sin(lat_j) ascending (south -> north), roots of P_J
+ *             (doc/description/ogcm/space_discretization.tex:27-38).
+ *   spectral  SPMODEL order l = n(n+1)+m+1 (1-based; 0-based index n(n+1)+m),
+ *             w(l_nm(n,m)) = Re s_n^m for m >= 0, w(l_nm(n,-m)) = Im s_n^m for m > 0,
+ *             q = sum s_n^m P_n^m(mu) e^{i m lambda}, s_n^{-m} = conj(s_n^m),
+ *             int_{-1}^{1} (P_n^m)^2 dmu = 2, no Condon-Shortley phase.
+ *   host API  arrays are Fortran column-major: xya(0:I-1,J,ncols), wa(L,ncols).
+ *   device API  pointers are device memory; grids: column c at xy + c*ldg;
+ *             spectra: element (l, c) at w + l*sl + c*sc  (DSHT layout "LC":
+ *             sl = ld >= ncols, sc = 1 is the fast layout; Fortran "CL": sl = 1, sc = ld).
+ *   All entry points return 0 on success or a DSHT_E* code; dsht_last_error()
+ *   gives the message (thread-local).  The host API is reentrant: each calling
+ *   thread gets its own HIP stream and buffers (the reference calls SpmlUtil from
+ *   inside `!$omp parallel do` loops, SpmlUtil_mod.f90:360,389,446).
+ */
+#ifndef DSHT_H
+#define DSHT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define DSHT_OK 0
+#define DSHT_EARG 1        /* invalid argument (sizes, null pointers, kind)  */
+#define DSHT_ENOMEM 2      /* device or host allocation failed               */
+#define DSHT_EDEVICE 3     /* HIP runtime error                              */
+#define DSHT_ESTATE 4      /* plan not initialised / already destroyed       */
+#define DSHT_EUNSUPPORTED 5
+
+/* transform kinds */
+#define DSHT_VAL 0        /* w_xy / xy_w                                      */
+#define DSHT_DIVLAMBDA 1  /* w_DivLambda_xy : (1/(1-mu^2)) d/dlambda          */
+#define DSHT_DIVMU 2      /* w_DivMu_xy     : d/dmu                            */
+#define DSHT_GRADLAMBDA 3 /* xy_GradLambda_w: d/dlambda                        */
+#define DSHT_GRADMU 4     /* xy_GradMu_w    : (1-mu^2) d/dmu                   */
+#define DSHT_GRADLON 5    /* xy_GradLon_w   : (1/cos phi) d/dlambda            */
+#define DSHT_GRADLAT 6    /* xy_GradLat_w   : d/dphi                           */
+
+/* Legendre-contraction precision forms */
+#define DSHT_PREC_FP64 0    /* FP64 MFMA end to end                          */
+#define DSHT_PREC_BF16X3 1  /* 2-term BF16 split, 3 products, FP32 accumulate  */
+#define DSHT_PREC_FP16 2    /* FP16 with per-m power-of-two scaling, FP32 acc  */
+#define DSHT_PREC_BF16X6 3  /* 3-term BF16 split, 6 products, FP32 accumulate  */
+#define DSHT_PREC_FP16X3 4  /* 2-term FP16 split (scaled), 3 products, FP32 acc */
+
+/* diagonal spectral coefficient tables (dsht_coef) */
+#define DSHT_COEF_LAPLA 0      /* -n(n+1)                 (w_Lapla_w)                          */
+#define DSHT_COEF_LAPLAINV 1   /* -1/(n(n+1)), 0 at n=0    (w_LaplaInv_w)                       */
+#define DSHT_COEF_HDIFF 2      /* params {kappa, numdiff, ord}: w_HDiffCoefH, LPhys_DIFF_spm_mod.f90:148-149 */
+#define DSHT_COEF_HVISC 3      /* params {nu, numdiff, ord}: w_HViscCoefH, LPhys_DIFF_spm_mod.f90:151-154    */
+#define DSHT_COEF_ADVFILTER 4  /* w_AdvFilter, HBEBaroc_hspm_vfvm_mod.f90:83-95                 */
+#define DSHT_COEF_HELMHOLTZ 5  /* params {diag, c0}: w_Coef, HBEBarot_hspm_vfvm_mod.f90:290-299 */
+#define DSHT_COEF_BEDIFF 6     /* params {dt, kappa|nu, numdiff, ord, momentum(0/1)}:
+                                  1/(1 - dt*c_n), DOGCM_Phys_hspm_vfvm_mod.f90:640,741-742 */
+
+typedef struct dsht_plan dsht_plan;
+
+/* ---------------------------------------------------------------- setup
+ * dsht_create replaces SpmlUtil_Init -> wa_Initial(nMax,iMax,jMax,kMax+1[,np])
+ * (SpmlUtil_mod.f90:192-241).  Builds Gauss nodes/weights and the P_n^m, H_n^m
+ * tables in FP64 on the current HIP device.  Requires im even, im >= 2*nmax+2,
+ * im/2 a product of 2, 3, 5, jm even, jm >= nmax+1.  kmax is a sizing hint.      */
+int dsht_create(int nmax, int im, int jm, int kmax, double radius, dsht_plan** plan);
+/* SpmlUtil_Final (SpmlUtil_mod.f90:246-255) */
+int dsht_destroy(dsht_plan* plan);
+const char* dsht_last_error(void);
+int dsht_get_dims(const dsht_plan* plan, int* nmax, int* im, int* jm, int* lm);
+int dsht_set_precision(dsht_plan* plan, int prec);
+int dsht_get_precision(const dsht_plan* plan);
+
+/* get_SpmlGridInfo (SpmlUtil_mod.f90:271-295) and the cascaded x_Lon, y_Lat,
+   x_Lon_Weight, y_Lat_Weight (sum 2), xy_Lon, xy_Lat; any pointer may be NULL.   */
+int dsht_grid_info(const dsht_plan* plan, double* x_lon, double* y_lat, double* x_lon_weight,
+                   double* y_lat_weight, double* xy_lon, double* xy_lat);
+/* nm_l / l_nm (cascaded, SpmlUtil_mod.f90:120): 1-based l; returns 0 for bad (n,m) */
+int dsht_nm_l(const dsht_plan* plan, int* n_of_l, int* m_of_l);
+int dsht_l_nm(const dsht_plan* plan, int n, int m);
+/* rn(l,1) = -n(n+1) (cascaded rn, used in LPhys_DIFF_spm_mod.f90:144) */
+int dsht_rn(const dsht_plan* plan, double* rn);
+/* diagonal coefficient table of length L into a host array */
+int dsht_coef(const dsht_plan* plan, int which, const double* params, int nparams, double* out);
+
+/* ---------------------------------------------------------------- host API
+ * Fortran arrays in host memory; ncols columns (layers).                       */
+/* w_xy, w_DivLambda_xy, w_DivMu_xy, wa_xya (kind = DSHT_VAL/DIVLAMBDA/DIVMU)    */
+int dsht_w_xy(dsht_plan* plan, int kind, const double* xy, double* w, int ncols);
+/* xy_w, xy_GradLambda_w, xy_GradMu_w, xy_GradLon_w, xy_GradLat_w, xya_wa          */
+int dsht_xy_w(dsht_plan* plan, int kind, const double* w, double* xy, int ncols);
+/* w_AlphaOptr_xy / wa_AlphaOptr_xya (SpmlUtil_mod.f90:398-412,453-473)           */
+int dsht_w_alpha_xy(dsht_plan* plan, const double* a, const double* b, double* w, int ncols);
+/* xy_AlphaOptr_w / xya_AlphaOptr_wa (SpmlUtil_mod.f90:414-451)                   */
+int dsht_xy_alpha_w(dsht_plan* plan, const double* wa, const double* wb, double* xy, int ncols);
+/* calc_UVCosLat2VorDiv (SpmlUtil_mod.f90:299-314, 342-368)                        */
+int dsht_uvcoslat2vordiv(dsht_plan* plan, const double* ucos, const double* vcos, double* vor,
+                         double* div, int ncols);
+/* calc_VorDiv2UV (SpmlUtil_mod.f90:316-340, 370-396)                              */
+int dsht_vordiv2uv(dsht_plan* plan, const double* vor, const double* div, double* u, double* v,
+                   int ncols);
+/* out = d[l] * w  (w_Lapla_w, w_LaplaInv_w, filters, implicit-diffusion factors) */
+int dsht_w_diag_w(dsht_plan* plan, const double* d, const double* w, double* out, int ncols);
+/* Forward transform, diagonal epilogue, inverse transform in one call:
+   xy_w( d * w_xy(xy) ) -- DOGCM_Phys_hspm_vfvm_mod.f90:636-644 with d = w_Filter/(1-dt*c) */
+int dsht_xy_diag_xy(dsht_plan* plan, const double* d, const double* xy, double* out, int ncols);
+/* IntLonLat_xy / AvrLonLat_xy per column; ya_IntLon_xya / ya_AvrLon_xya per (j, column) */
+int dsht_intlonlat(dsht_plan* plan, const double* xy, double* out, int ncols);
+int dsht_avrlonlat(dsht_plan* plan, const double* xy, double* out, int ncols);
+int dsht_ya_intlon(dsht_plan* plan, const double* xy, double* out, int ncols);
+int dsht_ya_avrlon(dsht_plan* plan, const double* xy, double* out, int ncols);
+/* Interpolate_w / a_Interpolate_wa: column w evaluated at npts (lon, lat) points */
+int dsht_interpolate(dsht_plan* plan, const double* w, const double* lon, const double* lat,
+                     double* out, int npts);
+
+/* ---------------------------------------------------------------- device API
+ * Device pointers, stream-ordered on `stream` (a hipStream_t; NULL = default).
+ * diag: optional device array of L multipliers fused into the spectral side
+ * (forward epilogue / inverse prologue); NULL means 1.                         */
+int dsht_fwd_d(dsht_plan* plan, int kind, const double* xy, long ldg, double* w, long sl, long sc,
+               int ncols, const double* diag, void* stream);
+int dsht_inv_d(dsht_plan* plan, int kind, const double* w, long sl, long sc, double* xy, long ldg,
+               int ncols, const double* diag, void* stream);
+int dsht_alpha_fwd_d(dsht_plan* plan, const double* a, const double* b, long ldg, double* w,
+                     long sl, long sc, int ncols, const double* diag, void* stream);
+int dsht_alpha_inv_d(dsht_plan* plan, const double* wa, const double* wb, long sl, long sc,
+                     double* xy, long ldg, int ncols, void* stream);
+int dsht_uv2vordiv_d(dsht_plan* plan, const double* ucos, const double* vcos, long ldg,
+                     double* vor, double* div, long sl, long sc, int ncols, const double* diag,
+                     void* stream);
+int dsht_vordiv2uv_d(dsht_plan* plan, const double* vor, const double* div, long sl, long sc,
+                     double* u, double* v, long ldg, int ncols, void* stream);
+int dsht_diag_d(dsht_plan* plan, const double* d, const double* w, double* out, long sl, long sc,
+                int ncols, void* stream);
+/* FP64 global sums; out is device memory */
+int dsht_intlonlat_d(dsht_plan* plan, const double* xy, long ldg, double* out, int ncols,
+                     void* stream);
+int dsht_avrlonlat_d(dsht_plan* plan, const double* xy, long ldg, double* out, int ncols,
+                     void* stream);
+int dsht_ya_intlon_d(dsht_plan* plan, const double* xy, long ldg, double* out, int ncols,
+                     void* stream);
+int dsht_interpolate_d(dsht_plan* plan, const double* w, long sl, const double* lon,
+                       const double* lat, double* out, int npts, void* stream);
+/* coefficient table written to device memory */
+int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, double* out,
+                void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSHT_H */

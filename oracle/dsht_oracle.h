@@ -1,0 +1,87 @@
+/*
+ * dsht_oracle.h -- FP64 CPU restatement of dogcm's spherical-harmonic transform path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the parity checker for the HIP library in
+ * dennou-ogcm_amd/ and the timed CPU baseline of bench.py.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the product
+ * path (libdsht.so, the Fortran SpmlUtil_mod shim, the Python host wrapper) never
+ * calls into it.
+ *
+ * Conventions (the same ones the HIP library implements):
+ *   grid     xy(0:I-1, 1:J) per column, longitude fastest: x[c*I*J + j*I + i],
+ *            lambda_i = 2*pi*i/I, mu_j ascending (south -> north), roots of P_J
+ *            (doc/description/ogcm/space_discretization.tex:27-38).
+ *   spectral w(l) per column, l = n*(n+1) + m (0-based; SPMODEL's l_nm(n,m)-1):
+ *            w(n,m)  = Re s_n^m  (m >= 0),  w(n,-m) = Im s_n^m (m > 0),
+ *            q(lambda,mu) = sum_{n,|m|<=n} s_n^m P_n^m(mu) e^{i m lambda},
+ *            s_n^{-m} = conj(s_n^m), P_n^m normalised to int_{-1}^{1} P^2 dmu = 2,
+ *            no Condon-Shortley phase (space_discretization.tex:160-170).
+ *   forward  G^m(mu_j) = (1/I) sum_i q_ij e^{-i m lambda_i};
+ *            s_n^m = sum_j w_j G^m(mu_j) P_n^m(mu_j), sum_j w_j = 1 (tex:188-208).
+ *   inverse  G^m(mu_j) = sum_n s_n^m P_n^m(mu_j); q_ij = sum_m G^m e^{i m lambda_i} (tex:211-219).
+ */
+#ifndef DSHT_ORACLE_H
+#define DSHT_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc orc_t;
+
+/* transform kinds (same numbering as include/dsht.h) */
+enum {
+  ORC_VAL = 0,        /* w_xy / xy_w                                   */
+  ORC_DIVLAMBDA = 1,  /* w_DivLambda_xy : (1/(1-mu^2)) d/dlambda        */
+  ORC_DIVMU = 2,      /* w_DivMu_xy     : d/dmu                          */
+  ORC_GRADLAMBDA = 3, /* xy_GradLambda_w: d/dlambda                      */
+  ORC_GRADMU = 4,     /* xy_GradMu_w    : (1-mu^2) d/dmu                 */
+  ORC_GRADLON = 5,    /* xy_GradLon_w   : (1/cos phi) d/dlambda          */
+  ORC_GRADLAT = 6     /* xy_GradLat_w   : d/dphi                         */
+};
+
+const char* orc_last_error(void);
+orc_t* orc_create(int nmax, int im, int jm, double radius);
+void orc_destroy(orc_t* h);
+int orc_set_threads(int n);
+
+/* grid: mu[J], w[J] (sum 1), lat[J], lon[I], y_lat_weight[J] (sum 2), x_lon_weight[I] */
+void orc_grid(const orc_t* h, double* mu, double* wq, double* lat, double* lon,
+              double* y_lat_weight, double* x_lon_weight);
+/* P_n^m(mu_j), n = m..nmax+1 and H_n^m = (1-mu^2) dP/dmu, n = m..nmax, for all j */
+int orc_legendre(const orc_t* h, int m, int n, double* p_out, double* h_out);
+/* longitude real FFT of nrow rows: G[r*(N+1)+m] (complex, interleaved re/im) */
+void orc_fft_fwd(const orc_t* h, const double* x, double* G, int nrow);
+void orc_fft_inv(const orc_t* h, const double* G, double* x, int nrow);
+
+/* transforms over ncols columns (grid column stride I*J, spectral column stride L) */
+int orc_fwd(const orc_t* h, int kind, const double* xy, double* w, int ncols);
+int orc_inv(const orc_t* h, int kind, const double* w, double* xy, int ncols);
+int orc_alpha_fwd(const orc_t* h, const double* a, const double* b, double* w, int ncols);
+int orc_alpha_inv(const orc_t* h, const double* wa, const double* wb, double* xy, int ncols);
+int orc_uv2vordiv(const orc_t* h, const double* ucos, const double* vcos,
+                  double* vor, double* div, int ncols);
+int orc_vordiv2uv(const orc_t* h, const double* vor, const double* div,
+                  double* u, double* v, int ncols);
+void orc_lapla(const orc_t* h, const double* w, double* out, int ncols);
+void orc_laplainv(const orc_t* h, const double* w, double* out, int ncols);
+/* integrals: IntLonLat_xy, AvrLonLat_xy per column; ya_IntLon_xya / ya_AvrLon_xya per row */
+void orc_intlonlat(const orc_t* h, const double* xy, double* out, int ncols);
+void orc_avrlonlat(const orc_t* h, const double* xy, double* out, int ncols);
+void orc_ya_intlon(const orc_t* h, const double* xy, double* out, int ncols);
+void orc_ya_avrlon(const orc_t* h, const double* xy, double* out, int ncols);
+/* Interpolate_w: value of column w at (lon, lat) points */
+void orc_interpolate(const orc_t* h, const double* w, const double* lon,
+                     const double* lat, double* out, int npts);
+/* diagonal coefficient tables (length L), LPhys_DIFF_spm_mod.f90:140-167,
+   HBEBaroc_hspm_vfvm_mod.f90:83-95, HBEBarot_hspm_vfvm_mod.f90:286-305 */
+void orc_coef_hdiff(const orc_t* h, double kappa, double numdiff, int ord, double* c);
+void orc_coef_hvisc(const orc_t* h, double nu, double numdiff, int ord, double* c);
+void orc_coef_advfilter(const orc_t* h, double* f);
+void orc_coef_helmholtz(const orc_t* h, double diag, double c0, double* c);
+/* w <- d[l] * w  (column-wise) */
+void orc_diag(const orc_t* h, const double* d, double* w, int ncols);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

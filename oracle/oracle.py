@@ -1,0 +1,253 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/dsht_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, as the checker.  The product path never imports this module.
+
+Array conventions (numpy, C order):
+  grid      (ncols, J, I)   -- SpmlUtil xy(0:I-1,1:J) per column, longitude fastest
+  spectral  (ncols, L)      -- SPMODEL order, l = n(n+1)+m, Im parts at m < 0
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libdsht_oracle.so")
+
+VAL, DIVLAMBDA, DIVMU, GRADLAMBDA, GRADMU, GRADLON, GRADLAT = range(7)
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        dp = ctypes.POINTER(ctypes.c_double)
+        vp = ctypes.c_void_p
+        i = ctypes.c_int
+        d = ctypes.c_double
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [i, i, i, d]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_set_threads.argtypes = [i]
+        L.orc_set_threads.restype = i
+        L.orc_grid.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.orc_legendre.argtypes = [vp, i, i, dp, dp]
+        L.orc_legendre.restype = i
+        L.orc_fft_fwd.argtypes = [vp, dp, dp, i]
+        L.orc_fft_inv.argtypes = [vp, dp, dp, i]
+        for f in ("orc_fwd", "orc_inv"):
+            getattr(L, f).argtypes = [vp, i, dp, dp, i]
+            getattr(L, f).restype = i
+        for f in ("orc_alpha_fwd", "orc_alpha_inv"):
+            getattr(L, f).argtypes = [vp, dp, dp, dp, i]
+            getattr(L, f).restype = i
+        for f in ("orc_uv2vordiv", "orc_vordiv2uv"):
+            getattr(L, f).argtypes = [vp, dp, dp, dp, dp, i]
+            getattr(L, f).restype = i
+        for f in ("orc_lapla", "orc_laplainv", "orc_intlonlat", "orc_avrlonlat",
+                  "orc_ya_intlon", "orc_ya_avrlon"):
+            getattr(L, f).argtypes = [vp, dp, dp, i]
+        L.orc_interpolate.argtypes = [vp, dp, dp, dp, dp, i]
+        L.orc_coef_hdiff.argtypes = [vp, d, d, i, dp]
+        L.orc_coef_hvisc.argtypes = [vp, d, d, i, dp]
+        L.orc_coef_advfilter.argtypes = [vp, dp]
+        L.orc_coef_helmholtz.argtypes = [vp, d, d, dp]
+        L.orc_diag.argtypes = [vp, dp, dp, i]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def set_threads(n):
+    return lib().orc_set_threads(int(n))
+
+
+class Oracle:
+    """One (nmax, im, jm, radius) transform setup on the CPU."""
+
+    def __init__(self, nmax, im, jm, radius=1.0):
+        L = lib()
+        self.h = L.orc_create(nmax, im, jm, float(radius))
+        if not self.h:
+            raise ValueError(L.orc_last_error().decode())
+        self.N, self.I, self.J, self.R = nmax, im, jm, float(radius)
+        self.L = (nmax + 1) ** 2
+        g = [np.zeros(jm) for _ in range(4)] + [np.zeros(im), np.zeros(im)]
+        L.orc_grid(self.h, _p(g[0]), _p(g[1]), _p(g[2]), _p(g[4]), _p(g[3]), _p(g[5]))
+        self.mu, self.wq, self.lat, self.y_lat_weight, self.lon, self.x_lon_weight = g
+        self.coslat = np.sqrt((1 - self.mu) * (1 + self.mu))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_destroy(self.h)
+            self.h = None
+
+    # index maps (SPMODEL l_nm / nm_l, 0-based)
+    def l_nm(self, n, m):
+        return n * (n + 1) + m
+
+    def nm_l(self):
+        n = np.floor(np.sqrt(np.arange(self.L))).astype(int)
+        m = np.arange(self.L) - n * (n + 1)
+        return n, m
+
+    def legendre(self, m, n):
+        p = np.zeros(self.J)
+        h = np.zeros(self.J)
+        if lib().orc_legendre(self.h, m, n, _p(p), _p(h)) != 0:
+            raise ValueError(lib().orc_last_error().decode())
+        return p, h
+
+    def fft_fwd(self, x):
+        x = _c(x)
+        nrow = x.size // self.I
+        G = np.zeros((nrow, self.N + 1, 2))
+        lib().orc_fft_fwd(self.h, _p(x), _p(G), nrow)
+        return G[..., 0] + 1j * G[..., 1]
+
+    def fft_inv(self, G):
+        G = np.ascontiguousarray(np.stack([G.real, G.imag], -1), dtype=np.float64)
+        nrow = G.size // (2 * (self.N + 1))
+        x = np.zeros((nrow, self.I))
+        lib().orc_fft_inv(self.h, _p(G), _p(x), nrow)
+        return x
+
+    def _cols_grid(self, a):
+        a = _c(a)
+        return a, a.size // (self.I * self.J)
+
+    def _cols_spec(self, a):
+        a = _c(a)
+        return a, a.size // self.L
+
+    def fwd(self, xy, kind=VAL):
+        xy, nc = self._cols_grid(xy)
+        w = np.zeros((nc, self.L))
+        if lib().orc_fwd(self.h, kind, _p(xy), _p(w), nc) != 0:
+            raise ValueError(lib().orc_last_error().decode())
+        return w
+
+    def inv(self, w, kind=VAL):
+        w, nc = self._cols_spec(w)
+        xy = np.zeros((nc, self.J, self.I))
+        if lib().orc_inv(self.h, kind, _p(w), _p(xy), nc) != 0:
+            raise ValueError(lib().orc_last_error().decode())
+        return xy
+
+    def alpha_fwd(self, a, b):
+        a, nc = self._cols_grid(a)
+        b = _c(b)
+        w = np.zeros((nc, self.L))
+        lib().orc_alpha_fwd(self.h, _p(a), _p(b), _p(w), nc)
+        return w
+
+    def alpha_inv(self, wa, wb):
+        wa, nc = self._cols_spec(wa)
+        wb = _c(wb)
+        xy = np.zeros((nc, self.J, self.I))
+        lib().orc_alpha_inv(self.h, _p(wa), _p(wb), _p(xy), nc)
+        return xy
+
+    def uv2vordiv(self, ucos, vcos):
+        ucos, nc = self._cols_grid(ucos)
+        vcos = _c(vcos)
+        vor = np.zeros((nc, self.L))
+        div = np.zeros((nc, self.L))
+        lib().orc_uv2vordiv(self.h, _p(ucos), _p(vcos), _p(vor), _p(div), nc)
+        return vor, div
+
+    def vordiv2uv(self, vor, div):
+        vor, nc = self._cols_spec(vor)
+        div = _c(div)
+        u = np.zeros((nc, self.J, self.I))
+        v = np.zeros((nc, self.J, self.I))
+        lib().orc_vordiv2uv(self.h, _p(vor), _p(div), _p(u), _p(v), nc)
+        return u, v
+
+    def lapla(self, w):
+        w, nc = self._cols_spec(w)
+        o = np.zeros((nc, self.L))
+        lib().orc_lapla(self.h, _p(w), _p(o), nc)
+        return o
+
+    def laplainv(self, w):
+        w, nc = self._cols_spec(w)
+        o = np.zeros((nc, self.L))
+        lib().orc_laplainv(self.h, _p(w), _p(o), nc)
+        return o
+
+    def intlonlat(self, xy):
+        xy, nc = self._cols_grid(xy)
+        o = np.zeros(nc)
+        lib().orc_intlonlat(self.h, _p(xy), _p(o), nc)
+        return o
+
+    def avrlonlat(self, xy):
+        xy, nc = self._cols_grid(xy)
+        o = np.zeros(nc)
+        lib().orc_avrlonlat(self.h, _p(xy), _p(o), nc)
+        return o
+
+    def ya_intlon(self, xy):
+        xy, nc = self._cols_grid(xy)
+        o = np.zeros((nc, self.J))
+        lib().orc_ya_intlon(self.h, _p(xy), _p(o), nc)
+        return o
+
+    def ya_avrlon(self, xy):
+        xy, nc = self._cols_grid(xy)
+        o = np.zeros((nc, self.J))
+        lib().orc_ya_avrlon(self.h, _p(xy), _p(o), nc)
+        return o
+
+    def interpolate(self, w, lon, lat):
+        w = _c(w)
+        lon = _c(np.atleast_1d(lon))
+        lat = _c(np.atleast_1d(lat))
+        o = np.zeros(lon.size)
+        lib().orc_interpolate(self.h, _p(w), _p(lon), _p(lat), _p(o), lon.size)
+        return o
+
+    def coef_hdiff(self, kappa, numdiff, ord_):
+        c = np.zeros(self.L)
+        lib().orc_coef_hdiff(self.h, kappa, numdiff, int(ord_), _p(c))
+        return c
+
+    def coef_hvisc(self, nu, numdiff, ord_):
+        c = np.zeros(self.L)
+        lib().orc_coef_hvisc(self.h, nu, numdiff, int(ord_), _p(c))
+        return c
+
+    def coef_advfilter(self):
+        c = np.zeros(self.L)
+        lib().orc_coef_advfilter(self.h, _p(c))
+        return c
+
+    def coef_helmholtz(self, diag, c0):
+        c = np.zeros(self.L)
+        lib().orc_coef_helmholtz(self.h, float(diag), float(c0), _p(c))
+        return c
+
+    def diag(self, d, w):
+        w, nc = self._cols_spec(np.array(w, dtype=np.float64, copy=True))
+        d = _c(d)
+        lib().orc_diag(self.h, _p(d), _p(w), nc)
+        return w

@@ -1,0 +1,156 @@
+"""GPU parity: libdsht (HIP, through the C ABI) against the FP64 CPU oracle.
+
+Every SpmlUtil operator on the horizontal path is checked on the same seeded inputs at
+sizes the oracle finishes in seconds.  Tolerance for the plain FP64 form: relative L2
+<= 1e-12 (the bar of the reference's FP64 ISPACK transforms).
+"""
+import numpy as np
+import pytest
+
+from conftest import red_spectrum, rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+SIZES = [(5, 16, 8), (21, 64, 32), (42, 128, 64), (31, 96, 48), (21, 72, 36)]
+
+
+def _pair(N, I, J, R=6.371e6):
+    import dsht
+    import oracle
+    return dsht.Plan(N, I, J, radius=R), oracle.Oracle(N, I, J, R)
+
+
+@pytest.mark.parametrize("N,I,J", SIZES)
+def test_grid_matches_oracle(gpu, N, I, J):
+    p, o = _pair(N, I, J)
+    assert np.abs(np.sin(p.y_Lat) - o.mu).max() < 1e-15
+    assert rel_l2(p.y_Lat_Weight, o.y_lat_weight) < 1e-14
+    assert abs(p.y_Lat_Weight.sum() - 2.0) < 1e-13
+
+
+@pytest.mark.parametrize("N,I,J", SIZES)
+@pytest.mark.parametrize("kind", ["VAL", "GRADLAMBDA", "GRADMU", "GRADLON", "GRADLAT"])
+def test_inverse(gpu, N, I, J, kind):
+    import dsht
+    p, o = _pair(N, I, J)
+    k = getattr(dsht, kind)
+    w = red_spectrum(N, 7, seed=N + k)
+    got = p._inv(w, k)
+    ref = o.inv(w, k)
+    assert rel_l2(got, ref) < TOL, (kind, rel_l2(got, ref))
+
+
+@pytest.mark.parametrize("N,I,J", SIZES)
+@pytest.mark.parametrize("kind", ["VAL", "DIVLAMBDA", "DIVMU"])
+def test_forward(gpu, N, I, J, kind):
+    import dsht
+    p, o = _pair(N, I, J)
+    k = getattr(dsht, kind)
+    rng = np.random.default_rng(100 + N + k)
+    xy = rng.standard_normal((5, J, I))
+    if k == dsht.DIVMU:  # a field vanishing at the poles like V cos(phi)
+        xy *= o.coslat[None, :, None] ** 2
+    got = p._fwd(xy, k)
+    ref = o.fwd(xy, k)
+    assert rel_l2(got, ref) < TOL, (kind, rel_l2(got, ref))
+
+
+@pytest.mark.parametrize("N,I,J", SIZES[:3])
+def test_alpha_and_vordiv(gpu, N, I, J):
+    p, o = _pair(N, I, J)
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((3, J, I)) * o.coslat[None, :, None]
+    B = rng.standard_normal((3, J, I)) * o.coslat[None, :, None]
+    assert rel_l2(p.w_AlphaOptr_xy(A, B), o.alpha_fwd(A, B)) < TOL
+    wa, wb = red_spectrum(N, 3, 1), red_spectrum(N, 3, 2)
+    assert rel_l2(p.xy_AlphaOptr_w(wa, wb), o.alpha_inv(wa, wb)) < TOL
+    vor, div = p.calc_UVCosLat2VorDiv(A, B)
+    rv, rd = o.uv2vordiv(A, B)
+    assert rel_l2(vor, rv) < TOL and rel_l2(div, rd) < TOL
+    u, v = p.calc_VorDiv2UV(wa, wb)
+    ru, rv2 = o.vordiv2uv(wa, wb)
+    assert rel_l2(u, ru) < TOL and rel_l2(v, rv2) < TOL
+
+
+@pytest.mark.parametrize("N,I,J", SIZES[1:3])
+def test_means_diag_interp(gpu, N, I, J):
+    import dsht
+    p, o = _pair(N, I, J)
+    rng = np.random.default_rng(3)
+    xy = rng.standard_normal((4, J, I)) + 2.0
+    assert np.allclose(p.AvrLonLat_xy(xy), o.avrlonlat(xy), rtol=1e-13, atol=0)
+    assert np.allclose(p.IntLonLat_xy(xy), o.intlonlat(xy), rtol=1e-13, atol=0)
+    assert np.allclose(p.ya_IntLon_xya(xy), o.ya_intlon(xy), rtol=1e-13, atol=0)
+    assert np.allclose(p.ya_AvrLon_xya(xy), o.ya_avrlon(xy), rtol=1e-13, atol=0)
+    w = red_spectrum(N, 2, 9)
+    assert rel_l2(p.w_Lapla_w(w), o.lapla(w)) < 1e-15
+    assert rel_l2(p.w_LaplaInv_w(w), o.laplainv(w)) < 1e-15
+    lon = rng.uniform(0, 2 * np.pi, 11)
+    lat = rng.uniform(-1.5, 1.5, 11)
+    got = p.Interpolate_w(w[0], lon, lat)
+    ref = o.interpolate(w[0], lon, lat)
+    assert rel_l2(got, ref) < 1e-12
+    R = p.radius
+    assert rel_l2(p.coef(dsht.COEF_HDIFF, [1e3, 1e-30, 8]), o.coef_hdiff(1e3, 1e-30, 8)) < 1e-14
+    assert rel_l2(p.coef(dsht.COEF_HVISC, [1e3, 1e-30, 8]), o.coef_hvisc(1e3, 1e-30, 8)) < 1e-14
+    assert rel_l2(p.coef(dsht.COEF_ADVFILTER), o.coef_advfilter()) < 1e-15
+    assert rel_l2(p.coef(dsht.COEF_HELMHOLTZ, [1e-13, 5.0]), o.coef_helmholtz(1e-13, 5.0)) < 1e-14
+    # implicit diffusion round trip: xy_w( w_xy(q)/(1 - dt c) )
+    dt = 43200.0
+    d = p.coef(dsht.COEF_BEDIFF, [dt, 1e3, -1.0, 8, 0, 86400.0])
+    c = o.coef_hdiff(1e3, (N * (N + 1) / R ** 2) ** -4 / 86400.0, 8)
+    assert rel_l2(d, 1.0 / (1.0 - dt * c)) < 1e-14
+    q = o.inv(red_spectrum(N, 3, 11))
+    got = p.xy_diag_xy(d, q)
+    ref = o.inv(o.diag(d, o.fwd(q)))
+    assert rel_l2(got, ref) < TOL
+
+
+@pytest.mark.parametrize("layout", ["LC", "CL"])
+def test_device_api_layouts(gpu, layout):
+    import torch
+    import dsht
+    N, I, J, nc = 42, 128, 64, 40
+    p, o = _pair(N, I, J)
+    w = red_spectrum(N, nc, 5)
+    xy = torch.empty((nc, J, I), dtype=torch.float64, device=gpu)
+    wt = torch.tensor(w.T.copy() if layout == "LC" else w, device=gpu)
+    p.inv_d(dsht.VAL, wt, xy, layout=layout)
+    torch.cuda.synchronize()
+    ref = o.inv(w)
+    assert rel_l2(xy.cpu().numpy(), ref) < TOL
+    w2 = torch.zeros_like(wt)
+    p.fwd_d(dsht.VAL, xy, w2, layout=layout)
+    torch.cuda.synchronize()
+    got = w2.cpu().numpy()
+    got = got.T if layout == "LC" else got
+    assert rel_l2(got, w) < TOL
+
+
+def test_roundtrip_t170_chunked(gpu):
+    """T170 (I=512, J=256) round trip over more columns than one scratch chunk."""
+    import torch
+    import dsht
+    N, I, J, nc = 170, 512, 256, 80
+    p = dsht.Plan(N, I, J, radius=6.371e6)
+    w = torch.tensor(red_spectrum(N, nc, 17).T.copy(), device=gpu)
+    xy = torch.empty((nc, J, I), dtype=torch.float64, device=gpu)
+    w2 = torch.zeros_like(w)
+    p.inv_d(dsht.VAL, w, xy)
+    p.fwd_d(dsht.VAL, xy, w2)
+    torch.cuda.synchronize()
+    assert rel_l2(w2.cpu().numpy(), w.cpu().numpy()) < TOL
+
+
+def test_errors(gpu):
+    import dsht
+    with pytest.raises(dsht.DshtError):
+        dsht.Plan(42, 64, 64)  # im < 2*nmax+2
+    with pytest.raises(dsht.DshtError):
+        dsht.Plan(42, 128, 63)  # odd jm
+    with pytest.raises(dsht.DshtError):
+        dsht.Plan(42, 2 * 7 * 11, 64)  # im/2 has factor 7
+    p = dsht.Plan(5, 16, 8)
+    with pytest.raises(dsht.DshtError):
+        p._inv(np.zeros(36), 99)
