@@ -32,6 +32,62 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+// ---------------------------------------------------------------- kernel timers
+// Per kernel class: HIP events around each launch (on the launch stream), plus the
+// algorithmic bytes and flops of the launch, so callers can report achieved GB/s and
+// TFLOP/s without an external profiler.  Named after the reference's rap timers
+// (model/dogcm/src/util/misc/ProfUtil_mod.f90:134-210).
+struct ProfEvent {
+  std::string name;
+  hipEvent_t a, b;
+  double bytes, flops;
+};
+static std::atomic<int> g_prof_on{0};
+static std::mutex g_prof_mu;
+static std::vector<ProfEvent> g_prof_pending;
+struct ProfAcc {
+  double ms = 0, bytes = 0, flops = 0;
+  long count = 0;
+};
+static std::map<std::string, ProfAcc> g_prof_acc;
+
+ProfScope::ProfScope(const char* name, hipStream_t st, double bytes, double flops) {
+  if (!g_prof_on.load(std::memory_order_relaxed)) return;
+  ProfEvent e;
+  e.name = name;
+  e.bytes = bytes;
+  e.flops = flops;
+  if (hipEventCreate(&e.a) != hipSuccess || hipEventCreate(&e.b) != hipSuccess) return;
+  (void)hipEventRecord(e.a, st);
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  slot = (int)g_prof_pending.size();
+  s = st;
+  g_prof_pending.push_back(e);
+}
+
+ProfScope::~ProfScope() {
+  if (slot < 0) return;
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  (void)hipEventRecord(g_prof_pending[slot].b, s);
+}
+
+static void prof_flush() {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  for (auto& e : g_prof_pending) {
+    float ms = 0.f;
+    (void)hipEventSynchronize(e.b);
+    (void)hipEventElapsedTime(&ms, e.a, e.b);
+    ProfAcc& acc = g_prof_acc[e.name];
+    acc.ms += ms;
+    acc.bytes += e.bytes;
+    acc.flops += e.flops;
+    acc.count++;
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  g_prof_pending.clear();
+}
+
 // ---------------------------------------------------------------- workspaces
 // One scratch buffer per (plan, stream); a call holds its lock while it enqueues, so
 // calls on one stream are serialised by the stream and calls on different streams never
@@ -260,6 +316,33 @@ static int ctx_buf(HostCtx* c, int i, size_t n, double** out) {
 extern "C" {
 
 const char* dsht_last_error(void) { return dsht::g_err; }
+
+int dsht_prof_enable(int on) {
+  if (!on) prof_flush();
+  g_prof_on.store(on ? 1 : 0);
+  return 0;
+}
+
+int dsht_prof_reset(void) {
+  prof_flush();
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_prof_acc.clear();
+  return 0;
+}
+
+int dsht_prof_query(const char* name, double* total_ms, long* count, double* bytes, double* flops) {
+  if (!name) return fail(DSHT_EARG, "dsht_prof_query: NULL name");
+  prof_flush();
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  auto it = g_prof_acc.find(name);
+  ProfAcc z;
+  const ProfAcc& acc = it == g_prof_acc.end() ? z : it->second;
+  if (total_ms) *total_ms = acc.ms;
+  if (count) *count = acc.count;
+  if (bytes) *bytes = acc.bytes;
+  if (flops) *flops = acc.flops;
+  return 0;
+}
 
 int dsht_create(int nmax, int im, int jm, int kmax, double radius, dsht_plan** plan) {
   if (!plan) return fail(DSHT_EARG, "dsht_create: plan pointer is NULL");

@@ -180,6 +180,13 @@ int launch_coef(const dsht_plan* p, int which, const double* params, int nparams
                 hipStream_t s);
 int launch_transpose(const double* in, long ld_in, double* out, long ld_out, int rows, int cols,
                      hipStream_t s);
+// api.hip: kernel timers (HIP events), enabled by dsht_prof_enable
+struct ProfScope {
+  int slot = -1;
+  hipStream_t s = nullptr;
+  ProfScope(const char* name, hipStream_t s, double bytes, double flops);
+  ~ProfScope();
+};
 // legendre_mixed.hip: split-precision forms
 void mixed_free(dsht_plan* p);
 int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, long ldg, double* out0,

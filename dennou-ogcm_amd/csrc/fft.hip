@@ -196,6 +196,179 @@ __global__ __launch_bounds__(256) void k_fft_inv(const double2* __restrict__ G, 
   }
 }
 
+// ---------------------------------------------------------------- power-of-two lengths
+// Compile-time radix plan (radix 4 stages, one radix-2 stage when log2 M is odd): index
+// arithmetic is shifts and masks, twiddles come from LDS, and ROWS rows share each
+// workgroup so the Fourier stores of consecutive rows form ROWS*16-byte runs per m.
+// Stages run in place: each thread reads all of its butterflies, the block synchronises,
+// then it writes them.
+template <int M>
+struct P2Plan {
+  static constexpr int LOG2 = __builtin_ctz(M);
+  static constexpr int N4 = LOG2 / 2;         // radix-4 stages
+  static constexpr bool HAS2 = (LOG2 & 1) != 0;  // trailing radix-2 stage
+};
+
+template <int M, int ROWS, int R, int NS, int SIGN>
+__device__ __forceinline__ void p2_stage(double2* __restrict__ buf, const double2* __restrict__ tw, int tid) {
+  constexpr int MR = M / R;
+  constexpr int NB = ROWS * MR;               // butterflies in the block
+  constexpr int PER = (NB + 255) / 256;       // per thread
+  constexpr int TS = M / (NS * R);            // twiddle stride
+  double2 v[PER][R];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int b = tid + 256 * q;
+    if (NB % 256 == 0 || b < NB) {
+      const int row = b / MR, jj = b % MR, k = jj % NS;
+      const double2* S = buf + row * M;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        double2 x = S[jj + r * MR];
+        if (r) {
+          double2 w = tw[k * r * TS];
+          x = SIGN < 0 ? cmul(x, w) : cmulc(x, w);
+        }
+        v[q][r] = x;
+      }
+      dft_small<SIGN>(v[q], R);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int b = tid + 256 * q;
+    if (NB % 256 == 0 || b < NB) {
+      const int row = b / MR, jj = b % MR, k = jj % NS;
+      double2* D = buf + row * M;
+      const int d = (jj / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) D[d + r * NS] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int M, int ROWS, int S4, int SIGN>
+__device__ __forceinline__ void p2_stages4(double2* buf, const double2* tw, int tid) {
+  if constexpr (S4 < P2Plan<M>::N4) {
+    constexpr int NS = 1 << (2 * S4);
+    p2_stage<M, ROWS, 4, NS, SIGN>(buf, tw, tid);
+    p2_stages4<M, ROWS, S4 + 1, SIGN>(buf, tw, tid);
+  }
+}
+
+template <int M, int ROWS, int SIGN>
+__device__ __forceinline__ void p2_fft(double2* buf, const double2* tw, int tid) {
+  p2_stages4<M, ROWS, 0, SIGN>(buf, tw, tid);
+  if constexpr (P2Plan<M>::HAS2) p2_stage<M, ROWS, 2, (1 << (2 * P2Plan<M>::N4)), SIGN>(buf, tw, tid);
+}
+
+template <int M, int ROWS>
+__global__ __launch_bounds__(256) void k_fft_fwd_p2(const double* __restrict__ xy, double2* __restrict__ G,
+                                                    FftArgs a) {
+  extern __shared__ double2 lds[];
+  double2* buf = lds;                  // ROWS * M
+  double2* tw = lds + ROWS * M;        // M
+  double2* twi = tw + M;               // M
+  const int tid = threadIdx.x;
+  for (int k = tid; k < M; k += 256) {
+    tw[k] = a.twM[k];
+    twi[k] = a.twI[k];
+  }
+  const long nrows = (long)a.J * a.ncols;
+  const int N = a.N;
+  const double inv_i = 1.0 / (2 * M);
+  for (long r0 = (long)blockIdx.x * ROWS; r0 < nrows; r0 += (long)gridDim.x * ROWS) {
+    __syncthreads();
+    // load ROWS rows (packed as complex) with every thread
+    for (int e = tid; e < ROWS * M; e += 256) {
+      const int rr = e / M, k = e % M;
+      const long r = r0 + rr;
+      double2 z = make_double2(0.0, 0.0);
+      if (r < nrows) {
+        const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
+        z = reinterpret_cast<const double2*>(xy + c * a.ldg + (long)j * (2 * M))[k];
+      }
+      buf[e] = z;
+    }
+    __syncthreads();
+    p2_fft<M, ROWS, -1>(buf, tw, tid);
+    for (int idx = tid; idx < (N + 1) * ROWS; idx += 256) {
+      const int m = idx / ROWS, rr = idx % ROWS;
+      const long r = r0 + rr;
+      if (r >= nrows) continue;
+      double2 zm = buf[rr * M + m];
+      double2 zc = buf[rr * M + ((M - m) & (M - 1))];
+      zc.y = -zc.y;
+      double2 E = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+      double2 O = make_double2(0.5 * (zm.y - zc.y), -0.5 * (zm.x - zc.x));
+      double2 g = cadd(E, cmul(twi[m], O));
+      const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
+      G[(long)m * a.gmst + (long)j * a.gld + c] = make_double2(g.x * inv_i, g.y * inv_i);
+    }
+  }
+}
+
+template <int M, int ROWS>
+__global__ __launch_bounds__(256) void k_fft_inv_p2(const double2* __restrict__ G, double* __restrict__ xy,
+                                                    FftArgs a) {
+  extern __shared__ double2 lds[];
+  double2* buf = lds;
+  double2* tw = lds + ROWS * M;
+  double2* twi = tw + M;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < M; k += 256) {
+    tw[k] = a.twM[k];
+    twi[k] = a.twI[k];
+  }
+  const long nrows = (long)a.J * a.ncols;
+  const int N = a.N;
+  for (long r0 = (long)blockIdx.x * ROWS; r0 < nrows; r0 += (long)gridDim.x * ROWS) {
+    __syncthreads();
+    for (int idx = tid; idx < M * ROWS; idx += 256) {
+      const int m = idx / ROWS, rr = idx % ROWS;
+      const long r = r0 + rr;
+      double2 g = make_double2(0.0, 0.0);
+      if (m <= N && r < nrows) {
+        const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
+        g = G[(long)m * a.gmst + (long)j * a.gld + c];
+        if (m == 0) g.y = 0.0;
+      }
+      buf[rr * M + m] = g;
+    }
+    __syncthreads();
+    // Z_m = A + iB in place: the thread owning (m, M-m) writes both
+    for (int idx = tid; idx < ROWS * (M / 2 + 1); idx += 256) {
+      const int rr = idx / (M / 2 + 1), m = idx % (M / 2 + 1);
+      double2* X = buf + rr * M;
+      const int mc = (M - m) & (M - 1);
+      double2 xm = X[m], xo = X[mc];
+      // Z_m from (X_m, X_{M-m}); Z_{M-m} from (X_{M-m}, X_m); X_M = 0 for m = 0
+      double2 c1 = m ? make_double2(xo.x, -xo.y) : make_double2(0.0, 0.0);
+      double2 A1 = cadd(xm, c1), B1 = cmulc(csub(xm, c1), twi[m]);
+      double2 z1 = make_double2(A1.x - B1.y, A1.y + B1.x);
+      if (m != 0 && m != M / 2) {
+        double2 c2 = make_double2(xm.x, -xm.y);
+        double2 A2 = cadd(xo, c2), B2 = cmulc(csub(xo, c2), twi[mc]);
+        X[mc] = make_double2(A2.x - B2.y, A2.y + B2.x);
+      }
+      X[m] = z1;
+    }
+    __syncthreads();
+    p2_fft<M, ROWS, +1>(buf, tw, tid);
+    for (int e = tid; e < ROWS * M; e += 256) {
+      const int rr = e / M, k = e % M;
+      const long r = r0 + rr;
+      if (r >= nrows) continue;
+      const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
+      const double s = a.rowscale ? a.rowscale[j] : 1.0;
+      double2 v = buf[e];
+      reinterpret_cast<double2*>(xy + c * a.ldg + (long)j * (2 * M))[k] = make_double2(v.x * s, v.y * s);
+    }
+  }
+}
+
 __global__ void k_twiddles(int M, int I, double2* twM, double2* twI) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < M) {
@@ -256,10 +429,50 @@ static FftArgs fft_args(const dsht_plan* p, long ldg, int ncols, long gld) {
   return a;
 }
 
+// rows per workgroup of the power-of-two kernels: ROWS*M complex doubles of LDS <= 64 KiB
+template <int M>
+struct P2Rows {
+  static constexpr int ROWS = (M >= 512) ? 8 : (M >= 256 ? 16 : (M >= 128 ? 16 : 32));
+};
+
+template <int M>
+static int p2_fwd(const dsht_plan* p, const double* xy, const FftArgs& a, double2* G, hipStream_t s) {
+  constexpr int ROWS = P2Rows<M>::ROWS;
+  long nrows = (long)p->J * a.ncols;
+  long nb = cdivl(nrows, ROWS);
+  if (nb > 4096) nb = 4096;
+  size_t lds = (size_t)(ROWS * M + 2 * M) * sizeof(double2);
+  hipLaunchKernelGGL((k_fft_fwd_p2<M, ROWS>), dim3((unsigned)nb), dim3(256), lds, s, xy, G, a);
+  DSHT_HIP(hipGetLastError());
+  return 0;
+}
+
+template <int M>
+static int p2_inv(const dsht_plan* p, const double2* G, const FftArgs& a, double* xy, hipStream_t s) {
+  constexpr int ROWS = P2Rows<M>::ROWS;
+  long nrows = (long)p->J * a.ncols;
+  long nb = cdivl(nrows, ROWS);
+  if (nb > 4096) nb = 4096;
+  size_t lds = (size_t)(ROWS * M + 2 * M) * sizeof(double2);
+  hipLaunchKernelGGL((k_fft_inv_p2<M, ROWS>), dim3((unsigned)nb), dim3(256), lds, s, G, xy, a);
+  DSHT_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_fft_fwd(const dsht_plan* p, const double* xy, long ldg, int ncols, double2* G, long gld,
                    hipStream_t s) {
   FftArgs a = fft_args(p, ldg, ncols, gld);
   long nrows = (long)p->J * ncols;
+  ProfScope ps("fft_fwd", s, (double)nrows * (8.0 * p->I + 16.0 * (p->N + 1)),
+               (double)nrows * 2.5 * p->I * std::log2((double)p->I));
+  switch (a.M) {
+    case 16: return p2_fwd<16>(p, xy, a, G, s);
+    case 32: return p2_fwd<32>(p, xy, a, G, s);
+    case 64: return p2_fwd<64>(p, xy, a, G, s);
+    case 128: return p2_fwd<128>(p, xy, a, G, s);
+    case 256: return p2_fwd<256>(p, xy, a, G, s);
+    case 512: return p2_fwd<512>(p, xy, a, G, s);
+  }
   int nb = (int)cdivl(nrows, a.rows_per_block);
   size_t lds = 2ul * a.rows_per_block * a.M * sizeof(double2);
   hipLaunchKernelGGL(k_fft_fwd, dim3(nb), dim3(256), lds, s, xy, G, a);
@@ -271,6 +484,16 @@ int launch_fft_inv(const dsht_plan* p, const double2* G, long gld, int ncols, do
                    hipStream_t s) {
   FftArgs a = fft_args(p, ldg, ncols, gld);
   long nrows = (long)p->J * ncols;
+  ProfScope ps("fft_inv", s, (double)nrows * (8.0 * p->I + 16.0 * (p->N + 1)),
+               (double)nrows * 2.5 * p->I * std::log2((double)p->I));
+  switch (a.M) {
+    case 16: return p2_inv<16>(p, G, a, xy, s);
+    case 32: return p2_inv<32>(p, G, a, xy, s);
+    case 64: return p2_inv<64>(p, G, a, xy, s);
+    case 128: return p2_inv<128>(p, G, a, xy, s);
+    case 256: return p2_inv<256>(p, G, a, xy, s);
+    case 512: return p2_inv<512>(p, G, a, xy, s);
+  }
   int nb = (int)cdivl(nrows, a.rows_per_block);
   size_t lds = 2ul * a.rows_per_block * a.M * sizeof(double2);
   hipLaunchKernelGGL(k_fft_inv, dim3(nb), dim3(256), lds, s, G, xy, a);

@@ -385,8 +385,24 @@ static int inv_launch(const dsht_plan* p, const LegInvArgs& a, hipStream_t s) {
   return 0;
 }
 
+// Algorithmic work of one contraction launch: every (input, table) term costs
+// 2 J T flops per column (even/odd split, re and im), T = (N+1)(N+2)/2; bytes are the
+// Fourier and spectral arrays touched once plus one read of each table used.
+static void leg_work(const dsht_plan* p, int nin, int nout, int nterm, int ntab, bool fwd, int ncols,
+                     double* bytes, double* flops) {
+  double T = 0.5 * (p->N + 1.0) * (p->N + 2.0);
+  double four = (double)(p->N + 1) * p->J * 16.0 * ncols, spec = (double)p->L * 8.0 * ncols;
+  double tab = 8.0 * (fwd ? p->sizeF : p->sizeI) * ntab;
+  *bytes = fwd ? nin * four + nout * spec + tab : nin * spec + nout * four + tab;
+  *flops = 2.0 * p->J * T * ncols * nterm;
+}
+
 int launch_leg_fwd(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t s) {
   const bool wide = a.ncols > 16;
+  double by, fl;
+  static const int nin[] = {1, 1, 1, 2, 2}, nout[] = {1, 1, 1, 1, 2}, nt[] = {1, 1, 1, 2, 4}, ntab[] = {1, 1, 1, 2, 2};
+  leg_work(p, nin[op], nout[op], nt[op], ntab[op], true, a.ncols, &by, &fl);
+  ProfScope ps("leg_fwd", s, by, fl);
   switch (op) {
     case F_VAL: return wide ? fwd_launch<F_VAL, 32>(p, a, s) : fwd_launch<F_VAL, 16>(p, a, s);
     case F_DIVLAMBDA:
@@ -400,6 +416,10 @@ int launch_leg_fwd(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t 
 
 int launch_leg_inv(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t s) {
   const bool wide = a.ncols > 16;
+  double by, fl;
+  static const int nin[] = {1, 1, 1, 2, 2}, nout[] = {1, 1, 1, 1, 2}, nt[] = {1, 1, 1, 2, 4}, ntab[] = {1, 1, 1, 2, 2};
+  leg_work(p, nin[op], nout[op], nt[op], ntab[op], false, a.ncols, &by, &fl);
+  ProfScope ps("leg_inv", s, by, fl);
   switch (op) {
     case I_VAL: return wide ? inv_launch<I_VAL, 32>(p, a, s) : inv_launch<I_VAL, 16>(p, a, s);
     case I_GRADLAMBDA:

@@ -39,7 +39,7 @@ EXPORTS = [
     "dsht_ya_intlon", "dsht_ya_avrlon", "dsht_interpolate", "dsht_fwd_d", "dsht_inv_d",
     "dsht_alpha_fwd_d", "dsht_alpha_inv_d", "dsht_uv2vordiv_d", "dsht_vordiv2uv_d",
     "dsht_diag_d", "dsht_intlonlat_d", "dsht_avrlonlat_d", "dsht_ya_intlon_d",
-    "dsht_interpolate_d", "dsht_coef_d",
+    "dsht_interpolate_d", "dsht_coef_d", "dsht_prof_enable", "dsht_prof_reset", "dsht_prof_query",
 ]
 
 
@@ -100,6 +100,9 @@ def load():
         "dsht_ya_intlon_d": ([vp, vp, lg, vp, i, vp], i),
         "dsht_interpolate_d": ([vp, vp, lg, vp, vp, vp, i, vp], i),
         "dsht_coef_d": ([vp, i, vp, i, vp, vp], i),
+        "dsht_prof_enable": ([i], i),
+        "dsht_prof_reset": ([], i),
+        "dsht_prof_query": ([ctypes.c_char_p, vp, vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -131,6 +134,23 @@ def _stream(stream=None):
         return ctypes.c_void_p(stream)
     import torch
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def prof_enable(on=True):
+    _check(load().dsht_prof_enable(1 if on else 0))
+
+
+def prof_reset():
+    _check(load().dsht_prof_reset())
+
+
+def prof_query(name):
+    """(total_ms, count, algorithmic_bytes, algorithmic_flops) of one kernel class."""
+    ms, by, fl = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    n = ctypes.c_long()
+    _check(load().dsht_prof_query(name.encode(), ctypes.byref(ms), ctypes.byref(n),
+                                  ctypes.byref(by), ctypes.byref(fl)))
+    return ms.value, n.value, by.value, fl.value
 
 
 class Plan:

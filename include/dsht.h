@@ -156,6 +156,15 @@ int dsht_interpolate_d(dsht_plan* plan, const double* w, long sl, const double* 
 int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, double* out,
                 void* stream);
 
+/* ---------------------------------------------------------------- kernel timers
+ * HIP-event timers around every kernel launch of the library, keyed by kernel class
+ * ("fft_fwd", "fft_inv", "leg_fwd", "leg_inv", ...), with the algorithmic bytes and
+ * flops of each launch.  The counterpart of the reference's ProfUtil rap timers
+ * (model/dogcm/src/util/misc/ProfUtil_mod.f90:134-269).  Off by default.           */
+int dsht_prof_enable(int on);
+int dsht_prof_reset(void);
+int dsht_prof_query(const char* name, double* total_ms, long* count, double* bytes, double* flops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -307,11 +307,13 @@ int orc_legendre(const orc_t* h, int m, int n, double* p_out, double* h_out) {
    DIVMU: s = -sum_j w_j/(1-mu^2) G H (integration by parts in mu, tex:229-268). */
 static void fwd_column(const orc_t* h, int kind, const double* x, double* w, cplx* G, cplx* work) {
   int N = h->N, J = h->J, I = h->I, J2 = J / 2;
-  cplx* z = work;           /* M */
-  cplx* Z = work + h->M;    /* M */
-  cplx* gs = work + 2 * h->M; /* J2 */
-  cplx* ga = gs + J2;         /* J2 */
-  double* cw = (double*)(ga + J2); /* J2 */
+  cplx* z = work;              /* M */
+  cplx* Z = work + h->M;       /* M */
+  double* gsr = (double*)(work + 2 * h->M); /* J2 each: sym re/im, anti re/im, weights */
+  double* gsi = gsr + J2;
+  double* gar = gsi + J2;
+  double* gai = gar + J2;
+  double* cw = gai + J2;
   for (int j = 0; j < J; j++) rfft_row(h, x + (long)j * I, G + (long)j * (N + 1), z, Z);
   for (int q = 0; q < J2; q++) {
     int jn = J2 + q;
@@ -321,17 +323,21 @@ static void fwd_column(const orc_t* h, int kind, const double* x, double* w, cpl
     for (int q = 0; q < J2; q++) {
       int jn = J2 + q, js = J2 - 1 - q;
       cplx a = G[(long)jn * (N + 1) + m], b = G[(long)js * (N + 1) + m];
-      gs[q] = cw[q] * (a + b);
-      ga[q] = cw[q] * (a - b);
+      gsr[q] = cw[q] * (creal(a) + creal(b));
+      gsi[q] = cw[q] * (cimag(a) + cimag(b));
+      gar[q] = cw[q] * (creal(a) - creal(b));
+      gai[q] = cw[q] * (cimag(a) - cimag(b));
     }
     const double* T = (kind == ORC_DIVMU) ? h->H + h->offH[m] : h->P + h->offP[m];
     for (int n = m; n <= N; n++) {
       const double* t = T + (long)(n - m) * J + J2;
       int odd = (n - m) & 1;
       /* P has parity (-1)^{n-m}; H has the opposite parity */
-      const cplx* g = (kind == ORC_DIVMU) ? (odd ? gs : ga) : (odd ? ga : gs);
+      int use_sym = (kind == ORC_DIVMU) ? odd : !odd;
+      const double* gr = use_sym ? gsr : gar;
+      const double* gi = use_sym ? gsi : gai;
       double ar = 0, ai = 0;
-      for (int q = 0; q < J2; q++) { ar += t[q] * creal(g[q]); ai += t[q] * cimag(g[q]); }
+      for (int q = 0; q < J2; q++) { ar += t[q] * gr[q]; ai += t[q] * gi[q]; }
       cplx s = ar + ai * _Complex_I;
       if (kind == ORC_DIVLAMBDA) s *= (double)m * _Complex_I;
       if (kind == ORC_DIVMU) s = -s;
@@ -347,25 +353,49 @@ static void inv_column(const orc_t* h, int kind, const double* w, double* x, cpl
   int N = h->N, J = h->J, I = h->I, J2 = J / 2;
   cplx* z = work;
   cplx* Z = work + h->M;
-  cplx* E = work + 2 * h->M;
-  cplx* O = E + J2;
+  double* Er = (double*)(work + 2 * h->M); /* J2 each */
+  double* Ei = Er + J2;
+  double* Or = Ei + J2;
+  double* Oi = Or + J2;
   int useH = (kind == ORC_GRADMU || kind == ORC_GRADLAT);
   int useIm = (kind == ORC_GRADLAMBDA || kind == ORC_GRADLON);
   for (int m = 0; m <= N; m++) {
-    for (int q = 0; q < J2; q++) E[q] = O[q] = 0;
+    for (int q = 0; q < J2; q++) Er[q] = Ei[q] = Or[q] = Oi[q] = 0;
     const double* T = useH ? h->H + h->offH[m] : h->P + h->offP[m];
-    for (int n = m; n <= N; n++) {
-      cplx s = w[lidx(n, m)] + (m > 0 ? w[lidx(n, -m)] : 0.0) * _Complex_I;
+    /* four wavenumbers per pass (two of each parity) to halve accumulator traffic */
+    int n = m;
+    for (; n + 3 <= N; n += 4) {
+      double s0r = w[lidx(n, m)], s1r = w[lidx(n + 1, m)], s2r = w[lidx(n + 2, m)], s3r = w[lidx(n + 3, m)];
+      double s0i = m > 0 ? w[lidx(n, -m)] : 0.0, s1i = m > 0 ? w[lidx(n + 1, -m)] : 0.0;
+      double s2i = m > 0 ? w[lidx(n + 2, -m)] : 0.0, s3i = m > 0 ? w[lidx(n + 3, -m)] : 0.0;
+      const double* t0 = T + (long)(n - m) * J + J2;
+      const double* t1 = t0 + J;
+      const double* t2 = t1 + J;
+      const double* t3 = t2 + J;
+      double* ar = ((n - m) & 1) ? Or : Er;
+      double* ai = ((n - m) & 1) ? Oi : Ei;
+      double* br = ((n - m) & 1) ? Er : Or;
+      double* bi = ((n - m) & 1) ? Ei : Oi;
+      for (int q = 0; q < J2; q++) {
+        ar[q] += t0[q] * s0r + t2[q] * s2r;
+        ai[q] += t0[q] * s0i + t2[q] * s2i;
+        br[q] += t1[q] * s1r + t3[q] * s3r;
+        bi[q] += t1[q] * s1i + t3[q] * s3i;
+      }
+    }
+    for (; n <= N; n++) {
+      double sr = w[lidx(n, m)], si = m > 0 ? w[lidx(n, -m)] : 0.0;
       const double* t = T + (long)(n - m) * J + J2;
-      cplx* acc = ((n - m) & 1) ? O : E;
-      double sr = creal(s), si = cimag(s);
-      for (int q = 0; q < J2; q++) acc[q] += t[q] * sr + t[q] * si * _Complex_I;
+      double* ar = ((n - m) & 1) ? Or : Er;
+      double* ai = ((n - m) & 1) ? Oi : Ei;
+      for (int q = 0; q < J2; q++) { ar[q] += t[q] * sr; ai[q] += t[q] * si; }
     }
     cplx f = useIm ? (double)m * _Complex_I : 1.0;
     for (int q = 0; q < J2; q++) {
       int jn = J2 + q, js = J2 - 1 - q;
-      G[(long)jn * (N + 1) + m] = f * (E[q] + O[q]);
-      G[(long)js * (N + 1) + m] = useH ? f * (O[q] - E[q]) : f * (E[q] - O[q]);
+      cplx E = Er[q] + Ei[q] * _Complex_I, O = Or[q] + Oi[q] * _Complex_I;
+      G[(long)jn * (N + 1) + m] = f * (E + O);
+      G[(long)js * (N + 1) + m] = useH ? f * (O - E) : f * (E - O);
     }
   }
   for (int j = 0; j < J; j++) {
@@ -375,7 +405,7 @@ static void inv_column(const orc_t* h, int kind, const double* w, double* x, cpl
   }
 }
 
-static size_t work_elems(const orc_t* h) { return (size_t)2 * h->M + (size_t)3 * h->J + 8; }
+static size_t work_elems(const orc_t* h) { return (size_t)2 * h->M + (size_t)3 * h->J + 8; } /* >= 2M cplx + 5 J/2 doubles */
 
 int orc_fwd(const orc_t* h, int kind, const double* xy, double* w, int ncols) {
   if (kind != ORC_VAL && kind != ORC_DIVLAMBDA && kind != ORC_DIVMU) { set_err("orc_fwd: bad kind"); return -1; }
