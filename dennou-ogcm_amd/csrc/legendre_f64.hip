@@ -24,6 +24,8 @@
 //   I_VORDIV2UV  psi, chi = -R^2/(n(n+1)) (vor, div);
 //                U = (i m P.chi - H.psi)/(R cos), V = (i m P.psi + H.chi)/(R cos)  (calc_VorDiv2UV)
 // plus an optional per-l diagonal (filters, implicit diffusion, Helmholtz) on the spectral side.
+#include <type_traits>
+
 #include "dsht_internal.hpp"
 
 namespace dsht {
@@ -52,13 +54,23 @@ struct InvTraits {
 };
 
 // ------------------------------------------------------------------ forward
-template <int OP, int CB, int MAXT>
-__global__ __launch_bounds__(256) void k_leg_fwd(LegFwdArgs a) {
+// WAVES waves per workgroup; wave w owns row tiles t = tbase + w + WAVES*tt (tt < MAXT).
+// The K loop (north-hemisphere latitudes) runs in chunks of KC and is software pipelined:
+// the Fourier values of chunk k+1 and its table fragments are loaded into registers while
+// the MFMAs of chunk k run (two table-fragment register sets used alternately), then the
+// data are staged to LDS behind one barrier.  Tables are padded to multiples of KCH = 16
+// only; partial chunks are predicated per 4-wide K step.
+template <int OP, int CB, int WAVES, int MAXT, int KC>
+__global__ __launch_bounds__(WAVES * 64) void k_leg_fwd(LegFwdArgs a) {
   using Tr = FwdTraits<OP>;
+  constexpr int NT = WAVES * 64;
   constexpr int NIN = Tr::NIN, NOUT = Tr::NOUT, NCT = CB / TILE;
   constexpr int PITCH = CB + 1;  // LDS row pitch (doubles)
-  // [input][S re, S im, A re, A im][KCH][PITCH]
-  __shared__ double sB[NIN][4][KCH * PITCH];
+  constexpr int EPT = (KC * CB + NT - 1) / NT;  // staged elements per thread
+  constexpr int NKS = KC / 4;
+  constexpr bool UP = Tr::USE_P, UH = Tr::USE_H;
+  // [input][S re, S im, A re, A im][KC][PITCH]
+  __shared__ double sB[NIN][4][KC * PITCH];
 
   const int m = blockIdx.y;
   const int c0 = blockIdx.x * CB;
@@ -67,13 +79,15 @@ __global__ __launch_bounds__(256) void k_leg_fwd(LegFwdArgs a) {
   const int nE = n_par(N, m, 0), nO = n_par(N, m, 1);
   const int tE = (nE + TILE - 1) / TILE, tO = (nO + TILE - 1) / TILE;
   const int RT = tE + tO;
+  const int KS = a.J2pad / 4;  // valid 4-wide K steps
+  const int nch = (a.J2pad + KC - 1) / KC;
   const long kstepsF = a.J2pad / 4;
   const double* tabP = a.tabP + a.offF[m];
   const double* tabH = a.tabH + a.offF[m];
   const double2* Gin[2] = {a.G0 + (long)m * a.gmst, a.G1 ? a.G1 + (long)m * a.gmst : nullptr};
   const double dm = (double)m;
 
-  for (int tbase = 0; tbase < RT; tbase += 4 * MAXT) {
+  for (int tbase = 0; tbase < RT; tbase += WAVES * MAXT) {
     double4_t acc[NOUT][MAXT][NCT][2];
 #pragma unroll
     for (int o = 0; o < NOUT; o++)
@@ -82,36 +96,66 @@ __global__ __launch_bounds__(256) void k_leg_fwd(LegFwdArgs a) {
 #pragma unroll
         for (int ct = 0; ct < NCT; ct++) acc[o][t][ct][0] = acc[o][t][ct][1] = double4_t{0, 0, 0, 0};
 
-    for (int k0 = 0; k0 < a.J2pad; k0 += KCH) {
-      // ---- stage G(north) +/- G(south), weighted, into LDS
+    double2 gN[NIN][EPT], gS[NIN][EPT];
+    double aP0[MAXT][NKS], aH0[MAXT][NKS], aP1[MAXT][NKS], aH1[MAXT][NKS];
+
+    auto load_b = [&](int k0) {
 #pragma unroll
-      for (int i = 0; i < NIN; i++) {
-        for (int e = tid; e < KCH * CB; e += 256) {
-          int kk = e / CB, cc = e % CB;
-          int jn = k0 + kk, c = c0 + cc;
-          double2 gn = make_double2(0.0, 0.0), gs = gn;
-          if (jn < J2 && c < a.ncols) {
-            gn = Gin[i][(long)(J2 + jn) * a.gld + c];
-            gs = Gin[i][(long)(J2 - 1 - jn) * a.gld + c];
+      for (int i = 0; i < NIN; i++)
+#pragma unroll
+        for (int q = 0; q < EPT; q++) {
+          const int e = tid + NT * q;
+          const int kk = e / CB, cc = e % CB;
+          const int jn = k0 + kk, c = c0 + cc;
+          gN[i][q] = gS[i][q] = make_double2(0.0, 0.0);
+          if (e < KC * CB && jn < J2 && c < a.ncols) {
+            gN[i][q] = Gin[i][(long)(J2 + jn) * a.gld + c];
+            gS[i][q] = Gin[i][(long)(J2 - 1 - jn) * a.gld + c];
           }
-          double cw = a.cw[jn];
-          sB[i][0][kk * PITCH + cc] = cw * (gn.x + gs.x);
-          sB[i][1][kk * PITCH + cc] = cw * (gn.y + gs.y);
-          sB[i][2][kk * PITCH + cc] = cw * (gn.x - gs.x);
-          sB[i][3][kk * PITCH + cc] = cw * (gn.y - gs.y);
+        }
+    };
+    auto load_a = [&](int k0, double (&xP)[MAXT][NKS], double (&xH)[MAXT][NKS]) {
+#pragma unroll
+      for (int tt = 0; tt < MAXT; tt++) {
+        const int t = tbase + w + WAVES * tt;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ks++) {
+          const int kg = k0 / 4 + ks;
+          const long fo = (long)t * kstepsF * 64 + (long)kg * 64 + lane;
+          const bool ok = t < RT && kg < KS;
+          xP[tt][ks] = (UP && ok) ? tabP[fo] : 0.0;
+          xH[tt][ks] = (UH && ok) ? tabH[fo] : 0.0;
         }
       }
-      __syncthreads();
+    };
+    auto stage = [&](int k0) {
 #pragma unroll
-      for (int ks = 0; ks < KCH / 4; ks++) {
+      for (int i = 0; i < NIN; i++)
+#pragma unroll
+        for (int q = 0; q < EPT; q++) {
+          const int e = tid + NT * q;
+          if (e < KC * CB) {
+            const int kk = e / CB, cc = e % CB;
+            const double cw = (k0 + kk < a.J2pad) ? a.cw[k0 + kk] : 0.0;
+            sB[i][0][kk * PITCH + cc] = cw * (gN[i][q].x + gS[i][q].x);
+            sB[i][1][kk * PITCH + cc] = cw * (gN[i][q].y + gS[i][q].y);
+            sB[i][2][kk * PITCH + cc] = cw * (gN[i][q].x - gS[i][q].x);
+            sB[i][3][kk * PITCH + cc] = cw * (gN[i][q].y - gS[i][q].y);
+          }
+        }
+    };
+    auto compute = [&](int k0, const double (&xPa)[MAXT][NKS], const double (&xHa)[MAXT][NKS]) {
+      const int kvalid = KS - k0 / 4;
+#pragma unroll
+      for (int ks = 0; ks < NKS; ks++) {
+        if (ks < kvalid) {
         const int krow = ks * 4 + (lane >> 4);
-        // B fragments: [input][sym/anti][ct][re/im]
         double b[NIN][2][NCT][2];
 #pragma unroll
         for (int i = 0; i < NIN; i++)
 #pragma unroll
           for (int ct = 0; ct < NCT; ct++) {
-            int col = ct * TILE + (lane & 15);
+            const int col = ct * TILE + (lane & 15);
             b[i][0][ct][0] = sB[i][0][krow * PITCH + col];
             b[i][0][ct][1] = sB[i][1][krow * PITCH + col];
             b[i][1][ct][0] = sB[i][2][krow * PITCH + col];
@@ -119,61 +163,71 @@ __global__ __launch_bounds__(256) void k_leg_fwd(LegFwdArgs a) {
           }
 #pragma unroll
         for (int tt = 0; tt < MAXT; tt++) {
-          const int t = tbase + tt * 4 + w;
+          const int t = tbase + w + WAVES * tt;
           if (t >= RT) continue;
           const int p = t < tE ? 0 : 1;
-          const long fo = (long)t * kstepsF * 64 + (long)(k0 / 4 + ks) * 64 + lane;
           // P pairs with (sym for even rows, anti for odd); H with the opposite
           const int gP = p, gH = 1 - p;
-          if constexpr (OP == F_VAL || OP == F_DIVLAMBDA) {
-            double aP = tabP[fo];
+          const double xP = xPa[tt][ks], xH = xHa[tt][ks];
 #pragma unroll
-            for (int ct = 0; ct < NCT; ct++) {
-              acc[0][tt][ct][0] = mfma(aP, b[0][gP][ct][0], acc[0][tt][ct][0]);
-              acc[0][tt][ct][1] = mfma(aP, b[0][gP][ct][1], acc[0][tt][ct][1]);
-            }
-          } else if constexpr (OP == F_DIVMU) {
-            double aH = tabH[fo];
-#pragma unroll
-            for (int ct = 0; ct < NCT; ct++) {
-              acc[0][tt][ct][0] = mfma(aH, b[0][gH][ct][0], acc[0][tt][ct][0]);
-              acc[0][tt][ct][1] = mfma(aH, b[0][gH][ct][1], acc[0][tt][ct][1]);
-            }
-          } else if constexpr (OP == F_ALPHA) {
-            // i m P.A - H.B :  (i m z) = (-m z.im, m z.re)
-            double aP = tabP[fo], aHn = -tabH[fo];
-#pragma unroll
-            for (int ct = 0; ct < NCT; ct++) {
-              acc[0][tt][ct][0] = mfma(aP, -dm * b[0][gP][ct][1], acc[0][tt][ct][0]);
-              acc[0][tt][ct][1] = mfma(aP, dm * b[0][gP][ct][0], acc[0][tt][ct][1]);
-              acc[0][tt][ct][0] = mfma(aHn, b[1][gH][ct][0], acc[0][tt][ct][0]);
-              acc[0][tt][ct][1] = mfma(aHn, b[1][gH][ct][1], acc[0][tt][ct][1]);
-            }
-          } else {  // F_UV2VORDIV, inputs U cos (0), V cos (1)
-            double aP = tabP[fo], aH = tabH[fo];
-#pragma unroll
-            for (int ct = 0; ct < NCT; ct++) {
+          for (int ct = 0; ct < NCT; ct++) {
+            if constexpr (OP == F_VAL || OP == F_DIVLAMBDA) {
+              acc[0][tt][ct][0] = mfma(xP, b[0][gP][ct][0], acc[0][tt][ct][0]);
+              acc[0][tt][ct][1] = mfma(xP, b[0][gP][ct][1], acc[0][tt][ct][1]);
+            } else if constexpr (OP == F_DIVMU) {
+              acc[0][tt][ct][0] = mfma(xH, b[0][gH][ct][0], acc[0][tt][ct][0]);
+              acc[0][tt][ct][1] = mfma(xH, b[0][gH][ct][1], acc[0][tt][ct][1]);
+            } else if constexpr (OP == F_ALPHA) {
+              // i m P.A - H.B :  (i m z) = (-m z.im, m z.re)
+              acc[0][tt][ct][0] = mfma(xP, -dm * b[0][gP][ct][1], acc[0][tt][ct][0]);
+              acc[0][tt][ct][1] = mfma(xP, dm * b[0][gP][ct][0], acc[0][tt][ct][1]);
+              acc[0][tt][ct][0] = mfma(-xH, b[1][gH][ct][0], acc[0][tt][ct][0]);
+              acc[0][tt][ct][1] = mfma(-xH, b[1][gH][ct][1], acc[0][tt][ct][1]);
+            } else {  // F_UV2VORDIV, inputs U cos (0), V cos (1)
               // vor = i m P.V + H.U
-              acc[0][tt][ct][0] = mfma(aP, -dm * b[1][gP][ct][1], acc[0][tt][ct][0]);
-              acc[0][tt][ct][1] = mfma(aP, dm * b[1][gP][ct][0], acc[0][tt][ct][1]);
-              acc[0][tt][ct][0] = mfma(aH, b[0][gH][ct][0], acc[0][tt][ct][0]);
-              acc[0][tt][ct][1] = mfma(aH, b[0][gH][ct][1], acc[0][tt][ct][1]);
+              acc[0][tt][ct][0] = mfma(xP, -dm * b[1][gP][ct][1], acc[0][tt][ct][0]);
+              acc[0][tt][ct][1] = mfma(xP, dm * b[1][gP][ct][0], acc[0][tt][ct][1]);
+              acc[0][tt][ct][0] = mfma(xH, b[0][gH][ct][0], acc[0][tt][ct][0]);
+              acc[0][tt][ct][1] = mfma(xH, b[0][gH][ct][1], acc[0][tt][ct][1]);
               // div = i m P.U - H.V
-              acc[1][tt][ct][0] = mfma(aP, -dm * b[0][gP][ct][1], acc[1][tt][ct][0]);
-              acc[1][tt][ct][1] = mfma(aP, dm * b[0][gP][ct][0], acc[1][tt][ct][1]);
-              acc[1][tt][ct][0] = mfma(-aH, b[1][gH][ct][0], acc[1][tt][ct][0]);
-              acc[1][tt][ct][1] = mfma(-aH, b[1][gH][ct][1], acc[1][tt][ct][1]);
+              acc[1][tt][ct][0] = mfma(xP, -dm * b[0][gP][ct][1], acc[1][tt][ct][0]);
+              acc[1][tt][ct][1] = mfma(xP, dm * b[0][gP][ct][0], acc[1][tt][ct][1]);
+              acc[1][tt][ct][0] = mfma(-xH, b[1][gH][ct][0], acc[1][tt][ct][0]);
+              acc[1][tt][ct][1] = mfma(-xH, b[1][gH][ct][1], acc[1][tt][ct][1]);
             }
           }
         }
+        }
       }
+    };
+
+    load_b(0);
+    load_a(0, aP0, aH0);
+    for (int ch = 0; ch < nch; ch += 2) {
+      const int k0 = ch * KC;
+      stage(k0);
+      __syncthreads();
+      if (ch + 1 < nch) {
+        load_b(k0 + KC);
+        load_a(k0 + KC, aP1, aH1);
+      }
+      compute(k0, aP0, aH0);
+      __syncthreads();
+      if (ch + 1 >= nch) break;
+      stage(k0 + KC);
+      __syncthreads();
+      if (ch + 2 < nch) {
+        load_b(k0 + 2 * KC);
+        load_a(k0 + 2 * KC, aP0, aH0);
+      }
+      compute(k0 + KC, aP1, aH1);
       __syncthreads();
     }
 
     // ---- epilogue: rotation/sign, scale, diagonal, scatter to SPMODEL order
 #pragma unroll
     for (int tt = 0; tt < MAXT; tt++) {
-      const int t = tbase + tt * 4 + w;
+      const int t = tbase + w + WAVES * tt;
       if (t >= RT) continue;
       const int p = t < tE ? 0 : 1;
       const int tp = p ? t - tE : t;
@@ -200,10 +254,10 @@ __global__ __launch_bounds__(256) void k_leg_fwd(LegFwdArgs a) {
             }
             const double* dg = o == 0 ? a.diag0 : a.diag1;
             double* W = o == 0 ? a.W0 : a.W1;
-            double sre = a.scale * (dg ? dg[lre] : 1.0);
+            const double sre = a.scale * (dg ? dg[lre] : 1.0);
             W[lre * a.sl + (long)c * a.sc] = re * sre;
             if (m > 0) {
-              double sim = a.scale * (dg ? dg[lim] : 1.0);
+              const double sim = a.scale * (dg ? dg[lim] : 1.0);
               W[lim * a.sl + (long)c * a.sc] = im * sim;
             }
           }
@@ -214,12 +268,19 @@ __global__ __launch_bounds__(256) void k_leg_fwd(LegFwdArgs a) {
 }
 
 // ------------------------------------------------------------------ inverse
-template <int OP, int CB, int MAXT>
-__global__ __launch_bounds__(256) void k_leg_inv(LegInvArgs a) {
+// Wave w owns latitude tiles t = tbase + w + WAVES*tt; the K loop runs over the total
+// wavenumbers of parity 0 then parity 1 in chunks of KC (never mixing parities in one
+// chunk), pipelined like the forward.
+template <int OP, int CB, int WAVES, int MAXT, int KC>
+__global__ __launch_bounds__(WAVES * 64) void k_leg_inv(LegInvArgs a) {
   using Tr = InvTraits<OP>;
+  constexpr int NT = WAVES * 64;
   constexpr int NIN = Tr::NIN, NOUT = Tr::NOUT, NCT = CB / TILE;
   constexpr int PITCH = CB + 1;
-  __shared__ double sB[NIN][2][KCH * PITCH];  // [input][re, im][KCH][PITCH]
+  constexpr int EPT = (KC * CB + NT - 1) / NT;
+  constexpr int NKS = KC / 4;
+  constexpr bool UP = Tr::USE_P, UH = Tr::USE_H;
+  __shared__ double sB[NIN][2][KC * PITCH];  // [input][re, im][KC][PITCH]
 
   const int m = blockIdx.y;
   const int c0 = blockIdx.x * CB;
@@ -227,11 +288,14 @@ __global__ __launch_bounds__(256) void k_leg_inv(LegInvArgs a) {
   const int N = a.N, J2 = a.J2;
   const int nE = n_par(N, m, 0), nO = n_par(N, m, 1);
   const int kp0 = (nE + KCH - 1) / KCH * KCH, kp1 = (nO + KCH - 1) / KCH * KCH;
+  const int nch0 = (kp0 + KC - 1) / KC, nch = nch0 + (kp1 + KC - 1) / KC;
   const int RT = a.J2pad / TILE;
+  const double* tabP = a.tabP + a.offI[m];
+  const double* tabH = a.tabH + a.offI[m];
   const double* Win[2] = {a.W0, a.W1};
   const double dm = (double)m;
 
-  for (int tbase = 0; tbase < RT; tbase += 4 * MAXT) {
+  for (int tbase = 0; tbase < RT; tbase += WAVES * MAXT) {
     // [output][tile][ct][re/im][sym/anti]
     double4_t acc[NOUT][MAXT][NCT][2][2];
 #pragma unroll
@@ -243,96 +307,154 @@ __global__ __launch_bounds__(256) void k_leg_inv(LegInvArgs a) {
 #pragma unroll
           for (int r = 0; r < 2; r++) acc[o][t][ct][r][0] = acc[o][t][ct][r][1] = double4_t{0, 0, 0, 0};
 
-    for (int p = 0; p < 2; p++) {
+    double bre[NIN][EPT], bim[NIN][EPT];
+    double aP0[MAXT][NKS], aH0[MAXT][NKS], aP1[MAXT][NKS], aH1[MAXT][NKS];
+    auto chunk_pos = [&](int ch, int& p, int& k0) {
+      p = ch < nch0 ? 0 : 1;
+      k0 = (p ? ch - nch0 : ch) * KC;
+    };
+    auto load_b = [&](int ch) {
+      int p, k0;
+      chunk_pos(ch, p, k0);
       const int np = p ? nO : nE;
+#pragma unroll
+      for (int i = 0; i < NIN; i++)
+#pragma unroll
+        for (int q = 0; q < EPT; q++) {
+          const int e = tid + NT * q;
+          const int kk = e / CB, cc = e % CB;
+          const int k = k0 + kk, c = c0 + cc;
+          bre[i][q] = bim[i][q] = 0.0;
+          if (e < KC * CB && k < np && c < a.ncols) {
+            const int n = m + p + 2 * k;
+            const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
+            const double pr = a.pro ? a.pro[lre] : 1.0;
+            bre[i][q] = pr * Win[i][lre * a.sl + (long)c * a.sc];
+            if (m > 0) bim[i][q] = pr * Win[i][lim * a.sl + (long)c * a.sc];
+          }
+        }
+    };
+    auto load_a = [&](int ch, double (&xP)[MAXT][NKS], double (&xH)[MAXT][NKS]) {
+      int p, k0;
+      chunk_pos(ch, p, k0);
       const int kp = p ? kp1 : kp0;
       const long pbase = p ? (long)a.J2pad * kp0 : 0;
-      for (int k0 = 0; k0 < kp; k0 += KCH) {
 #pragma unroll
-        for (int i = 0; i < NIN; i++) {
-          for (int e = tid; e < KCH * CB; e += 256) {
-            int kk = e / CB, cc = e % CB;
-            int k = k0 + kk, c = c0 + cc;
-            double re = 0.0, im = 0.0;
-            if (k < np && c < a.ncols) {
-              int n = m + p + 2 * k;
-              long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
-              double pr = a.pro ? a.pro[lre] : 1.0;
-              re = pr * Win[i][lre * a.sl + (long)c * a.sc];
-              if (m > 0) im = pr * Win[i][lim * a.sl + (long)c * a.sc];
-            }
-            sB[i][0][kk * PITCH + cc] = re;
-            sB[i][1][kk * PITCH + cc] = im;
+      for (int tt = 0; tt < MAXT; tt++) {
+        const int t = tbase + w + WAVES * tt;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ks++) {
+          const int kg = k0 / 4 + ks;
+          const long io = pbase + (long)t * (kp / 4) * 64 + (long)kg * 64 + lane;
+          const bool ok = t < RT && kg < kp / 4;
+          xP[tt][ks] = (UP && ok) ? tabP[io] : 0.0;
+          xH[tt][ks] = (UH && ok) ? tabH[io] : 0.0;
+        }
+      }
+    };
+    auto stage = [&]() {
+#pragma unroll
+      for (int i = 0; i < NIN; i++)
+#pragma unroll
+        for (int q = 0; q < EPT; q++) {
+          const int e = tid + NT * q;
+          if (e < KC * CB) {
+            const int kk = e / CB, cc = e % CB;
+            sB[i][0][kk * PITCH + cc] = bre[i][q];
+            sB[i][1][kk * PITCH + cc] = bim[i][q];
           }
         }
-        __syncthreads();
+    };
+    // P-terms land in sym for even parity, anti for odd; H-terms the opposite.  The
+    // parity is uniform per chunk and a template argument, so accumulator selection is static.
+    auto compute_p = [&](auto PARITY, int kvalid, const double (&xPa)[MAXT][NKS],
+                         const double (&xHa)[MAXT][NKS]) {
+      constexpr int sP = decltype(PARITY)::value, sH = 1 - decltype(PARITY)::value;
 #pragma unroll
-        for (int ks = 0; ks < KCH / 4; ks++) {
-          const int krow = ks * 4 + (lane >> 4);
-          double b[NIN][NCT][2];
+      for (int ks = 0; ks < NKS; ks++) {
+        if (ks < kvalid) {
+        const int krow = ks * 4 + (lane >> 4);
+        double b[NIN][NCT][2];
 #pragma unroll
-          for (int i = 0; i < NIN; i++)
+        for (int i = 0; i < NIN; i++)
 #pragma unroll
-            for (int ct = 0; ct < NCT; ct++) {
-              int col = ct * TILE + (lane & 15);
-              b[i][ct][0] = sB[i][0][krow * PITCH + col];
-              b[i][ct][1] = sB[i][1][krow * PITCH + col];
-            }
-          // P-terms land in sym for even parity, anti for odd; H-terms the opposite
-          const int sP = p, sH = 1 - p;
+          for (int ct = 0; ct < NCT; ct++) {
+            const int col = ct * TILE + (lane & 15);
+            b[i][ct][0] = sB[i][0][krow * PITCH + col];
+            b[i][ct][1] = sB[i][1][krow * PITCH + col];
+          }
 #pragma unroll
-          for (int tt = 0; tt < MAXT; tt++) {
-            const int t = tbase + tt * 4 + w;
-            if (t >= RT) continue;
-            const long io = pbase + (long)t * (kp / 4) * 64 + (long)(k0 / 4 + ks) * 64 + lane;
+        for (int tt = 0; tt < MAXT; tt++) {
+          const int t = tbase + w + WAVES * tt;
+          if (t >= RT) continue;
+          const double xP = xPa[tt][ks], xH = xHa[tt][ks];
+#pragma unroll
+          for (int ct = 0; ct < NCT; ct++) {
             if constexpr (OP == I_VAL || OP == I_GRADLAMBDA) {
-              double aP = a.tabP[a.offI[m] + io];
 #pragma unroll
-              for (int ct = 0; ct < NCT; ct++)
-#pragma unroll
-                for (int r = 0; r < 2; r++) acc[0][tt][ct][r][sP] = mfma(aP, b[0][ct][r], acc[0][tt][ct][r][sP]);
+              for (int r = 0; r < 2; r++) acc[0][tt][ct][r][sP] = mfma(xP, b[0][ct][r], acc[0][tt][ct][r][sP]);
             } else if constexpr (OP == I_GRADMU) {
-              double aH = a.tabH[a.offI[m] + io];
 #pragma unroll
-              for (int ct = 0; ct < NCT; ct++)
-#pragma unroll
-                for (int r = 0; r < 2; r++) acc[0][tt][ct][r][sH] = mfma(aH, b[0][ct][r], acc[0][tt][ct][r][sH]);
+              for (int r = 0; r < 2; r++) acc[0][tt][ct][r][sH] = mfma(xH, b[0][ct][r], acc[0][tt][ct][r][sH]);
             } else if constexpr (OP == I_ALPHA) {
               // i m P.A + H.B
-              double aP = a.tabP[a.offI[m] + io], aH = a.tabH[a.offI[m] + io];
-#pragma unroll
-              for (int ct = 0; ct < NCT; ct++) {
-                acc[0][tt][ct][0][sP] = mfma(aP, -dm * b[0][ct][1], acc[0][tt][ct][0][sP]);
-                acc[0][tt][ct][1][sP] = mfma(aP, dm * b[0][ct][0], acc[0][tt][ct][1][sP]);
-                acc[0][tt][ct][0][sH] = mfma(aH, b[1][ct][0], acc[0][tt][ct][0][sH]);
-                acc[0][tt][ct][1][sH] = mfma(aH, b[1][ct][1], acc[0][tt][ct][1][sH]);
-              }
+              acc[0][tt][ct][0][sP] = mfma(xP, -dm * b[0][ct][1], acc[0][tt][ct][0][sP]);
+              acc[0][tt][ct][1][sP] = mfma(xP, dm * b[0][ct][0], acc[0][tt][ct][1][sP]);
+              acc[0][tt][ct][0][sH] = mfma(xH, b[1][ct][0], acc[0][tt][ct][0][sH]);
+              acc[0][tt][ct][1][sH] = mfma(xH, b[1][ct][1], acc[0][tt][ct][1][sH]);
             } else {  // I_VORDIV2UV: inputs psi (0), chi (1) after the prologue
-              double aP = a.tabP[a.offI[m] + io], aH = a.tabH[a.offI[m] + io];
-#pragma unroll
-              for (int ct = 0; ct < NCT; ct++) {
-                // U = i m P.chi - H.psi
-                acc[0][tt][ct][0][sP] = mfma(aP, -dm * b[1][ct][1], acc[0][tt][ct][0][sP]);
-                acc[0][tt][ct][1][sP] = mfma(aP, dm * b[1][ct][0], acc[0][tt][ct][1][sP]);
-                acc[0][tt][ct][0][sH] = mfma(-aH, b[0][ct][0], acc[0][tt][ct][0][sH]);
-                acc[0][tt][ct][1][sH] = mfma(-aH, b[0][ct][1], acc[0][tt][ct][1][sH]);
-                // V = i m P.psi + H.chi
-                acc[1][tt][ct][0][sP] = mfma(aP, -dm * b[0][ct][1], acc[1][tt][ct][0][sP]);
-                acc[1][tt][ct][1][sP] = mfma(aP, dm * b[0][ct][0], acc[1][tt][ct][1][sP]);
-                acc[1][tt][ct][0][sH] = mfma(aH, b[1][ct][0], acc[1][tt][ct][0][sH]);
-                acc[1][tt][ct][1][sH] = mfma(aH, b[1][ct][1], acc[1][tt][ct][1][sH]);
-              }
+              // U = i m P.chi - H.psi
+              acc[0][tt][ct][0][sP] = mfma(xP, -dm * b[1][ct][1], acc[0][tt][ct][0][sP]);
+              acc[0][tt][ct][1][sP] = mfma(xP, dm * b[1][ct][0], acc[0][tt][ct][1][sP]);
+              acc[0][tt][ct][0][sH] = mfma(-xH, b[0][ct][0], acc[0][tt][ct][0][sH]);
+              acc[0][tt][ct][1][sH] = mfma(-xH, b[0][ct][1], acc[0][tt][ct][1][sH]);
+              // V = i m P.psi + H.chi
+              acc[1][tt][ct][0][sP] = mfma(xP, -dm * b[0][ct][1], acc[1][tt][ct][0][sP]);
+              acc[1][tt][ct][1][sP] = mfma(xP, dm * b[0][ct][0], acc[1][tt][ct][1][sP]);
+              acc[1][tt][ct][0][sH] = mfma(xH, b[1][ct][0], acc[1][tt][ct][0][sH]);
+              acc[1][tt][ct][1][sH] = mfma(xH, b[1][ct][1], acc[1][tt][ct][1][sH]);
             }
           }
         }
-        __syncthreads();
+        }
       }
+    };
+    auto compute = [&](int ch, const double (&xPa)[MAXT][NKS], const double (&xHa)[MAXT][NKS]) {
+      int p, k0;
+      chunk_pos(ch, p, k0);
+      const int kvalid = (p ? kp1 : kp0) / 4 - k0 / 4;
+      if (p == 0)
+        compute_p(std::integral_constant<int, 0>(), kvalid, xPa, xHa);
+      else
+        compute_p(std::integral_constant<int, 1>(), kvalid, xPa, xHa);
+    };
+
+    load_b(0);
+    load_a(0, aP0, aH0);
+    for (int ch = 0; ch < nch; ch += 2) {
+      stage();
+      __syncthreads();
+      if (ch + 1 < nch) {
+        load_b(ch + 1);
+        load_a(ch + 1, aP1, aH1);
+      }
+      compute(ch, aP0, aH0);
+      __syncthreads();
+      if (ch + 1 >= nch) break;
+      stage();
+      __syncthreads();
+      if (ch + 2 < nch) {
+        load_b(ch + 2);
+        load_a(ch + 2, aP0, aH0);
+      }
+      compute(ch + 1, aP1, aH1);
+      __syncthreads();
     }
 
     // ---- epilogue: north = sym + anti, south = sym - anti, row scale
 #pragma unroll
     for (int tt = 0; tt < MAXT; tt++) {
-      const int t = tbase + tt * 4 + w;
+      const int t = tbase + w + WAVES * tt;
       if (t >= RT) continue;
 #pragma unroll
       for (int ct = 0; ct < NCT; ct++) {
@@ -369,20 +491,44 @@ __global__ __launch_bounds__(256) void k_leg_inv(LegInvArgs a) {
 }
 
 // ------------------------------------------------------------------ launchers
-template <int OP, int CB>
+template <int OP, int CB, int WAVES, int MAXT, int KC>
 static int fwd_launch(const dsht_plan* p, const LegFwdArgs& a, hipStream_t s) {
   dim3 grid(cdiv(a.ncols, CB), p->N + 1);
-  hipLaunchKernelGGL((k_leg_fwd<OP, CB, 3>), grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_leg_fwd<OP, CB, WAVES, MAXT, KC>), grid, dim3(WAVES * 64), 0, s, a);
   DSHT_HIP(hipGetLastError());
   return 0;
 }
 
-template <int OP, int CB>
+template <int OP, int CB, int WAVES, int MAXT, int KC>
 static int inv_launch(const dsht_plan* p, const LegInvArgs& a, hipStream_t s) {
   dim3 grid(cdiv(a.ncols, CB), p->N + 1);
-  hipLaunchKernelGGL((k_leg_inv<OP, CB, 2>), grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_leg_inv<OP, CB, WAVES, MAXT, KC>), grid, dim3(WAVES * 64), 0, s, a);
   DSHT_HIP(hipGetLastError());
   return 0;
+}
+
+// forward: row tiles of the heaviest wavenumber (m = 0) decide waves and tiles per wave
+template <int OP, int CB>
+static int fwd_pick(const dsht_plan* p, const LegFwdArgs& a, hipStream_t s) {
+  const int RT0 = cdiv(n_par(p->N, 0, 0), TILE) + cdiv(n_par(p->N, 0, 1), TILE);
+  // two-input forms keep 16-column blocks when a wave owns several tiles (register budget)
+  constexpr bool TWO = FwdTraits<OP>::NIN == 2;
+  constexpr int CB1 = FwdTraits<OP>::NOUT == 2 ? 16 : CB, CB2 = TWO ? 16 : CB;
+  constexpr int KC2 = TWO ? 16 : 32;
+  if (RT0 <= 4) return fwd_launch<OP, CB1, 4, 1, 32>(p, a, s);
+  if (RT0 <= 8) return fwd_launch<OP, CB1, 8, 1, 32>(p, a, s);
+  if (RT0 <= 16) return fwd_launch<OP, CB2, 8, 2, KC2>(p, a, s);
+  return fwd_launch<OP, CB2, 8, 3, 16>(p, a, s);
+}
+
+template <int OP, int CB>
+static int inv_pick(const dsht_plan* p, const LegInvArgs& a, hipStream_t s) {
+  const int RT = a.J2pad / TILE;
+  constexpr int KC = InvTraits<OP>::NIN == 2 ? 32 : 64;
+  constexpr int CB1 = InvTraits<OP>::NOUT == 2 ? 16 : CB;
+  if (RT <= 2) return inv_launch<OP, CB1, 2, 1, KC>(p, a, s);
+  if (RT <= 4) return inv_launch<OP, CB1, 4, 1, KC>(p, a, s);
+  return inv_launch<OP, CB1, 8, 1, KC>(p, a, s);
 }
 
 // Algorithmic work of one contraction launch: every (input, table) term costs
@@ -404,12 +550,11 @@ int launch_leg_fwd(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t 
   leg_work(p, nin[op], nout[op], nt[op], ntab[op], true, a.ncols, &by, &fl);
   ProfScope ps("leg_fwd", s, by, fl);
   switch (op) {
-    case F_VAL: return wide ? fwd_launch<F_VAL, 32>(p, a, s) : fwd_launch<F_VAL, 16>(p, a, s);
-    case F_DIVLAMBDA:
-      return wide ? fwd_launch<F_DIVLAMBDA, 32>(p, a, s) : fwd_launch<F_DIVLAMBDA, 16>(p, a, s);
-    case F_DIVMU: return wide ? fwd_launch<F_DIVMU, 32>(p, a, s) : fwd_launch<F_DIVMU, 16>(p, a, s);
-    case F_ALPHA: return fwd_launch<F_ALPHA, 16>(p, a, s);
-    case F_UV2VORDIV: return fwd_launch<F_UV2VORDIV, 16>(p, a, s);
+    case F_VAL: return wide ? fwd_pick<F_VAL, 32>(p, a, s) : fwd_pick<F_VAL, 16>(p, a, s);
+    case F_DIVLAMBDA: return wide ? fwd_pick<F_DIVLAMBDA, 32>(p, a, s) : fwd_pick<F_DIVLAMBDA, 16>(p, a, s);
+    case F_DIVMU: return wide ? fwd_pick<F_DIVMU, 32>(p, a, s) : fwd_pick<F_DIVMU, 16>(p, a, s);
+    case F_ALPHA: return wide ? fwd_pick<F_ALPHA, 32>(p, a, s) : fwd_pick<F_ALPHA, 16>(p, a, s);
+    case F_UV2VORDIV: return wide ? fwd_pick<F_UV2VORDIV, 32>(p, a, s) : fwd_pick<F_UV2VORDIV, 16>(p, a, s);
   }
   return fail(DSHT_EARG, "launch_leg_fwd: bad op %d", op);
 }
@@ -421,12 +566,11 @@ int launch_leg_inv(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t 
   leg_work(p, nin[op], nout[op], nt[op], ntab[op], false, a.ncols, &by, &fl);
   ProfScope ps("leg_inv", s, by, fl);
   switch (op) {
-    case I_VAL: return wide ? inv_launch<I_VAL, 32>(p, a, s) : inv_launch<I_VAL, 16>(p, a, s);
-    case I_GRADLAMBDA:
-      return wide ? inv_launch<I_GRADLAMBDA, 32>(p, a, s) : inv_launch<I_GRADLAMBDA, 16>(p, a, s);
-    case I_GRADMU: return wide ? inv_launch<I_GRADMU, 32>(p, a, s) : inv_launch<I_GRADMU, 16>(p, a, s);
-    case I_ALPHA: return inv_launch<I_ALPHA, 16>(p, a, s);
-    case I_VORDIV2UV: return inv_launch<I_VORDIV2UV, 16>(p, a, s);
+    case I_VAL: return wide ? inv_pick<I_VAL, 32>(p, a, s) : inv_pick<I_VAL, 16>(p, a, s);
+    case I_GRADLAMBDA: return wide ? inv_pick<I_GRADLAMBDA, 32>(p, a, s) : inv_pick<I_GRADLAMBDA, 16>(p, a, s);
+    case I_GRADMU: return wide ? inv_pick<I_GRADMU, 32>(p, a, s) : inv_pick<I_GRADMU, 16>(p, a, s);
+    case I_ALPHA: return wide ? inv_pick<I_ALPHA, 32>(p, a, s) : inv_pick<I_ALPHA, 16>(p, a, s);
+    case I_VORDIV2UV: return wide ? inv_pick<I_VORDIV2UV, 32>(p, a, s) : inv_pick<I_VORDIV2UV, 16>(p, a, s);
   }
   return fail(DSHT_EARG, "launch_leg_inv: bad op %d", op);
 }
