@@ -92,12 +92,7 @@ static void prof_flush() {
 // One scratch buffer per (plan, stream); a call holds its lock while it enqueues, so
 // calls on one stream are serialised by the stream and calls on different streams never
 // share scratch.
-struct WsLock {
-  Workspace* ws;
-  std::unique_lock<std::mutex> lk;
-};
-
-static int acquire_ws(dsht_plan* p, hipStream_t s, size_t bytes, WsLock& out) {
+int workspace_acquire(dsht_plan* p, hipStream_t s, size_t bytes, WsHandle& out) {
   Workspace* ws;
   {
     std::lock_guard<std::mutex> g(p->ws_mu);
@@ -105,7 +100,6 @@ static int acquire_ws(dsht_plan* p, hipStream_t s, size_t bytes, WsLock& out) {
     if (!slot) slot.reset(new Workspace());
     ws = slot.get();
   }
-  out.ws = ws;
   out.lk = std::unique_lock<std::mutex>(ws->mu);
   if (ws->bytes < bytes) {
     if (ws->buf) {
@@ -120,6 +114,7 @@ static int acquire_ws(dsht_plan* p, hipStream_t s, size_t bytes, WsLock& out) {
     }
     ws->bytes = bytes;
   }
+  out.buf = ws->buf;
   return 0;
 }
 
@@ -149,10 +144,10 @@ static int run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, l
   if (ncols <= 0) return 0;
   const int nin = (op == F_ALPHA || op == F_UV2VORDIV) ? 2 : 1;
   const int ch = chunk_cols(p, ncols);
-  WsLock wl;
-  int rc = acquire_ws(p, s, nin * fourier_bytes(p, ch), wl);
+  WsHandle wl;
+  int rc = workspace_acquire(p, s, nin * fourier_bytes(p, ch), wl);
   if (rc) return rc;
-  double2* G0 = (double2*)wl.ws->buf;
+  double2* G0 = (double2*)wl.buf;
   double2* G1 = G0 + (size_t)(p->N + 1) * p->J * ch;
   for (int c0 = 0; c0 < ncols; c0 += ch) {
     int nc = ncols - c0 < ch ? ncols - c0 : ch;
@@ -190,10 +185,10 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
   if (ncols <= 0) return 0;
   const int nout = (op == I_VORDIV2UV) ? 2 : 1;
   const int ch = chunk_cols(p, ncols);
-  WsLock wl;
-  int rc = acquire_ws(p, s, nout * fourier_bytes(p, ch), wl);
+  WsHandle wl;
+  int rc = workspace_acquire(p, s, nout * fourier_bytes(p, ch), wl);
   if (rc) return rc;
-  double2* G0 = (double2*)wl.ws->buf;
+  double2* G0 = (double2*)wl.buf;
   double2* G1 = G0 + (size_t)(p->N + 1) * p->J * ch;
   for (int c0 = 0; c0 < ncols; c0 += ch) {
     int nc = ncols - c0 < ch ? ncols - c0 : ch;

@@ -35,6 +35,24 @@ __host__ __device__ inline int n_par(int N, int m, int p) {
 enum FwdOp { F_VAL = 0, F_DIVLAMBDA = 1, F_DIVMU = 2, F_ALPHA = 3, F_UV2VORDIV = 4 };
 enum InvOp { I_VAL = 0, I_GRADLAMBDA = 1, I_GRADMU = 2, I_ALPHA = 3, I_VORDIV2UV = 4 };
 
+// inputs, outputs and tables of each operator
+template <int OP>
+struct FwdTraits {
+  static constexpr int NIN = (OP == F_ALPHA || OP == F_UV2VORDIV) ? 2 : 1;
+  static constexpr int NOUT = (OP == F_UV2VORDIV) ? 2 : 1;
+  static constexpr bool USE_P = (OP != F_DIVMU);
+  static constexpr bool USE_H = (OP == F_DIVMU || OP == F_ALPHA || OP == F_UV2VORDIV);
+};
+
+template <int OP>
+struct InvTraits {
+  static constexpr int NIN = (OP == I_ALPHA || OP == I_VORDIV2UV) ? 2 : 1;
+  static constexpr int NOUT = (OP == I_VORDIV2UV) ? 2 : 1;
+  static constexpr bool USE_P = (OP != I_GRADMU);
+  static constexpr bool USE_H = (OP == I_GRADMU || OP == I_ALPHA || OP == I_VORDIV2UV);
+};
+
+
 // Arguments of the forward Legendre contraction (Fourier -> spectral).
 struct LegFwdArgs {
   const double2* G0;  // Fourier coefficients [m][j*gld + c] (complex), input 0
@@ -105,6 +123,12 @@ struct Workspace {
 
 struct HostCtx;  // per-thread host-API context (api.hip)
 
+// a locked scratch buffer of the (plan, stream) pair; unlocks when destroyed
+struct WsHandle {
+  void* buf = nullptr;
+  std::unique_lock<std::mutex> lk;
+};
+
 // ---------------------------------------------------------------- plan
 }  // namespace dsht
 
@@ -155,6 +179,8 @@ int fail(int code, const char* fmt, ...);
                           hipGetErrorString(e_));                                 \
   } while (0)
 
+// api.hip: per-stream scratch
+int workspace_acquire(dsht_plan* p, hipStream_t s, size_t bytes, WsHandle& out);
 // tables.hip
 int build_tables(dsht_plan* p);
 // fft.hip
@@ -164,6 +190,12 @@ int launch_fft_fwd(const dsht_plan* p, const double* xy, long ldg, int ncols, do
                    hipStream_t s);
 int launch_fft_inv(const dsht_plan* p, const double2* G, long gld, int ncols, double* xy, long ldg,
                    hipStream_t s);
+// FP32 Fourier intermediates of the split-precision forms; colmax (nullable) receives the
+// per-column max |G| as float bits
+int launch_fft_fwd32(const dsht_plan* p, const double* xy, long ldg, int ncols, float2* G, long gld,
+                     unsigned* colmax, hipStream_t s);
+int launch_fft_inv32(const dsht_plan* p, const float2* G, long gld, int ncols, double* xy, long ldg,
+                     hipStream_t s);
 // legendre_f64.hip
 int launch_leg_fwd(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t s);
 int launch_leg_inv(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t s);

@@ -29,6 +29,16 @@ __device__ inline double2 mul_i(double2 a) {
   return SIGN < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
 }
 
+// Fourier coefficients are stored as double2 (FP64 form) or float2 (split-precision forms);
+// arithmetic is FP64 either way.
+__device__ inline double2 load_g(const double2* p) { return *p; }
+__device__ inline double2 load_g(const float2* p) {
+  float2 v = *p;
+  return make_double2((double)v.x, (double)v.y);
+}
+__device__ inline void store_g(double2* p, double2 v) { *p = v; }
+__device__ inline void store_g(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+
 // in-register radix-R DFT, y_q = sum_p v_p exp(SIGN 2 pi i p q / R)
 template <int SIGN>
 __device__ inline void dft_small(double2* v, int R) {
@@ -105,8 +115,9 @@ __device__ double2* stockham(double2* src, double2* dst, int M, int nrad, const 
   return src;
 }
 
-__global__ __launch_bounds__(256) void k_fft_fwd(const double* __restrict__ xy, double2* __restrict__ G,
-                                                 FftArgs a) {
+template <typename GT>
+__global__ __launch_bounds__(256) void k_fft_fwd(const double* __restrict__ xy, GT* __restrict__ G, FftArgs a,
+                                                 unsigned* __restrict__ colmax) {
   extern __shared__ double2 lds[];
   const int M = a.M, rpb = a.rows_per_block;
   const int tpr = blockDim.x / rpb;
@@ -142,12 +153,14 @@ __global__ __launch_bounds__(256) void k_fft_fwd(const double* __restrict__ xy, 
     double2 O = make_double2(0.5 * (zm.y - zc.y), -0.5 * (zm.x - zc.x));  // -i (zm - zc) / 2
     double2 g = cadd(E, cmul(a.twI[m], O));
     int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
-    G[(long)m * a.gmst + (long)j * a.gld + c] = make_double2(g.x * inv_i, g.y * inv_i);
+    g = make_double2(g.x * inv_i, g.y * inv_i);
+    store_g(&G[(long)m * a.gmst + (long)j * a.gld + c], g);
+    if (colmax) atomicMax(colmax + c, __float_as_uint((float)fmax(fabs(g.x), fabs(g.y))));
   }
 }
 
-__global__ __launch_bounds__(256) void k_fft_inv(const double2* __restrict__ G, double* __restrict__ xy,
-                                                 FftArgs a) {
+template <typename GT>
+__global__ __launch_bounds__(256) void k_fft_inv(const GT* __restrict__ G, double* __restrict__ xy, FftArgs a) {
   extern __shared__ double2 lds[];
   const int M = a.M, rpb = a.rows_per_block, N = a.N;
   const int tpr = blockDim.x / rpb;
@@ -163,7 +176,7 @@ __global__ __launch_bounds__(256) void k_fft_inv(const double2* __restrict__ G, 
     double2 g = make_double2(0.0, 0.0);
     if (m <= N && r < nrows) {
       int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
-      g = G[(long)m * a.gmst + (long)j * a.gld + c];
+      g = load_g(&G[(long)m * a.gmst + (long)j * a.gld + c]);
       if (m == 0) g.y = 0.0;
     }
     X[(long)rr * M + m] = g;
@@ -264,10 +277,11 @@ __device__ __forceinline__ void p2_fft(double2* buf, const double2* tw, int tid)
   if constexpr (P2Plan<M>::HAS2) p2_stage<M, ROWS, 2, (1 << (2 * P2Plan<M>::N4)), SIGN>(buf, tw, tid);
 }
 
-template <int M, int ROWS>
-__global__ __launch_bounds__(256) void k_fft_fwd_p2(const double* __restrict__ xy, double2* __restrict__ G,
-                                                    FftArgs a) {
+template <int M, int ROWS, typename GT>
+__global__ __launch_bounds__(256) void k_fft_fwd_p2(const double* __restrict__ xy, GT* __restrict__ G,
+                                                    FftArgs a, unsigned* __restrict__ colmax) {
   extern __shared__ double2 lds[];
+  __shared__ unsigned smax[ROWS];
   double2* buf = lds;                  // ROWS * M
   double2* tw = lds + ROWS * M;        // M
   double2* twi = tw + M;               // M
@@ -293,7 +307,10 @@ __global__ __launch_bounds__(256) void k_fft_fwd_p2(const double* __restrict__ x
       buf[e] = z;
     }
     __syncthreads();
+    if (colmax && tid < ROWS) smax[tid] = 0u;
     p2_fft<M, ROWS, -1>(buf, tw, tid);
+    // each thread keeps one row (256 is a multiple of ROWS)
+    float gmax = 0.f;
     for (int idx = tid; idx < (N + 1) * ROWS; idx += 256) {
       const int m = idx / ROWS, rr = idx % ROWS;
       const long r = r0 + rr;
@@ -305,13 +322,20 @@ __global__ __launch_bounds__(256) void k_fft_fwd_p2(const double* __restrict__ x
       double2 O = make_double2(0.5 * (zm.y - zc.y), -0.5 * (zm.x - zc.x));
       double2 g = cadd(E, cmul(twi[m], O));
       const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
-      G[(long)m * a.gmst + (long)j * a.gld + c] = make_double2(g.x * inv_i, g.y * inv_i);
+      g = make_double2(g.x * inv_i, g.y * inv_i);
+      store_g(&G[(long)m * a.gmst + (long)j * a.gld + c], g);
+      gmax = fmaxf(gmax, (float)fmax(fabs(g.x), fabs(g.y)));
+    }
+    if (colmax) {  // per-column magnitude for the scaled FP16 contraction
+      atomicMax(&smax[tid % ROWS], __float_as_uint(gmax));
+      __syncthreads();
+      if (tid < ROWS && r0 + tid < nrows) atomicMax(colmax + (int)((r0 + tid) % a.ncols), smax[tid]);
     }
   }
 }
 
-template <int M, int ROWS>
-__global__ __launch_bounds__(256) void k_fft_inv_p2(const double2* __restrict__ G, double* __restrict__ xy,
+template <int M, int ROWS, typename GT>
+__global__ __launch_bounds__(256) void k_fft_inv_p2(const GT* __restrict__ G, double* __restrict__ xy,
                                                     FftArgs a) {
   extern __shared__ double2 lds[];
   double2* buf = lds;
@@ -332,7 +356,7 @@ __global__ __launch_bounds__(256) void k_fft_inv_p2(const double2* __restrict__ 
       double2 g = make_double2(0.0, 0.0);
       if (m <= N && r < nrows) {
         const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
-        g = G[(long)m * a.gmst + (long)j * a.gld + c];
+        g = load_g(&G[(long)m * a.gmst + (long)j * a.gld + c]);
         if (m == 0) g.y = 0.0;
       }
       buf[rr * M + m] = g;
@@ -435,56 +459,59 @@ struct P2Rows {
   static constexpr int ROWS = (M >= 512) ? 8 : (M >= 256 ? 16 : (M >= 128 ? 16 : 32));
 };
 
-template <int M>
-static int p2_fwd(const dsht_plan* p, const double* xy, const FftArgs& a, double2* G, hipStream_t s) {
+template <int M, typename GT>
+static int p2_fwd(const dsht_plan* p, const double* xy, const FftArgs& a, GT* G, unsigned* colmax,
+                  hipStream_t s) {
   constexpr int ROWS = P2Rows<M>::ROWS;
   long nrows = (long)p->J * a.ncols;
   long nb = cdivl(nrows, ROWS);
   if (nb > 4096) nb = 4096;
   size_t lds = (size_t)(ROWS * M + 2 * M) * sizeof(double2);
-  hipLaunchKernelGGL((k_fft_fwd_p2<M, ROWS>), dim3((unsigned)nb), dim3(256), lds, s, xy, G, a);
+  hipLaunchKernelGGL((k_fft_fwd_p2<M, ROWS, GT>), dim3((unsigned)nb), dim3(256), lds, s, xy, G, a, colmax);
   DSHT_HIP(hipGetLastError());
   return 0;
 }
 
-template <int M>
-static int p2_inv(const dsht_plan* p, const double2* G, const FftArgs& a, double* xy, hipStream_t s) {
+template <int M, typename GT>
+static int p2_inv(const dsht_plan* p, const GT* G, const FftArgs& a, double* xy, hipStream_t s) {
   constexpr int ROWS = P2Rows<M>::ROWS;
   long nrows = (long)p->J * a.ncols;
   long nb = cdivl(nrows, ROWS);
   if (nb > 4096) nb = 4096;
   size_t lds = (size_t)(ROWS * M + 2 * M) * sizeof(double2);
-  hipLaunchKernelGGL((k_fft_inv_p2<M, ROWS>), dim3((unsigned)nb), dim3(256), lds, s, G, xy, a);
+  hipLaunchKernelGGL((k_fft_inv_p2<M, ROWS, GT>), dim3((unsigned)nb), dim3(256), lds, s, G, xy, a);
   DSHT_HIP(hipGetLastError());
   return 0;
 }
 
-int launch_fft_fwd(const dsht_plan* p, const double* xy, long ldg, int ncols, double2* G, long gld,
-                   hipStream_t s) {
+template <typename GT>
+static int fft_fwd_any(const dsht_plan* p, const double* xy, long ldg, int ncols, GT* G, long gld,
+                       unsigned* colmax, hipStream_t s) {
   FftArgs a = fft_args(p, ldg, ncols, gld);
   long nrows = (long)p->J * ncols;
-  ProfScope ps("fft_fwd", s, (double)nrows * (8.0 * p->I + 16.0 * (p->N + 1)),
+  ProfScope ps("fft_fwd", s, (double)nrows * (8.0 * p->I + (double)sizeof(GT) * (p->N + 1)),
                (double)nrows * 2.5 * p->I * std::log2((double)p->I));
   switch (a.M) {
-    case 16: return p2_fwd<16>(p, xy, a, G, s);
-    case 32: return p2_fwd<32>(p, xy, a, G, s);
-    case 64: return p2_fwd<64>(p, xy, a, G, s);
-    case 128: return p2_fwd<128>(p, xy, a, G, s);
-    case 256: return p2_fwd<256>(p, xy, a, G, s);
-    case 512: return p2_fwd<512>(p, xy, a, G, s);
+    case 16: return p2_fwd<16>(p, xy, a, G, colmax, s);
+    case 32: return p2_fwd<32>(p, xy, a, G, colmax, s);
+    case 64: return p2_fwd<64>(p, xy, a, G, colmax, s);
+    case 128: return p2_fwd<128>(p, xy, a, G, colmax, s);
+    case 256: return p2_fwd<256>(p, xy, a, G, colmax, s);
+    case 512: return p2_fwd<512>(p, xy, a, G, colmax, s);
   }
   int nb = (int)cdivl(nrows, a.rows_per_block);
   size_t lds = 2ul * a.rows_per_block * a.M * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_fwd, dim3(nb), dim3(256), lds, s, xy, G, a);
+  hipLaunchKernelGGL(k_fft_fwd<GT>, dim3(nb), dim3(256), lds, s, xy, G, a, colmax);
   DSHT_HIP(hipGetLastError());
   return 0;
 }
 
-int launch_fft_inv(const dsht_plan* p, const double2* G, long gld, int ncols, double* xy, long ldg,
-                   hipStream_t s) {
+template <typename GT>
+static int fft_inv_any(const dsht_plan* p, const GT* G, long gld, int ncols, double* xy, long ldg,
+                       hipStream_t s) {
   FftArgs a = fft_args(p, ldg, ncols, gld);
   long nrows = (long)p->J * ncols;
-  ProfScope ps("fft_inv", s, (double)nrows * (8.0 * p->I + 16.0 * (p->N + 1)),
+  ProfScope ps("fft_inv", s, (double)nrows * (8.0 * p->I + (double)sizeof(GT) * (p->N + 1)),
                (double)nrows * 2.5 * p->I * std::log2((double)p->I));
   switch (a.M) {
     case 16: return p2_inv<16>(p, G, a, xy, s);
@@ -496,9 +523,29 @@ int launch_fft_inv(const dsht_plan* p, const double2* G, long gld, int ncols, do
   }
   int nb = (int)cdivl(nrows, a.rows_per_block);
   size_t lds = 2ul * a.rows_per_block * a.M * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_inv, dim3(nb), dim3(256), lds, s, G, xy, a);
+  hipLaunchKernelGGL(k_fft_inv<GT>, dim3(nb), dim3(256), lds, s, G, xy, a);
   DSHT_HIP(hipGetLastError());
   return 0;
+}
+
+int launch_fft_fwd(const dsht_plan* p, const double* xy, long ldg, int ncols, double2* G, long gld,
+                   hipStream_t s) {
+  return fft_fwd_any<double2>(p, xy, ldg, ncols, G, gld, nullptr, s);
+}
+
+int launch_fft_inv(const dsht_plan* p, const double2* G, long gld, int ncols, double* xy, long ldg,
+                   hipStream_t s) {
+  return fft_inv_any<double2>(p, G, gld, ncols, xy, ldg, s);
+}
+
+int launch_fft_fwd32(const dsht_plan* p, const double* xy, long ldg, int ncols, float2* G, long gld,
+                     unsigned* colmax, hipStream_t s) {
+  return fft_fwd_any<float2>(p, xy, ldg, ncols, G, gld, colmax, s);
+}
+
+int launch_fft_inv32(const dsht_plan* p, const float2* G, long gld, int ncols, double* xy, long ldg,
+                     hipStream_t s) {
+  return fft_inv_any<float2>(p, G, gld, ncols, xy, ldg, s);
 }
 
 }  // namespace dsht

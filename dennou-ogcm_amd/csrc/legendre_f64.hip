@@ -37,22 +37,6 @@ __device__ __forceinline__ double4_t mfma(double a, double b, double4_t c) {
 // result element i of lane l holds D[l/16 + 4i][l%16].
 __device__ __forceinline__ int acc_row(int lane, int i) { return (lane >> 4) + 4 * i; }
 
-template <int OP>
-struct FwdTraits {
-  static constexpr int NIN = (OP == F_ALPHA || OP == F_UV2VORDIV) ? 2 : 1;
-  static constexpr int NOUT = (OP == F_UV2VORDIV) ? 2 : 1;
-  static constexpr bool USE_P = (OP != F_DIVMU);
-  static constexpr bool USE_H = (OP == F_DIVMU || OP == F_ALPHA || OP == F_UV2VORDIV);
-};
-
-template <int OP>
-struct InvTraits {
-  static constexpr int NIN = (OP == I_ALPHA || OP == I_VORDIV2UV) ? 2 : 1;
-  static constexpr int NOUT = (OP == I_VORDIV2UV) ? 2 : 1;
-  static constexpr bool USE_P = (OP != I_GRADMU);
-  static constexpr bool USE_H = (OP == I_GRADMU || OP == I_ALPHA || OP == I_VORDIV2UV);
-};
-
 // ------------------------------------------------------------------ forward
 // WAVES waves per workgroup; wave w owns row tiles t = tbase + w + WAVES*tt (tt < MAXT).
 // The K loop (north-hemisphere latitudes) runs in chunks of KC and is software pipelined:

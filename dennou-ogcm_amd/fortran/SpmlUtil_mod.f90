@@ -1,0 +1,454 @@
+!-------------------------------------------------------------
+! SpmlUtil_mod -- drop-in replacement of the horizontal half of dogcm's SpmlUtil_mod
+! (model/dogcm/src/util/math/SpmlUtil_mod.f90:71-135) and of the SPMODEL wa_module names it
+! cascades, running on an MI355X through the C ABI of libdsht.so (include/dsht.h).
+!
+! Same procedure names, argument shapes and meanings as the reference:
+!   grid     xy(0:im-1, jm), xya(0:im-1, jm, :)     (longitude fastest, mu ascending)
+!   spectral w(lm), wa(lm, :)  with lm = (nm+1)**2, SPMODEL order l = n*(n+1)+m+1
+! Vertical (Chebyshev) operators of the reference module (xyz_DSig_xyz, IntSig_*, t_g,
+! g_t, ...) are not part of this module; they stay with the caller's vertical code.
+! Errors from the library stop the program with the library's message, as the
+! reference's MessageNotify('E', ...) does.  Calls are thread safe (OpenMP loops).
+!-------------------------------------------------------------
+module SpmlUtil_mod
+  use iso_c_binding
+  implicit none
+  private
+
+  integer, parameter :: DP = c_double
+
+  integer(c_int), parameter :: DSHT_VAL = 0, DSHT_DIVLAMBDA = 1, DSHT_DIVMU = 2
+  integer(c_int), parameter :: DSHT_GRADLAMBDA = 3, DSHT_GRADMU = 4, DSHT_GRADLON = 5, DSHT_GRADLAT = 6
+  integer(c_int), parameter :: DSHT_COEF_LAPLA = 0, DSHT_COEF_LAPLAINV = 1
+
+  interface
+    integer(c_int) function c_create(nmax, im, jm, kmax, radius, plan) bind(C, name="dsht_create")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: nmax, im, jm, kmax
+      real(c_double), value :: radius
+      type(c_ptr), intent(out) :: plan
+    end function
+    integer(c_int) function c_destroy(plan) bind(C, name="dsht_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: plan
+    end function
+    type(c_ptr) function c_last_error() bind(C, name="dsht_last_error")
+      import :: c_ptr
+    end function
+    integer(c_int) function c_grid_info(plan, x_lon, y_lat, x_w, y_w, xy_lon, xy_lat) bind(C, name="dsht_grid_info")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      real(c_double) :: x_lon(*), y_lat(*), x_w(*), y_w(*), xy_lon(*), xy_lat(*)
+    end function
+    integer(c_int) function c_coef(plan, which, params, np, out) bind(C, name="dsht_coef")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: which, np
+      real(c_double) :: params(*), out(*)
+    end function
+    integer(c_int) function c_w_xy(plan, kind, xy, w, ncols) bind(C, name="dsht_w_xy")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: kind, ncols
+      real(c_double), intent(in) :: xy(*)
+      real(c_double) :: w(*)
+    end function
+    integer(c_int) function c_xy_w(plan, kind, w, xy, ncols) bind(C, name="dsht_xy_w")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: kind, ncols
+      real(c_double), intent(in) :: w(*)
+      real(c_double) :: xy(*)
+    end function
+    integer(c_int) function c_w_alpha_xy(plan, a, b, w, ncols) bind(C, name="dsht_w_alpha_xy")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: ncols
+      real(c_double), intent(in) :: a(*), b(*)
+      real(c_double) :: w(*)
+    end function
+    integer(c_int) function c_xy_alpha_w(plan, wa, wb, xy, ncols) bind(C, name="dsht_xy_alpha_w")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: ncols
+      real(c_double), intent(in) :: wa(*), wb(*)
+      real(c_double) :: xy(*)
+    end function
+    integer(c_int) function c_uv2vordiv(plan, u, v, vor, div, ncols) bind(C, name="dsht_uvcoslat2vordiv")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: ncols
+      real(c_double), intent(in) :: u(*), v(*)
+      real(c_double) :: vor(*), div(*)
+    end function
+    integer(c_int) function c_vordiv2uv(plan, vor, div, u, v, ncols) bind(C, name="dsht_vordiv2uv")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: ncols
+      real(c_double), intent(in) :: vor(*), div(*)
+      real(c_double) :: u(*), v(*)
+    end function
+    integer(c_int) function c_w_diag_w(plan, d, w, out, ncols) bind(C, name="dsht_w_diag_w")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: ncols
+      real(c_double), intent(in) :: d(*), w(*)
+      real(c_double) :: out(*)
+    end function
+    integer(c_int) function c_intlonlat(plan, xy, out, ncols) bind(C, name="dsht_intlonlat")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: ncols
+      real(c_double), intent(in) :: xy(*)
+      real(c_double) :: out(*)
+    end function
+    integer(c_int) function c_avrlonlat(plan, xy, out, ncols) bind(C, name="dsht_avrlonlat")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: ncols
+      real(c_double), intent(in) :: xy(*)
+      real(c_double) :: out(*)
+    end function
+    integer(c_int) function c_ya_intlon(plan, xy, out, ncols) bind(C, name="dsht_ya_intlon")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: ncols
+      real(c_double), intent(in) :: xy(*)
+      real(c_double) :: out(*)
+    end function
+    integer(c_int) function c_ya_avrlon(plan, xy, out, ncols) bind(C, name="dsht_ya_avrlon")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: ncols
+      real(c_double), intent(in) :: xy(*)
+      real(c_double) :: out(*)
+    end function
+    integer(c_int) function c_interpolate(plan, w, lon, lat, out, npts) bind(C, name="dsht_interpolate")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: npts
+      real(c_double), intent(in) :: w(*), lon(*), lat(*)
+      real(c_double) :: out(*)
+    end function
+  end interface
+
+  ! public procedures (SpmlUtil_mod.f90:71-135, horizontal part)
+  public :: SpmlUtil_Init, SpmlUtil_Final, isInitialzed, get_SpmlGridInfo
+  public :: w_AlphaOptr_xy, xy_AlphaOptr_w, wa_AlphaOptr_xya, xya_AlphaOptr_wa
+  public :: calc_VorDiv2UV, calc_UVCosLat2VorDiv
+  public :: w_xy, xy_w, wa_xya, xya_wa
+  public :: xya_GradLon_wa, xya_GradLambda_wa, xya_GradLat_wa, xya_GradMu_wa
+  public :: xy_GradLon_w, xy_GradLambda_w, xy_GradLat_w, xy_GradMu_w
+  public :: w_DivLambda_xy, w_DivMu_xy, w_Lapla_w, w_LaplaInv_w
+  public :: l_nm, nm_l
+  public :: IntLonLat_xy, ya_IntLon_xya, AvrLonLat_xy, ya_AvrLon_xya
+  public :: a_Interpolate_wa, Interpolate_w
+
+  interface calc_VorDiv2UV
+    module procedure calc_VorDiv2UV_w_xy
+    module procedure calc_VorDiv2UV_wa_xya
+  end interface calc_VorDiv2UV
+  interface calc_UVCosLat2VorDiv
+    module procedure calc_UVCosLat2VorDiv_xy_w
+    module procedure calc_UVCosLat2VorDiv_xya_wa
+  end interface calc_UVCosLat2VorDiv
+
+  ! public variables (cascaded grid information and spectral arrays)
+  real(DP), public, save, allocatable :: xy_CosLat(:,:)
+  real(DP), public, save, allocatable :: x_Lon(:), y_Lat(:), x_Lon_Weight(:), y_Lat_Weight(:)
+  real(DP), public, save, allocatable :: xy_Lon(:,:), xy_Lat(:,:)
+  real(DP), public, save, allocatable :: rn(:,:)   ! rn(:,1) = -n(n+1); rn(:,2) = n
+
+  type(c_ptr), save :: plan = c_null_ptr
+  integer, save :: im, jm, km, nm, lm
+  real(DP), save :: Radius
+  logical, save :: initalizedFlag = .false.
+
+contains
+
+  subroutine chk(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    character(len=512) :: text
+    integer :: i
+    if (rc == 0) return
+    call c_f_pointer(c_last_error(), msg, [512])
+    text = ''
+    do i = 1, 512
+      if (msg(i) == c_null_char) exit
+      text(i:i) = msg(i)
+    end do
+    write(*, '(a)') 'SpmlUtil_mod: ' // where // ': ' // trim(text)
+    error stop 1
+  end subroutine chk
+
+  subroutine SpmlUtil_Init(iMax, jMax, kMax, nMax, tMax, RPlanet, np)
+    integer, intent(in) :: iMax, jMax, kMax, nMax, tMax
+    real(DP), intent(in) :: RPlanet
+    integer, intent(in), optional :: np
+    integer :: n, m, l
+    im = iMax; jm = jMax; km = kMax; nm = nMax
+    lm = (nm + 1)**2
+    Radius = RPlanet
+    call chk(c_create(int(nMax, c_int), int(iMax, c_int), int(jMax, c_int), int(kMax + 1, c_int), &
+         & real(RPlanet, c_double), plan), 'SpmlUtil_Init')
+    allocate(x_Lon(0:im-1), x_Lon_Weight(0:im-1), y_Lat(jm), y_Lat_Weight(jm))
+    allocate(xy_Lon(0:im-1, jm), xy_Lat(0:im-1, jm), xy_CosLat(0:im-1, jm), rn(lm, 2))
+    call chk(c_grid_info(plan, x_Lon, y_Lat, x_Lon_Weight, y_Lat_Weight, xy_Lon, xy_Lat), 'get grid')
+    xy_CosLat = cos(xy_Lat)
+    do n = 0, nm
+      do m = -n, n
+        l = n*(n + 1) + m + 1
+        rn(l, 1) = -dble(n*(n + 1))
+        rn(l, 2) = dble(n)
+      end do
+    end do
+    if (present(np)) continue
+    if (tMax < 0) continue
+    initalizedFlag = .true.
+  end subroutine SpmlUtil_Init
+
+  subroutine SpmlUtil_Final()
+    integer(c_int) :: rc
+    if (initalizedFlag) then
+      rc = c_destroy(plan)
+      plan = c_null_ptr
+      deallocate(x_Lon, x_Lon_Weight, y_Lat, y_Lat_Weight, xy_Lon, xy_Lat, xy_CosLat, rn)
+      initalizedFlag = .false.
+    end if
+  end subroutine SpmlUtil_Final
+
+  logical function isInitialzed()
+    isInitialzed = initalizedFlag
+  end function isInitialzed
+
+  subroutine get_SpmlGridInfo(x_Lon_, y_Lat_, z_Sig_, xy_Lon_, xy_Lat_, x_Lon_Weight_, y_Lat_Weight_, z_Sig_Weight_)
+    real(DP), dimension(0:im-1), intent(out), optional :: x_Lon_, x_Lon_Weight_
+    real(DP), dimension(jm), intent(out), optional :: y_Lat_, y_Lat_Weight_
+    real(DP), dimension(0:km), intent(out), optional :: z_Sig_, z_Sig_Weight_
+    real(DP), dimension(0:im-1, jm), intent(out), optional :: xy_Lon_, xy_Lat_
+    if (present(x_Lon_)) x_Lon_ = x_Lon
+    if (present(y_Lat_)) y_Lat_ = y_Lat
+    if (present(x_Lon_Weight_)) x_Lon_Weight_ = x_Lon_Weight
+    if (present(y_Lat_Weight_)) y_Lat_Weight_ = y_Lat_Weight
+    if (present(xy_Lon_)) xy_Lon_ = xy_Lon
+    if (present(xy_Lat_)) xy_Lat_ = xy_Lat
+    ! the vertical (sigma) grid belongs to the caller's vertical discretisation
+    if (present(z_Sig_)) z_Sig_ = 0d0
+    if (present(z_Sig_Weight_)) z_Sig_Weight_ = 0d0
+  end subroutine get_SpmlGridInfo
+
+  ! ---------------------------------------------------- index maps
+  integer function l_nm(n, m)
+    integer, intent(in) :: n, m
+    l_nm = n*(n + 1) + m + 1
+  end function l_nm
+
+  function nm_l(l)
+    integer, intent(in) :: l
+    integer :: nm_l(2)
+    nm_l(1) = int(sqrt(dble(l - 1)))
+    nm_l(2) = l - 1 - nm_l(1)*(nm_l(1) + 1)
+  end function nm_l
+
+  ! ---------------------------------------------------- transforms
+  function w_xy(xy)
+    real(DP), intent(in) :: xy(0:im-1, jm)
+    real(DP) :: w_xy(lm)
+    call chk(c_w_xy(plan, DSHT_VAL, xy, w_xy, 1_c_int), 'w_xy')
+  end function w_xy
+
+  function xy_w(w)
+    real(DP), intent(in) :: w(lm)
+    real(DP) :: xy_w(0:im-1, jm)
+    call chk(c_xy_w(plan, DSHT_VAL, w, xy_w, 1_c_int), 'xy_w')
+  end function xy_w
+
+  function wa_xya(xya)
+    real(DP), intent(in) :: xya(:,:,:)
+    real(DP) :: wa_xya(lm, size(xya, 3))
+    call chk(c_w_xy(plan, DSHT_VAL, xya, wa_xya, int(size(xya, 3), c_int)), 'wa_xya')
+  end function wa_xya
+
+  function xya_wa(wa)
+    real(DP), intent(in) :: wa(:,:)
+    real(DP) :: xya_wa(0:im-1, jm, size(wa, 2))
+    call chk(c_xy_w(plan, DSHT_VAL, wa, xya_wa, int(size(wa, 2), c_int)), 'xya_wa')
+  end function xya_wa
+
+  function w_DivLambda_xy(xy)
+    real(DP), intent(in) :: xy(0:im-1, jm)
+    real(DP) :: w_DivLambda_xy(lm)
+    call chk(c_w_xy(plan, DSHT_DIVLAMBDA, xy, w_DivLambda_xy, 1_c_int), 'w_DivLambda_xy')
+  end function w_DivLambda_xy
+
+  function w_DivMu_xy(xy)
+    real(DP), intent(in) :: xy(0:im-1, jm)
+    real(DP) :: w_DivMu_xy(lm)
+    call chk(c_w_xy(plan, DSHT_DIVMU, xy, w_DivMu_xy, 1_c_int), 'w_DivMu_xy')
+  end function w_DivMu_xy
+
+  function xy_GradLambda_w(w)
+    real(DP), intent(in) :: w(lm)
+    real(DP) :: xy_GradLambda_w(0:im-1, jm)
+    call chk(c_xy_w(plan, DSHT_GRADLAMBDA, w, xy_GradLambda_w, 1_c_int), 'xy_GradLambda_w')
+  end function xy_GradLambda_w
+
+  function xy_GradMu_w(w)
+    real(DP), intent(in) :: w(lm)
+    real(DP) :: xy_GradMu_w(0:im-1, jm)
+    call chk(c_xy_w(plan, DSHT_GRADMU, w, xy_GradMu_w, 1_c_int), 'xy_GradMu_w')
+  end function xy_GradMu_w
+
+  function xy_GradLon_w(w)
+    real(DP), intent(in) :: w(lm)
+    real(DP) :: xy_GradLon_w(0:im-1, jm)
+    call chk(c_xy_w(plan, DSHT_GRADLON, w, xy_GradLon_w, 1_c_int), 'xy_GradLon_w')
+  end function xy_GradLon_w
+
+  function xy_GradLat_w(w)
+    real(DP), intent(in) :: w(lm)
+    real(DP) :: xy_GradLat_w(0:im-1, jm)
+    call chk(c_xy_w(plan, DSHT_GRADLAT, w, xy_GradLat_w, 1_c_int), 'xy_GradLat_w')
+  end function xy_GradLat_w
+
+  function xya_GradLambda_wa(wa)
+    real(DP), intent(in) :: wa(:,:)
+    real(DP) :: xya_GradLambda_wa(0:im-1, jm, size(wa, 2))
+    call chk(c_xy_w(plan, DSHT_GRADLAMBDA, wa, xya_GradLambda_wa, int(size(wa, 2), c_int)), 'xya_GradLambda_wa')
+  end function xya_GradLambda_wa
+
+  function xya_GradMu_wa(wa)
+    real(DP), intent(in) :: wa(:,:)
+    real(DP) :: xya_GradMu_wa(0:im-1, jm, size(wa, 2))
+    call chk(c_xy_w(plan, DSHT_GRADMU, wa, xya_GradMu_wa, int(size(wa, 2), c_int)), 'xya_GradMu_wa')
+  end function xya_GradMu_wa
+
+  function xya_GradLon_wa(wa)
+    real(DP), intent(in) :: wa(:,:)
+    real(DP) :: xya_GradLon_wa(0:im-1, jm, size(wa, 2))
+    call chk(c_xy_w(plan, DSHT_GRADLON, wa, xya_GradLon_wa, int(size(wa, 2), c_int)), 'xya_GradLon_wa')
+  end function xya_GradLon_wa
+
+  function xya_GradLat_wa(wa)
+    real(DP), intent(in) :: wa(:,:)
+    real(DP) :: xya_GradLat_wa(0:im-1, jm, size(wa, 2))
+    call chk(c_xy_w(plan, DSHT_GRADLAT, wa, xya_GradLat_wa, int(size(wa, 2), c_int)), 'xya_GradLat_wa')
+  end function xya_GradLat_wa
+
+  function w_Lapla_w(w)
+    real(DP), intent(in) :: w(lm)
+    real(DP) :: w_Lapla_w(lm)
+    real(DP) :: d(lm), p(1)
+    p = 0d0
+    call chk(c_coef(plan, DSHT_COEF_LAPLA, p, 0_c_int, d), 'w_Lapla_w')
+    call chk(c_w_diag_w(plan, d, w, w_Lapla_w, 1_c_int), 'w_Lapla_w')
+  end function w_Lapla_w
+
+  function w_LaplaInv_w(w)
+    real(DP), intent(in) :: w(lm)
+    real(DP) :: w_LaplaInv_w(lm)
+    real(DP) :: d(lm), p(1)
+    p = 0d0
+    call chk(c_coef(plan, DSHT_COEF_LAPLAINV, p, 0_c_int, d), 'w_LaplaInv_w')
+    call chk(c_w_diag_w(plan, d, w, w_LaplaInv_w, 1_c_int), 'w_LaplaInv_w')
+  end function w_LaplaInv_w
+
+  ! ---------------------------------------------------- vector operators (SpmlUtil_mod.f90:297-473)
+  function w_AlphaOptr_xy(xy_A, xy_B)
+    real(DP), intent(in) :: xy_A(0:im-1, jm), xy_B(0:im-1, jm)
+    real(DP) :: w_AlphaOptr_xy(lm)
+    call chk(c_w_alpha_xy(plan, xy_A, xy_B, w_AlphaOptr_xy, 1_c_int), 'w_AlphaOptr_xy')
+  end function w_AlphaOptr_xy
+
+  function xy_AlphaOptr_w(w_A, w_B)
+    real(DP), intent(in) :: w_A(lm), w_B(lm)
+    real(DP) :: xy_AlphaOptr_w(0:im-1, jm)
+    call chk(c_xy_alpha_w(plan, w_A, w_B, xy_AlphaOptr_w, 1_c_int), 'xy_AlphaOptr_w')
+  end function xy_AlphaOptr_w
+
+  function wa_AlphaOptr_xya(xya_A, xya_B)
+    real(DP), intent(in) :: xya_A(:,:,:), xya_B(:,:,:)
+    real(DP) :: wa_AlphaOptr_xya(lm, size(xya_A, 3))
+    call chk(c_w_alpha_xy(plan, xya_A, xya_B, wa_AlphaOptr_xya, int(size(xya_A, 3), c_int)), 'wa_AlphaOptr_xya')
+  end function wa_AlphaOptr_xya
+
+  function xya_AlphaOptr_wa(wa_A, wa_B)
+    real(DP), intent(in) :: wa_A(:,:), wa_B(:,:)
+    real(DP) :: xya_AlphaOptr_wa(0:im-1, jm, size(wa_A, 2))
+    call chk(c_xy_alpha_w(plan, wa_A, wa_B, xya_AlphaOptr_wa, int(size(wa_A, 2), c_int)), 'xya_AlphaOptr_wa')
+  end function xya_AlphaOptr_wa
+
+  subroutine calc_UVCosLat2VorDiv_xy_w(xy_UCosLat, xy_VCosLat, w_Vor, w_Div)
+    real(DP), intent(in) :: xy_UCosLat(0:im-1, jm), xy_VCosLat(0:im-1, jm)
+    real(DP), intent(out) :: w_Vor(lm), w_Div(lm)
+    call chk(c_uv2vordiv(plan, xy_UCosLat, xy_VCosLat, w_Vor, w_Div, 1_c_int), 'calc_UVCosLat2VorDiv')
+  end subroutine calc_UVCosLat2VorDiv_xy_w
+
+  subroutine calc_UVCosLat2VorDiv_xya_wa(xya_UCosLat, xya_VCosLat, wa_Vor, wa_Div)
+    real(DP), intent(in) :: xya_UCosLat(:,:,:), xya_VCosLat(:,:,:)
+    real(DP), intent(out) :: wa_Vor(lm, size(xya_UCosLat, 3)), wa_Div(lm, size(xya_UCosLat, 3))
+    call chk(c_uv2vordiv(plan, xya_UCosLat, xya_VCosLat, wa_Vor, wa_Div, int(size(xya_UCosLat, 3), c_int)), &
+         & 'calc_UVCosLat2VorDiv')
+  end subroutine calc_UVCosLat2VorDiv_xya_wa
+
+  subroutine calc_VorDiv2UV_w_xy(w_Vor, w_Div, xy_U, xy_V)
+    real(DP), intent(in) :: w_Vor(lm), w_Div(lm)
+    real(DP), intent(out) :: xy_U(0:im-1, jm), xy_V(0:im-1, jm)
+    call chk(c_vordiv2uv(plan, w_Vor, w_Div, xy_U, xy_V, 1_c_int), 'calc_VorDiv2UV')
+  end subroutine calc_VorDiv2UV_w_xy
+
+  subroutine calc_VorDiv2UV_wa_xya(wa_Vor, wa_Div, xya_U, xya_V)
+    real(DP), intent(in) :: wa_Vor(:,:), wa_Div(:,:)
+    real(DP), intent(out) :: xya_U(0:im-1, jm, size(wa_Vor, 2)), xya_V(0:im-1, jm, size(wa_Vor, 2))
+    call chk(c_vordiv2uv(plan, wa_Vor, wa_Div, xya_U, xya_V, int(size(wa_Vor, 2), c_int)), 'calc_VorDiv2UV')
+  end subroutine calc_VorDiv2UV_wa_xya
+
+  ! ---------------------------------------------------- integrals and means (FP64 sums)
+  real(DP) function IntLonLat_xy(xy)
+    real(DP), intent(in) :: xy(0:im-1, jm)
+    real(DP) :: out(1)
+    call chk(c_intlonlat(plan, xy, out, 1_c_int), 'IntLonLat_xy')
+    IntLonLat_xy = out(1)
+  end function IntLonLat_xy
+
+  real(DP) function AvrLonLat_xy(xy)
+    real(DP), intent(in) :: xy(0:im-1, jm)
+    real(DP) :: out(1)
+    call chk(c_avrlonlat(plan, xy, out, 1_c_int), 'AvrLonLat_xy')
+    AvrLonLat_xy = out(1)
+  end function AvrLonLat_xy
+
+  function ya_IntLon_xya(xya)
+    real(DP), intent(in) :: xya(:,:,:)
+    real(DP) :: ya_IntLon_xya(jm, size(xya, 3))
+    call chk(c_ya_intlon(plan, xya, ya_IntLon_xya, int(size(xya, 3), c_int)), 'ya_IntLon_xya')
+  end function ya_IntLon_xya
+
+  function ya_AvrLon_xya(xya)
+    real(DP), intent(in) :: xya(:,:,:)
+    real(DP) :: ya_AvrLon_xya(jm, size(xya, 3))
+    call chk(c_ya_avrlon(plan, xya, ya_AvrLon_xya, int(size(xya, 3), c_int)), 'ya_AvrLon_xya')
+  end function ya_AvrLon_xya
+
+  ! ---------------------------------------------------- point evaluation
+  real(DP) function Interpolate_w(w, alon, alat)
+    real(DP), intent(in) :: w(lm), alon, alat
+    real(DP) :: lon(1), lat(1), out(1)
+    lon(1) = alon; lat(1) = alat
+    call chk(c_interpolate(plan, w, lon, lat, out, 1_c_int), 'Interpolate_w')
+    Interpolate_w = out(1)
+  end function Interpolate_w
+
+  function a_Interpolate_wa(wa, alon, alat)
+    real(DP), intent(in) :: wa(:,:), alon, alat
+    real(DP) :: a_Interpolate_wa(size(wa, 2))
+    integer :: k
+    do k = 1, size(wa, 2)
+      a_Interpolate_wa(k) = Interpolate_w(wa(:, k), alon, alat)
+    end do
+  end function a_Interpolate_wa
+
+end module SpmlUtil_mod

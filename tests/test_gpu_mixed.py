@@ -1,0 +1,72 @@
+"""Split-precision Legendre forms against the FP64 oracle.
+
+Error bounds (relative L2 over the whole output), from the split width of each form:
+  bf16x6, fp16x3   <= 1e-6   (the bar the T170 metric is quoted at)
+  bf16x3           <= 3e-5   (2 x 8-bit terms: ~2^-16 per product)
+  fp16             <= 2e-3   (11-bit single term)
+Every form must also report its error at T170 in bench.py.
+"""
+import numpy as np
+import pytest
+
+from conftest import red_spectrum, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+BOUND = {"bf16x6": 1e-6, "fp16x3": 1e-6, "bf16x3": 3e-5, "fp16": 2e-3}
+
+
+def _prec(name):
+    import dsht
+    return {v: k for k, v in dsht.PREC_NAMES.items()}[name]
+
+
+@pytest.mark.parametrize("form", list(BOUND))
+@pytest.mark.parametrize("N,I,J", [(21, 64, 32), (42, 128, 64), (31, 96, 48)])
+def test_mixed_transforms(gpu, form, N, I, J):
+    import dsht
+    import oracle
+    R = 6.371e6
+    p = dsht.Plan(N, I, J, radius=R)
+    o = oracle.Oracle(N, I, J, R)
+    p.set_precision(_prec(form))
+    tol = BOUND[form]
+    w = red_spectrum(N, 6, 3) * 1e-5  # magnitudes of a vorticity field
+    for kind in ("VAL", "GRADLAMBDA", "GRADMU", "GRADLAT"):
+        k = getattr(dsht, kind)
+        assert rel_l2(p._inv(w, k), o.inv(w, k)) < tol, (form, kind)
+    rng = np.random.default_rng(4)
+    xy = (rng.standard_normal((5, J, I)) + 20.0) * o.coslat[None, :, None]
+    for kind in ("VAL", "DIVLAMBDA", "DIVMU"):
+        k = getattr(dsht, kind)
+        assert rel_l2(p._fwd(xy, k), o.fwd(xy, k)) < tol, (form, kind)
+    A, B = xy[:3], xy[2:] * 1e-3
+    assert rel_l2(p.w_AlphaOptr_xy(A, B), o.alpha_fwd(A, B)) < tol
+    vor, div = p.calc_UVCosLat2VorDiv(A, B)
+    rv, rd = o.uv2vordiv(A, B)
+    assert rel_l2(vor, rv) < tol and rel_l2(div, rd) < tol
+    wa, wb = red_spectrum(N, 3, 7), red_spectrum(N, 3, 8) * 1e-2
+    assert rel_l2(p.xy_AlphaOptr_w(wa, wb), o.alpha_inv(wa, wb)) < tol
+    u, v = p.calc_VorDiv2UV(wa * 1e-5, wb * 1e-6)
+    ru, rv2 = o.vordiv2uv(wa * 1e-5, wb * 1e-6)
+    assert rel_l2(u, ru) < tol and rel_l2(v, rv2) < tol
+
+
+@pytest.mark.parametrize("form", ["bf16x6", "fp16x3"])
+def test_mixed_t170_roundtrip(gpu, form):
+    """T170 round trip and inverse accuracy at the metric's tolerance (1e-6)."""
+    import torch
+    import dsht
+    N, I, J, nc = 170, 512, 256, 40
+    p = dsht.Plan(N, I, J, radius=6.371e6)
+    w = torch.tensor(red_spectrum(N, nc, 21).T.copy(), device=gpu)
+    ref = torch.empty((nc, J, I), dtype=torch.float64, device=gpu)
+    p.inv_d(dsht.VAL, w, ref)
+    p.set_precision(_prec(form))
+    got = torch.empty_like(ref)
+    p.inv_d(dsht.VAL, w, got)
+    w2 = torch.zeros_like(w)
+    p.fwd_d(dsht.VAL, ref, w2)
+    torch.cuda.synchronize()
+    assert rel_l2(got.cpu().numpy(), ref.cpu().numpy()) < 1e-6
+    assert rel_l2(w2.cpu().numpy(), w.cpu().numpy()) < 1e-6

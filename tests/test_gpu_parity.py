@@ -154,3 +154,16 @@ def test_errors(gpu):
     p = dsht.Plan(5, 16, 8)
     with pytest.raises(dsht.DshtError):
         p._inv(np.zeros(36), 99)
+
+
+def test_fortran_dropin(gpu):
+    """The Fortran SpmlUtil_mod shim (amdflang, ISO_C_BINDING) run end to end on the GPU."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(__file__)), "dennou-ogcm_amd", "spml_dropin_test")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran test program not built (amdflang missing at build time)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "PASS" in r.stdout
