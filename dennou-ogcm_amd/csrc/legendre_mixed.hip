@@ -49,6 +49,15 @@ __device__ __forceinline__ float4_t mfma16(f16x8_t a, f16x8_t b, float4_t c) {
 // c + 16*(k/8), element k%8; result element i of lane l holds D[4*(l/16) + i][l%16].
 __device__ __forceinline__ int acc_row16(int lane, int i) { return 4 * (lane >> 4) + i; }
 
+// FP16 split terms are stored pre-scaled by 2^(11 t) so the low parts of small values stay
+// in FP16's normal range (the MFMA flushes FP16 subnormals); products of total order o are
+// weighted by 2^(-11 o).  BF16 has FP32's exponent range and needs no term scaling.
+template <int FORM>
+struct TermScale {
+  static constexpr float UP = MxForm<FORM>::BF ? 1.0f : 2048.0f;
+  static constexpr float DOWN = MxForm<FORM>::BF ? 1.0f : (1.0f / 2048.0f);
+};
+
 template <int FORM>
 __device__ __forceinline__ void split_to(float x, typename MxForm<FORM>::T* out) {
   using T = typename MxForm<FORM>::T;
@@ -56,7 +65,7 @@ __device__ __forceinline__ void split_to(float x, typename MxForm<FORM>::T* out)
   for (int t = 0; t < MxForm<FORM>::NT; t++) {
     T h = (T)x;
     out[t] = h;
-    x -= (float)h;
+    x = (x - (float)h) * TermScale<FORM>::UP;
   }
 }
 
@@ -130,7 +139,7 @@ __global__ void k_mx_tables(int N, int J2, int J2pad, int J2p32, int nterm, cons
                             const long* offI64, const long* offF, const long* offI, const double* FP,
                             const double* FH, const double* IP, const double* IH, const double* minv,
                             T* fP0, T* fP1, T* fP2, T* fH0, T* fH1, T* fH2, T* iP0, T* iP1, T* iP2, T* iH0,
-                            T* iH1, T* iH2, int forward) {
+                            T* iH1, T* iH2, int forward, double tscale) {
   const int m = blockIdx.y;
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int nE = n_par(N, m, 0), nO = n_par(N, m, 1);
@@ -190,8 +199,8 @@ __global__ void k_mx_tables(int N, int J2, int J2pad, int J2p32, int nterm, cons
     T hp = (T)vP, hh = (T)vH;
     oP[t][dst] = hp;
     oH[t][dst] = hh;
-    vP -= (double)hp;
-    vH -= (double)hh;
+    vP = (vP - (double)hp) * tscale;
+    vH = (vH - (double)hh) * tscale;
   }
 }
 
@@ -254,13 +263,13 @@ static int build_mixed(dsht_plan* p, int family) {
       hipLaunchKernelGGL((k_mx_tables<T>), grid, dim3(256), 0, 0, N, p->J2, p->J2pad, t.J2p32, 3, p->d_offF,
                          p->d_offI, t.d_offF, t.d_offI, p->d_tabFP, p->d_tabFH, p->d_tabIP, p->d_tabIH, t.d_minv,
                          (T*)t.fP[0], (T*)t.fP[1], (T*)t.fP[2], (T*)t.fH[0], (T*)t.fH[1], (T*)t.fH[2],
-                         (T*)t.iP[0], (T*)t.iP[1], (T*)t.iP[2], (T*)t.iH[0], (T*)t.iH[1], (T*)t.iH[2], fwd);
+                         (T*)t.iP[0], (T*)t.iP[1], (T*)t.iP[2], (T*)t.iH[0], (T*)t.iH[1], (T*)t.iH[2], fwd, 1.0);
     } else {
       typedef _Float16 T;
       hipLaunchKernelGGL((k_mx_tables<T>), grid, dim3(256), 0, 0, N, p->J2, p->J2pad, t.J2p32, 3, p->d_offF,
                          p->d_offI, t.d_offF, t.d_offI, p->d_tabFP, p->d_tabFH, p->d_tabIP, p->d_tabIH, t.d_minv,
                          (T*)t.fP[0], (T*)t.fP[1], (T*)t.fP[2], (T*)t.fH[0], (T*)t.fH[1], (T*)t.fH[2],
-                         (T*)t.iP[0], (T*)t.iP[1], (T*)t.iP[2], (T*)t.iH[0], (T*)t.iH[1], (T*)t.iH[2], fwd);
+                         (T*)t.iP[0], (T*)t.iP[1], (T*)t.iP[2], (T*)t.iH[0], (T*)t.iH[1], (T*)t.iH[2], fwd, 2048.0);
     }
     DSHT_HIP(hipGetLastError());
   }
@@ -356,10 +365,19 @@ __device__ __forceinline__ float4_t mx_prod(const typename MxForm<FORM>::V8 (&A)
   V8 B[F::NT];
 #pragma unroll
   for (int t = 0; t < F::NT; t++) B[t] = *reinterpret_cast<const V8*>(b + t * tstride);
-  float4_t acc = float4_t{0, 0, 0, 0};
+  if constexpr (F::BF) {
+    float4_t acc = float4_t{0, 0, 0, 0};
 #pragma unroll
-  for (int pr = 0; pr < F::NP; pr++) acc = mfma16(A[PA[pr]], B[PB[pr]], acc);
-  return acc;
+    for (int pr = 0; pr < F::NP; pr++) acc = mfma16(A[PA[pr]], B[PB[pr]], acc);
+    return acc;
+  } else {
+    // accumulate each product order separately, then weight by 2^(-11 order)
+    float4_t acc[3] = {float4_t{0, 0, 0, 0}, float4_t{0, 0, 0, 0}, float4_t{0, 0, 0, 0}};
+#pragma unroll
+    for (int pr = 0; pr < F::NP; pr++) acc[PA[pr] + PB[pr]] = mfma16(A[PA[pr]], B[PB[pr]], acc[PA[pr] + PB[pr]]);
+    constexpr float D = TermScale<FORM>::DOWN;
+    return acc[0] + D * (acc[1] + D * acc[2]);
+  }
 }
 
 // ---------------------------------------------------------------- forward
