@@ -165,6 +165,7 @@ struct dsht_plan {
   // mixed-precision tables (built lazily)
   void* mixed = nullptr;
   int chunk_cols = 0;  // column chunk of the device path (0 = auto)
+  int leg_cb = 0;      // column block of the contractions (0 = auto, 16, 32)
 };
 
 namespace dsht {

@@ -528,7 +528,7 @@ static void leg_work(const dsht_plan* p, int nin, int nout, int nterm, int ntab,
 }
 
 int launch_leg_fwd(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t s) {
-  const bool wide = a.ncols > 16;
+  const bool wide = p->leg_cb ? p->leg_cb == 32 : a.ncols > 16;
   double by, fl;
   static const int nin[] = {1, 1, 1, 2, 2}, nout[] = {1, 1, 1, 1, 2}, nt[] = {1, 1, 1, 2, 4}, ntab[] = {1, 1, 1, 2, 2};
   leg_work(p, nin[op], nout[op], nt[op], ntab[op], true, a.ncols, &by, &fl);
@@ -544,7 +544,7 @@ int launch_leg_fwd(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t 
 }
 
 int launch_leg_inv(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t s) {
-  const bool wide = a.ncols > 16;
+  const bool wide = p->leg_cb ? p->leg_cb == 32 : a.ncols > 16;
   double by, fl;
   static const int nin[] = {1, 1, 1, 2, 2}, nout[] = {1, 1, 1, 1, 2}, nt[] = {1, 1, 1, 2, 4}, ntab[] = {1, 1, 1, 2, 2};
   leg_work(p, nin[op], nout[op], nt[op], ntab[op], false, a.ncols, &by, &fl);

@@ -40,7 +40,11 @@ EXPORTS = [
     "dsht_alpha_fwd_d", "dsht_alpha_inv_d", "dsht_uv2vordiv_d", "dsht_vordiv2uv_d",
     "dsht_diag_d", "dsht_intlonlat_d", "dsht_avrlonlat_d", "dsht_ya_intlon_d",
     "dsht_interpolate_d", "dsht_coef_d", "dsht_prof_enable", "dsht_prof_reset", "dsht_prof_query",
+    "dsht_set_tuning", "dsht_get_tuning", "dsht_batch_create", "dsht_batch_add", "dsht_batch_finalize",
+    "dsht_batch_info", "dsht_batch_ngroups", "dsht_batch_buffers", "dsht_batch_step",
+    "dsht_batch_step_sequential", "dsht_batch_capture", "dsht_batch_launch", "dsht_batch_destroy",
 ]
+TUNE_CHUNK_COLS, TUNE_LEG_CB = 0, 1
 
 
 class DshtError(RuntimeError):
@@ -103,6 +107,19 @@ def load():
         "dsht_prof_enable": ([i], i),
         "dsht_prof_reset": ([], i),
         "dsht_prof_query": ([ctypes.c_char_p, vp, vp, vp, vp], i),
+        "dsht_set_tuning": ([vp, i, i], i),
+        "dsht_get_tuning": ([vp, i], i),
+        "dsht_batch_create": ([pp], i),
+        "dsht_batch_add": ([vp, vp, i, ip], i),
+        "dsht_batch_finalize": ([vp], i),
+        "dsht_batch_info": ([vp, i, ip, ip, ip, ctypes.POINTER(ctypes.c_long)], i),
+        "dsht_batch_ngroups": ([vp], i),
+        "dsht_batch_buffers": ([vp, i, pp, pp, pp, pp], i),
+        "dsht_batch_step": ([vp, vp], i),
+        "dsht_batch_step_sequential": ([vp, vp], i),
+        "dsht_batch_capture": ([vp, vp], i),
+        "dsht_batch_launch": ([vp, vp], i),
+        "dsht_batch_destroy": ([vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -424,8 +441,95 @@ class Plan:
                                        _stream(stream)))
         return out
 
+    def set_tuning(self, key, value):
+        _check(load().dsht_set_tuning(self.h, int(key), int(value)))
+
+    def get_tuning(self, key):
+        return load().dsht_get_tuning(self.h, int(key))
+
     def coef_d(self, which, params, out, stream=None):
         p = np.asarray(params, dtype=np.float64).reshape(-1)
         pp = _f64(p) if p.size else np.zeros(1)
         _check(load().dsht_coef_d(self.h, int(which), _ptr(pp), int(p.size), _tptr(out), _stream(stream)))
         return out
+
+
+class Batch:
+    """Independent instances of uneven truncation/layer count advanced together
+    (dsht_batch_*).  ``add(plan, km)`` returns an instance id; after ``finalize()`` each
+    truncation group exposes torch views of its device buffers via ``group_views``."""
+
+    NPROG, NTEND, NINV = 4, 6, 3
+
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+        _check(load().dsht_batch_create(ctypes.byref(self.h)))
+        self.plans = []
+
+    def add(self, plan, km):
+        i = ctypes.c_int()
+        _check(load().dsht_batch_add(self.h, plan.h, int(km), ctypes.byref(i)))
+        self.plans.append((plan, km))
+        return i.value
+
+    def finalize(self):
+        _check(load().dsht_batch_finalize(self.h))
+
+    def info(self, inst):
+        g, c0, gc, by = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
+        _check(load().dsht_batch_info(self.h, int(inst), ctypes.byref(g), ctypes.byref(c0), ctypes.byref(gc),
+                                      ctypes.byref(by)))
+        return g.value, c0.value, gc.value, by.value
+
+    def group_views(self, group, plan, cols):
+        """torch tensors over the group's device buffers (no copies)."""
+        import torch
+        ptrs = [ctypes.c_void_p() for _ in range(4)]
+        _check(load().dsht_batch_buffers(self.h, int(group), *[ctypes.byref(p) for p in ptrs]))
+        L, J, I = plan.lm, plan.jm, plan.im
+        shapes = [(L, self.NPROG * cols), (self.NINV, self.NPROG * cols, J, I), (self.NTEND * cols, J, I),
+                  (L, self.NTEND * cols)]
+        return [_tensor_from_ptr(p.value, s) for p, s in zip(ptrs, shapes)]
+
+    def ngroups(self):
+        return load().dsht_batch_ngroups(self.h)
+
+    def step(self, stream=None):
+        _check(load().dsht_batch_step(self.h, _stream(stream)))
+
+    def step_sequential(self, stream=None):
+        _check(load().dsht_batch_step_sequential(self.h, _stream(stream)))
+
+    def capture(self, stream):
+        _check(load().dsht_batch_capture(self.h, ctypes.c_void_p(stream)))
+
+    def launch(self, stream=None):
+        _check(load().dsht_batch_launch(self.h, _stream(stream)))
+
+    def close(self):
+        if self.h and self.h.value:
+            load().dsht_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _tensor_from_ptr(ptr, shape):
+    """A float64 CUDA tensor viewing device memory owned by the library."""
+    import torch
+
+    n = 1
+    for s in shape:
+        n *= s
+
+    class _Holder:
+        pass
+
+    h = _Holder()
+    h.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": "<f8", "data": (ptr, False), "version": 2,
+                                  "strides": None}
+    return torch.as_tensor(h, device="cuda")

@@ -156,6 +156,38 @@ int dsht_interpolate_d(dsht_plan* plan, const double* w, long sl, const double* 
 int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, double* out,
                 void* stream);
 
+/* ---------------------------------------------------------------- tuning
+ * Knobs chosen per (truncation, column count) by the autotuner (dsht/autotune.py):
+ * the column chunk of the device path (Fourier scratch per chunk) and the column block
+ * of the Legendre contractions.  No counterpart in the reference.                   */
+#define DSHT_TUNE_CHUNK_COLS 0 /* 0 = automatic */
+#define DSHT_TUNE_LEG_CB 1     /* 0 = automatic, 16 or 32 */
+int dsht_set_tuning(dsht_plan* plan, int key, int value);
+int dsht_get_tuning(const dsht_plan* plan, int key);
+
+/* ---------------------------------------------------------------- batches
+ * Many independent instances (planets) of uneven truncation and layer count advanced
+ * together.  Instances sharing a plan are packed as columns of one set of device buffers
+ * (field f, layer column c at f*cols + c; spectra in the LC layout) so each transform
+ * launch covers all of them; one step of every group is captured into a hipGraph.  The
+ * reference runs one model per process (OpenMP over layers, SpmlUtil_mod.f90:360-366).
+ * Step per instance: 4 fields spectral -> grid with lon/lat derivatives, 6 tendency
+ * fields grid -> spectral.                                                            */
+typedef struct dsht_batch dsht_batch;
+int dsht_batch_create(dsht_batch** batch);
+int dsht_batch_add(dsht_batch* batch, dsht_plan* plan, int km, int* instance_id);
+int dsht_batch_finalize(dsht_batch* batch);
+int dsht_batch_info(const dsht_batch* batch, int instance_id, int* group, int* col0, int* group_cols,
+                    long* bytes);
+int dsht_batch_ngroups(const dsht_batch* batch);
+int dsht_batch_buffers(const dsht_batch* batch, int group, double** spec, double** grid, double** tend,
+                       double** spect);
+int dsht_batch_step(dsht_batch* batch, void* stream);
+int dsht_batch_step_sequential(dsht_batch* batch, void* stream);
+int dsht_batch_capture(dsht_batch* batch, void* stream);
+int dsht_batch_launch(dsht_batch* batch, void* stream);
+int dsht_batch_destroy(dsht_batch* batch);
+
 /* ---------------------------------------------------------------- kernel timers
  * HIP-event timers around every kernel launch of the library, keyed by kernel class
  * ("fft_fwd", "fft_inv", "leg_fwd", "leg_inv", ...), with the algorithmic bytes and
