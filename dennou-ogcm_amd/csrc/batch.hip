@@ -225,8 +225,8 @@ int dsht_batch_destroy(dsht_batch* b) {
   return 0;
 }
 
-/* tuning knobs of a plan (set by the autotuner): column chunk of the device path and the
-   column block of the Legendre contractions */
+/* tuning knobs of a plan (set by the autotuner): column chunk of the device path, the
+   column block of the Legendre contractions and the FFT kernel family */
 int dsht_set_tuning(dsht_plan* p, int key, int value) {
   if (!p) return fail(DSHT_ESTATE, "NULL plan");
   switch (key) {
@@ -238,6 +238,10 @@ int dsht_set_tuning(dsht_plan* p, int key, int value) {
       if (value != 0 && value != 16 && value != 32) return fail(DSHT_EARG, "column block must be 0, 16 or 32");
       p->leg_cb = value;
       return 0;
+    case DSHT_TUNE_FFT:
+      if (value != 0 && value != 1) return fail(DSHT_EARG, "fft kernel must be 0 or 1");
+      p->fft_kind = value;
+      return 0;
   }
   return fail(DSHT_EARG, "unknown tuning key %d", key);
 }
@@ -247,6 +251,7 @@ int dsht_get_tuning(const dsht_plan* p, int key) {
   switch (key) {
     case DSHT_TUNE_CHUNK_COLS: return p->chunk_cols;
     case DSHT_TUNE_LEG_CB: return p->leg_cb;
+    case DSHT_TUNE_FFT: return p->fft_kind;
   }
   return -1;
 }

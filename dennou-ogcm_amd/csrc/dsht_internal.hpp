@@ -166,6 +166,7 @@ struct dsht_plan {
   void* mixed = nullptr;
   int chunk_cols = 0;  // column chunk of the device path (0 = auto)
   int leg_cb = 0;      // column block of the contractions (0 = auto, 16, 32)
+  int fft_kind = 0;    // 0 = register two-pass FFT where available, 1 = LDS Stockham
 };
 
 namespace dsht {
@@ -193,6 +194,11 @@ int launch_fft_inv(const dsht_plan* p, const double2* G, long gld, int ncols, do
                    hipStream_t s);
 // FP32 Fourier intermediates of the split-precision forms; colmax (nullable) receives the
 // per-column max |G| as float bits
+bool reg_fft_supported(int M);
+template <typename GT>
+int reg_fft_fwd(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s);
+template <typename GT>
+int reg_fft_inv(const GT* G, const FftArgs& a, double* xy, hipStream_t s);
 int launch_fft_fwd32(const dsht_plan* p, const double* xy, long ldg, int ncols, float2* G, long gld,
                      unsigned* colmax, hipStream_t s);
 int launch_fft_inv32(const dsht_plan* p, const float2* G, long gld, int ncols, double* xy, long ldg,

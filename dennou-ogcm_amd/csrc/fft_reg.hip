@@ -1,0 +1,444 @@
+// fft_reg.hip -- register-resident longitude FFTs for the grid lengths in use
+// (I = 2M with M = R1 * R2, both factors small).  Same transform as fft.hip:
+//
+// Forward:  G^m(mu_j) = (1/I) sum_i q_ij e^{-i m lambda_i},  m = 0..N
+// Inverse:  q_ij = G^0 + 2 Re sum_{m=1}^{N} G^m e^{i m lambda_i}
+//
+// The complex length-M FFT of a packed row z_k = x_{2k} + i x_{2k+1} runs as two passes
+// of in-register DFTs (index n = n1 + R1 n2, k = k2 + R2 k1):
+//   pass 1, thread n1 of the row:  Y[n1][k2] = W_M^{n1 k2} sum_{n2} z[n1 + R1 n2] W_R2^{n2 k2}
+//   pass 2, thread k2 of the row:  Z[k2 + R2 k1] = sum_{n1} Y[n1][k2] W_R1^{n1 k1}
+// so a row crosses LDS once between the passes (plus once more for the real/half-complex
+// split), against 2 log4 M times for a radix-4 Stockham chain.  Small DFTs are fully
+// unrolled with compile-time twiddles.  The grid side is read or written straight from
+// registers in 256-byte runs; the Fourier side goes through LDS so that consecutive
+// threads take consecutive rows (r = j*ncols + c) and each m is stored as one run.
+// LDS rows are padded to an odd number of complex values so that both the per-row and
+// the across-row access patterns are free of bank conflicts.
+#include <cmath>
+#include <type_traits>
+
+#include "dsht_internal.hpp"
+
+namespace dsht {
+
+namespace {
+
+__device__ __forceinline__ double2 rcmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 rcmulc(double2 a, double2 b) {  // a * conj(b)
+  return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ double2 radd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 rsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+template <int SIGN>  // multiply by SIGN * i
+__device__ __forceinline__ double2 rmul_i(double2 a) {
+  return SIGN < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+}
+
+// ---- compile-time twiddles: cos and sin of 2 pi e / R by reduction to [0, pi/4] and
+// Taylor series (terms to x^27, exact to rounding for |x| <= pi/4)
+constexpr double ct_sinp(double x) {
+  double x2 = x * x, t = x, s = x;
+  for (int k = 1; k < 14; k++) {
+    t *= -x2 / ((2.0 * k) * (2.0 * k + 1.0));
+    s += t;
+  }
+  return s;
+}
+constexpr double ct_cosp(double x) {
+  double x2 = x * x, t = 1.0, s = 1.0;
+  for (int k = 1; k < 14; k++) {
+    t *= -x2 / ((2.0 * k - 1.0) * (2.0 * k));
+    s += t;
+  }
+  return s;
+}
+constexpr double ct_cs(int e, int R, bool want_sin) {
+  e = ((e % R) + R) % R;
+  const int q = (4 * e) / R, r = 4 * e - q * R;  // angle = (pi/2)(q + r/R), 0 <= r < R
+  const double h = 1.57079632679489661923;
+  double c = 1.0, s = 0.0;
+  if (2 * r <= R) {
+    const double x = h * r / R;
+    c = ct_cosp(x);
+    s = ct_sinp(x);
+  } else {
+    const double x = h * (R - r) / R;
+    c = ct_sinp(x);
+    s = ct_cosp(x);
+  }
+  for (int i = 0; i < q; i++) {
+    const double t = c;
+    c = -s;
+    s = t;
+  }
+  return want_sin ? s : c;
+}
+
+// x * exp(SIGN 2 pi i E / R), E a compile-time constant
+template <int E, int R, int SIGN>
+__device__ __forceinline__ double2 twc(double2 x) {
+  constexpr int e = ((E % R) + R) % R;
+  if constexpr (e == 0) {
+    return x;
+  } else if constexpr (2 * e == R) {
+    return make_double2(-x.x, -x.y);
+  } else if constexpr (4 * e == R) {
+    return rmul_i<SIGN>(x);
+  } else if constexpr (4 * e == 3 * R) {
+    return rmul_i<-SIGN>(x);
+  } else {
+    constexpr double c = ct_cs(e, R, false), s = SIGN * ct_cs(e, R, true);
+    return make_double2(x.x * c - x.y * s, x.x * s + x.y * c);
+  }
+}
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void sfor_impl(F& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    sfor_impl<I + 1, N>(f);
+  }
+}
+template <int N, typename F>
+__device__ __forceinline__ void sfor(F&& f) {
+  sfor_impl<0, N>(f);
+}
+
+template <int R>
+struct Split {
+  static constexpr int P = (R % 4 == 0 && R > 4)   ? 4
+                           : (R % 2 == 0 && R > 2) ? 2
+                           : (R % 3 == 0 && R > 3) ? 3
+                           : (R % 5 == 0 && R > 5) ? 5
+                                                   : 1;
+};
+
+// in-register DFT y_q = sum_p x_p exp(SIGN 2 pi i p q / R), natural order in and out
+template <int R, int SIGN>
+__device__ __forceinline__ void dft(double2* x) {
+  if constexpr (R == 1) {
+    return;
+  } else if constexpr (R == 2) {
+    const double2 a = x[0], b = x[1];
+    x[0] = radd(a, b);
+    x[1] = rsub(a, b);
+  } else if constexpr (R == 4) {
+    const double2 t0 = radd(x[0], x[2]), t1 = rsub(x[0], x[2]);
+    const double2 t2 = radd(x[1], x[3]), t3 = rmul_i<SIGN>(rsub(x[1], x[3]));
+    x[0] = radd(t0, t2);
+    x[1] = radd(t1, t3);
+    x[2] = rsub(t0, t2);
+    x[3] = rsub(t1, t3);
+  } else if constexpr (R == 3) {
+    constexpr double s3 = 0.86602540378443864676;
+    const double2 t1 = radd(x[1], x[2]), t2 = rsub(x[1], x[2]);
+    const double2 a = make_double2(x[0].x - 0.5 * t1.x, x[0].y - 0.5 * t1.y);
+    const double2 b = rmul_i<SIGN>(make_double2(s3 * t2.x, s3 * t2.y));
+    x[0] = radd(x[0], t1);
+    x[1] = radd(a, b);
+    x[2] = rsub(a, b);
+  } else if constexpr (R == 5) {
+    constexpr double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+    constexpr double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+    const double2 t1 = radd(x[1], x[4]), t4 = rsub(x[1], x[4]);
+    const double2 t2 = radd(x[2], x[3]), t3 = rsub(x[2], x[3]);
+    const double2 a1 = make_double2(x[0].x + c1 * t1.x + c2 * t2.x, x[0].y + c1 * t1.y + c2 * t2.y);
+    const double2 a2 = make_double2(x[0].x + c2 * t1.x + c1 * t2.x, x[0].y + c2 * t1.y + c1 * t2.y);
+    const double2 b1 = rmul_i<SIGN>(make_double2(s1 * t4.x + s2 * t3.x, s1 * t4.y + s2 * t3.y));
+    const double2 b2 = rmul_i<SIGN>(make_double2(s2 * t4.x - s1 * t3.x, s2 * t4.y - s1 * t3.y));
+    x[0] = radd(x[0], radd(t1, t2));
+    x[1] = radd(a1, b1);
+    x[4] = rsub(a1, b1);
+    x[2] = radd(a2, b2);
+    x[3] = rsub(a2, b2);
+  } else {
+    // R = P * Q: n = p + P q, k = k1 + Q k2
+    constexpr int P = Split<R>::P, Q = R / P;
+    static_assert(P > 1, "radix must factor into 2, 3, 4, 5");
+    double2 y[R];
+    sfor<P>([&](auto pc) {
+      constexpr int p = decltype(pc)::value;
+      double2 t[Q];
+#pragma unroll
+      for (int q = 0; q < Q; q++) t[q] = x[p + P * q];
+      dft<Q, SIGN>(t);
+      sfor<Q>([&](auto kc) {
+        constexpr int k1 = decltype(kc)::value;
+        y[p * Q + k1] = twc<p * k1, R, SIGN>(t[k1]);
+      });
+    });
+#pragma unroll
+    for (int k1 = 0; k1 < Q; k1++) {
+      double2 s[P];
+#pragma unroll
+      for (int p = 0; p < P; p++) s[p] = y[p * Q + k1];
+      dft<P, SIGN>(s);
+#pragma unroll
+      for (int k2 = 0; k2 < P; k2++) x[k1 + Q * k2] = s[k2];
+    }
+  }
+}
+
+__device__ __forceinline__ double2 ld_g(const double2* p) { return *p; }
+__device__ __forceinline__ double2 ld_g(const float2* p) {
+  const float2 v = *p;
+  return make_double2((double)v.x, (double)v.y);
+}
+__device__ __forceinline__ void st_g(double2* p, double2 v) { *p = v; }
+__device__ __forceinline__ void st_g(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+
+constexpr int odd_up(int x) { return x | 1; }
+
+template <int R1, int R2>
+struct RegFft {
+  static constexpr int M = R1 * R2;
+  static constexpr int THREADS = 256;
+  static constexpr int ROWS = THREADS / R1;                      // rows per workgroup
+  static constexpr int PY = (R2 % 2 == 0) ? R2 + 1 : R2 + 2;     // padded pass-1 stride (odd)
+  static constexpr int SR = odd_up(R1 * PY > M + 1 ? R1 * PY : M + 1);  // padded row (odd)
+  static constexpr size_t LDS = (size_t)(ROWS * SR + 2 * M) * sizeof(double2);
+};
+
+}  // namespace
+
+template <int R1, int R2, typename GT>
+__global__ __launch_bounds__(256) void k_fft_fwd_r(const double* __restrict__ xy, GT* __restrict__ G,
+                                                   FftArgs a, unsigned* __restrict__ colmax) {
+  using C = RegFft<R1, R2>;
+  constexpr int M = C::M, ROWS = C::ROWS, PY = C::PY, SR = C::SR;
+  extern __shared__ double2 lds[];
+  __shared__ unsigned smax[ROWS];
+  double2* buf = lds;              // ROWS * SR
+  double2* tw2 = lds + ROWS * SR;  // W_M^{n1 k2} at k2 * R1 + n1
+  double2* twi = tw2 + M;          // W_I^m
+  const int tid = threadIdx.x;
+  for (int e = tid; e < M; e += C::THREADS) {
+    const int k2 = e / R1, n1 = e % R1;
+    tw2[e] = a.twM[(n1 * k2) % M];
+    twi[e] = a.twI[e];
+  }
+  const long nrows = (long)a.J * a.ncols;
+  const int N = a.N;
+  const double inv_i = 1.0 / (2 * M);
+  const int rr1 = tid / R1, n1 = tid % R1;  // pass-1 role
+  const int rr2 = tid / R2, k2 = tid % R2;  // pass-2 role
+  for (long r0 = (long)blockIdx.x * ROWS; r0 < nrows; r0 += (long)gridDim.x * ROWS) {
+    double2 v[R2];
+    {
+      const long r = r0 + rr1;
+      if (rr1 < ROWS && r < nrows) {
+        const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
+        const double2* x = reinterpret_cast<const double2*>(xy + c * a.ldg + (long)j * (2 * M));
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) v[n2] = x[n1 + R1 * n2];
+      } else {
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) v[n2] = make_double2(0.0, 0.0);
+      }
+    }
+    dft<R2, -1>(v);
+    if (colmax && tid < ROWS) smax[tid] = 0u;
+    __syncthreads();  // the previous rows are out of buf; twiddles are in place
+    if (rr1 < ROWS) {
+      double2* Y = buf + rr1 * SR + n1 * PY;
+      Y[0] = v[0];
+#pragma unroll
+      for (int q = 1; q < R2; q++) Y[q] = rcmul(v[q], tw2[q * R1 + n1]);
+    }
+    __syncthreads();
+    double2 u[R1];
+    if (rr2 < ROWS) {
+      const double2* Y = buf + rr2 * SR + k2;
+#pragma unroll
+      for (int p = 0; p < R1; p++) u[p] = Y[p * PY];
+      dft<R1, -1>(u);
+    }
+    __syncthreads();
+    if (rr2 < ROWS) {
+      double2* X = buf + rr2 * SR;
+#pragma unroll
+      for (int k1 = 0; k1 < R1; k1++) X[k2 + R2 * k1] = u[k1];
+    }
+    __syncthreads();
+    // half-complex split of the packed transform, 1/I, store; rows fastest across threads
+    for (int idx = tid; idx < (N + 1) * ROWS; idx += C::THREADS) {
+      const int m = idx / ROWS, rr = idx % ROWS;
+      const long r = r0 + rr;
+      if (r >= nrows) continue;
+      const double2* X = buf + rr * SR;
+      const double2 zm = X[m];
+      double2 zc = X[m ? M - m : 0];
+      zc.y = -zc.y;
+      const double2 E = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+      const double2 O = make_double2(0.5 * (zm.y - zc.y), -0.5 * (zm.x - zc.x));
+      double2 g = radd(E, rcmul(twi[m], O));
+      g = make_double2(g.x * inv_i, g.y * inv_i);
+      const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
+      st_g(&G[(long)m * a.gmst + (long)j * a.gld + c], g);
+      if (colmax) atomicMax(&smax[rr], __float_as_uint((float)fmax(fabs(g.x), fabs(g.y))));
+    }
+    if (colmax) {  // per-column magnitude for the scaled FP16 contraction
+      __syncthreads();
+      if (tid < ROWS && r0 + tid < nrows) atomicMax(colmax + (int)((r0 + tid) % a.ncols), smax[tid]);
+    }
+  }
+}
+
+template <int R1, int R2, typename GT>
+__global__ __launch_bounds__(256) void k_fft_inv_r(const GT* __restrict__ G, double* __restrict__ xy,
+                                                   FftArgs a) {
+  using C = RegFft<R1, R2>;
+  constexpr int M = C::M, ROWS = C::ROWS, PY = C::PY, SR = C::SR;
+  extern __shared__ double2 lds[];
+  double2* buf = lds;
+  double2* tw2 = lds + ROWS * SR;
+  double2* twi = tw2 + M;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < M; e += C::THREADS) {
+    const int k2 = e / R1, n1 = e % R1;
+    tw2[e] = a.twM[(n1 * k2) % M];
+    twi[e] = a.twI[e];
+  }
+  const long nrows = (long)a.J * a.ncols;
+  const int N = a.N;
+  const int rr1 = tid / R1, n1 = tid % R1;
+  const int rr2 = tid / R2, k2 = tid % R2;
+  for (long r0 = (long)blockIdx.x * ROWS; r0 < nrows; r0 += (long)gridDim.x * ROWS) {
+    __syncthreads();  // the previous rows are out of buf
+    for (int idx = tid; idx < M * ROWS; idx += C::THREADS) {
+      const int m = idx / ROWS, rr = idx % ROWS;
+      const long r = r0 + rr;
+      double2 g = make_double2(0.0, 0.0);
+      if (m <= N && r < nrows) {
+        const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
+        g = ld_g(&G[(long)m * a.gmst + (long)j * a.gld + c]);
+        if (m == 0) g.y = 0.0;
+      }
+      buf[rr * SR + m] = g;
+    }
+    __syncthreads();
+    // packed spectrum Z_m from the half-complex pair (X_m, X_{M-m}); X_M = 0 for m = 0
+    for (int idx = tid; idx < ROWS * (M / 2 + 1); idx += C::THREADS) {
+      const int m = idx / ROWS, rr = idx % ROWS;
+      double2* X = buf + rr * SR;
+      const int mc = m ? M - m : 0;
+      const double2 xm = X[m], xo = X[mc];
+      const double2 c1 = m ? make_double2(xo.x, -xo.y) : make_double2(0.0, 0.0);
+      const double2 A1 = radd(xm, c1), B1 = rcmulc(rsub(xm, c1), twi[m]);
+      const double2 z1 = make_double2(A1.x - B1.y, A1.y + B1.x);
+      if (m != 0 && 2 * m != M) {
+        const double2 c2 = make_double2(xm.x, -xm.y);
+        const double2 A2 = radd(xo, c2), B2 = rcmulc(rsub(xo, c2), twi[mc]);
+        X[mc] = make_double2(A2.x - B2.y, A2.y + B2.x);
+      }
+      X[m] = z1;
+    }
+    __syncthreads();
+    double2 v[R2];
+    if (rr1 < ROWS) {
+      const double2* Z = buf + rr1 * SR + n1;
+#pragma unroll
+      for (int n2 = 0; n2 < R2; n2++) v[n2] = Z[R1 * n2];
+    }
+    dft<R2, +1>(v);
+    __syncthreads();
+    if (rr1 < ROWS) {
+      double2* Y = buf + rr1 * SR + n1 * PY;
+      Y[0] = v[0];
+#pragma unroll
+      for (int q = 1; q < R2; q++) Y[q] = rcmulc(v[q], tw2[q * R1 + n1]);
+    }
+    __syncthreads();
+    if (rr2 < ROWS) {
+      double2 u[R1];
+      const double2* Y = buf + rr2 * SR + k2;
+#pragma unroll
+      for (int p = 0; p < R1; p++) u[p] = Y[p * PY];
+      dft<R1, +1>(u);
+      const long r = r0 + rr2;
+      if (r < nrows) {
+        const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
+        const double s = a.rowscale ? a.rowscale[j] : 1.0;
+        double2* x = reinterpret_cast<double2*>(xy + c * a.ldg + (long)j * (2 * M));
+#pragma unroll
+        for (int k1 = 0; k1 < R1; k1++) x[k2 + R2 * k1] = make_double2(u[k1].x * s, u[k1].y * s);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- dispatch
+template <int R1, int R2, typename GT>
+static int reg_fwd(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
+  using C = RegFft<R1, R2>;
+  const long nrows = (long)a.J * a.ncols;
+  long nb = cdivl(nrows, C::ROWS);
+  if (nb > 8192) nb = 8192;
+  hipLaunchKernelGGL((k_fft_fwd_r<R1, R2, GT>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, xy, G, a,
+                     colmax);
+  DSHT_HIP(hipGetLastError());
+  return 0;
+}
+
+template <int R1, int R2, typename GT>
+static int reg_inv(const GT* G, const FftArgs& a, double* xy, hipStream_t s) {
+  using C = RegFft<R1, R2>;
+  const long nrows = (long)a.J * a.ncols;
+  long nb = cdivl(nrows, C::ROWS);
+  if (nb > 8192) nb = 8192;
+  hipLaunchKernelGGL((k_fft_inv_r<R1, R2, GT>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, G, xy, a);
+  DSHT_HIP(hipGetLastError());
+  return 0;
+}
+
+// factor pairs (R1, R2) with M = R1 * R2 that have register kernels
+#define DSHT_REG_FFT_SIZES(X) \
+  X(4, 2)                     \
+  X(4, 4)                     \
+  X(6, 6)                     \
+  X(8, 4)                     \
+  X(8, 6)                     \
+  X(8, 8)                     \
+  X(9, 8)                     \
+  X(12, 8)                    \
+  X(16, 8)                    \
+  X(15, 9)                    \
+  X(16, 12)                   \
+  X(16, 16)                   \
+  X(18, 15)
+
+bool reg_fft_supported(int M) {
+#define X(a, b) \
+  if (M == (a) * (b)) return true;
+  DSHT_REG_FFT_SIZES(X)
+#undef X
+  return false;
+}
+
+template <typename GT>
+int reg_fft_fwd(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
+#define X(r1, r2) \
+  if (a.M == (r1) * (r2)) return reg_fwd<r1, r2, GT>(xy, a, G, colmax, s);
+  DSHT_REG_FFT_SIZES(X)
+#undef X
+  return fail(DSHT_EUNSUPPORTED, "no register FFT for M = %d", a.M);
+}
+
+template <typename GT>
+int reg_fft_inv(const GT* G, const FftArgs& a, double* xy, hipStream_t s) {
+#define X(r1, r2) \
+  if (a.M == (r1) * (r2)) return reg_inv<r1, r2, GT>(G, a, xy, s);
+  DSHT_REG_FFT_SIZES(X)
+#undef X
+  return fail(DSHT_EUNSUPPORTED, "no register FFT for M = %d", a.M);
+}
+
+template int reg_fft_fwd<double2>(const double*, const FftArgs&, double2*, unsigned*, hipStream_t);
+template int reg_fft_fwd<float2>(const double*, const FftArgs&, float2*, unsigned*, hipStream_t);
+template int reg_fft_inv<double2>(const double2*, const FftArgs&, double*, hipStream_t);
+template int reg_fft_inv<float2>(const float2*, const FftArgs&, double*, hipStream_t);
+
+}  // namespace dsht

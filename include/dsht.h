@@ -162,6 +162,7 @@ int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, d
  * of the Legendre contractions.  No counterpart in the reference.                   */
 #define DSHT_TUNE_CHUNK_COLS 0 /* 0 = automatic */
 #define DSHT_TUNE_LEG_CB 1     /* 0 = automatic, 16 or 32 */
+#define DSHT_TUNE_FFT 2        /* 0 = register two-pass kernel where the length has one, 1 = LDS Stockham */
 int dsht_set_tuning(dsht_plan* plan, int key, int value);
 int dsht_get_tuning(const dsht_plan* plan, int key);
 

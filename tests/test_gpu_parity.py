@@ -143,6 +143,31 @@ def test_roundtrip_t170_chunked(gpu):
     assert rel_l2(w2.cpu().numpy(), w.cpu().numpy()) < TOL
 
 
+# every FFT length with a register two-pass kernel (M = I/2 = R1*R2), plus the Stockham
+# kernels of the same lengths: both against each other and against the oracle
+FFT_SIZES = [(5, 16, 8), (10, 32, 16), (21, 64, 32), (23, 72, 36), (31, 96, 48), (42, 128, 64),
+             (47, 144, 72), (63, 192, 96), (85, 256, 128), (85, 270, 136), (127, 384, 192),
+             (170, 512, 256), (170, 540, 270)]
+
+
+@pytest.mark.parametrize("N,I,J", FFT_SIZES)
+def test_fft_kernels(gpu, N, I, J):
+    import dsht
+    p, o = _pair(N, I, J)
+    nc = 3
+    w = red_spectrum(N, nc, seed=I)
+    xy = np.random.default_rng(I).standard_normal((nc, J, I))
+    ref_i, ref_f = o.inv(w), o.fwd(xy)
+    out = {}
+    for kind in (0, 1):
+        p.set_tuning(dsht.TUNE_FFT, kind)
+        out[kind] = (p._inv(w, dsht.VAL), p._fwd(xy, dsht.VAL))
+        assert rel_l2(out[kind][0], ref_i) < TOL, (kind, rel_l2(out[kind][0], ref_i))
+        assert rel_l2(out[kind][1], ref_f) < TOL, (kind, rel_l2(out[kind][1], ref_f))
+    assert rel_l2(out[0][0], out[1][0]) < 1e-14
+    assert rel_l2(out[0][1], out[1][1]) < 1e-14
+
+
 def test_errors(gpu):
     import dsht
     with pytest.raises(dsht.DshtError):
