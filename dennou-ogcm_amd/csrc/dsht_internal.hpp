@@ -167,6 +167,7 @@ struct dsht_plan {
   int chunk_cols = 0;  // column chunk of the device path (0 = auto)
   int leg_cb = 0;      // column block of the contractions (0 = auto, 16, 32)
   int fft_kind = 0;    // 0 = register two-pass FFT where available, 1 = LDS Stockham
+  int leg_kind = 0;    // split-precision contractions: 0 = per-wave fragment streams, 1 = LDS-staged
 };
 
 namespace dsht {

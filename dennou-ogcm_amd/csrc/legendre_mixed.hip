@@ -16,94 +16,9 @@
 #include <cstdint>
 #include <type_traits>
 
-#include "dsht_internal.hpp"
+#include "legendre_mixed.hpp"
 
 namespace dsht {
-
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
-
-constexpr int KM = 32;  // MFMA K of the 16-bit forms (one chunk)
-
-// form index: 0 bf16x3, 1 bf16x6, 2 fp16, 3 fp16x3
-template <int FORM>
-struct MxForm {
-  static constexpr bool BF = (FORM == 0 || FORM == 1);
-  static constexpr int NT = FORM == 1 ? 3 : (FORM == 2 ? 1 : 2);  // split terms
-  static constexpr int NP = FORM == 1 ? 6 : (FORM == 2 ? 1 : 3);  // products
-  static constexpr bool SCALED = !BF;
-  using T = typename std::conditional<BF, __bf16, _Float16>::type;
-  using V8 = typename std::conditional<BF, bf16x8_t, f16x8_t>::type;
-};
-// product list (table term, data term)
-__device__ constexpr int PA[6] = {0, 0, 1, 0, 1, 2};
-__device__ constexpr int PB[6] = {0, 1, 0, 2, 1, 0};
-
-__device__ __forceinline__ float4_t mfma16(bf16x8_t a, bf16x8_t b, float4_t c) {
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
-__device__ __forceinline__ float4_t mfma16(f16x8_t a, f16x8_t b, float4_t c) {
-  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-}
-// 16x16x32 16-bit MFMA lane layout: A[r][k] at lane r + 16*(k/8), element k%8; B[k][c] at lane
-// c + 16*(k/8), element k%8; result element i of lane l holds D[4*(l/16) + i][l%16].
-__device__ __forceinline__ int acc_row16(int lane, int i) { return 4 * (lane >> 4) + i; }
-
-// FP16 split terms are stored pre-scaled by 2^(11 t) so the low parts of small values stay
-// in FP16's normal range (the MFMA flushes FP16 subnormals); products of total order o are
-// weighted by 2^(-11 o).  BF16 has FP32's exponent range and needs no term scaling.
-template <int FORM>
-struct TermScale {
-  static constexpr float UP = MxForm<FORM>::BF ? 1.0f : 2048.0f;
-  static constexpr float DOWN = MxForm<FORM>::BF ? 1.0f : (1.0f / 2048.0f);
-};
-
-template <int FORM>
-__device__ __forceinline__ void split_to(float x, typename MxForm<FORM>::T* out) {
-  using T = typename MxForm<FORM>::T;
-#pragma unroll
-  for (int t = 0; t < MxForm<FORM>::NT; t++) {
-    T h = (T)x;
-    out[t] = h;
-    x = (x - (float)h) * TermScale<FORM>::UP;
-  }
-}
-
-// ---------------------------------------------------------------- tables
-struct MixedTables {
-  int form_family = -1;  // 0: bf16, 1: fp16 (each holds up to 3 terms)
-  int nterm = 0;
-  uint16_t* fP[3] = {nullptr, nullptr, nullptr};
-  uint16_t* fH[3] = {nullptr, nullptr, nullptr};
-  uint16_t* iP[3] = {nullptr, nullptr, nullptr};
-  uint16_t* iH[3] = {nullptr, nullptr, nullptr};
-  long* d_offF = nullptr;  // per-m offsets (elements) of the forward tables
-  long* d_offI = nullptr;
-  long sizeF = 0, sizeI = 0;
-  double* d_minv = nullptr;  // per-m table unscale 2^e (1 for bf16)
-  float* d_cwv = nullptr;    // north-half quadrature weights, FP32
-  float* d_cwd = nullptr;
-  int J2p32 = 0;
-};
-
-struct MixedState {
-  MixedTables tab[2];  // [family]
-};
-
-// bf16 forms share one table family, fp16 forms the other
-static inline int MxFormFamily(int form) { return form <= 1 ? 0 : 1; }
-
-// algorithmic work of one split-precision contraction launch (FP64-equivalent flops;
-// Fourier arrays in FP32, spectral in FP64, one read of each 16-bit table term used)
-static void mixed_work(const dsht_plan* p, int op, bool fwd, int ncols, double* bytes, double* flops) {
-  static const int nin[] = {1, 1, 1, 2, 2}, nout[] = {1, 1, 1, 1, 2}, nt[] = {1, 1, 1, 2, 4};
-  double T = 0.5 * (p->N + 1.0) * (p->N + 2.0);
-  double four = (double)(p->N + 1) * p->J * 8.0 * ncols, spec = (double)p->L * 8.0 * ncols;
-  *bytes = fwd ? nin[op] * four + nout[op] * spec : nin[op] * spec + nout[op] * four;
-  *flops = 2.0 * p->J * T * ncols * nt[op];
-}
-
-__host__ __device__ inline int kpad32(int n) { return (n + KM - 1) / KM * KM; }
 
 // per-m table scale: 2^-e with e = ceil(log2(max |P|, |H|)) over the m block
 __global__ void k_mx_scale(int N, int J2pad, const long* offF, const double* tP, const double* tH, double* minv,
@@ -232,6 +147,18 @@ static int build_mixed(dsht_plan* p, int family) {
   offI[N + 1] = i;
   t.sizeF = f;
   t.sizeI = i;
+  {
+    const long per = (long)t.J2p32 * KM;
+    std::vector<int> cbase(N + 2), chunk_m;
+    for (int m = 0; m <= N + 1; m++) cbase[m] = (int)(offI[m] / per);
+    for (int m = 0; m <= N; m++)
+      for (int g = cbase[m]; g < cbase[m + 1]; g++) chunk_m.push_back(m);
+    t.nchunks = (int)chunk_m.size();
+    DSHT_HIP(hipMalloc(&t.d_cbase, sizeof(int) * (N + 2)));
+    DSHT_HIP(hipMalloc(&t.d_chunk_m, sizeof(int) * (chunk_m.size() + 1)));
+    DSHT_HIP(hipMemcpy(t.d_cbase, cbase.data(), sizeof(int) * (N + 2), hipMemcpyHostToDevice));
+    DSHT_HIP(hipMemcpy(t.d_chunk_m, chunk_m.data(), sizeof(int) * chunk_m.size(), hipMemcpyHostToDevice));
+  }
   DSHT_HIP(hipMalloc(&t.d_offF, sizeof(long) * (N + 2)));
   DSHT_HIP(hipMalloc(&t.d_offI, sizeof(long) * (N + 2)));
   DSHT_HIP(hipMemcpy(t.d_offF, offF.data(), sizeof(long) * (N + 2), hipMemcpyHostToDevice));
@@ -301,6 +228,8 @@ void mixed_free(dsht_plan* p) {
     if (t.d_minv) (void)hipFree(t.d_minv);
     if (t.d_cwv) (void)hipFree(t.d_cwv);
     if (t.d_cwd) (void)hipFree(t.d_cwd);
+    if (t.d_cbase) (void)hipFree(t.d_cbase);
+    if (t.d_chunk_m) (void)hipFree(t.d_chunk_m);
   }
   delete st;
   p->mixed = nullptr;
@@ -944,6 +873,26 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
     double by, fl;
     mixed_work(p, op, true, nc, &by, &fl);
     ProfScope ps("leg_fwd", s, by, fl);
+    if (p->leg_kind == 0) {
+      Mx2FwdJob j;
+      j.G0 = G0;
+      j.G1 = nin > 1 ? G1 : nullptr;
+      j.gld = nc;
+      j.gmst = (long)p->J * nc;
+      j.ncols = nc;
+      j.W0 = a.W0;
+      j.W1 = a.W1;
+      j.sl = sl;
+      j.sc = sc;
+      j.cw = a.cw;
+      j.colmax = colmax;
+      j.bound = bound;
+      j.diag0 = diag0;
+      j.diag1 = diag1;
+      j.scale = scale;
+      if ((rc = mx2_fwd(p, t, form, op, j, s))) return rc;
+      continue;
+    }
     switch (form) {
       case 0: rc = mx_fwd_op<0>(p, op, a, s); break;
       case 1: rc = mx_fwd_op<1>(p, op, a, s); break;
@@ -969,20 +918,25 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
   const bool scaled = form >= 2;
   const int ch = mixed_chunk_cols(p, ncols);
   const size_t gsz = (size_t)(p->N + 1) * p->J * ch;
-  size_t bytes = nout * gsz * sizeof(float2) + 2 * (size_t)ch * sizeof(unsigned) + 256;
+  const size_t cmb = (2 * (size_t)ch * sizeof(unsigned) + 255) / 256 * 256;
+  const size_t pkb = p->leg_kind == 0 ? mx2_inv_pack_bytes(t, form, op, ch) : 0;
+  size_t bytes = nout * gsz * sizeof(float2) + cmb + pkb;
   WsHandle wh;
   if (int rc = workspace_acquire(p, s, bytes, wh)) return rc;
   float2* G0 = (float2*)wh.buf;
   float2* G1 = G0 + gsz;
   unsigned* colmax = (unsigned*)(G0 + nout * gsz);
+  uint16_t* pack = (uint16_t*)((char*)colmax + cmb);
   for (int c0 = 0; c0 < ncols; c0 += ch) {
     const int nc = ncols - c0 < ch ? ncols - c0 : ch;
     const double* w0 = in0 + (long)c0 * sc;
     const double* w1 = in1 ? in1 + (long)c0 * sc : nullptr;
     if (scaled) {
-      hipLaunchKernelGGL(k_spec_colmax, dim3(nc), dim3(256), 0, s, w0, sl, sc, p->L, pro, colmax);
-      if (nin > 1) hipLaunchKernelGGL(k_spec_colmax, dim3(nc), dim3(256), 0, s, w1, sl, sc, p->L, pro, colmax + nc);
+      hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(2 * nc, 256)), dim3(256), 0, s, colmax, 2 * nc);
       DSHT_HIP(hipGetLastError());
+      if (int rc = launch_spec_colmax(w0, sl, sc, p->L, nc, pro, colmax, s)) return rc;
+      if (nin > 1)
+        if (int rc = launch_spec_colmax(w1, sl, sc, p->L, nc, pro, colmax + nc, s)) return rc;
     }
     MxInvArgs a;
     a.W0 = w0;
@@ -1011,6 +965,23 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
       double by, fl;
       mixed_work(p, op, false, nc, &by, &fl);
       ProfScope ps("leg_inv", s, by, fl);
+      if (p->leg_kind == 0) {
+        Mx2InvJob j;
+        j.W0 = w0;
+        j.W1 = w1;
+        j.sl = sl;
+        j.sc = sc;
+        j.ncols = nc;
+        j.pro = pro;
+        j.rs = rs;
+        j.colmax = colmax;
+        j.pack = pack;
+        j.G0 = G0;
+        j.G1 = nout > 1 ? G1 : nullptr;
+        j.gld = nc;
+        j.gmst = (long)p->J * nc;
+        rc = mx2_inv(p, t, form, op, j, s);
+      } else
       switch (form) {
         case 0: rc = mx_inv_op<0>(p, op, a, s); break;
         case 1: rc = mx_inv_op<1>(p, op, a, s); break;

@@ -1,0 +1,671 @@
+// legendre_mx2.hip -- split-precision Legendre contractions with per-wave fragment streams.
+//
+// Same operators and split forms as legendre_mixed.hip (the Legendre half of SPMODEL
+// w_xy / xy_w and their derivative forms, doc/description/ogcm/space_discretization.tex:
+// 199-268).  Every wave owns its output tiles and streams both MFMA operands straight from
+// memory into registers, so there is no LDS staging and no barrier in the K loop:
+//  * table fragments are pre-split and pre-swizzled (one 16-byte load per lane per term);
+//  * inverse: the spectral data are first packed by k_mx2_ipack into split-term panels in
+//    MFMA B-fragment order (prefactor, column scale and FP16 term scaling applied there),
+//    so the contraction loads them like the tables;
+//  * forward: each wave reads its Fourier fragment (north and south rows) and forms the
+//    symmetric or antisymmetric combination its row parity needs, then splits it.
+// Products of one output accumulate in one MFMA chain per (table, input) group; the i*m
+// factors, signs, north/south recombination and unscaling happen once in the epilogue.
+// Blocks are placed so that all blocks of one zonal wavenumber run on one XCD (shared
+// table and data panels stay in that XCD's L2) and wavenumbers alternate across XCDs.
+#include <cmath>
+
+#include "legendre_mixed.hpp"
+
+namespace dsht {
+
+namespace {
+
+// B-operand terms and product list of each form.  FP16 forms keep the whole sum scaled by
+// 2^11 (undone in the epilogue) so that no product needs a separate weight:
+//   FP16    b -> fp16(2^11 b);                           products (a0, b0)
+//   FP16X3  b -> {2^11 h, fp16(2^11 (b - h)), h}, h = fp16(b); products (a0,b0) (a0,b1) (a1,b2)
+template <int FORM>
+struct Mx2 {
+  static constexpr int NTA = FORM == 1 ? 3 : (FORM == 2 ? 1 : 2);
+  static constexpr int NTB = FORM == 1 ? 3 : (FORM == 2 ? 1 : (FORM == 3 ? 3 : 2));
+  static constexpr int NP = FORM == 1 ? 6 : (FORM == 2 ? 1 : 3);
+  static constexpr double DSCALE = (FORM >= 2) ? 1.0 / 2048.0 : 1.0;
+  __host__ __device__ static constexpr int pa(int i) {
+    return FORM == 3 ? (i == 2 ? 1 : 0) : (i == 2 || i == 4 ? 1 : (i == 5 ? 2 : 0));
+  }
+  __host__ __device__ static constexpr int pb(int i) {
+    return FORM == 3 ? i : (i == 1 || i == 4 ? 1 : (i == 3 ? 2 : 0));
+  }
+};
+
+template <int FORM>
+__device__ __forceinline__ void bsplit(float x, typename MxForm<FORM>::T* out) {
+  using T = typename MxForm<FORM>::T;
+  if constexpr (FORM == 0 || FORM == 1) {
+#pragma unroll
+    for (int t = 0; t < Mx2<FORM>::NTB; t++) {
+      const T h = (T)x;
+      out[t] = h;
+      x -= (float)h;
+    }
+  } else if constexpr (FORM == 2) {
+    out[0] = (T)(2048.0f * x);
+  } else {
+    const T h = (T)x;
+    out[0] = (T)(2048.0f * (float)h);
+    out[1] = (T)(2048.0f * (x - (float)h));
+    out[2] = h;
+  }
+}
+
+// operator groups: each is one (table, input) product stream; table 0 = P, 1 = H
+__host__ __device__ constexpr int inv_ng(int op) { return op == I_ALPHA ? 2 : (op == I_VORDIV2UV ? 4 : 1); }
+__host__ __device__ constexpr int inv_tab(int op, int g) {
+  return op == I_GRADMU ? 1 : (op == I_ALPHA ? (g == 1 ? 1 : 0) : (op == I_VORDIV2UV ? (g >= 2 ? 1 : 0) : 0));
+}
+__host__ __device__ constexpr int inv_in(int op, int g) {
+  return op == I_ALPHA ? g : (op == I_VORDIV2UV ? (g & 1) : 0);
+}
+// forward UV2VORDIV groups: (P,V) (H,U) (P,U) (H,V)
+__host__ __device__ constexpr int fwd_ng(int op) { return op == F_ALPHA ? 2 : (op == F_UV2VORDIV ? 4 : 1); }
+__host__ __device__ constexpr int fwd_tab(int op, int g) {
+  return op == F_DIVMU ? 1 : (op == F_ALPHA ? (g == 1 ? 1 : 0) : (op == F_UV2VORDIV ? (g & 1) : 0));
+}
+__host__ __device__ constexpr int fwd_in(int op, int g) {
+  return op == F_ALPHA ? g : (op == F_UV2VORDIV ? ((g == 0 || g == 3) ? 1 : 0) : 0);
+}
+
+// block -> (m, sub-block): wavenumbers m = x, x+8, ... run on XCD x (blocks are dispatched
+// to the 8 XCDs round robin), the CBLK blocks of one m are consecutive on that XCD
+__device__ __forceinline__ bool xcd_decode(int N, int cblk, int& m, int& sub) {
+  const int b = blockIdx.x;
+  const int idx = b >> 3;
+  m = (b & 7) + 8 * (idx / cblk);
+  sub = idx % cblk;
+  return m <= N;
+}
+
+template <int NIN>
+__device__ __forceinline__ float colscale(const unsigned* colmax, int ncols, int c, float bound) {
+  float mx = 0.f;
+#pragma unroll
+  for (int i = 0; i < NIN; i++) mx = fmaxf(mx, __uint_as_float(colmax[(long)i * ncols + c]));
+  mx *= bound;
+  return mx > 0.f ? exp2f(-ceilf(log2f(mx))) : 1.f;
+}
+
+struct Mx2InvArgs {
+  const double* W0;
+  const double* W1;
+  long sl, sc;
+  int ncols, CT;
+  const double* pro;
+  const double* rs;
+  const unsigned* colmax;
+  uint16_t* pack;
+  float2* G0;
+  float2* G1;
+  long gld, gmst;
+  const uint16_t* tP[3];
+  const uint16_t* tH[3];
+  const long* off;
+  const int* cbase;
+  const int* chunk_m;
+  int nchunks;
+  const double* minv;
+  int N, J2, J2p32, RT;
+  int CG, RG;  // column groups and row groups per wavenumber
+};
+
+// ---------------------------------------------------------------- inverse data panels
+// panel (chunk g, column tile ct, input i): [re, im][term][lane][8], lane = c + 16*(k/8)
+template <int FORM, int NIN>
+__global__ __launch_bounds__(256) void k_mx2_ipack(Mx2InvArgs a) {
+  using F = MxForm<FORM>;
+  using T = typename F::T;
+  using V8 = typename F::V8;
+  constexpr int NTB = Mx2<FORM>::NTB;
+  const long wv = ((long)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wv >= (long)a.nchunks * a.CT * NIN) return;
+  const int i = (int)(wv % NIN), ct = (int)((wv / NIN) % a.CT);
+  const int g = (int)(wv / NIN / a.CT);
+  const int m = a.chunk_m[g], ch = g - a.cbase[m];
+  const int nE = n_par(a.N, m, 0), nO = n_par(a.N, m, 1);
+  const int nch0 = kpad32(nE) / KM;
+  const int p = ch >= nch0 ? 1 : 0, ks = p ? ch - nch0 : ch, np = p ? nO : nE;
+  const int c = ct * TILE + (lane & 15);
+  const double* W = i ? a.W1 : a.W0;
+  float dsc = 1.f;
+  if constexpr (F::SCALED) {
+    if (c < a.ncols) dsc = colscale<NIN>(a.colmax, a.ncols, c, 1.f);
+  }
+  T tre[NTB][8], tim[NTB][8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    const int k = ks * KM + 8 * (lane >> 4) + e;
+    double vre = 0.0, vim = 0.0;
+    if (k < np && c < a.ncols) {
+      const int n = m + p + 2 * k;
+      const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
+      const double pr = a.pro ? a.pro[lre] : 1.0;
+      vre = pr * W[lre * a.sl + (long)c * a.sc];
+      if (m > 0) vim = pr * W[lim * a.sl + (long)c * a.sc];
+    }
+    T t1[NTB], t2[NTB];
+    bsplit<FORM>((float)vre * dsc, t1);
+    bsplit<FORM>((float)vim * dsc, t2);
+#pragma unroll
+    for (int t = 0; t < NTB; t++) {
+      tre[t][e] = t1[t];
+      tim[t][e] = t2[t];
+    }
+  }
+  uint16_t* dst = a.pack + ((((long)g * a.CT + ct) * NIN + i) * 2 * NTB) * 512 + lane * 8;
+#pragma unroll
+  for (int t = 0; t < NTB; t++) {
+    V8 v1, v2;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      v1[e] = tre[t][e];
+      v2[e] = tim[t][e];
+    }
+    *reinterpret_cast<V8*>(dst + (long)t * 512) = v1;
+    *reinterpret_cast<V8*>(dst + (long)(NTB + t) * 512) = v2;
+  }
+}
+
+// ---------------------------------------------------------------- inverse contraction
+// block = 4 waves of one wavenumber m and one group of NCT column tiles; wave w owns the
+// MAXT latitude tiles starting at (rg*4 + w)*MAXT.  K runs over the parity-0 chunks, then
+// the parity-1 chunks; P of parity p accumulates into sym (p = 0) or anti (p = 1), H the
+// opposite; north = sym + anti, south = sym - anti.
+template <int OP, int FORM, int NCT, int MAXT>
+__global__ __launch_bounds__(256) void k_mx2_inv(Mx2InvArgs a) {
+  using F = MxForm<FORM>;
+  using X = Mx2<FORM>;
+  using V8 = typename F::V8;
+  constexpr int NG = inv_ng(OP), NIN = InvTraits<OP>::NIN, NOUT = InvTraits<OP>::NOUT;
+  constexpr int NTA = X::NTA, NTB = X::NTB;
+  constexpr bool UP = InvTraits<OP>::USE_P, UH = InvTraits<OP>::USE_H;
+  int m, sub;
+  if (!xcd_decode(a.N, a.CG * a.RG, m, sub)) return;
+  const int cg = sub % a.CG, rg = sub / a.CG;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int t0 = (rg * 4 + w) * MAXT;
+  if (t0 >= a.RT) return;
+  const int nE = n_par(a.N, m, 0), nO = n_par(a.N, m, 1);
+  const int kq0 = kpad32(nE), kq1 = kpad32(nO);
+  const int nch0 = kq0 / KM, nch1 = kq1 / KM;
+  const long toff = a.off[m];
+  const int gc0 = a.cbase[m];
+  const int ct0 = cg * NCT;
+
+  float4_t acc[NG][MAXT][NCT][2][2];
+#pragma unroll
+  for (int g = 0; g < NG; g++)
+#pragma unroll
+    for (int tt = 0; tt < MAXT; tt++)
+#pragma unroll
+      for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+        for (int r = 0; r < 2; r++) acc[g][tt][ct][r][0] = acc[g][tt][ct][r][1] = float4_t{0, 0, 0, 0};
+
+  auto run = [&](auto PAR) {
+    constexpr int PP = decltype(PAR)::value;
+    const int nch = PP ? nch1 : nch0;
+    const long pbase = toff + (PP ? (long)a.J2p32 * kq0 : 0);
+    for (int ks = 0; ks < nch; ks++) {
+      V8 A[2][MAXT][NTA];
+#pragma unroll
+      for (int tt = 0; tt < MAXT; tt++) {
+        const int t = t0 + tt;
+        const long o = pbase + ((long)t * nch + ks) * 512 + lane * 8;
+#pragma unroll
+        for (int k = 0; k < NTA; k++) {
+          if constexpr (UP) A[0][tt][k] = t < a.RT ? *reinterpret_cast<const V8*>(a.tP[k] + o) : V8{};
+          if constexpr (UH) A[1][tt][k] = t < a.RT ? *reinterpret_cast<const V8*>(a.tH[k] + o) : V8{};
+        }
+      }
+      V8 B[NIN][NCT][2][NTB];
+      const long g = gc0 + (PP ? nch0 : 0) + ks;
+#pragma unroll
+      for (int i = 0; i < NIN; i++)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) {
+          const int ctg = ct0 + ct;
+          const uint16_t* src = a.pack + (((g * a.CT + ctg) * NIN + i) * 2 * NTB) * 512 + lane * 8;
+#pragma unroll
+          for (int r = 0; r < 2; r++)
+#pragma unroll
+            for (int t = 0; t < NTB; t++)
+              B[i][ct][r][t] = ctg < a.CT ? *reinterpret_cast<const V8*>(src + (long)(r * NTB + t) * 512) : V8{};
+        }
+#pragma unroll
+      for (int gi = 0; gi < NG; gi++) {
+        const int tb = inv_tab(OP, gi), in = inv_in(OP, gi);
+        const int sidx = tb == 0 ? PP : 1 - PP;
+#pragma unroll
+        for (int tt = 0; tt < MAXT; tt++)
+#pragma unroll
+          for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+            for (int r = 0; r < 2; r++)
+#pragma unroll
+              for (int pr = 0; pr < X::NP; pr++)
+                acc[gi][tt][ct][r][sidx] =
+                    mfma16(A[tb][tt][X::pa(pr)], B[in][ct][r][X::pb(pr)], acc[gi][tt][ct][r][sidx]);
+      }
+    }
+  };
+  run(std::integral_constant<int, 0>());
+  run(std::integral_constant<int, 1>());
+
+  // ---- epilogue: recombine, i*m factors, unscale, row scale, FP32 store
+  const double mi = a.minv[m] * X::DSCALE;
+  const double dm = (double)m;
+#pragma unroll
+  for (int tt = 0; tt < MAXT; tt++) {
+    const int t = t0 + tt;
+    if (t >= a.RT) continue;
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      const int c = (ct0 + ct) * TILE + (lane & 15);
+      if (c >= a.ncols) continue;
+      double u = mi;
+      if constexpr (F::SCALED) u = mi / (double)colscale<NIN>(a.colmax, a.ncols, c, 1.f);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int jr = t * TILE + acc_row16(lane, i);
+        if (jr >= a.J2) continue;
+        const int jN = a.J2 + jr, jS = a.J2 - 1 - jr;
+        // per group: north and south complex values
+        double nr[NG], ni[NG], sr[NG], si[NG];
+#pragma unroll
+        for (int gi = 0; gi < NG; gi++) {
+          const double a0 = acc[gi][tt][ct][0][0][i], a1 = acc[gi][tt][ct][0][1][i];
+          const double b0 = acc[gi][tt][ct][1][0][i], b1 = acc[gi][tt][ct][1][1][i];
+          nr[gi] = a0 + a1;
+          sr[gi] = a0 - a1;
+          ni[gi] = b0 + b1;
+          si[gi] = b0 - b1;
+        }
+        double onr[NOUT], oni[NOUT], osr[NOUT], osi[NOUT];
+        if constexpr (OP == I_VAL || OP == I_GRADMU) {
+          onr[0] = nr[0], oni[0] = ni[0], osr[0] = sr[0], osi[0] = si[0];
+        } else if constexpr (OP == I_GRADLAMBDA) {  // i m X
+          onr[0] = -dm * ni[0], oni[0] = dm * nr[0], osr[0] = -dm * si[0], osi[0] = dm * sr[0];
+        } else if constexpr (OP == I_ALPHA) {  // i m P.A + H.B
+          onr[0] = -dm * ni[0] + nr[1], oni[0] = dm * nr[0] + ni[1];
+          osr[0] = -dm * si[0] + sr[1], osi[0] = dm * sr[0] + si[1];
+        } else {  // groups (P,psi) (P,chi) (H,psi) (H,chi): U = i m P.chi - H.psi, V = i m P.psi + H.chi
+          onr[0] = -dm * ni[1] - nr[2], oni[0] = dm * nr[1] - ni[2];
+          osr[0] = -dm * si[1] - sr[2], osi[0] = dm * sr[1] - si[2];
+          onr[1] = -dm * ni[0] + nr[3], oni[1] = dm * nr[0] + ni[3];
+          osr[1] = -dm * si[0] + sr[3], osi[1] = dm * sr[0] + si[3];
+        }
+        const double sN = u * (a.rs ? a.rs[jN] : 1.0), sS = u * (a.rs ? a.rs[jS] : 1.0);
+#pragma unroll
+        for (int o = 0; o < NOUT; o++) {
+          float2* G = (o == 0 ? a.G0 : a.G1) + (long)m * a.gmst;
+          G[(long)jN * a.gld + c] = make_float2((float)(onr[o] * sN), m ? (float)(oni[o] * sN) : 0.f);
+          G[(long)jS * a.gld + c] = make_float2((float)(osr[o] * sS), m ? (float)(osi[o] * sS) : 0.f);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- forward contraction
+struct Mx2FwdArgs {
+  const float2* G0;
+  const float2* G1;
+  long gld, gmst;
+  int ncols, CT;
+  double* W0;
+  double* W1;
+  long sl, sc;
+  const float* cw;
+  const unsigned* colmax;
+  float bound;
+  const double* diag0;
+  const double* diag1;
+  double scale;
+  const uint16_t* tP[3];
+  const uint16_t* tH[3];
+  const long* off;
+  const double* minv;
+  int N, J2, J2p32;
+};
+
+// block = 4 waves of one wavenumber m and one column tile; the row tiles of each parity
+// are cut into slots of MAXT tiles and the waves take slots in turn.  Rows of parity 0 pair
+// the P table with the symmetric combination G_N + G_S and H with G_N - G_S; parity 1 the
+// other way round.
+template <int OP, int FORM, int MAXT>
+__global__ __launch_bounds__(256) void k_mx2_fwd(Mx2FwdArgs a) {
+  using F = MxForm<FORM>;
+  using X = Mx2<FORM>;
+  using T = typename F::T;
+  using V8 = typename F::V8;
+  constexpr int NG = fwd_ng(OP), NIN = FwdTraits<OP>::NIN, NOUT = FwdTraits<OP>::NOUT;
+  constexpr int NTA = X::NTA, NTB = X::NTB;
+  constexpr bool UP = FwdTraits<OP>::USE_P, UH = FwdTraits<OP>::USE_H;
+  int m, ct;
+  if (!xcd_decode(a.N, a.CT, m, ct)) return;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nE = n_par(a.N, m, 0), nO = n_par(a.N, m, 1);
+  const int tE = cdiv(nE, TILE), tO = cdiv(nO, TILE);
+  const int sE = cdiv(tE, MAXT), sO = cdiv(tO, MAXT);
+  const int KS = a.J2p32 / KM;
+  const long toff = a.off[m];
+  const int c = ct * TILE + (lane & 15);
+  const bool cok = c < a.ncols;
+  float dsc = 1.f;
+  if constexpr (F::SCALED) {
+    if (cok) dsc = colscale<NIN>(a.colmax, a.ncols, c, a.bound);
+  }
+  double u = a.minv[m] * X::DSCALE * a.scale;
+  if constexpr (F::SCALED) u /= (double)dsc;
+  const float2* Gm[2] = {a.G0 + (long)m * a.gmst, NIN > 1 ? a.G1 + (long)m * a.gmst : nullptr};
+  const double dm = (double)m;
+
+  auto slot = [&](auto PAR, int tp0) {
+    constexpr int PP = decltype(PAR)::value;
+    const int tpar = PP ? tO : tE;
+    float4_t acc[NG][MAXT][2];
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+#pragma unroll
+      for (int tt = 0; tt < MAXT; tt++) acc[g][tt][0] = acc[g][tt][1] = float4_t{0, 0, 0, 0};
+    for (int ks = 0; ks < KS; ks++) {
+      V8 A[2][MAXT][NTA];
+#pragma unroll
+      for (int tt = 0; tt < MAXT; tt++) {
+        const int tp = tp0 + tt;
+        const bool ok = tp < tpar;
+        const long o = toff + ((long)(PP ? tE + tp : tp) * KS + ks) * 512 + lane * 8;
+#pragma unroll
+        for (int k = 0; k < NTA; k++) {
+          if constexpr (UP) A[0][tt][k] = ok ? *reinterpret_cast<const V8*>(a.tP[k] + o) : V8{};
+          if constexpr (UH) A[1][tt][k] = ok ? *reinterpret_cast<const V8*>(a.tH[k] + o) : V8{};
+        }
+      }
+      // data fragments: [input][q: 0 = combination for P, 1 = for H][re, im][term]
+      V8 B[NIN][2][2][NTB];
+#pragma unroll
+      for (int i = 0; i < NIN; i++) {
+        T part[2][2][NTB][8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          const int jn = ks * KM + 8 * (lane >> 4) + e;
+          float2 gn = make_float2(0.f, 0.f), gs = make_float2(0.f, 0.f);
+          if (jn < a.J2 && cok) {
+            gn = Gm[i][(long)(a.J2 + jn) * a.gld + c];
+            gs = Gm[i][(long)(a.J2 - 1 - jn) * a.gld + c];
+          }
+          const float cw = a.cw[jn] * dsc;
+          const float symr = cw * (gn.x + gs.x), symi = cw * (gn.y + gs.y);
+          const float antr = cw * (gn.x - gs.x), anti = cw * (gn.y - gs.y);
+          // P pairs with sym at parity 0, H with anti; swapped at parity 1
+          const float vq[2][2] = {{PP ? antr : symr, PP ? anti : symi}, {PP ? symr : antr, PP ? symi : anti}};
+#pragma unroll
+          for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+              T tmp[NTB];
+              bsplit<FORM>(vq[q][r], tmp);
+#pragma unroll
+              for (int t = 0; t < NTB; t++) part[q][r][t][e] = tmp[t];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+          for (int r = 0; r < 2; r++)
+#pragma unroll
+            for (int t = 0; t < NTB; t++) {
+              V8 v;
+#pragma unroll
+              for (int e = 0; e < 8; e++) v[e] = part[q][r][t][e];
+              B[i][q][r][t] = v;
+            }
+      }
+#pragma unroll
+      for (int gi = 0; gi < NG; gi++) {
+        const int tb = fwd_tab(OP, gi), in = fwd_in(OP, gi);
+#pragma unroll
+        for (int tt = 0; tt < MAXT; tt++)
+#pragma unroll
+          for (int r = 0; r < 2; r++)
+#pragma unroll
+            for (int pr = 0; pr < X::NP; pr++)
+              acc[gi][tt][r] = mfma16(A[tb][tt][X::pa(pr)], B[in][tb][r][X::pb(pr)], acc[gi][tt][r]);
+      }
+    }
+    // ---- epilogue (FP64): combine, unscale, diagonal, scatter to SPMODEL order
+    if (!cok) return;
+#pragma unroll
+    for (int tt = 0; tt < MAXT; tt++) {
+      const int tp = tp0 + tt;
+      if (tp >= tpar) continue;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int k = tp * TILE + acc_row16(lane, i);
+        const int n = m + PP + 2 * k;
+        if (n > a.N) continue;
+        double xr[NG], xi[NG];
+#pragma unroll
+        for (int gi = 0; gi < NG; gi++) {
+          xr[gi] = acc[gi][tt][0][i];
+          xi[gi] = acc[gi][tt][1][i];
+        }
+        double orr[NOUT], oii[NOUT];
+        if constexpr (OP == F_VAL) {
+          orr[0] = xr[0], oii[0] = xi[0];
+        } else if constexpr (OP == F_DIVLAMBDA) {  // i m P.G
+          orr[0] = -dm * xi[0], oii[0] = dm * xr[0];
+        } else if constexpr (OP == F_DIVMU) {  // -H.G
+          orr[0] = -xr[0], oii[0] = -xi[0];
+        } else if constexpr (OP == F_ALPHA) {  // i m P.A - H.B
+          orr[0] = -dm * xi[0] - xr[1], oii[0] = dm * xr[0] - xi[1];
+        } else {  // vor = i m P.V + H.U ; div = i m P.U - H.V
+          orr[0] = -dm * xi[0] + xr[1], oii[0] = dm * xr[0] + xi[1];
+          orr[1] = -dm * xi[2] - xr[3], oii[1] = dm * xr[2] - xi[3];
+        }
+        const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
+#pragma unroll
+        for (int o = 0; o < NOUT; o++) {
+          const double* dg = o == 0 ? a.diag0 : a.diag1;
+          double* W = o == 0 ? a.W0 : a.W1;
+          W[lre * a.sl + (long)c * a.sc] = orr[o] * u * (dg ? dg[lre] : 1.0);
+          if (m > 0) W[lim * a.sl + (long)c * a.sc] = oii[o] * u * (dg ? dg[lim] : 1.0);
+        }
+      }
+    }
+  };
+
+  for (int s = w; s < sE + sO; s += 4) {
+    if (s < sE)
+      slot(std::integral_constant<int, 0>(), s * MAXT);
+    else
+      slot(std::integral_constant<int, 1>(), (s - sE) * MAXT);
+  }
+}
+
+// per-column max |pro[l] w(l, c)| (float bits), threads along columns
+__global__ __launch_bounds__(256) void k_spec_colmax2(const double* __restrict__ w, long sl, long sc, int L,
+                                                      int ncols, const double* __restrict__ pro,
+                                                      unsigned* __restrict__ out) {
+  __shared__ float red[256];
+  const int tid = threadIdx.x;
+  const int c = blockIdx.x * 64 + (tid & 63);
+  float mx = 0.f;
+  if (c < ncols) {
+    for (int l = blockIdx.y * 4 + (tid >> 6); l < L; l += 4 * gridDim.y) {
+      double v = w[(long)l * sl + (long)c * sc];
+      if (pro) v *= pro[l];
+      mx = fmaxf(mx, fabsf((float)v));
+    }
+  }
+  red[tid] = mx;
+  __syncthreads();
+  if (tid < 64) {
+    mx = fmaxf(fmaxf(red[tid], red[tid + 64]), fmaxf(red[tid + 128], red[tid + 192]));
+    if (c < ncols) atomicMax(out + c, __float_as_uint(mx));
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+// tile shapes per operator: one-group operators take 2x2 tiles per wave, the two- and
+// four-group ones fewer so that their accumulators stay within the register budget
+template <int OP>
+struct InvShape {
+  static constexpr int NCT = inv_ng(OP) == 4 ? 1 : 2;
+  static constexpr int MAXT = inv_ng(OP) == 1 ? 2 : 1;
+};
+template <int OP>
+struct FwdShape {
+  static constexpr int MAXT = fwd_ng(OP) == 1 ? 2 : 1;
+};
+
+template <int OP, int FORM>
+int inv_launch(const dsht_plan* p, const MixedTables& t, const Mx2InvJob& j, hipStream_t s) {
+  constexpr int NIN = InvTraits<OP>::NIN;
+  Mx2InvArgs a;
+  a.W0 = j.W0;
+  a.W1 = j.W1;
+  a.sl = j.sl;
+  a.sc = j.sc;
+  a.ncols = j.ncols;
+  a.CT = cdiv(j.ncols, TILE);
+  a.pro = j.pro;
+  a.rs = j.rs;
+  a.colmax = j.colmax;
+  a.pack = j.pack;
+  a.G0 = j.G0;
+  a.G1 = j.G1;
+  a.gld = j.gld;
+  a.gmst = j.gmst;
+  for (int k = 0; k < 3; k++) {
+    a.tP[k] = t.iP[k];
+    a.tH[k] = t.iH[k];
+  }
+  a.off = t.d_offI;
+  a.cbase = t.d_cbase;
+  a.chunk_m = t.d_chunk_m;
+  a.nchunks = t.nchunks;
+  a.minv = t.d_minv;
+  a.N = p->N;
+  a.J2 = p->J2;
+  a.J2p32 = t.J2p32;
+  a.RT = t.J2p32 / TILE;
+  constexpr int NCT = InvShape<OP>::NCT, MAXT = InvShape<OP>::MAXT;
+  a.CG = cdiv(a.CT, NCT);
+  a.RG = cdiv(a.RT, 4 * MAXT);
+  const long pw = (long)t.nchunks * a.CT * NIN;  // packing waves
+  hipLaunchKernelGGL((k_mx2_ipack<FORM, NIN>), dim3((unsigned)cdivl(pw, 4)), dim3(256), 0, s, a);
+  DSHT_HIP(hipGetLastError());
+  const long nb = 8l * cdiv(p->N + 1, 8) * a.CG * a.RG;
+  hipLaunchKernelGGL((k_mx2_inv<OP, FORM, NCT, MAXT>), dim3((unsigned)nb), dim3(256), 0, s, a);
+  DSHT_HIP(hipGetLastError());
+  return 0;
+}
+
+template <int OP, int FORM>
+int fwd_launch(const dsht_plan* p, const MixedTables& t, const Mx2FwdJob& j, hipStream_t s) {
+  Mx2FwdArgs a;
+  a.G0 = j.G0;
+  a.G1 = j.G1;
+  a.gld = j.gld;
+  a.gmst = j.gmst;
+  a.ncols = j.ncols;
+  a.CT = cdiv(j.ncols, TILE);
+  a.W0 = j.W0;
+  a.W1 = j.W1;
+  a.sl = j.sl;
+  a.sc = j.sc;
+  a.cw = j.cw;
+  a.colmax = j.colmax;
+  a.bound = j.bound;
+  a.diag0 = j.diag0;
+  a.diag1 = j.diag1;
+  a.scale = j.scale;
+  for (int k = 0; k < 3; k++) {
+    a.tP[k] = t.fP[k];
+    a.tH[k] = t.fH[k];
+  }
+  a.off = t.d_offF;
+  a.minv = t.d_minv;
+  a.N = p->N;
+  a.J2 = p->J2;
+  a.J2p32 = t.J2p32;
+  const long nb = 8l * cdiv(p->N + 1, 8) * a.CT;
+  hipLaunchKernelGGL((k_mx2_fwd<OP, FORM, FwdShape<OP>::MAXT>), dim3((unsigned)nb), dim3(256), 0, s, a);
+  DSHT_HIP(hipGetLastError());
+  return 0;
+}
+
+template <int FORM>
+int inv_form(const dsht_plan* p, const MixedTables& t, int op, const Mx2InvJob& j, hipStream_t s) {
+  switch (op) {
+    case I_VAL: return inv_launch<I_VAL, FORM>(p, t, j, s);
+    case I_GRADLAMBDA: return inv_launch<I_GRADLAMBDA, FORM>(p, t, j, s);
+    case I_GRADMU: return inv_launch<I_GRADMU, FORM>(p, t, j, s);
+    case I_ALPHA: return inv_launch<I_ALPHA, FORM>(p, t, j, s);
+    case I_VORDIV2UV: return inv_launch<I_VORDIV2UV, FORM>(p, t, j, s);
+  }
+  return fail(DSHT_EARG, "bad inverse op %d", op);
+}
+
+template <int FORM>
+int fwd_form(const dsht_plan* p, const MixedTables& t, int op, const Mx2FwdJob& j, hipStream_t s) {
+  switch (op) {
+    case F_VAL: return fwd_launch<F_VAL, FORM>(p, t, j, s);
+    case F_DIVLAMBDA: return fwd_launch<F_DIVLAMBDA, FORM>(p, t, j, s);
+    case F_DIVMU: return fwd_launch<F_DIVMU, FORM>(p, t, j, s);
+    case F_ALPHA: return fwd_launch<F_ALPHA, FORM>(p, t, j, s);
+    case F_UV2VORDIV: return fwd_launch<F_UV2VORDIV, FORM>(p, t, j, s);
+  }
+  return fail(DSHT_EARG, "bad forward op %d", op);
+}
+
+}  // namespace
+
+size_t mx2_inv_pack_bytes(const MixedTables& t, int form, int op, int ncols) {
+  const int ntb = form == 1 ? 3 : (form == 2 ? 1 : (form == 3 ? 3 : 2));
+  const int nin = (op == I_ALPHA || op == I_VORDIV2UV) ? 2 : 1;
+  return (size_t)t.nchunks * cdiv(ncols, TILE) * nin * 2 * ntb * 512 * sizeof(uint16_t);
+}
+
+int mx2_inv(const dsht_plan* p, const MixedTables& t, int form, int op, const Mx2InvJob& j, hipStream_t s) {
+  switch (form) {
+    case 0: return inv_form<0>(p, t, op, j, s);
+    case 1: return inv_form<1>(p, t, op, j, s);
+    case 2: return inv_form<2>(p, t, op, j, s);
+    case 3: return inv_form<3>(p, t, op, j, s);
+  }
+  return fail(DSHT_EARG, "bad form %d", form);
+}
+
+int mx2_fwd(const dsht_plan* p, const MixedTables& t, int form, int op, const Mx2FwdJob& j, hipStream_t s) {
+  switch (form) {
+    case 0: return fwd_form<0>(p, t, op, j, s);
+    case 1: return fwd_form<1>(p, t, op, j, s);
+    case 2: return fwd_form<2>(p, t, op, j, s);
+    case 3: return fwd_form<3>(p, t, op, j, s);
+  }
+  return fail(DSHT_EARG, "bad form %d", form);
+}
+
+int launch_spec_colmax(const double* w, long sl, long sc, int L, int ncols, const double* pro, unsigned* out,
+                       hipStream_t s) {
+  dim3 grid(cdiv(ncols, 64), 32);
+  hipLaunchKernelGGL(k_spec_colmax2, grid, dim3(256), 0, s, w, sl, sc, L, ncols, pro, out);
+  DSHT_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace dsht

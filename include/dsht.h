@@ -163,6 +163,7 @@ int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, d
 #define DSHT_TUNE_CHUNK_COLS 0 /* 0 = automatic */
 #define DSHT_TUNE_LEG_CB 1     /* 0 = automatic, 16 or 32 */
 #define DSHT_TUNE_FFT 2        /* 0 = register two-pass kernel where the length has one, 1 = LDS Stockham */
+#define DSHT_TUNE_LEG 3        /* split-precision contractions: 0 = per-wave fragment streams, 1 = LDS-staged */
 int dsht_set_tuning(dsht_plan* plan, int key, int value);
 int dsht_get_tuning(const dsht_plan* plan, int key);
 
