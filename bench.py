@@ -73,10 +73,17 @@ def rel_l2(a, b):
     return d / n if n > 0 else d
 
 
-def time_steps(torch, plan, dsht, bufs, steps, warmup):
+def time_steps(torch, plan, dsht, bufs, steps, warmup, min_warm_s=0.3):
+    """W warmup steps (continued until `min_warm_s` of GPU work so clocks have settled),
+    then K timed steps back to back; per-step event pairs give the latency distribution."""
     stream = torch.cuda.current_stream().cuda_stream
-    for _ in range(warmup):
+    t0 = time.perf_counter()
+    done = 0
+    while done < warmup or time.perf_counter() - t0 < min_warm_s:
         run_step(plan, dsht, bufs, stream)
+        done += 1
+        if done % 8 == 0:
+            torch.cuda.synchronize()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t0 = time.perf_counter()
@@ -147,6 +154,193 @@ def oracle_check(torch, spec, g_out, NCHK=3):
     return float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
 
 
+def chunk_columns(form):
+    """Column chunk the library picks by default: ~48 MiB of Fourier scratch per chunk."""
+    per = (NMAX + 1) * JM * (16 if form == "fp64" else 8)
+    return max(32, (48 << 20) // per // 32 * 32)
+
+
+def traffic_table(path=os.path.join(ROOT, "profiles", "hbm_traffic_T170L60.md")):
+    """Measured HBM bytes per full-chunk launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-
+    calibrated) from the committed profile summary: {(form, scope): bytes}."""
+    out = {}
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 4 and f[0] == "traffic":
+                out[(f[1], f[2])] = float(f[3])
+    except OSError:
+        pass
+    return out
+
+
+def roofline_probe(torch, plan, dsht, form, scope, bufs, reps=10):
+    """Average duration and algorithmic bytes/flops of `scope` on full-chunk launches of the
+    step's own call (the launch shape the committed counter profile measured)."""
+    spec, g_out, tend, spec_t = bufs
+    stream = torch.cuda.current_stream().cuda_stream
+    nc = chunk_columns(form)
+    pid = {v: k for k, v in dsht.PREC_NAMES.items()}[form]
+    plan.set_precision(pid)
+    inv = scope.endswith("inv")
+
+    def call():
+        if inv:
+            plan.inv_d(dsht.VAL, spec[:, :nc], g_out[0][:nc], stream=stream)
+        else:
+            plan.fwd_d(dsht.VAL, tend[:nc], spec_t[:, :nc], stream=stream)
+
+    for _ in range(3):
+        call()
+    torch.cuda.synchronize()
+    dsht.prof_reset()
+    dsht.prof_enable(True)
+    for _ in range(reps):
+        call()
+    torch.cuda.synchronize()
+    dsht.prof_enable(False)
+    ms, cnt, by, fl = dsht.prof_query(scope)
+    plan.set_precision(dsht.PREC_FP64)
+    t = ms * 1e-3 / cnt
+    gbs, tfs = by / cnt / t / 1e9, fl / cnt / t / 1e12
+    mpeak = FP64_PEAK_TFS if form == "fp64" else 2500.0
+    if scope.startswith("leg") and tfs / mpeak > gbs / HBM_PEAK_GBS:
+        roof = {"bound": "mfma", "achieved": tfs, "peak": mpeak, "unit": "TFLOP/s", "frac": tfs / mpeak}
+    else:
+        roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+    roof["traffic"] = traffic_table().get((form, scope))
+    roof.update({"kernel": scope, "columns_per_launch": nc, "avg_launch_us": 1e6 * t,
+                 "algorithmic_bytes_per_launch": by / cnt, "algorithmic_flops_per_launch": fl / cnt,
+                 "traffic_source": "profiles/hbm_traffic_T170L60.md" if roof["traffic"] else None})
+    return roof
+
+
+# ---------------------------------------------------------------- batched form
+BATCH_TRUNC = (21, 42, 85, 170)
+BATCH_KM = (20, 30, 60)
+# IM = 3(N+1) rounded up to a length with a register FFT (M = IM/2 = R1*R2), JM = IM/2 made even
+BATCH_GRIDS = {21: (72, 36), 42: (144, 72), 85: (270, 136), 170: (540, 270)}
+
+
+def batched_bench(torch, dsht, n_inst=64, steps=10, seq_steps=2, seed=0xB47C4, prec="bf16x3"):
+    """64 independent instances of uneven truncation and layer count advanced by one
+    hipGraph-captured batched step (tile/chunk knobs from the autotuner), against the same
+    instances run as sequential per-instance calls.  Error of the split form against the
+    FP64 batched step (itself checked against the oracle in the test suite)."""
+    from dsht import autotune
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(seed)
+    inst = [(int(rng.choice(BATCH_TRUNC)), int(rng.choice(BATCH_KM))) for _ in range(n_inst)]
+    plans = {N: dsht.Plan(N, *BATCH_GRIDS[N], radius=RADIUS) for N in sorted({n for n, _ in inst})}
+    b = dsht.Batch()
+    ids = [b.add(plans[N], km) for N, km in inst]
+    b.finalize()
+    groups = {}
+    for iid, (N, km) in zip(ids, inst):
+        g, _, gc, nbytes = b.info(iid)
+        groups[g] = (N, gc)
+    views = {}
+    for g, (N, gc) in groups.items():
+        p = plans[N]
+        spec, grid, tend, spect = b.group_views(g, p, gc)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed + g)
+        n = torch.floor(torch.sqrt(torch.arange(p.lm, device=dev, dtype=torch.float64)))
+        spec.copy_(torch.randn(tuple(spec.shape), generator=gen, device=dev, dtype=torch.float64)
+                   * torch.sqrt(0.5 * (n + 1.0) ** -3)[:, None])
+        tend.copy_(torch.randn(tuple(tend.shape), generator=gen, device=dev, dtype=torch.float64))
+        views[g] = (p, gc, spec, grid, tend, spect)
+    units = sum(km for _, km in inst) * (N_PROG * len(INV_KINDS) + N_TEND)
+    s = torch.cuda.Stream()
+    ss = s.cuda_stream
+    # FP64 reference step
+    for p in plans.values():
+        p.set_precision(dsht.PREC_FP64)
+    b.step(ss)
+    s.synchronize()
+    ref = {g: (v[3].clone(), v[5].clone()) for g, v in views.items()}
+    pid = {v: k for k, v in dsht.PREC_NAMES.items()}[prec]
+    tuning = {}
+    with torch.cuda.stream(s):
+        for g, (p, gc, *_r) in views.items():
+            p.set_precision(pid)
+            t = autotune.tune(p, N_PROG * gc, use_cache=False)
+            tuning[f"T{p.nmax}"] = {"chunk": t["chunk"], "cb": t["cb"], "columns": N_PROG * gc}
+    s.synchronize()
+    b.step(ss)
+    s.synchronize()
+    err = {}
+    for g, (p, gc, spec, grid, tend, spect) in views.items():
+        err[f"T{p.nmax}"] = {"grid": rel_l2(grid, ref[g][0]), "spectral": rel_l2(spect, ref[g][1])}
+    b.capture(ss)
+
+    def timed(fn, k):
+        for _ in range(2):
+            fn()
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        s.synchronize()
+        return (time.perf_counter() - t0) / k
+
+    t_graph = timed(lambda: b.launch(ss), steps)
+    t_eager = timed(lambda: b.step(ss), steps)
+    t_seq = timed(lambda: b.step_sequential(ss), seq_steps)
+    b.close()
+    return {
+        "workload": f"{n_inst} instances, truncation from {list(BATCH_TRUNC)}, KM from {list(BATCH_KM)}, "
+                    "grids " + ", ".join(f"T{n}: {i}x{j}" for n, (i, j) in BATCH_GRIDS.items()) +
+                    "; per instance and step the T170L60 transform set (12 inverse + 6 forward units/layer)",
+        "precision": prec,
+        "instances": [{"nmax": n, "km": km} for n, km in inst],
+        "units_per_step": units,
+        "graph_pairs_per_s": units / 2 / t_graph,
+        "graph_ms_per_step": 1e3 * t_graph,
+        "eager_batched_ms_per_step": 1e3 * t_eager,
+        "sequential_ms_per_step": 1e3 * t_seq,
+        "sequential_pairs_per_s": units / 2 / t_seq,
+        "speedup_vs_sequential": t_seq / t_graph,
+        "rel_l2_vs_fp64": err,
+        "autotuned": tuning,
+    }
+
+
+# ---------------------------------------------------------------- serving form
+def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E):
+    """Poisson arrivals of independent runs (truncation, KM, rotation rate drawn per run),
+    served with continuous batching and, separately, one run at a time."""
+    from dsht import serve
+
+    def stream():
+        return serve.poisson_stream(n_req, rate, seed=seed, nsteps=nsteps)
+
+    out = {}
+    for mode in ("batched", "one_at_a_time"):
+        srv = serve.Server()
+        out[mode] = srv.run(stream(), batched=(mode == "batched"))
+        srv = None
+        torch.cuda.synchronize()
+    # the largest run alone: its step time is the latency reference
+    big = max(stream(), key=lambda r: (r.nmax, r.km))
+    solo = serve.Server()
+    r = serve.Request(0, big.nmax, big.km, 1.0, 30, 0.0, 1)
+    solo.run([r], batched=True)
+    solo_ms = float(np.median(r.step_ms[5:]))
+    b, u = out["batched"], out["one_at_a_time"]
+    return {
+        "workload": f"{n_req} runs, Poisson arrivals at {rate:g}/s, {nsteps} steps each, truncation from "
+                    f"{list(serve.GRIDS)}, KM from [20, 30, 60], rotation 0.25-4x Earth; FP64 transforms",
+        "batched": b,
+        "one_at_a_time": u,
+        "throughput_ratio": b["pairs_per_s"] / u["pairs_per_s"],
+        "largest_run_alone_step_ms": solo_ms,
+        "largest_run": {"nmax": big.nmax, "km": big.km},
+        "p99_over_largest_alone": b["p99_step_ms"] / solo_ms,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +349,10 @@ def main():
     ap.add_argument("--prec", default="auto")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device copy 3 times (HBM counter calibration) and exit")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: the transform path has no "
@@ -162,10 +360,20 @@ def main():
         sys.exit(2)
 
     import torch
-    import dsht
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        a = torch.ones(32 << 20, dtype=torch.float64, device=dev)  # 256 MiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8, "launches": 3}))
+        return
+
+    import dsht
+
     print(f"[bench] T{NMAX} I={IM} J={JM} KM={KM}: building plan", file=sys.stderr, flush=True)
     plan = dsht.Plan(NMAX, IM, JM, kmax=KM, radius=RADIUS)
     L = plan.lm
@@ -219,20 +427,20 @@ def main():
     ok = [f for f in results if f == "fp64" or max(results[f]["errors"].values()) <= TOL_MIXED]
     best = max(ok, key=lambda f: results[f]["pairs_per_s"])
     r = results[best]
-    # roofline of the dominant kernel of the reported form
-    kern = max(r["kernels"].items(), key=lambda kv: kv[1]["ms_total"])
-    kname, k = kern
-    hbm_frac = k["GBps"] / HBM_PEAK_GBS
-    mfma_frac = k["TFLOPs"] / FP64_PEAK_TFS if best == "fp64" else k["TFLOPs"] / 2500.0
-    if kname.startswith("leg") and mfma_frac > hbm_frac:
-        roof = {"kernel": kname, "bound": "mfma", "achieved": k["TFLOPs"],
-                "peak": FP64_PEAK_TFS if best == "fp64" else 2500.0, "unit": "TFLOP/s", "frac": mfma_frac}
-    else:
-        roof = {"kernel": kname, "bound": "hbm", "achieved": k["GBps"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": hbm_frac}
-    roof["traffic"] = None
-    roof["avg_launch_us"] = k["avg_us"]
-    roof["algorithmic_bytes_per_launch"] = k["bytes_per_launch"]
+    # roofline of the dominant kernel of the reported form, probed on full-chunk launches
+    kname = max(r["kernels"].items(), key=lambda kv: kv[1]["ms_total"])[0]
+    roof = roofline_probe(torch, plan, dsht, best, kname, bufs)
+    batched = serving = None
+    if not args.no_batch:
+        print("[bench] batched form (64 instances)", file=sys.stderr, flush=True)
+        batched = batched_bench(torch, dsht)
+        print(f"[bench] batched: graph {batched['graph_ms_per_step']:.2f} ms/step, sequential "
+              f"{batched['sequential_ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    if not args.no_serve:
+        print("[bench] serving form", file=sys.stderr, flush=True)
+        serving = serving_bench(torch, dsht)
+        print(f"[bench] serving: throughput ratio {serving['throughput_ratio']:.2f}, p99 "
+              f"{serving['batched']['p99_step_ms']:.2f} ms", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -266,6 +474,8 @@ def main():
         "mixed_speedup_vs_fp64": (r["pairs_per_s"] / results["fp64"]["pairs_per_s"]) if "fp64" in results else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line))
 

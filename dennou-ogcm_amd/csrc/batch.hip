@@ -239,11 +239,11 @@ int dsht_set_tuning(dsht_plan* p, int key, int value) {
       p->leg_cb = value;
       return 0;
     case DSHT_TUNE_FFT:
-      if (value != 0 && value != 1) return fail(DSHT_EARG, "fft kernel must be 0 or 1");
+      if (value < 0 || value > 2) return fail(DSHT_EARG, "fft kernel must be 0, 1 or 2");
       p->fft_kind = value;
       return 0;
     case DSHT_TUNE_LEG:
-      if (value != 0 && value != 1) return fail(DSHT_EARG, "contraction kernel must be 0 or 1");
+      if (value < 0 || value > 2) return fail(DSHT_EARG, "contraction kernel must be 0, 1 or 2");
       p->leg_kind = value;
       return 0;
   }

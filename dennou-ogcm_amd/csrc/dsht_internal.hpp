@@ -112,6 +112,7 @@ struct FftArgs {
   const double2* twI;
   int rows_per_block;
   const double* rowscale;  // optional per-latitude scale applied after the inverse (nullable)
+  int threads;             // workgroup size of the register kernels (128 or 256)
 };
 
 // ---------------------------------------------------------------- workspace
@@ -166,8 +167,10 @@ struct dsht_plan {
   void* mixed = nullptr;
   int chunk_cols = 0;  // column chunk of the device path (0 = auto)
   int leg_cb = 0;      // column block of the contractions (0 = auto, 16, 32)
-  int fft_kind = 0;    // 0 = register two-pass FFT where available, 1 = LDS Stockham
-  int leg_kind = 0;    // split-precision contractions: 0 = per-wave fragment streams, 1 = LDS-staged
+  int fft_kind = 0;    // 0 = register two-pass FFT (256-thread blocks) where available, 1 = LDS
+                       // Stockham, 2 = register FFT in 128-thread blocks
+  int leg_kind = 0;    // split-precision contractions: 0 = fragment streams over packed data
+                       // panels, 1 = LDS-staged, 2 = fragment streams splitting data in place
 };
 
 namespace dsht {

@@ -450,6 +450,7 @@ static FftArgs fft_args(const dsht_plan* p, long ldg, int ncols, long gld) {
   a.twI = p->fft.twI;
   a.rows_per_block = fft_rows_per_block(p->fft.M);
   a.rowscale = nullptr;
+  a.threads = p->fft_kind == 2 ? 128 : 256;
   return a;
 }
 
@@ -491,7 +492,7 @@ static int fft_fwd_any(const dsht_plan* p, const double* xy, long ldg, int ncols
   long nrows = (long)p->J * ncols;
   ProfScope ps("fft_fwd", s, (double)nrows * (8.0 * p->I + (double)sizeof(GT) * (p->N + 1)),
                (double)nrows * 2.5 * p->I * std::log2((double)p->I));
-  if (p->fft_kind == 0 && reg_fft_supported(a.M)) return reg_fft_fwd<GT>(xy, a, G, colmax, s);
+  if (p->fft_kind != 1 && reg_fft_supported(a.M)) return reg_fft_fwd<GT>(xy, a, G, colmax, s);
   switch (a.M) {
     case 16: return p2_fwd<16>(p, xy, a, G, colmax, s);
     case 32: return p2_fwd<32>(p, xy, a, G, colmax, s);
@@ -514,7 +515,7 @@ static int fft_inv_any(const dsht_plan* p, const GT* G, long gld, int ncols, dou
   long nrows = (long)p->J * ncols;
   ProfScope ps("fft_inv", s, (double)nrows * (8.0 * p->I + (double)sizeof(GT) * (p->N + 1)),
                (double)nrows * 2.5 * p->I * std::log2((double)p->I));
-  if (p->fft_kind == 0 && reg_fft_supported(a.M)) return reg_fft_inv<GT>(G, a, xy, s);
+  if (p->fft_kind != 1 && reg_fft_supported(a.M)) return reg_fft_inv<GT>(G, a, xy, s);
   switch (a.M) {
     case 16: return p2_inv<16>(p, G, a, xy, s);
     case 32: return p2_inv<32>(p, G, a, xy, s);

@@ -192,10 +192,10 @@ __device__ __forceinline__ void st_g(float2* p, double2 v) { *p = make_float2((f
 
 constexpr int odd_up(int x) { return x | 1; }
 
-template <int R1, int R2>
+template <int R1, int R2, int NT>
 struct RegFft {
   static constexpr int M = R1 * R2;
-  static constexpr int THREADS = 256;
+  static constexpr int THREADS = NT;
   static constexpr int ROWS = THREADS / R1;                      // rows per workgroup
   static constexpr int PY = (R2 % 2 == 0) ? R2 + 1 : R2 + 2;     // padded pass-1 stride (odd)
   static constexpr int SR = odd_up(R1 * PY > M + 1 ? R1 * PY : M + 1);  // padded row (odd)
@@ -204,10 +204,10 @@ struct RegFft {
 
 }  // namespace
 
-template <int R1, int R2, typename GT>
-__global__ __launch_bounds__(256) void k_fft_fwd_r(const double* __restrict__ xy, GT* __restrict__ G,
-                                                   FftArgs a, unsigned* __restrict__ colmax) {
-  using C = RegFft<R1, R2>;
+template <int R1, int R2, int NT, typename GT>
+__global__ __launch_bounds__(NT) void k_fft_fwd_r(const double* __restrict__ xy, GT* __restrict__ G,
+                                                  FftArgs a, unsigned* __restrict__ colmax) {
+  using C = RegFft<R1, R2, NT>;
   constexpr int M = C::M, ROWS = C::ROWS, PY = C::PY, SR = C::SR;
   extern __shared__ double2 lds[];
   __shared__ unsigned smax[ROWS];
@@ -287,10 +287,10 @@ __global__ __launch_bounds__(256) void k_fft_fwd_r(const double* __restrict__ xy
   }
 }
 
-template <int R1, int R2, typename GT>
-__global__ __launch_bounds__(256) void k_fft_inv_r(const GT* __restrict__ G, double* __restrict__ xy,
-                                                   FftArgs a) {
-  using C = RegFft<R1, R2>;
+template <int R1, int R2, int NT, typename GT>
+__global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, double* __restrict__ xy,
+                                                  FftArgs a) {
+  using C = RegFft<R1, R2, NT>;
   constexpr int M = C::M, ROWS = C::ROWS, PY = C::PY, SR = C::SR;
   extern __shared__ double2 lds[];
   double2* buf = lds;
@@ -307,17 +307,27 @@ __global__ __launch_bounds__(256) void k_fft_inv_r(const GT* __restrict__ G, dou
   const int rr1 = tid / R1, n1 = tid % R1;
   const int rr2 = tid / R2, k2 = tid % R2;
   for (long r0 = (long)blockIdx.x * ROWS; r0 < nrows; r0 += (long)gridDim.x * ROWS) {
-    __syncthreads();  // the previous rows are out of buf
-    for (int idx = tid; idx < M * ROWS; idx += C::THREADS) {
+    // Fourier coefficients of the ROWS rows: every load is issued before the first LDS
+    // store (consecutive threads take consecutive rows, zero above the truncation)
+    constexpr int NL = (M * ROWS + C::THREADS - 1) / C::THREADS;
+    double2 gv[NL];
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+      const int idx = tid + q * C::THREADS;
       const int m = idx / ROWS, rr = idx % ROWS;
       const long r = r0 + rr;
-      double2 g = make_double2(0.0, 0.0);
-      if (m <= N && r < nrows) {
+      gv[q] = make_double2(0.0, 0.0);
+      if (idx < M * ROWS && m <= N && r < nrows) {
         const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
-        g = ld_g(&G[(long)m * a.gmst + (long)j * a.gld + c]);
-        if (m == 0) g.y = 0.0;
+        gv[q] = ld_g(&G[(long)m * a.gmst + (long)j * a.gld + c]);
+        if (m == 0) gv[q].y = 0.0;
       }
-      buf[rr * SR + m] = g;
+    }
+    __syncthreads();  // the previous rows are out of buf
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+      const int idx = tid + q * C::THREADS;
+      if (idx < M * ROWS) buf[(idx % ROWS) * SR + idx / ROWS] = gv[q];
     }
     __syncthreads();
     // packed spectrum Z_m from the half-complex pair (X_m, X_{M-m}); X_M = 0 for m = 0
@@ -371,27 +381,40 @@ __global__ __launch_bounds__(256) void k_fft_inv_r(const GT* __restrict__ G, dou
 }
 
 // ---------------------------------------------------------------- dispatch
-template <int R1, int R2, typename GT>
-static int reg_fwd(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
-  using C = RegFft<R1, R2>;
+template <int R1, int R2, int NT, typename GT>
+static int reg_fwd_nt(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
+  using C = RegFft<R1, R2, NT>;
   const long nrows = (long)a.J * a.ncols;
   long nb = cdivl(nrows, C::ROWS);
-  if (nb > 8192) nb = 8192;
-  hipLaunchKernelGGL((k_fft_fwd_r<R1, R2, GT>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, xy, G, a,
+  if (nb > 16384) nb = 16384;
+  hipLaunchKernelGGL((k_fft_fwd_r<R1, R2, NT, GT>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, xy, G, a,
                      colmax);
   DSHT_HIP(hipGetLastError());
   return 0;
 }
 
-template <int R1, int R2, typename GT>
-static int reg_inv(const GT* G, const FftArgs& a, double* xy, hipStream_t s) {
-  using C = RegFft<R1, R2>;
+template <int R1, int R2, int NT, typename GT>
+static int reg_inv_nt(const GT* G, const FftArgs& a, double* xy, hipStream_t s) {
+  using C = RegFft<R1, R2, NT>;
   const long nrows = (long)a.J * a.ncols;
   long nb = cdivl(nrows, C::ROWS);
-  if (nb > 8192) nb = 8192;
-  hipLaunchKernelGGL((k_fft_inv_r<R1, R2, GT>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, G, xy, a);
+  if (nb > 16384) nb = 16384;
+  hipLaunchKernelGGL((k_fft_inv_r<R1, R2, NT, GT>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, G, xy, a);
   DSHT_HIP(hipGetLastError());
   return 0;
+}
+
+// workgroup size: 256 threads, or 128 (half the LDS per workgroup, more workgroups per CU)
+template <int R1, int R2, typename GT>
+static int reg_fwd(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
+  if (a.threads == 128 && R1 <= 128) return reg_fwd_nt<R1, R2, 128, GT>(xy, a, G, colmax, s);
+  return reg_fwd_nt<R1, R2, 256, GT>(xy, a, G, colmax, s);
+}
+
+template <int R1, int R2, typename GT>
+static int reg_inv(const GT* G, const FftArgs& a, double* xy, hipStream_t s) {
+  if (a.threads == 128 && R1 <= 128) return reg_inv_nt<R1, R2, 128, GT>(G, a, xy, s);
+  return reg_inv_nt<R1, R2, 256, GT>(G, a, xy, s);
 }
 
 // factor pairs (R1, R2) with M = R1 * R2 that have register kernels

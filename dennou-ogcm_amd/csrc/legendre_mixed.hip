@@ -873,7 +873,7 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
     double by, fl;
     mixed_work(p, op, true, nc, &by, &fl);
     ProfScope ps("leg_fwd", s, by, fl);
-    if (p->leg_kind == 0) {
+    if (p->leg_kind != 1) {
       Mx2FwdJob j;
       j.G0 = G0;
       j.G1 = nin > 1 ? G1 : nullptr;
@@ -965,7 +965,7 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
       double by, fl;
       mixed_work(p, op, false, nc, &by, &fl);
       ProfScope ps("leg_inv", s, by, fl);
-      if (p->leg_kind == 0) {
+      if (p->leg_kind != 1) {
         Mx2InvJob j;
         j.W0 = w0;
         j.W1 = w1;
@@ -975,7 +975,7 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
         j.pro = pro;
         j.rs = rs;
         j.colmax = colmax;
-        j.pack = pack;
+        j.pack = p->leg_kind == 0 ? pack : nullptr;
         j.G0 = G0;
         j.G1 = nout > 1 ? G1 : nullptr;
         j.gld = nc;

@@ -14,6 +14,7 @@
 // factors, signs, north/south recombination and unscaling happen once in the epilogue.
 // Blocks are placed so that all blocks of one zonal wavenumber run on one XCD (shared
 // table and data panels stay in that XCD's L2) and wavenumbers alternate across XCDs.
+#include <algorithm>
 #include <cmath>
 
 #include "legendre_mixed.hpp"
@@ -182,10 +183,11 @@ __global__ __launch_bounds__(256) void k_mx2_ipack(Mx2InvArgs a) {
 // MAXT latitude tiles starting at (rg*4 + w)*MAXT.  K runs over the parity-0 chunks, then
 // the parity-1 chunks; P of parity p accumulates into sym (p = 0) or anti (p = 1), H the
 // opposite; north = sym + anti, south = sym - anti.
-template <int OP, int FORM, int NCT, int MAXT>
+template <int OP, int FORM, int NCT, int MAXT, bool PACKED>
 __global__ __launch_bounds__(256) void k_mx2_inv(Mx2InvArgs a) {
   using F = MxForm<FORM>;
   using X = Mx2<FORM>;
+  using T = typename F::T;
   using V8 = typename F::V8;
   constexpr int NG = inv_ng(OP), NIN = InvTraits<OP>::NIN, NOUT = InvTraits<OP>::NOUT;
   constexpr int NTA = X::NTA, NTB = X::NTB;
@@ -202,6 +204,15 @@ __global__ __launch_bounds__(256) void k_mx2_inv(Mx2InvArgs a) {
   const long toff = a.off[m];
   const int gc0 = a.cbase[m];
   const int ct0 = cg * NCT;
+  float dsc[NCT];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ct++) {
+    const int c = (ct0 + ct) * TILE + (lane & 15);
+    dsc[ct] = 1.f;
+    if constexpr (F::SCALED && !PACKED) {
+      if (c < a.ncols) dsc[ct] = colscale<NIN>(a.colmax, a.ncols, c, 1.f);
+    }
+  }
 
   float4_t acc[NG][MAXT][NCT][2][2];
 #pragma unroll
@@ -230,19 +241,60 @@ __global__ __launch_bounds__(256) void k_mx2_inv(Mx2InvArgs a) {
         }
       }
       V8 B[NIN][NCT][2][NTB];
-      const long g = gc0 + (PP ? nch0 : 0) + ks;
+      if constexpr (PACKED) {
+        const long g = gc0 + (PP ? nch0 : 0) + ks;
 #pragma unroll
-      for (int i = 0; i < NIN; i++)
+        for (int i = 0; i < NIN; i++)
 #pragma unroll
-        for (int ct = 0; ct < NCT; ct++) {
-          const int ctg = ct0 + ct;
-          const uint16_t* src = a.pack + (((g * a.CT + ctg) * NIN + i) * 2 * NTB) * 512 + lane * 8;
+          for (int ct = 0; ct < NCT; ct++) {
+            const int ctg = ct0 + ct;
+            const uint16_t* src = a.pack + (((g * a.CT + ctg) * NIN + i) * 2 * NTB) * 512 + lane * 8;
 #pragma unroll
-          for (int r = 0; r < 2; r++)
+            for (int r = 0; r < 2; r++)
+#pragma unroll
+              for (int t = 0; t < NTB; t++)
+                B[i][ct][r][t] = ctg < a.CT ? *reinterpret_cast<const V8*>(src + (long)(r * NTB + t) * 512) : V8{};
+          }
+      } else {
+        // split the FP64 coefficients in place: lane (c, k-group) reads 8 total wavenumbers
+        const int np = PP ? nO : nE;
+#pragma unroll
+        for (int i = 0; i < NIN; i++) {
+          const double* W = i ? a.W1 : a.W0;
+#pragma unroll
+          for (int ct = 0; ct < NCT; ct++) {
+            const int c = (ct0 + ct) * TILE + (lane & 15);
+            T tre[NTB][8], tim[NTB][8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+              const int k = ks * KM + 8 * (lane >> 4) + e;
+              double vre = 0.0, vim = 0.0;
+              if (k < np && c < a.ncols) {
+                const int n = m + PP + 2 * k;
+                const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
+                const double pr = a.pro ? a.pro[lre] : 1.0;
+                vre = pr * W[lre * a.sl + (long)c * a.sc];
+                if (m > 0) vim = pr * W[lim * a.sl + (long)c * a.sc];
+              }
+              T t1[NTB], t2[NTB];
+              bsplit<FORM>((float)vre * dsc[ct], t1);
+              bsplit<FORM>((float)vim * dsc[ct], t2);
+#pragma unroll
+              for (int t = 0; t < NTB; t++) {
+                tre[t][e] = t1[t];
+                tim[t][e] = t2[t];
+              }
+            }
 #pragma unroll
             for (int t = 0; t < NTB; t++)
-              B[i][ct][r][t] = ctg < a.CT ? *reinterpret_cast<const V8*>(src + (long)(r * NTB + t) * 512) : V8{};
+#pragma unroll
+              for (int e = 0; e < 8; e++) {
+                B[i][ct][0][t][e] = tre[t][e];
+                B[i][ct][1][t][e] = tim[t][e];
+              }
+          }
         }
+      }
 #pragma unroll
       for (int gi = 0; gi < NG; gi++) {
         const int tb = inv_tab(OP, gi), in = inv_in(OP, gi);
@@ -502,28 +554,40 @@ __global__ __launch_bounds__(256) void k_spec_colmax2(const double* __restrict__
   __shared__ float red[256];
   const int tid = threadIdx.x;
   const int c = blockIdx.x * 64 + (tid & 63);
-  float mx = 0.f;
+  float mx[4] = {0.f, 0.f, 0.f, 0.f};
   if (c < ncols) {
-    for (int l = blockIdx.y * 4 + (tid >> 6); l < L; l += 4 * gridDim.y) {
+    const int step = 4 * gridDim.y;
+    int l = blockIdx.y * 4 + (tid >> 6);
+    for (; l + 3 * step < L; l += 4 * step) {  // four independent loads in flight
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        double v = w[(long)(l + u * step) * sl + (long)c * sc];
+        if (pro) v *= pro[l + u * step];
+        mx[u] = fmaxf(mx[u], fabsf((float)v));
+      }
+    }
+    for (; l < L; l += step) {
       double v = w[(long)l * sl + (long)c * sc];
       if (pro) v *= pro[l];
-      mx = fmaxf(mx, fabsf((float)v));
+      mx[0] = fmaxf(mx[0], fabsf((float)v));
     }
   }
-  red[tid] = mx;
+  red[tid] = fmaxf(fmaxf(mx[0], mx[1]), fmaxf(mx[2], mx[3]));
   __syncthreads();
   if (tid < 64) {
-    mx = fmaxf(fmaxf(red[tid], red[tid + 64]), fmaxf(red[tid + 128], red[tid + 192]));
-    if (c < ncols) atomicMax(out + c, __float_as_uint(mx));
+    const float m2 = fmaxf(fmaxf(red[tid], red[tid + 64]), fmaxf(red[tid + 128], red[tid + 192]));
+    if (c < ncols) atomicMax(out + c, __float_as_uint(m2));
   }
 }
+
 
 // ---------------------------------------------------------------- launchers
 // tile shapes per operator: one-group operators take 2x2 tiles per wave, the two- and
 // four-group ones fewer so that their accumulators stay within the register budget
-template <int OP>
+// (NCT: column tiles per wave; the plan's column-block knob selects 1 tile for 16, else 2)
+template <int OP, int NCT_>
 struct InvShape {
-  static constexpr int NCT = inv_ng(OP) == 4 ? 1 : 2;
+  static constexpr int NCT = inv_ng(OP) == 4 ? 1 : NCT_;
   static constexpr int MAXT = inv_ng(OP) == 1 ? 2 : 1;
 };
 template <int OP>
@@ -531,7 +595,7 @@ struct FwdShape {
   static constexpr int MAXT = fwd_ng(OP) == 1 ? 2 : 1;
 };
 
-template <int OP, int FORM>
+template <int OP, int FORM, int NCT_>
 int inv_launch(const dsht_plan* p, const MixedTables& t, const Mx2InvJob& j, hipStream_t s) {
   constexpr int NIN = InvTraits<OP>::NIN;
   Mx2InvArgs a;
@@ -562,14 +626,18 @@ int inv_launch(const dsht_plan* p, const MixedTables& t, const Mx2InvJob& j, hip
   a.J2 = p->J2;
   a.J2p32 = t.J2p32;
   a.RT = t.J2p32 / TILE;
-  constexpr int NCT = InvShape<OP>::NCT, MAXT = InvShape<OP>::MAXT;
+  constexpr int NCT = InvShape<OP, NCT_>::NCT, MAXT = InvShape<OP, NCT_>::MAXT;
   a.CG = cdiv(a.CT, NCT);
   a.RG = cdiv(a.RT, 4 * MAXT);
-  const long pw = (long)t.nchunks * a.CT * NIN;  // packing waves
-  hipLaunchKernelGGL((k_mx2_ipack<FORM, NIN>), dim3((unsigned)cdivl(pw, 4)), dim3(256), 0, s, a);
-  DSHT_HIP(hipGetLastError());
   const long nb = 8l * cdiv(p->N + 1, 8) * a.CG * a.RG;
-  hipLaunchKernelGGL((k_mx2_inv<OP, FORM, NCT, MAXT>), dim3((unsigned)nb), dim3(256), 0, s, a);
+  if (j.pack) {
+    const long pw = (long)t.nchunks * a.CT * NIN;  // packing waves
+    hipLaunchKernelGGL((k_mx2_ipack<FORM, NIN>), dim3((unsigned)cdivl(pw, 4)), dim3(256), 0, s, a);
+    DSHT_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_mx2_inv<OP, FORM, NCT, MAXT, true>), dim3((unsigned)nb), dim3(256), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_mx2_inv<OP, FORM, NCT, MAXT, false>), dim3((unsigned)nb), dim3(256), 0, s, a);
+  }
   DSHT_HIP(hipGetLastError());
   return 0;
 }
@@ -608,16 +676,21 @@ int fwd_launch(const dsht_plan* p, const MixedTables& t, const Mx2FwdJob& j, hip
   return 0;
 }
 
-template <int FORM>
-int inv_form(const dsht_plan* p, const MixedTables& t, int op, const Mx2InvJob& j, hipStream_t s) {
+template <int FORM, int NCT>
+int inv_form_nct(const dsht_plan* p, const MixedTables& t, int op, const Mx2InvJob& j, hipStream_t s) {
   switch (op) {
-    case I_VAL: return inv_launch<I_VAL, FORM>(p, t, j, s);
-    case I_GRADLAMBDA: return inv_launch<I_GRADLAMBDA, FORM>(p, t, j, s);
-    case I_GRADMU: return inv_launch<I_GRADMU, FORM>(p, t, j, s);
-    case I_ALPHA: return inv_launch<I_ALPHA, FORM>(p, t, j, s);
-    case I_VORDIV2UV: return inv_launch<I_VORDIV2UV, FORM>(p, t, j, s);
+    case I_VAL: return inv_launch<I_VAL, FORM, NCT>(p, t, j, s);
+    case I_GRADLAMBDA: return inv_launch<I_GRADLAMBDA, FORM, NCT>(p, t, j, s);
+    case I_GRADMU: return inv_launch<I_GRADMU, FORM, NCT>(p, t, j, s);
+    case I_ALPHA: return inv_launch<I_ALPHA, FORM, NCT>(p, t, j, s);
+    case I_VORDIV2UV: return inv_launch<I_VORDIV2UV, FORM, NCT>(p, t, j, s);
   }
   return fail(DSHT_EARG, "bad inverse op %d", op);
+}
+
+template <int FORM>
+int inv_form(const dsht_plan* p, const MixedTables& t, int op, const Mx2InvJob& j, hipStream_t s) {
+  return p->leg_cb == 16 ? inv_form_nct<FORM, 1>(p, t, op, j, s) : inv_form_nct<FORM, 2>(p, t, op, j, s);
 }
 
 template <int FORM>
@@ -662,7 +735,8 @@ int mx2_fwd(const dsht_plan* p, const MixedTables& t, int form, int op, const Mx
 
 int launch_spec_colmax(const double* w, long sl, long sc, int L, int ncols, const double* pro, unsigned* out,
                        hipStream_t s) {
-  dim3 grid(cdiv(ncols, 64), 32);
+  // enough workgroups to cover the chip: ~16 rows of l per thread
+  dim3 grid(cdiv(ncols, 64), std::max(1, std::min(1024, cdiv(L, 64))));
   hipLaunchKernelGGL(k_spec_colmax2, grid, dim3(256), 0, s, w, sl, sc, L, ncols, pro, out);
   DSHT_HIP(hipGetLastError());
   return 0;

@@ -162,8 +162,11 @@ int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, d
  * of the Legendre contractions.  No counterpart in the reference.                   */
 #define DSHT_TUNE_CHUNK_COLS 0 /* 0 = automatic */
 #define DSHT_TUNE_LEG_CB 1     /* 0 = automatic, 16 or 32 */
-#define DSHT_TUNE_FFT 2        /* 0 = register two-pass kernel where the length has one, 1 = LDS Stockham */
-#define DSHT_TUNE_LEG 3        /* split-precision contractions: 0 = per-wave fragment streams, 1 = LDS-staged */
+#define DSHT_TUNE_FFT 2        /* 0 = register two-pass kernel where the length has one (256-thread
+                                  blocks), 1 = LDS Stockham, 2 = register kernel, 128-thread blocks */
+#define DSHT_TUNE_LEG 3        /* split-precision contractions: 0 = per-wave fragment streams over
+                                  pre-packed data panels, 1 = LDS-staged, 2 = fragment streams
+                                  splitting the data in place */
 int dsht_set_tuning(dsht_plan* plan, int key, int value);
 int dsht_get_tuning(const dsht_plan* plan, int key);
 
