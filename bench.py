@@ -136,9 +136,27 @@ def cpu_baseline(seconds=12.0):
         if time.perf_counter() - t0 >= seconds:
             break
     dt = time.perf_counter() - t0
+    # one thread, a shorter sample of the same work
+    oracle.set_threads(1)
+    p1 = 0
+    t1 = time.perf_counter()
+    while time.perf_counter() - t1 < min(3.0, seconds / 4):
+        o.fwd(o.inv(w[:4]))
+        p1 += 4
+    d1 = time.perf_counter() - t1
+    oracle.set_threads(threads)
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": pairs / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
             "sample": f"T170 (I=512, J=256) inverse+forward of {KM} columns, repeated {pairs // KM}x "
-                      f"in {dt:.1f} s; FP64 oracle (oracle/dsht_oracle.c), OpenMP over columns"}
+                      f"in {dt:.1f} s; FP64 oracle (oracle/dsht_oracle.c), OpenMP over columns",
+            "one_thread_value": p1 / d1, "cpu_model": model, "nproc": os.cpu_count()}
 
 
 def oracle_check(torch, spec, g_out, NCHK=3):
@@ -152,6 +170,44 @@ def oracle_check(torch, spec, g_out, NCHK=3):
     ref = o.inv(w)
     got = g_out[0][cols].cpu().numpy()
     return float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+
+
+FIELDS = ("vorticity", "divergence", "temperature", "salinity")
+
+
+def per_field_errors(outs, ref):
+    """Relative L2 and relative max error per prognostic field over its three grid outputs
+    (value, lon and lat derivative), against the FP64 form."""
+    out = {}
+    for f, name in enumerate(FIELDS):
+        sl = slice(f * KM, (f + 1) * KM)
+        d2 = n2 = 0.0
+        dmax = rmax = 0.0
+        for a, b in zip(outs[:3], ref[:3]):
+            d = (a[sl] - b[sl]).double()
+            d2 += float((d * d).sum())
+            n2 += float((b[sl].double() ** 2).sum())
+            dmax = max(dmax, float(d.abs().max()))
+            rmax = max(rmax, float(b[sl].abs().max()))
+        out[name] = {"rel_l2": (d2 / n2) ** 0.5 if n2 > 0 else d2 ** 0.5, "rel_max": dmax / rmax if rmax else dmax}
+    return out
+
+
+def energy_drift(torch, plan, dsht, form, spec, trips=100):
+    """Relative change of spectral energy sum_l w_l^2 over `trips` inverse+forward round
+    trips of the prognostic state in `form`."""
+    stream = torch.cuda.current_stream().cuda_stream
+    plan.set_precision({v: k for k, v in dsht.PREC_NAMES.items()}[form])
+    w = spec.clone()
+    w[0] = 0.0  # drop the depth backgrounds: drift of the perturbation energy
+    g = torch.empty((w.shape[1], JM, IM), dtype=torch.float64, device=spec.device)
+    e0 = float((w * w).sum())
+    for _ in range(trips):
+        plan.inv_d(dsht.VAL, w, g, stream=stream)
+        plan.fwd_d(dsht.VAL, g, w, stream=stream)
+    torch.cuda.synchronize()
+    plan.set_precision(dsht.PREC_FP64)
+    return (float((w * w).sum()) - e0) / e0
 
 
 def chunk_columns(form):
@@ -392,6 +448,7 @@ def main():
     forms = ["fp64"] if args.prec == "fp64" else (
         ["fp64", args.prec] if args.prec != "auto" else ["fp64", "bf16x3", "fp16x3", "bf16x6", "fp16"])
     results = {}
+    field_err = {}
     ref_out = None
     for form in forms:
         pid = {v: k for k, v in dsht.PREC_NAMES.items()}[form]
@@ -409,6 +466,7 @@ def main():
         else:
             err = {f"rel_l2_{nm}": rel_l2(a, b) for nm, a, b in
                    zip(list(INV_KINDS) + ["forward"], outs, ref_out)}
+            field_err[form] = per_field_errors(outs, ref_out)
         print(f"[bench] form {form}: timing {args.steps} steps", file=sys.stderr, flush=True)
         wall, lat = time_steps(torch, plan, dsht, bufs, args.steps, args.warmup)
         prof = kernel_profile(torch, plan, dsht, bufs, max(2, min(args.steps, 5)))
@@ -427,6 +485,7 @@ def main():
     ok = [f for f in results if f == "fp64" or max(results[f]["errors"].values()) <= TOL_MIXED]
     best = max(ok, key=lambda f: results[f]["pairs_per_s"])
     r = results[best]
+    drift = {f: energy_drift(torch, plan, dsht, f, spec) for f in results}
     # roofline of the dominant kernel of the reported form, probed on full-chunk launches
     kname = max(r["kernels"].items(), key=lambda kv: kv[1]["ms_total"])[0]
     roof = roofline_probe(torch, plan, dsht, best, kname, bufs)
@@ -471,6 +530,8 @@ def main():
                       "kernels": {kk: {"avg_us": kv["avg_us"], "GBps": kv["GBps"], "TFLOPs": kv["TFLOPs"]}
                                   for kk, kv in v["kernels"].items()}}
                   for f, v in results.items()},
+        "per_field_errors_vs_fp64": field_err,
+        "energy_drift_100_roundtrips": drift,
         "mixed_speedup_vs_fp64": (r["pairs_per_s"] / results["fp64"]["pairs_per_s"]) if "fp64" in results else None,
         "roofline": roof,
         "cpu_baseline": cpu,
