@@ -364,36 +364,52 @@ def batched_bench(torch, dsht, n_inst=64, steps=10, seq_steps=2, seed=0xB47C4, p
 
 
 # ---------------------------------------------------------------- serving form
-def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E):
+def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E, factor=2.0):
     """Poisson arrivals of independent runs (truncation, KM, rotation rate drawn per run),
-    served with continuous batching and, separately, one run at a time."""
+    served with continuous batching (per-truncation pools stepping asynchronously on their
+    own streams; admission capped so a pool's step stays within `factor` x the step time
+    of one KM=60 run of that truncation), without the latency cap, and one run at a time."""
     from dsht import serve
 
     def stream():
         return serve.poisson_stream(n_req, rate, seed=seed, nsteps=nsteps)
 
     out = {}
-    for mode in ("batched", "one_at_a_time"):
-        srv = serve.Server()
-        out[mode] = srv.run(stream(), batched=(mode == "batched"))
+    caps = {}
+    for mode in ("batched", "batched_uncapped", "one_at_a_time"):
+        srv = serve.Server(latency_factor=factor if mode == "batched" else None)
+        if mode == "batched":  # calibrate every pool before the clock starts
+            for n in serve.GRIDS:
+                p = srv.pool(n)
+                caps[f"T{n}"] = {"step_ms_km60": p.t_ref, "max_active_columns": p.max_cols}
+        reqs = stream()
+        out[mode] = srv.run(reqs, batched=(mode != "one_at_a_time"))
+        if mode == "batched":
+            out[mode]["p99_step_ms_by_truncation"] = {
+                f"T{n}": float(np.percentile(np.concatenate([r.step_ms for r in reqs if r.nmax == n]), 99))
+                for n in sorted({r.nmax for r in reqs})}
         srv = None
         torch.cuda.synchronize()
-    # the largest run alone: its step time is the latency reference
-    big = max(stream(), key=lambda r: (r.nmax, r.km))
-    solo = serve.Server()
-    r = serve.Request(0, big.nmax, big.km, 1.0, 30, 0.0, 1)
-    solo.run([r], batched=True)
-    solo_ms = float(np.median(r.step_ms[5:]))
+    # each truncation's KM=60 run alone: the latency reference
+    solo = {}
+    for n in sorted({r.nmax for r in stream()}):
+        r = serve.Request(0, n, 60, 1.0, 30, 0.0, 1)
+        serve.Server().run([r], batched=True)
+        solo[f"T{n}"] = float(np.median(r.step_ms[5:]))
     b, u = out["batched"], out["one_at_a_time"]
+    big = max(solo, key=lambda k: int(k[1:]))
     return {
         "workload": f"{n_req} runs, Poisson arrivals at {rate:g}/s, {nsteps} steps each, truncation from "
                     f"{list(serve.GRIDS)}, KM from [20, 30, 60], rotation 0.25-4x Earth; FP64 transforms",
         "batched": b,
+        "batched_uncapped": out["batched_uncapped"],
         "one_at_a_time": u,
+        "latency_caps": caps,
         "throughput_ratio": b["pairs_per_s"] / u["pairs_per_s"],
-        "largest_run_alone_step_ms": solo_ms,
-        "largest_run": {"nmax": big.nmax, "km": big.km},
-        "p99_over_largest_alone": b["p99_step_ms"] / solo_ms,
+        "throughput_ratio_uncapped": out["batched_uncapped"]["pairs_per_s"] / u["pairs_per_s"],
+        "solo_step_ms_km60": solo,
+        "p99_over_largest_alone": b["p99_step_ms"] / solo[big],
+        "p99_over_alone_by_truncation": {k: v / solo[k] for k, v in b["p99_step_ms_by_truncation"].items()},
     }
 
 

@@ -3,11 +3,13 @@
 Each request is a run (truncation N, KM layers, rotation rate Omega, number of steps).
 Runs of the same truncation share a slot pool: device buffers of `capacity` layer columns
 that hold every run's spectral state resident in HBM.  A run is admitted when a
-contiguous column range is free and the total footprint of admitted runs stays within
-the memory budget; requests that arrive while others are on the GPU wait in a queue and
-are admitted as slots free up (continuous batching).  Every scheduler iteration advances
-all admitted runs by one step, with one set of transform launches per truncation covering
-all of that pool's active columns.  Each column is computed independently, so a run's
+contiguous column range is free, the total footprint of admitted runs stays within the
+memory budget and (optionally) the pool's predicted step time stays within a latency
+budget; requests that arrive while others are on the GPU wait in a queue and are admitted
+as slots free up (continuous batching).  Each pool has its own HIP stream and steps
+asynchronously: whenever a pool's previous step has completed, one set of transform
+launches advances all of that pool's admitted runs, while the other pools' steps run
+concurrently on their own streams.  Each column is computed independently, so a run's
 result does not depend on what else is in the pool.
 
 The step per layer is a reduced form of the hydrostatic-Boussinesq transform workload of
@@ -22,7 +24,6 @@ then forward Euler with backward-Euler hyperdiffusion 1/(1 - dt c_n)
 (DOGCM_Phys_hspm_vfvm_mod.f90:636-644, LPhys_DIFF_spm_mod.f90:140-167).
 16 transform units (8 pairs) per layer per step.
 """
-import heapq
 import time
 
 import numpy as np
@@ -33,6 +34,7 @@ UNITS_PER_LAYER_STEP = 16
 EARTH_OMEGA = 7.292115e-5
 RADIUS = 6.371e6
 GRIDS = {21: (64, 32), 42: (128, 64), 85: (256, 128), 170: (512, 256)}
+GRAPH_COLS = 32  # step graphs are captured for active-column counts rounded up to this
 
 
 class Request:
@@ -59,21 +61,27 @@ def poisson_stream(n, rate, seed=0x0D0C2026, nsteps=200, truncations=(21, 42, 85
     return out
 
 
-def initial_state(nmax, km, seed):
-    """Red-spectrum vorticity/divergence/T'/S' with depth-dependent T/S means (SPMODEL order)."""
-    rng = np.random.default_rng(seed)
+def initial_state(nmax, km, seed, out):
+    """Red-spectrum vorticity/divergence/T'/S' with depth-dependent T/S means (SPMODEL order),
+    generated on the device into `out` (4, L, km) on the current stream."""
+    import torch
+
+    dev = out.device
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
     L = (nmax + 1) ** 2
-    n = np.floor(np.sqrt(np.arange(L)))
-    sd = np.sqrt(0.5 * (n + 1.0) ** -3)
-    st = rng.standard_normal((4, L, km)) * sd[None, :, None]
-    st[0] *= 1e-5 / np.sqrt((st[0] ** 2).sum(0)).mean()
-    st[1] *= 1e-6 / np.sqrt((st[1] ** 2).sum(0)).mean()
+    n = torch.floor(torch.sqrt(torch.arange(L, device=dev, dtype=torch.float64)))
+    sd = torch.sqrt(0.5 * (n + 1.0) ** -3)
+    st = torch.randn((4, L, km), generator=g, device=dev, dtype=torch.float64) * sd[None, :, None]
+    st[0] *= 1e-5 / torch.sqrt((st[0] ** 2).sum(0)).mean()
+    st[1] *= 1e-6 / torch.sqrt((st[1] ** 2).sum(0)).mean()
     st[3] *= 0.05
-    st[0, 0] = st[1, 0] = 0.0
-    z = -(np.arange(km) + 0.5) * (5000.0 / km)
-    st[2, 0] = 2.0 + 23.0 * np.exp(z / 1000.0)
-    st[3, 0] = 34.5 + 1.0 * np.exp(z / 1000.0)
-    return st
+    st[0, 0] = 0.0
+    st[1, 0] = 0.0
+    z = -(torch.arange(km, device=dev, dtype=torch.float64) + 0.5) * (5000.0 / km)
+    st[2, 0] = 2.0 + 23.0 * torch.exp(z / 1000.0)
+    st[3, 0] = 34.5 + 1.0 * torch.exp(z / 1000.0)
+    out.copy_(st)
 
 
 class Pool:
@@ -82,6 +90,10 @@ class Pool:
     def __init__(self, nmax, capacity, dt, dev, precision=0):
         import torch
 
+        self.stream = torch.cuda.Stream(device=dev)
+        self.graphs = {}
+        self.mempool = None
+        self.max_cols = capacity  # latency cap on the active columns (set by calibrate)
         im, jm = GRIDS[nmax]
         self.plan = Plan(nmax, im, jm, kmax=capacity, radius=RADIUS)
         self.plan.set_precision(precision)
@@ -105,6 +117,11 @@ class Pool:
         self.free = [(0, capacity)]  # free column ranges
         self.runs = []
         self.bytes_per_col = 8 * (9 * L + 11 * im * jm)
+        torch.cuda.synchronize(dev)  # buffers are initialised before the pool stream uses them
+        # one full-width step on the zero state sizes every workspace of the pool's stream
+        with torch.cuda.stream(self.stream):
+            self.step(capacity, self.stream.cuda_stream)
+        self.stream.synchronize()
 
     def alloc(self, km):
         for i, (a, b) in enumerate(self.free):
@@ -129,11 +146,61 @@ class Pool:
     def active_hi(self):
         return max((r.col0 + r.km for r in self.runs), default=0)
 
+    def active_cols(self):
+        return sum(r.km for r in self.runs)
+
     def load(self, req):
+        """Initial state of an admitted run, written on the pool's stream so that it lands
+        after any step still in flight over the same columns."""
         import torch
-        st = initial_state(self.nmax, req.km, req.seed)
-        self.state[:, :, req.col0:req.col0 + req.km] = torch.tensor(st, device=self.state.device)
-        self.omega[req.col0:req.col0 + req.km] = req.omega
+        with torch.cuda.stream(self.stream):
+            initial_state(self.nmax, req.km, req.seed, self.state[:, :, req.col0:req.col0 + req.km])
+            self.omega[req.col0:req.col0 + req.km] = req.omega
+
+    def calibrate(self, ref_cols, factor, reps=3):
+        """Step time t(c) ~ a + b c from two column counts; caps the active columns so that
+        t stays within `factor` x t(ref_cols) (the step time of one run of ref_cols layers)."""
+        import torch
+        c1, c2 = min(ref_cols, self.cap), self.cap
+        ts = []
+        for c in (c1, c2):
+            with torch.cuda.stream(self.stream):
+                self.step(c, self.stream.cuda_stream)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(self.stream)
+                for _ in range(reps):
+                    self.step(c, self.stream.cuda_stream)
+                e1.record(self.stream)
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1) / reps)
+        with torch.cuda.stream(self.stream):
+            self.state.zero_()
+        b = max((ts[1] - ts[0]) / max(c2 - c1, 1), 1e-9)
+        a = ts[0] - b * c1
+        self.t_ref = ts[0]
+        self.max_cols = max(ref_cols, min(self.cap, int((factor * ts[0] - a) / b)))
+        return self.t_ref, self.max_cols
+
+    def launch(self, hi):
+        """Enqueue one step of columns [0, hi) on the pool's stream as a hipGraph replay.
+        hi is rounded up to a multiple of GRAPH_COLS (idle columns hold zeros or a retired
+        run's last state and are never read back), so a pool keeps a handful of graphs.
+        The library's workspaces were sized by the full-width step at construction, so a
+        capture allocates nothing."""
+        import torch
+        hi = min(self.cap, -(-hi // GRAPH_COLS) * GRAPH_COLS)
+        g = self.graphs.get(hi)
+        with torch.cuda.stream(self.stream):
+            if g is None:
+                g = torch.cuda.CUDAGraph()
+                # one memory pool per slot pool: a step's temporaries are dead when it ends
+                # and the pool's graphs never run concurrently (one stream)
+                if self.mempool is None:
+                    self.mempool = torch.cuda.graph_pool_handle()
+                with torch.cuda.graph(g, pool=self.mempool, stream=self.stream):
+                    self.step(hi, self.stream.cuda_stream)
+                self.graphs[hi] = g
+            g.replay()
 
     def step(self, hi, stream):
         """One step for columns [0, hi) of the pool."""
@@ -168,28 +235,35 @@ class Pool:
 
 
 class Server:
-    """Continuous-batching request loop over per-truncation slot pools."""
+    """Continuous-batching request loop over per-truncation slot pools, one stream each."""
 
-    def __init__(self, capacity=None, budget_bytes=64 << 30, dt=600.0, precision=0, dev=None):
-        import torch
-
-        self.dev = dev or torch.device("cuda:0")
+    def __init__(self, capacity=None, budget_bytes=64 << 30, dt=600.0, precision=0, dev=None,
+                 latency_factor=None, ref_km=60):
+        self.dev = dev
         self.capacity = capacity or {21: 240, 42: 240, 85: 240, 170: 240}
         self.budget = budget_bytes
         self.dt, self.precision = dt, precision
+        self.latency_factor, self.ref_km = latency_factor, ref_km
         self.pools = {}
         self.used = 0
-        self.stream = torch.cuda.current_stream(self.dev).cuda_stream
 
     def pool(self, nmax):
+        import torch
+
         if nmax not in self.pools:
-            self.pools[nmax] = Pool(nmax, self.capacity[nmax], self.dt, self.dev, self.precision)
+            dev = self.dev or torch.device("cuda:0")
+            p = Pool(nmax, self.capacity[nmax], self.dt, dev, self.precision)
+            if self.latency_factor:
+                p.calibrate(self.ref_km, self.latency_factor)
+            self.pools[nmax] = p
         return self.pools[nmax]
 
     def try_admit(self, req, now):
         pool = self.pool(req.nmax)
         need = req.km * pool.bytes_per_col
         if self.used + need > self.budget:
+            return False
+        if pool.runs and pool.active_cols() + req.km > pool.max_cols:
             return False
         c0 = pool.alloc(req.km)
         if c0 is None:
@@ -209,7 +283,8 @@ class Server:
 
     def run(self, requests, batched=True, time_scale=1.0, on_finish=None):
         """Serve `requests` (arrival times in seconds from start); batched=False runs them
-        one at a time in arrival order.  Returns summary statistics."""
+        one at a time in arrival order.  A run's step latency is the wall time from the
+        launch of its pool's step to the observed completion.  Returns summary statistics."""
         import torch
 
         pending = sorted(requests, key=lambda r: r.arrival)
@@ -219,44 +294,64 @@ class Server:
         units = 0
         active = []
         max_active = 0
+        inflight = {}  # nmax -> (event, launch time, runs in the step)
         while pending or queue or active:
             tnow = now()
             while pending and pending[0].arrival <= tnow:
                 queue.append(pending.pop(0))
-            if not active and not queue:
-                wait = (pending[0].arrival - tnow) * time_scale
-                if wait > 0:
-                    time.sleep(wait)
-                continue
+            progressed = False
+            # completions
+            for nmax, (ev, tl, runs) in list(inflight.items()):
+                if not ev.query():
+                    continue
+                del inflight[nmax]
+                progressed = True
+                ms = 1e3 * (time.perf_counter() - tl)
+                tnow = now()
+                for r in runs:
+                    r.step_ms.append(ms)
+                    r.steps_done += 1
+                    units += UNITS_PER_LAYER_STEP * r.km
+                    if r.steps_done >= r.nsteps:
+                        if on_finish:
+                            torch.cuda.current_stream().wait_stream(self.pools[r.nmax].stream)
+                            on_finish(r, self.pools[r.nmax])
+                        self.retire(r, tnow)
+                        active.remove(r)
             # admission (first fit over the queue; one run at a time when unbatched)
             if batched:
                 for r in list(queue):
                     if self.try_admit(r, tnow):
                         queue.remove(r)
                         active.append(r)
+                        progressed = True
             elif not active and queue:
                 r = queue.pop(0)
                 if not self.try_admit(r, tnow):
                     raise DshtError(2, "run does not fit in an empty pool")
                 active.append(r)
+                progressed = True
             max_active = max(max_active, len(active))
-            ts = time.perf_counter()
-            for pool in self.pools.values():
+            # launches: every pool with runs and no step in flight
+            for nmax, pool in self.pools.items():
+                if nmax in inflight or not pool.runs:
+                    continue
+                runs = list(pool.runs)
                 hi = pool.active_hi()
-                if hi:
-                    pool.step(hi, self.stream)
-            torch.cuda.synchronize(self.dev)
-            ms = 1e3 * (time.perf_counter() - ts)
-            tnow = now()
-            for r in list(active):
-                r.step_ms.append(ms)
-                r.steps_done += 1
-                units += UNITS_PER_LAYER_STEP * r.km
-                if r.steps_done >= r.nsteps:
-                    if on_finish:
-                        on_finish(r, self.pools[r.nmax])
-                    self.retire(r, tnow)
-                    active.remove(r)
+                tl = time.perf_counter()
+                pool.launch(hi)
+                ev = torch.cuda.Event()
+                ev.record(pool.stream)
+                inflight[nmax] = (ev, tl, runs)
+                progressed = True
+            if not progressed:
+                if not inflight and not active and not queue and pending:
+                    wait = (pending[0].arrival - now()) * time_scale
+                    if wait > 0:
+                        time.sleep(wait)
+                else:
+                    time.sleep(2e-5)
+        torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         lat = np.concatenate([np.array(r.step_ms) for r in requests])
         return {
