@@ -192,6 +192,13 @@ __device__ __forceinline__ void st_g(float2* p, double2 v) { *p = make_float2((f
 
 constexpr int odd_up(int x) { return x | 1; }
 
+template <typename GT>
+__device__ __forceinline__ GT zero_of();
+template <>
+__device__ __forceinline__ double2 zero_of<double2>() { return make_double2(0.0, 0.0); }
+template <>
+__device__ __forceinline__ float2 zero_of<float2>() { return make_float2(0.f, 0.f); }
+
 template <int R1, int R2, int NT>
 struct RegFft {
   static constexpr int M = R1 * R2;
@@ -225,20 +232,24 @@ __global__ __launch_bounds__(NT) void k_fft_fwd_r(const double* __restrict__ xy,
   const double inv_i = 1.0 / (2 * M);
   const int rr1 = tid / R1, n1 = tid % R1;  // pass-1 role
   const int rr2 = tid / R2, k2 = tid % R2;  // pass-2 role
-  for (long r0 = (long)blockIdx.x * ROWS; r0 < nrows; r0 += (long)gridDim.x * ROWS) {
-    double2 v[R2];
-    {
-      const long r = r0 + rr1;
-      if (rr1 < ROWS && r < nrows) {
-        const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
-        const double2* x = reinterpret_cast<const double2*>(xy + c * a.ldg + (long)j * (2 * M));
+  // pass-1 inputs straight from global memory; the next group's rows are loaded into the
+  // same registers once the current group's pass-1 output is in LDS
+  double2 v[R2];
+  auto fetch = [&](long rb) {
+    const long r = rb + rr1;
+    if (rr1 < ROWS && r < nrows) {
+      const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
+      const double2* x = reinterpret_cast<const double2*>(xy + c * a.ldg + (long)j * (2 * M));
 #pragma unroll
-        for (int n2 = 0; n2 < R2; n2++) v[n2] = x[n1 + R1 * n2];
-      } else {
+      for (int n2 = 0; n2 < R2; n2++) v[n2] = x[n1 + R1 * n2];
+    } else {
 #pragma unroll
-        for (int n2 = 0; n2 < R2; n2++) v[n2] = make_double2(0.0, 0.0);
-      }
+      for (int n2 = 0; n2 < R2; n2++) v[n2] = make_double2(0.0, 0.0);
     }
+  };
+  const long rstep = (long)gridDim.x * ROWS;
+  if ((long)blockIdx.x * ROWS < nrows) fetch((long)blockIdx.x * ROWS);
+  for (long r0 = (long)blockIdx.x * ROWS; r0 < nrows; r0 += rstep) {
     dft<R2, -1>(v);
     if (colmax && tid < ROWS) smax[tid] = 0u;
     __syncthreads();  // the previous rows are out of buf; twiddles are in place
@@ -248,6 +259,7 @@ __global__ __launch_bounds__(NT) void k_fft_fwd_r(const double* __restrict__ xy,
 #pragma unroll
       for (int q = 1; q < R2; q++) Y[q] = rcmul(v[q], tw2[q * R1 + n1]);
     }
+    if (r0 + rstep < nrows) fetch(r0 + rstep);
     __syncthreads();
     double2 u[R1];
     if (rr2 < ROWS) {
@@ -306,29 +318,42 @@ __global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, doub
   const int N = a.N;
   const int rr1 = tid / R1, n1 = tid % R1;
   const int rr2 = tid / R2, k2 = tid % R2;
-  for (long r0 = (long)blockIdx.x * ROWS; r0 < nrows; r0 += (long)gridDim.x * ROWS) {
-    // Fourier coefficients of the ROWS rows: every load is issued before the first LDS
-    // store (consecutive threads take consecutive rows, zero above the truncation)
-    constexpr int NL = (M * ROWS + C::THREADS - 1) / C::THREADS;
-    double2 gv[NL];
+  // Fourier coefficients of a group of ROWS rows, raw: consecutive threads take consecutive
+  // rows, zero above the truncation.  The next group's loads are issued as soon as the
+  // current group is in LDS, so they overlap its FFT passes and grid stores.
+  constexpr int NL = (M * ROWS + C::THREADS - 1) / C::THREADS;
+  GT gr[NL];
+  auto fetch = [&](long rb) {
 #pragma unroll
     for (int q = 0; q < NL; q++) {
       const int idx = tid + q * C::THREADS;
       const int m = idx / ROWS, rr = idx % ROWS;
-      const long r = r0 + rr;
-      gv[q] = make_double2(0.0, 0.0);
+      const long r = rb + rr;
+      gr[q] = zero_of<GT>();
       if (idx < M * ROWS && m <= N && r < nrows) {
         const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
-        gv[q] = ld_g(&G[(long)m * a.gmst + (long)j * a.gld + c]);
-        if (m == 0) gv[q].y = 0.0;
+        gr[q] = G[(long)m * a.gmst + (long)j * a.gld + c];
       }
     }
+  };
+  // prefetching holds the next group in registers through the whole pipeline: only where
+  // that fits the register budget (FP32 coefficients, M <= 256)
+  constexpr bool PF = sizeof(GT) == 8 && M <= 256;
+  const long rstep = (long)gridDim.x * ROWS;
+  if (PF && (long)blockIdx.x * ROWS < nrows) fetch((long)blockIdx.x * ROWS);
+  for (long r0 = (long)blockIdx.x * ROWS; r0 < nrows; r0 += rstep) {
+    if (!PF) fetch(r0);
     __syncthreads();  // the previous rows are out of buf
 #pragma unroll
     for (int q = 0; q < NL; q++) {
       const int idx = tid + q * C::THREADS;
-      if (idx < M * ROWS) buf[(idx % ROWS) * SR + idx / ROWS] = gv[q];
+      if (idx < M * ROWS) {
+        double2 v = make_double2((double)gr[q].x, (double)gr[q].y);
+        if (idx < ROWS) v.y = 0.0;  // m = 0
+        buf[(idx % ROWS) * SR + idx / ROWS] = v;
+      }
     }
+    if (PF && r0 + rstep < nrows) fetch(r0 + rstep);
     __syncthreads();
     // packed spectrum Z_m from the half-complex pair (X_m, X_{M-m}); X_M = 0 for m = 0
     for (int idx = tid; idx < ROWS * (M / 2 + 1); idx += C::THREADS) {
@@ -381,12 +406,26 @@ __global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, doub
 }
 
 // ---------------------------------------------------------------- dispatch
+// workgroups resident at once: the register kernels are held to two per CU by their LDS
+// rows, so the grid is capped at four per CU and each workgroup walks several row groups
+// (its loads for the next group overlap the current one)
+static long fft_grid_cap() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
+  return 4l * cus;
+}
+
 template <int R1, int R2, int NT, typename GT>
 static int reg_fwd_nt(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
   using C = RegFft<R1, R2, NT>;
   const long nrows = (long)a.J * a.ncols;
   long nb = cdivl(nrows, C::ROWS);
-  if (nb > 16384) nb = 16384;
+  if (nb > fft_grid_cap()) nb = fft_grid_cap();
   hipLaunchKernelGGL((k_fft_fwd_r<R1, R2, NT, GT>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, xy, G, a,
                      colmax);
   DSHT_HIP(hipGetLastError());
@@ -398,7 +437,7 @@ static int reg_inv_nt(const GT* G, const FftArgs& a, double* xy, hipStream_t s) 
   using C = RegFft<R1, R2, NT>;
   const long nrows = (long)a.J * a.ncols;
   long nb = cdivl(nrows, C::ROWS);
-  if (nb > 16384) nb = 16384;
+  if (nb > fft_grid_cap()) nb = fft_grid_cap();
   hipLaunchKernelGGL((k_fft_inv_r<R1, R2, NT, GT>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, G, xy, a);
   DSHT_HIP(hipGetLastError());
   return 0;
