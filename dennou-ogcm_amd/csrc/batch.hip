@@ -243,7 +243,7 @@ int dsht_set_tuning(dsht_plan* p, int key, int value) {
       p->fft_kind = value;
       return 0;
     case DSHT_TUNE_LEG:
-      if (value < 0 || value > 2) return fail(DSHT_EARG, "contraction kernel must be 0, 1 or 2");
+      if (value < 0 || value > 3) return fail(DSHT_EARG, "contraction kernel must be 0, 1, 2 or 3");
       p->leg_kind = value;
       return 0;
   }

@@ -976,6 +976,7 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
         j.rs = rs;
         j.colmax = colmax;
         j.pack = p->leg_kind == 0 ? pack : nullptr;
+        j.staged = p->leg_kind == 3;
         j.G0 = G0;
         j.G1 = nout > 1 ? G1 : nullptr;
         j.gld = nc;

@@ -109,6 +109,7 @@ struct Mx2InvJob {
   const double* rs;         // per-latitude output scale (nullable)
   const unsigned* colmax;   // [input][column] max |pro w| (float bits); scaled forms only
   uint16_t* pack;           // packed data panels (workspace, mx2_inv_pack_bytes)
+  bool staged;              // split the data once per block into LDS instead of packing
   float2* G0;
   float2* G1;
   long gld, gmst;

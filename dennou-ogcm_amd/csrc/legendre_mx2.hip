@@ -121,49 +121,58 @@ struct Mx2InvArgs {
 };
 
 // ---------------------------------------------------------------- inverse data panels
-// panel (chunk g, column tile ct, input i): [re, im][term][lane][8], lane = c + 16*(k/8)
+// panel (chunk g, column tile ct, input i): [re, im][term][lane][8], lane = c + 16*(k/8).
+// Threads of one (chunk, input) run along the columns (64 consecutive columns per wave, so
+// every gathered coefficient row is read in 512-byte runs); thread (k-group q, column c)
+// fills fragment lane c%16 + 16 q of tile c/16 with the 8 wavenumbers of its group.
 template <int FORM, int NIN>
 __global__ __launch_bounds__(256) void k_mx2_ipack(Mx2InvArgs a) {
   using F = MxForm<FORM>;
   using T = typename F::T;
   using V8 = typename F::V8;
   constexpr int NTB = Mx2<FORM>::NTB;
-  const long wv = ((long)blockIdx.x * 256 + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (wv >= (long)a.nchunks * a.CT * NIN) return;
-  const int i = (int)(wv % NIN), ct = (int)((wv / NIN) % a.CT);
-  const int g = (int)(wv / NIN / a.CT);
+  const int CC = a.CT * TILE;  // padded columns
+  const long per = 4l * CC;    // threads per (chunk, input)
+  const long gt = (long)blockIdx.x * 256 + threadIdx.x;
+  if (gt >= per * a.nchunks * NIN) return;
+  const int gi = (int)(gt / per), rem = (int)(gt % per);
+  const int i = gi % NIN, g = gi / NIN;
+  const int q = rem / CC, c = rem % CC, ct = c / TILE;
   const int m = a.chunk_m[g], ch = g - a.cbase[m];
   const int nE = n_par(a.N, m, 0), nO = n_par(a.N, m, 1);
   const int nch0 = kpad32(nE) / KM;
   const int p = ch >= nch0 ? 1 : 0, ks = p ? ch - nch0 : ch, np = p ? nO : nE;
-  const int c = ct * TILE + (lane & 15);
   const double* W = i ? a.W1 : a.W0;
   float dsc = 1.f;
   if constexpr (F::SCALED) {
     if (c < a.ncols) dsc = colscale<NIN>(a.colmax, a.ncols, c, 1.f);
   }
-  T tre[NTB][8], tim[NTB][8];
+  double vre[8], vim[8];
 #pragma unroll
   for (int e = 0; e < 8; e++) {
-    const int k = ks * KM + 8 * (lane >> 4) + e;
-    double vre = 0.0, vim = 0.0;
+    const int k = ks * KM + 8 * q + e;
+    vre[e] = vim[e] = 0.0;
     if (k < np && c < a.ncols) {
       const int n = m + p + 2 * k;
       const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
       const double pr = a.pro ? a.pro[lre] : 1.0;
-      vre = pr * W[lre * a.sl + (long)c * a.sc];
-      if (m > 0) vim = pr * W[lim * a.sl + (long)c * a.sc];
+      vre[e] = pr * W[lre * a.sl + (long)c * a.sc];
+      if (m > 0) vim[e] = pr * W[lim * a.sl + (long)c * a.sc];
     }
+  }
+  T tre[NTB][8], tim[NTB][8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
     T t1[NTB], t2[NTB];
-    bsplit<FORM>((float)vre * dsc, t1);
-    bsplit<FORM>((float)vim * dsc, t2);
+    bsplit<FORM>((float)vre[e] * dsc, t1);
+    bsplit<FORM>((float)vim[e] * dsc, t2);
 #pragma unroll
     for (int t = 0; t < NTB; t++) {
       tre[t][e] = t1[t];
       tim[t][e] = t2[t];
     }
   }
+  const int lane = (c % TILE) + TILE * q;
   uint16_t* dst = a.pack + ((((long)g * a.CT + ct) * NIN + i) * 2 * NTB) * 512 + lane * 8;
 #pragma unroll
   for (int t = 0; t < NTB; t++) {
@@ -175,6 +184,68 @@ __global__ __launch_bounds__(256) void k_mx2_ipack(Mx2InvArgs a) {
     }
     *reinterpret_cast<V8*>(dst + (long)t * 512) = v1;
     *reinterpret_cast<V8*>(dst + (long)(NTB + t) * 512) = v2;
+  }
+}
+
+// ---- inverse epilogue: recombine north/south, i*m factors, unscale, row scale, FP32 store
+template <int OP, int FORM, int NCT, int MAXT>
+__device__ __forceinline__ void inv_epilogue(const Mx2InvArgs& a,
+                                             const float4_t (&acc)[inv_ng(OP)][MAXT][NCT][2][2], int m,
+                                             int t0, int ct0, int lane) {
+  using F = MxForm<FORM>;
+  using X = Mx2<FORM>;
+  constexpr int NG = inv_ng(OP), NIN = InvTraits<OP>::NIN, NOUT = InvTraits<OP>::NOUT;
+  const double mi = a.minv[m] * X::DSCALE;
+  const double dm = (double)m;
+#pragma unroll
+  for (int tt = 0; tt < MAXT; tt++) {
+    const int t = t0 + tt;
+    if (t >= a.RT) continue;
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      const int c = (ct0 + ct) * TILE + (lane & 15);
+      if (c >= a.ncols) continue;
+      double u = mi;
+      if constexpr (F::SCALED) u = mi / (double)colscale<NIN>(a.colmax, a.ncols, c, 1.f);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int jr = t * TILE + acc_row16(lane, i);
+        if (jr >= a.J2) continue;
+        const int jN = a.J2 + jr, jS = a.J2 - 1 - jr;
+        // per group: north and south complex values
+        double nr[NG], ni[NG], sr[NG], si[NG];
+#pragma unroll
+        for (int gi = 0; gi < NG; gi++) {
+          const double a0 = acc[gi][tt][ct][0][0][i], a1 = acc[gi][tt][ct][0][1][i];
+          const double b0 = acc[gi][tt][ct][1][0][i], b1 = acc[gi][tt][ct][1][1][i];
+          nr[gi] = a0 + a1;
+          sr[gi] = a0 - a1;
+          ni[gi] = b0 + b1;
+          si[gi] = b0 - b1;
+        }
+        double onr[NOUT], oni[NOUT], osr[NOUT], osi[NOUT];
+        if constexpr (OP == I_VAL || OP == I_GRADMU) {
+          onr[0] = nr[0], oni[0] = ni[0], osr[0] = sr[0], osi[0] = si[0];
+        } else if constexpr (OP == I_GRADLAMBDA) {  // i m X
+          onr[0] = -dm * ni[0], oni[0] = dm * nr[0], osr[0] = -dm * si[0], osi[0] = dm * sr[0];
+        } else if constexpr (OP == I_ALPHA) {  // i m P.A + H.B
+          onr[0] = -dm * ni[0] + nr[1], oni[0] = dm * nr[0] + ni[1];
+          osr[0] = -dm * si[0] + sr[1], osi[0] = dm * sr[0] + si[1];
+        } else {  // groups (P,psi) (P,chi) (H,psi) (H,chi): U = i m P.chi - H.psi, V = i m P.psi + H.chi
+          onr[0] = -dm * ni[1] - nr[2], oni[0] = dm * nr[1] - ni[2];
+          osr[0] = -dm * si[1] - sr[2], osi[0] = dm * sr[1] - si[2];
+          onr[1] = -dm * ni[0] + nr[3], oni[1] = dm * nr[0] + ni[3];
+          osr[1] = -dm * si[0] + sr[3], osi[1] = dm * sr[0] + si[3];
+        }
+        const double sN = u * (a.rs ? a.rs[jN] : 1.0), sS = u * (a.rs ? a.rs[jS] : 1.0);
+#pragma unroll
+        for (int o = 0; o < NOUT; o++) {
+          float2* G = (o == 0 ? a.G0 : a.G1) + (long)m * a.gmst;
+          G[(long)jN * a.gld + c] = make_float2((float)(onr[o] * sN), m ? (float)(oni[o] * sN) : 0.f);
+          G[(long)jS * a.gld + c] = make_float2((float)(osr[o] * sS), m ? (float)(osi[o] * sS) : 0.f);
+        }
+      }
+    }
   }
 }
 
@@ -315,59 +386,175 @@ __global__ __launch_bounds__(256) void k_mx2_inv(Mx2InvArgs a) {
   run(std::integral_constant<int, 0>());
   run(std::integral_constant<int, 1>());
 
-  // ---- epilogue: recombine, i*m factors, unscale, row scale, FP32 store
-  const double mi = a.minv[m] * X::DSCALE;
-  const double dm = (double)m;
+  inv_epilogue<OP, FORM, NCT, MAXT>(a, acc, m, t0, ct0, lane);
+}
+
+// Inverse with the split data shared through LDS: per K chunk the block's threads gather
+// the FP64 coefficients of its NCT column tiles once, split them and store B fragments to a
+// double-buffered LDS panel (one barrier per chunk); each wave then reads its fragments
+// with one 16-byte LDS read per (input, column tile, re/im, term) and streams its table
+// fragments from memory one chunk ahead.  Same tiling and epilogue as k_mx2_inv.
+template <int OP, int FORM, int NCT, int MAXT>
+__global__ __launch_bounds__(256) void k_mx2_inv_lds(Mx2InvArgs a) {
+  using F = MxForm<FORM>;
+  using X = Mx2<FORM>;
+  using T = typename F::T;
+  using V8 = typename F::V8;
+  constexpr int NG = inv_ng(OP), NIN = InvTraits<OP>::NIN;
+  constexpr int NTA = X::NTA, NTB = X::NTB;
+  constexpr bool UP = InvTraits<OP>::USE_P, UH = InvTraits<OP>::USE_H;
+  constexpr int PANEL = NIN * NCT * 2 * NTB * 512;  // elements of one chunk's B panel
+  constexpr int ITEMS = NIN * NCT * 2 * 64;         // (input, tile, re/im, lane) staging items
+  constexpr int IPT = (ITEMS + 255) / 256;          // items per thread
+  __shared__ __attribute__((aligned(16))) T sB[2][PANEL];
+  int m, sub;
+  if (!xcd_decode(a.N, a.CG * a.RG, m, sub)) return;  // uniform over the block
+  const int cg = sub % a.CG, rg = sub / a.CG;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int t0 = (rg * 4 + w) * MAXT;
+  const int nE = n_par(a.N, m, 0), nO = n_par(a.N, m, 1);
+  const int kq0 = kpad32(nE), kq1 = kpad32(nO);
+  const int nch0 = kq0 / KM, nch1 = kq1 / KM, nch = nch0 + nch1;
+  const long toff = a.off[m];
+  const int ct0 = cg * NCT;
+
+  // staging: item -> (input i, tile ct, re/im r, fragment lane fl); 8 coefficients each
+  double raw[IPT][8];
+  float dsc[IPT];
 #pragma unroll
-  for (int tt = 0; tt < MAXT; tt++) {
-    const int t = t0 + tt;
-    if (t >= a.RT) continue;
+  for (int q = 0; q < IPT; q++) {
+    const int it = tid + 256 * q;
+    const int fl = it & 63, ct = (it >> 6) % NCT, i = (it >> 6) / NCT / 2;
+    const int c = (ct0 + ct) * TILE + (fl & 15);
+    dsc[q] = 1.f;
+    if constexpr (F::SCALED) {
+      if (it < ITEMS && c < a.ncols) dsc[q] = colscale<NIN>(a.colmax, a.ncols, c, 1.f);
+    }
+    (void)i;
+  }
+  auto gather = [&](int ch) {
+    const int p = ch >= nch0 ? 1 : 0, ks = p ? ch - nch0 : ch, np = p ? nO : nE;
 #pragma unroll
-    for (int ct = 0; ct < NCT; ct++) {
-      const int c = (ct0 + ct) * TILE + (lane & 15);
-      if (c >= a.ncols) continue;
-      double u = mi;
-      if constexpr (F::SCALED) u = mi / (double)colscale<NIN>(a.colmax, a.ncols, c, 1.f);
+    for (int q = 0; q < IPT; q++) {
+      const int it = tid + 256 * q;
+      const int fl = it & 63, ct = (it >> 6) % NCT, r = ((it >> 6) / NCT) & 1, i = (it >> 6) / NCT / 2;
+      const int c = (ct0 + ct) * TILE + (fl & 15);
+      const double* W = i ? a.W1 : a.W0;
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int jr = t * TILE + acc_row16(lane, i);
-        if (jr >= a.J2) continue;
-        const int jN = a.J2 + jr, jS = a.J2 - 1 - jr;
-        // per group: north and south complex values
-        double nr[NG], ni[NG], sr[NG], si[NG];
-#pragma unroll
-        for (int gi = 0; gi < NG; gi++) {
-          const double a0 = acc[gi][tt][ct][0][0][i], a1 = acc[gi][tt][ct][0][1][i];
-          const double b0 = acc[gi][tt][ct][1][0][i], b1 = acc[gi][tt][ct][1][1][i];
-          nr[gi] = a0 + a1;
-          sr[gi] = a0 - a1;
-          ni[gi] = b0 + b1;
-          si[gi] = b0 - b1;
-        }
-        double onr[NOUT], oni[NOUT], osr[NOUT], osi[NOUT];
-        if constexpr (OP == I_VAL || OP == I_GRADMU) {
-          onr[0] = nr[0], oni[0] = ni[0], osr[0] = sr[0], osi[0] = si[0];
-        } else if constexpr (OP == I_GRADLAMBDA) {  // i m X
-          onr[0] = -dm * ni[0], oni[0] = dm * nr[0], osr[0] = -dm * si[0], osi[0] = dm * sr[0];
-        } else if constexpr (OP == I_ALPHA) {  // i m P.A + H.B
-          onr[0] = -dm * ni[0] + nr[1], oni[0] = dm * nr[0] + ni[1];
-          osr[0] = -dm * si[0] + sr[1], osi[0] = dm * sr[0] + si[1];
-        } else {  // groups (P,psi) (P,chi) (H,psi) (H,chi): U = i m P.chi - H.psi, V = i m P.psi + H.chi
-          onr[0] = -dm * ni[1] - nr[2], oni[0] = dm * nr[1] - ni[2];
-          osr[0] = -dm * si[1] - sr[2], osi[0] = dm * sr[1] - si[2];
-          onr[1] = -dm * ni[0] + nr[3], oni[1] = dm * nr[0] + ni[3];
-          osr[1] = -dm * si[0] + sr[3], osi[1] = dm * sr[0] + si[3];
-        }
-        const double sN = u * (a.rs ? a.rs[jN] : 1.0), sS = u * (a.rs ? a.rs[jS] : 1.0);
-#pragma unroll
-        for (int o = 0; o < NOUT; o++) {
-          float2* G = (o == 0 ? a.G0 : a.G1) + (long)m * a.gmst;
-          G[(long)jN * a.gld + c] = make_float2((float)(onr[o] * sN), m ? (float)(oni[o] * sN) : 0.f);
-          G[(long)jS * a.gld + c] = make_float2((float)(osr[o] * sS), m ? (float)(osi[o] * sS) : 0.f);
+      for (int e = 0; e < 8; e++) {
+        const int k = ks * KM + 8 * (fl >> 4) + e;
+        raw[q][e] = 0.0;
+        if (it < ITEMS && k < np && c < a.ncols && (r == 0 || m > 0)) {
+          const int n = m + p + 2 * k;
+          const long l = (long)n * (n + 1) + (r ? -m : m);
+          raw[q][e] = (a.pro ? a.pro[(long)n * (n + 1) + m] : 1.0) * W[l * a.sl + (long)c * a.sc];
         }
       }
     }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+      const int it = tid + 256 * q;
+      if (it >= ITEMS) continue;
+      const int fl = it & 63, rest = it >> 6;  // rest = (i * 2 + r) * NCT + ct
+      T tt[NTB][8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        T tmp[NTB];
+        bsplit<FORM>((float)raw[q][e] * dsc[q], tmp);
+#pragma unroll
+        for (int t = 0; t < NTB; t++) tt[t][e] = tmp[t];
+      }
+      const int ct = rest % NCT, r = (rest / NCT) & 1, i = rest / NCT / 2;
+      T* dst = sB[buf] + ((((i * NCT + ct) * 2 + r) * NTB) * 512) + fl * 8;
+#pragma unroll
+      for (int t = 0; t < NTB; t++) {
+        V8 v;
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = tt[t][e];
+        *reinterpret_cast<V8*>(dst + t * 512) = v;
+      }
+    }
+  };
+
+  float4_t acc[NG][MAXT][NCT][2][2];
+#pragma unroll
+  for (int g = 0; g < NG; g++)
+#pragma unroll
+    for (int tt = 0; tt < MAXT; tt++)
+#pragma unroll
+      for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+        for (int r = 0; r < 2; r++) acc[g][tt][ct][r][0] = acc[g][tt][ct][r][1] = float4_t{0, 0, 0, 0};
+
+  V8 A[2][2][MAXT][NTA];  // [slot][P/H][tile][term]
+  auto load_a = [&](int ch, int slot) {
+    const int p = ch >= nch0 ? 1 : 0, ks = p ? ch - nch0 : ch, nchp = p ? nch1 : nch0;
+    const long pbase = toff + (p ? (long)a.J2p32 * kq0 : 0);
+#pragma unroll
+    for (int tt = 0; tt < MAXT; tt++) {
+      const int t = t0 + tt;
+      const long o = pbase + ((long)t * nchp + ks) * 512 + lane * 8;
+#pragma unroll
+      for (int k = 0; k < NTA; k++) {
+        if constexpr (UP) A[slot][0][tt][k] = t < a.RT ? *reinterpret_cast<const V8*>(a.tP[k] + o) : V8{};
+        if constexpr (UH) A[slot][1][tt][k] = t < a.RT ? *reinterpret_cast<const V8*>(a.tH[k] + o) : V8{};
+      }
+    }
+  };
+  auto compute = [&](auto PAR, int slot, int buf) {
+    constexpr int PP = decltype(PAR)::value;
+    if (t0 >= a.RT) return;
+    V8 B[NIN][NCT][2][NTB];
+#pragma unroll
+    for (int i = 0; i < NIN; i++)
+#pragma unroll
+      for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+#pragma unroll
+          for (int t = 0; t < NTB; t++)
+            B[i][ct][r][t] =
+                *reinterpret_cast<const V8*>(sB[buf] + ((((i * NCT + ct) * 2 + r) * NTB + t) * 512) + lane * 8);
+#pragma unroll
+    for (int gi = 0; gi < NG; gi++) {
+      const int tb = inv_tab(OP, gi), in = inv_in(OP, gi);
+      const int sidx = tb == 0 ? PP : 1 - PP;
+#pragma unroll
+      for (int tt = 0; tt < MAXT; tt++)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+          for (int r = 0; r < 2; r++)
+#pragma unroll
+            for (int pr = 0; pr < X::NP; pr++)
+              acc[gi][tt][ct][r][sidx] =
+                  mfma16(A[slot][tb][tt][X::pa(pr)], B[in][ct][r][X::pb(pr)], acc[gi][tt][ct][r][sidx]);
+    }
+  };
+  auto step = [&](int ch, int slot) {
+    if (ch + 1 < nch) {
+      gather(ch + 1);
+      load_a(ch + 1, slot ^ 1);
+    }
+    if (ch < nch0)
+      compute(std::integral_constant<int, 0>(), slot, ch & 1);
+    else
+      compute(std::integral_constant<int, 1>(), slot, ch & 1);
+    if (ch + 1 < nch) stage((ch + 1) & 1);
+    __syncthreads();
+  };
+
+  gather(0);
+  load_a(0, 0);
+  stage(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ch += 2) {
+    step(ch, 0);
+    if (ch + 1 < nch) step(ch + 1, 1);
   }
+  if (t0 < a.RT) inv_epilogue<OP, FORM, NCT, MAXT>(a, acc, m, t0, ct0, lane);
 }
 
 // ---------------------------------------------------------------- forward contraction
@@ -630,9 +817,11 @@ int inv_launch(const dsht_plan* p, const MixedTables& t, const Mx2InvJob& j, hip
   a.CG = cdiv(a.CT, NCT);
   a.RG = cdiv(a.RT, 4 * MAXT);
   const long nb = 8l * cdiv(p->N + 1, 8) * a.CG * a.RG;
-  if (j.pack) {
-    const long pw = (long)t.nchunks * a.CT * NIN;  // packing waves
-    hipLaunchKernelGGL((k_mx2_ipack<FORM, NIN>), dim3((unsigned)cdivl(pw, 4)), dim3(256), 0, s, a);
+  if (j.staged) {
+    hipLaunchKernelGGL((k_mx2_inv_lds<OP, FORM, NCT, MAXT>), dim3((unsigned)nb), dim3(256), 0, s, a);
+  } else if (j.pack) {
+    const long pt = 4l * a.CT * TILE * t.nchunks * NIN;  // packing threads
+    hipLaunchKernelGGL((k_mx2_ipack<FORM, NIN>), dim3((unsigned)cdivl(pt, 256)), dim3(256), 0, s, a);
     DSHT_HIP(hipGetLastError());
     hipLaunchKernelGGL((k_mx2_inv<OP, FORM, NCT, MAXT, true>), dim3((unsigned)nb), dim3(256), 0, s, a);
   } else {

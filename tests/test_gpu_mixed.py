@@ -25,15 +25,19 @@ def _prec(name):
     return {v: k for k, v in dsht.PREC_NAMES.items()}[name]
 
 
+# contraction kernel families (dsht_set_tuning DSHT_TUNE_LEG): packed panels, LDS-staged
+# (first design), in-place split, LDS-shared split
+@pytest.mark.parametrize("leg", [0, 1, 2, 3])
 @pytest.mark.parametrize("form", list(BOUND))
 @pytest.mark.parametrize("N,I,J", [(21, 64, 32), (42, 128, 64), (31, 96, 48)])
-def test_mixed_transforms(gpu, form, N, I, J):
+def test_mixed_transforms(gpu, form, N, I, J, leg):
     import dsht
     import oracle
     R = 6.371e6
     p = dsht.Plan(N, I, J, radius=R)
     o = oracle.Oracle(N, I, J, R)
     p.set_precision(_prec(form))
+    p.set_tuning(dsht.TUNE_LEG, leg)
     tol = BOUND[form]
     w = red_spectrum(N, 6, 3) * 1e-5  # magnitudes of a vorticity field
     for kind in ("VAL", "GRADLAMBDA", "GRADMU", "GRADLAT"):
@@ -56,13 +60,15 @@ def test_mixed_transforms(gpu, form, N, I, J):
     assert rel_l2(u, ru) < tol and rel_l2(v, rv2) < tol
 
 
+@pytest.mark.parametrize("leg", [0, 3])
 @pytest.mark.parametrize("form", ["bf16x6", "fp16x3"])
-def test_mixed_t170_roundtrip(gpu, form):
+def test_mixed_t170_roundtrip(gpu, form, leg):
     """T170 round trip and inverse accuracy at the metric's tolerance (1e-6)."""
     import torch
     import dsht
     N, I, J, nc = 170, 512, 256, 40
     p = dsht.Plan(N, I, J, radius=6.371e6)
+    p.set_tuning(dsht.TUNE_LEG, leg)
     w = torch.tensor(red_spectrum(N, nc, 21).T.copy(), device=gpu)
     ref = torch.empty((nc, J, I), dtype=torch.float64, device=gpu)
     p.inv_d(dsht.VAL, w, ref)
