@@ -115,7 +115,39 @@ int workspace_acquire(dsht_plan* p, hipStream_t s, size_t bytes, WsHandle& out) 
     ws->bytes = bytes;
   }
   out.buf = ws->buf;
+  out.ws = ws;
   return 0;
+}
+
+int pipe_begin(WsHandle& wh, hipStream_t s, const dsht_plan* p, int nchunks, ChunkPipe& cp) {
+  cp.on = p->pipeline && nchunks > 1 && wh.ws;
+  const int device = p->device;
+  cp.s = s;
+  if (!cp.on) return 0;
+  Workspace* ws = wh.ws;
+  if (!ws->aux) {
+    DSHT_HIP(hipSetDevice(device));
+    DSHT_HIP(hipStreamCreateWithFlags(&ws->aux, hipStreamNonBlocking));
+    for (auto& e : ws->ev) DSHT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  cp.aux = ws->aux;
+  cp.evL[0] = ws->ev[0];
+  cp.evL[1] = ws->ev[1];
+  cp.evF[0] = ws->ev[2];
+  cp.evF[1] = ws->ev[3];
+  cp.evX = ws->ev[4];
+  return pipe_after(cp.evX, s, cp.aux);  // fork
+}
+
+int pipe_after(hipEvent_t e, hipStream_t from, hipStream_t to) {
+  DSHT_HIP(hipEventRecord(e, from));
+  DSHT_HIP(hipStreamWaitEvent(to, e, 0));
+  return 0;
+}
+
+int pipe_join(ChunkPipe& cp) {
+  if (!cp.on) return 0;
+  return pipe_after(cp.evX, cp.aux, cp.s);
 }
 
 // column chunk: keeps each Fourier scratch buffer near 48 MiB (MALL resident)
@@ -144,15 +176,26 @@ static int run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, l
   if (ncols <= 0) return 0;
   const int nin = (op == F_ALPHA || op == F_UV2VORDIV) ? 2 : 1;
   const int ch = chunk_cols(p, ncols);
+  const int nchunks = cdiv(ncols, ch);
+  const size_t slot_bytes = nin * fourier_bytes(p, ch);
   WsHandle wl;
-  int rc = workspace_acquire(p, s, nin * fourier_bytes(p, ch), wl);
+  int rc = workspace_acquire(p, s, (p->pipeline && nchunks > 1 ? 2 : 1) * slot_bytes, wl);
   if (rc) return rc;
-  double2* G0 = (double2*)wl.buf;
-  double2* G1 = G0 + (size_t)(p->N + 1) * p->J * ch;
-  for (int c0 = 0; c0 < ncols; c0 += ch) {
-    int nc = ncols - c0 < ch ? ncols - c0 : ch;
-    if ((rc = launch_fft_fwd(p, in0 + (long)c0 * ldg, ldg, nc, G0, nc, s))) return rc;
-    if (nin > 1 && (rc = launch_fft_fwd(p, in1 + (long)c0 * ldg, ldg, nc, G1, nc, s))) return rc;
+  ChunkPipe cp;
+  if ((rc = pipe_begin(wl, s, p, nchunks, cp))) return rc;
+  for (int k = 0, c0 = 0; c0 < ncols; k++, c0 += ch) {
+    const int nc = ncols - c0 < ch ? ncols - c0 : ch;
+    const int slot = cp.on ? (k & 1) : 0;
+    double2* G0 = (double2*)((char*)wl.buf + slot * slot_bytes);
+    double2* G1 = G0 + (size_t)(p->N + 1) * p->J * ch;
+    hipStream_t fs = s;
+    if (cp.on) {
+      fs = cp.aux;
+      if (k >= 2) DSHT_HIP(hipStreamWaitEvent(fs, cp.evL[slot], 0));  // slot free again
+    }
+    if ((rc = launch_fft_fwd(p, in0 + (long)c0 * ldg, ldg, nc, G0, nc, fs))) return rc;
+    if (nin > 1 && (rc = launch_fft_fwd(p, in1 + (long)c0 * ldg, ldg, nc, G1, nc, fs))) return rc;
+    if (cp.on && (rc = pipe_after(cp.evF[slot], fs, s))) return rc;
     LegFwdArgs a;
     a.G0 = G0;
     a.G1 = nin > 1 ? G1 : nullptr;
@@ -174,8 +217,9 @@ static int run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, l
     a.J2 = p->J2;
     a.J2pad = p->J2pad;
     if ((rc = launch_leg_fwd(p, op, a, s))) return rc;
+    if (cp.on) DSHT_HIP(hipEventRecord(cp.evL[slot], s));
   }
-  return 0;
+  return pipe_join(cp);
 }
 
 // inverse family: spectra -> grids
@@ -185,13 +229,19 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
   if (ncols <= 0) return 0;
   const int nout = (op == I_VORDIV2UV) ? 2 : 1;
   const int ch = chunk_cols(p, ncols);
+  const int nchunks = cdiv(ncols, ch);
+  const size_t slot_bytes = nout * fourier_bytes(p, ch);
   WsHandle wl;
-  int rc = workspace_acquire(p, s, nout * fourier_bytes(p, ch), wl);
+  int rc = workspace_acquire(p, s, (p->pipeline && nchunks > 1 ? 2 : 1) * slot_bytes, wl);
   if (rc) return rc;
-  double2* G0 = (double2*)wl.buf;
-  double2* G1 = G0 + (size_t)(p->N + 1) * p->J * ch;
-  for (int c0 = 0; c0 < ncols; c0 += ch) {
-    int nc = ncols - c0 < ch ? ncols - c0 : ch;
+  ChunkPipe cp;
+  if ((rc = pipe_begin(wl, s, p, nchunks, cp))) return rc;
+  for (int k = 0, c0 = 0; c0 < ncols; k++, c0 += ch) {
+    const int nc = ncols - c0 < ch ? ncols - c0 : ch;
+    const int slot = cp.on ? (k & 1) : 0;
+    double2* G0 = (double2*)((char*)wl.buf + slot * slot_bytes);
+    double2* G1 = G0 + (size_t)(p->N + 1) * p->J * ch;
+    if (cp.on && k >= 2) DSHT_HIP(hipStreamWaitEvent(s, cp.evF[slot], 0));  // slot free again
     LegInvArgs a;
     a.W0 = in0 + (long)c0 * sc;
     a.W1 = in1 ? in1 + (long)c0 * sc : nullptr;
@@ -212,10 +262,16 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
     a.J2 = p->J2;
     a.J2pad = p->J2pad;
     if ((rc = launch_leg_inv(p, op, a, s))) return rc;
-    if ((rc = launch_fft_inv(p, G0, nc, nc, out0 + (long)c0 * ldg, ldg, s))) return rc;
-    if (nout > 1 && (rc = launch_fft_inv(p, G1, nc, nc, out1 + (long)c0 * ldg, ldg, s))) return rc;
+    hipStream_t fs = s;
+    if (cp.on) {
+      fs = cp.aux;
+      if ((rc = pipe_after(cp.evL[slot], s, fs))) return rc;
+    }
+    if ((rc = launch_fft_inv(p, G0, nc, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (nout > 1 && (rc = launch_fft_inv(p, G1, nc, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (cp.on) DSHT_HIP(hipEventRecord(cp.evF[slot], fs));
   }
-  return 0;
+  return pipe_join(cp);
 }
 
 static int check_grid_ptr(const void* x, long ldg, const dsht_plan* p) {
@@ -414,8 +470,13 @@ int dsht_destroy(dsht_plan* p) {
     g_serial.erase(p);
   }
   (void)hipDeviceSynchronize();
-  for (auto& kv : p->ws)
-    if (kv.second && kv.second->buf) (void)hipFree(kv.second->buf);
+  for (auto& kv : p->ws) {
+    if (!kv.second) continue;
+    if (kv.second->buf) (void)hipFree(kv.second->buf);
+    for (auto& e : kv.second->ev)
+      if (e) (void)hipEventDestroy(e);
+    if (kv.second->aux) (void)hipStreamDestroy(kv.second->aux);
+  }
   double* ds[] = {p->d_mu, p->d_wq, p->d_wl, p->d_cwv, p->d_cwd, p->d_rs_icos, p->d_rs_alpha, p->d_rs_uv,
                   p->d_pro_psi, p->d_tabFP, p->d_tabFH, p->d_tabIP, p->d_tabIH};
   for (double* d : ds)

@@ -120,6 +120,10 @@ struct Workspace {
   void* buf = nullptr;
   size_t bytes = 0;
   std::mutex mu;
+  // chunk pipeline of multi-chunk calls: a helper stream for the FFT stage and the events
+  // that order the two stages over two scratch slots (created on first use)
+  hipStream_t aux = nullptr;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 struct HostCtx;  // per-thread host-API context (api.hip)
@@ -127,7 +131,19 @@ struct HostCtx;  // per-thread host-API context (api.hip)
 // a locked scratch buffer of the (plan, stream) pair; unlocks when destroyed
 struct WsHandle {
   void* buf = nullptr;
+  Workspace* ws = nullptr;
   std::unique_lock<std::mutex> lk;
+};
+
+// Two-stage pipeline over the column chunks of one call.  Stage L (the Legendre
+// contraction) runs on the caller's stream, stage F (the FFT) on the workspace's helper
+// stream; chunk k uses scratch slot k%2, which chunk k+2 reuses only after both stages of
+// chunk k are done with it.  fork() makes the helper stream see the caller's earlier work,
+// join() makes the caller's stream see all of the helper's.
+struct ChunkPipe {
+  bool on = false;
+  hipStream_t s = nullptr, aux = nullptr;
+  hipEvent_t evL[2] = {nullptr, nullptr}, evF[2] = {nullptr, nullptr}, evX = nullptr;
 };
 
 // ---------------------------------------------------------------- plan
@@ -167,10 +183,13 @@ struct dsht_plan {
   void* mixed = nullptr;
   int chunk_cols = 0;  // column chunk of the device path (0 = auto)
   int leg_cb = 0;      // column block of the contractions (0 = auto, 16, 32)
+  int pipeline = 0;    // 1 = two-stream chunk pipeline (FFT stage on a helper stream)
   int fft_kind = 0;    // 0 = register two-pass FFT (256-thread blocks) where available, 1 = LDS
                        // Stockham, 2 = register FFT in 128-thread blocks
-  int leg_kind = 0;    // split-precision contractions: 0 = fragment streams over packed data
-                       // panels, 1 = LDS-staged, 2 = fragment streams splitting data in place
+  int leg_kind = 0;    // contractions.  FP64: 0/1 = LDS-staged, 2 = per-wave fragment streams.
+                       // Split forms: 0 = fragment streams over packed data panels, 1 = LDS-
+                       // staged, 2 = fragment streams splitting data in place, 3 = split data
+                       // shared through LDS
 };
 
 namespace dsht {
@@ -186,6 +205,9 @@ int fail(int code, const char* fmt, ...);
   } while (0)
 
 // api.hip: per-stream scratch
+int pipe_begin(WsHandle& wh, hipStream_t s, const dsht_plan* p, int nchunks, ChunkPipe& cp);
+int pipe_after(hipEvent_t e, hipStream_t from, hipStream_t to);  // record on from, to waits
+int pipe_join(ChunkPipe& cp);
 int workspace_acquire(dsht_plan* p, hipStream_t s, size_t bytes, WsHandle& out);
 // tables.hip
 int build_tables(dsht_plan* p);
@@ -203,6 +225,9 @@ template <typename GT>
 int reg_fft_fwd(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s);
 template <typename GT>
 int reg_fft_inv(const GT* G, const FftArgs& a, double* xy, hipStream_t s);
+// legendre_f64d.hip: FP64 contractions as per-wave fragment streams (DSHT_TUNE_LEG = 2)
+int launch_leg_inv_direct(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t s);
+int launch_leg_fwd_direct(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t s);
 int launch_fft_fwd32(const dsht_plan* p, const double* xy, long ldg, int ncols, float2* G, long gld,
                      unsigned* colmax, hipStream_t s);
 int launch_fft_inv32(const dsht_plan* p, const float2* G, long gld, int ncols, double* xy, long ldg,

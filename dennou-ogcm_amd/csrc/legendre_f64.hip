@@ -533,6 +533,7 @@ int launch_leg_fwd(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t 
   static const int nin[] = {1, 1, 1, 2, 2}, nout[] = {1, 1, 1, 1, 2}, nt[] = {1, 1, 1, 2, 4}, ntab[] = {1, 1, 1, 2, 2};
   leg_work(p, nin[op], nout[op], nt[op], ntab[op], true, a.ncols, &by, &fl);
   ProfScope ps("leg_fwd", s, by, fl);
+  if (p->leg_kind == 2) return launch_leg_fwd_direct(p, op, a, s);
   switch (op) {
     case F_VAL: return wide ? fwd_pick<F_VAL, 32>(p, a, s) : fwd_pick<F_VAL, 16>(p, a, s);
     case F_DIVLAMBDA: return wide ? fwd_pick<F_DIVLAMBDA, 32>(p, a, s) : fwd_pick<F_DIVLAMBDA, 16>(p, a, s);
@@ -549,6 +550,7 @@ int launch_leg_inv(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t 
   static const int nin[] = {1, 1, 1, 2, 2}, nout[] = {1, 1, 1, 1, 2}, nt[] = {1, 1, 1, 2, 4}, ntab[] = {1, 1, 1, 2, 2};
   leg_work(p, nin[op], nout[op], nt[op], ntab[op], false, a.ncols, &by, &fl);
   ProfScope ps("leg_inv", s, by, fl);
+  if (p->leg_kind == 2) return launch_leg_inv_direct(p, op, a, s);
   switch (op) {
     case I_VAL: return wide ? inv_pick<I_VAL, 32>(p, a, s) : inv_pick<I_VAL, 16>(p, a, s);
     case I_GRADLAMBDA: return wide ? inv_pick<I_GRADLAMBDA, 32>(p, a, s) : inv_pick<I_GRADLAMBDA, 16>(p, a, s);

@@ -827,24 +827,35 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
   const int nin = (op == F_ALPHA || op == F_UV2VORDIV) ? 2 : 1;
   const bool scaled = form >= 2;
   const int ch = mixed_chunk_cols(p, ncols);
+  const int nchunks = cdiv(ncols, ch);
   const size_t gsz = (size_t)(p->N + 1) * p->J * ch;
-  size_t bytes = nin * gsz * sizeof(float2) + 2 * (size_t)ch * sizeof(unsigned) + 256;
+  const size_t cmb = (2 * (size_t)ch * sizeof(unsigned) + 255) / 256 * 256;
+  const size_t slot_bytes = nin * gsz * sizeof(float2) + cmb;  // Fourier inputs + column maxima
   WsHandle wh;
-  if (int rc = workspace_acquire(p, s, bytes, wh)) return rc;
-  float2* G0 = (float2*)wh.buf;
-  float2* G1 = G0 + gsz;
-  unsigned* colmax = (unsigned*)(G0 + nin * gsz);
+  if (int rc = workspace_acquire(p, s, (p->pipeline && nchunks > 1 ? 2 : 1) * slot_bytes, wh)) return rc;
+  ChunkPipe cp;
+  if (int rc = pipe_begin(wh, s, p, nchunks, cp)) return rc;
   const bool vdiv = cw != p->d_cwv;
   const float bound = weight_bound(p, vdiv);
-  for (int c0 = 0; c0 < ncols; c0 += ch) {
+  for (int k = 0, c0 = 0; c0 < ncols; k++, c0 += ch) {
     const int nc = ncols - c0 < ch ? ncols - c0 : ch;
+    const int slot = cp.on ? (k & 1) : 0;
+    float2* G0 = (float2*)((char*)wh.buf + slot * slot_bytes);
+    float2* G1 = G0 + gsz;
+    unsigned* colmax = (unsigned*)(G0 + nin * gsz);
+    hipStream_t fs = s;  // FFT stage
+    if (cp.on) {
+      fs = cp.aux;
+      if (k >= 2) DSHT_HIP(hipStreamWaitEvent(fs, cp.evL[slot], 0));  // slot free again
+    }
     if (scaled) {
-      hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(2 * nc, 256)), dim3(256), 0, s, colmax, 2 * nc);
+      hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(2 * nc, 256)), dim3(256), 0, fs, colmax, 2 * nc);
       DSHT_HIP(hipGetLastError());
     }
-    int rc = launch_fft_fwd32(p, in0 + (long)c0 * ldg, ldg, nc, G0, nc, scaled ? colmax : nullptr, s);
-    if (!rc && nin > 1) rc = launch_fft_fwd32(p, in1 + (long)c0 * ldg, ldg, nc, G1, nc, scaled ? colmax + nc : nullptr, s);
+    int rc = launch_fft_fwd32(p, in0 + (long)c0 * ldg, ldg, nc, G0, nc, scaled ? colmax : nullptr, fs);
+    if (!rc && nin > 1) rc = launch_fft_fwd32(p, in1 + (long)c0 * ldg, ldg, nc, G1, nc, scaled ? colmax + nc : nullptr, fs);
     if (rc) return rc;
+    if (cp.on && (rc = pipe_after(cp.evF[slot], fs, s))) return rc;
     MxFwdArgs a;
     a.G0 = G0;
     a.G1 = nin > 1 ? G1 : nullptr;
@@ -855,9 +866,9 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
     a.ncols = nc;
     a.gld = nc;
     a.gmst = (long)p->J * nc;
-    for (int k = 0; k < 3; k++) {
-      a.tP[k] = t.fP[k];
-      a.tH[k] = t.fH[k];
+    for (int q = 0; q < 3; q++) {
+      a.tP[q] = t.fP[q];
+      a.tH[q] = t.fH[q];
     }
     a.off = t.d_offF;
     a.cw = vdiv ? t.d_cwd : t.d_cwv;
@@ -870,38 +881,41 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
     a.N = p->N;
     a.J2 = p->J2;
     a.J2p32 = t.J2p32;
-    double by, fl;
-    mixed_work(p, op, true, nc, &by, &fl);
-    ProfScope ps("leg_fwd", s, by, fl);
-    if (p->leg_kind != 1) {
-      Mx2FwdJob j;
-      j.G0 = G0;
-      j.G1 = nin > 1 ? G1 : nullptr;
-      j.gld = nc;
-      j.gmst = (long)p->J * nc;
-      j.ncols = nc;
-      j.W0 = a.W0;
-      j.W1 = a.W1;
-      j.sl = sl;
-      j.sc = sc;
-      j.cw = a.cw;
-      j.colmax = colmax;
-      j.bound = bound;
-      j.diag0 = diag0;
-      j.diag1 = diag1;
-      j.scale = scale;
-      if ((rc = mx2_fwd(p, t, form, op, j, s))) return rc;
-      continue;
-    }
-    switch (form) {
-      case 0: rc = mx_fwd_op<0>(p, op, a, s); break;
-      case 1: rc = mx_fwd_op<1>(p, op, a, s); break;
-      case 2: rc = mx_fwd_op<2>(p, op, a, s); break;
-      default: rc = mx_fwd_op<3>(p, op, a, s); break;
+    {
+      double by, fl;
+      mixed_work(p, op, true, nc, &by, &fl);
+      ProfScope ps("leg_fwd", s, by, fl);
+      if (p->leg_kind != 1) {
+        Mx2FwdJob j;
+        j.G0 = G0;
+        j.G1 = nin > 1 ? G1 : nullptr;
+        j.gld = nc;
+        j.gmst = (long)p->J * nc;
+        j.ncols = nc;
+        j.W0 = a.W0;
+        j.W1 = a.W1;
+        j.sl = sl;
+        j.sc = sc;
+        j.cw = a.cw;
+        j.colmax = colmax;
+        j.bound = bound;
+        j.diag0 = diag0;
+        j.diag1 = diag1;
+        j.scale = scale;
+        rc = mx2_fwd(p, t, form, op, j, s);
+      } else {
+        switch (form) {
+          case 0: rc = mx_fwd_op<0>(p, op, a, s); break;
+          case 1: rc = mx_fwd_op<1>(p, op, a, s); break;
+          case 2: rc = mx_fwd_op<2>(p, op, a, s); break;
+          default: rc = mx_fwd_op<3>(p, op, a, s); break;
+        }
+      }
     }
     if (rc) return rc;
+    if (cp.on) DSHT_HIP(hipEventRecord(cp.evL[slot], s));
   }
-  return 0;
+  return pipe_join(cp);
 }
 
 int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, long sl, long sc,
@@ -917,20 +931,27 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
   const int nout = (op == I_VORDIV2UV) ? 2 : 1;
   const bool scaled = form >= 2;
   const int ch = mixed_chunk_cols(p, ncols);
+  const int nchunks = cdiv(ncols, ch);
   const size_t gsz = (size_t)(p->N + 1) * p->J * ch;
+  const size_t slot_bytes = nout * gsz * sizeof(float2);  // Fourier outputs of one chunk
+  const int nslot = p->pipeline && nchunks > 1 ? 2 : 1;
   const size_t cmb = (2 * (size_t)ch * sizeof(unsigned) + 255) / 256 * 256;
   const size_t pkb = p->leg_kind == 0 ? mx2_inv_pack_bytes(t, form, op, ch) : 0;
-  size_t bytes = nout * gsz * sizeof(float2) + cmb + pkb;
   WsHandle wh;
-  if (int rc = workspace_acquire(p, s, bytes, wh)) return rc;
-  float2* G0 = (float2*)wh.buf;
-  float2* G1 = G0 + gsz;
-  unsigned* colmax = (unsigned*)(G0 + nout * gsz);
+  if (int rc = workspace_acquire(p, s, nslot * slot_bytes + cmb + pkb, wh)) return rc;
+  ChunkPipe cp;
+  if (int rc = pipe_begin(wh, s, p, nchunks, cp)) return rc;
+  // column maxima and data panels are used by the contraction stage only (caller's stream)
+  unsigned* colmax = (unsigned*)((char*)wh.buf + nslot * slot_bytes);
   uint16_t* pack = (uint16_t*)((char*)colmax + cmb);
-  for (int c0 = 0; c0 < ncols; c0 += ch) {
+  for (int k = 0, c0 = 0; c0 < ncols; k++, c0 += ch) {
     const int nc = ncols - c0 < ch ? ncols - c0 : ch;
+    const int slot = cp.on ? (k & 1) : 0;
+    float2* G0 = (float2*)((char*)wh.buf + slot * slot_bytes);
+    float2* G1 = G0 + gsz;
     const double* w0 = in0 + (long)c0 * sc;
     const double* w1 = in1 ? in1 + (long)c0 * sc : nullptr;
+    if (cp.on && k >= 2) DSHT_HIP(hipStreamWaitEvent(s, cp.evF[slot], 0));  // slot free again
     if (scaled) {
       hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(2 * nc, 256)), dim3(256), 0, s, colmax, 2 * nc);
       DSHT_HIP(hipGetLastError());
@@ -948,9 +969,9 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
     a.ncols = nc;
     a.gld = nc;
     a.gmst = (long)p->J * nc;
-    for (int k = 0; k < 3; k++) {
-      a.tP[k] = t.iP[k];
-      a.tH[k] = t.iH[k];
+    for (int q = 0; q < 3; q++) {
+      a.tP[q] = t.iP[q];
+      a.tH[q] = t.iH[q];
     }
     a.off = t.d_offI;
     a.minv = t.d_minv;
@@ -982,19 +1003,26 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
         j.gld = nc;
         j.gmst = (long)p->J * nc;
         rc = mx2_inv(p, t, form, op, j, s);
-      } else
-      switch (form) {
-        case 0: rc = mx_inv_op<0>(p, op, a, s); break;
-        case 1: rc = mx_inv_op<1>(p, op, a, s); break;
-        case 2: rc = mx_inv_op<2>(p, op, a, s); break;
-        default: rc = mx_inv_op<3>(p, op, a, s); break;
+      } else {
+        switch (form) {
+          case 0: rc = mx_inv_op<0>(p, op, a, s); break;
+          case 1: rc = mx_inv_op<1>(p, op, a, s); break;
+          case 2: rc = mx_inv_op<2>(p, op, a, s); break;
+          default: rc = mx_inv_op<3>(p, op, a, s); break;
+        }
       }
     }
     if (rc) return rc;
-    if ((rc = launch_fft_inv32(p, G0, nc, nc, out0 + (long)c0 * ldg, ldg, s))) return rc;
-    if (nout > 1 && (rc = launch_fft_inv32(p, G1, nc, nc, out1 + (long)c0 * ldg, ldg, s))) return rc;
+    hipStream_t fs = s;  // FFT stage
+    if (cp.on) {
+      fs = cp.aux;
+      if ((rc = pipe_after(cp.evL[slot], s, fs))) return rc;
+    }
+    if ((rc = launch_fft_inv32(p, G0, nc, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (nout > 1 && (rc = launch_fft_inv32(p, G1, nc, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (cp.on) DSHT_HIP(hipEventRecord(cp.evF[slot], fs));
   }
-  return 0;
+  return pipe_join(cp);
 }
 
 }  // namespace dsht

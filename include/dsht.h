@@ -164,9 +164,13 @@ int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, d
 #define DSHT_TUNE_LEG_CB 1     /* 0 = automatic, 16 or 32 */
 #define DSHT_TUNE_FFT 2        /* 0 = register two-pass kernel where the length has one (256-thread
                                   blocks), 1 = LDS Stockham, 2 = register kernel, 128-thread blocks */
-#define DSHT_TUNE_LEG 3        /* split-precision contractions: 0 = per-wave fragment streams over
-                                  pre-packed data panels, 1 = LDS-staged, 2 = fragment streams
-                                  splitting the data in place */
+#define DSHT_TUNE_LEG 3        /* Legendre contractions.  FP64: 0 or 1 = LDS-staged, 2 = per-wave
+                                  fragment streams.  Split forms: 0 = per-wave fragment streams
+                                  over pre-packed data panels, 1 = LDS-staged, 2 = fragment
+                                  streams splitting the data in place, 3 = split data shared
+                                  through LDS */
+#define DSHT_TUNE_PIPELINE 4   /* 1 = run each call's column chunks as a two-stream pipeline (FFT of
+                                  chunk k on a helper stream beside the contraction of chunk k+1) */
 int dsht_set_tuning(dsht_plan* plan, int key, int value);
 int dsht_get_tuning(const dsht_plan* plan, int key);
 
