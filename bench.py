@@ -322,7 +322,8 @@ def batched_bench(torch, dsht, n_inst=64, steps=10, seq_steps=2, seed=0xB47C4, p
         for g, (p, gc, *_r) in views.items():
             p.set_precision(pid)
             t = autotune.tune(p, N_PROG * gc, use_cache=False)
-            tuning[f"T{p.nmax}"] = {"chunk": t["chunk"], "cb": t["cb"], "columns": N_PROG * gc}
+            tuning[f"T{p.nmax}"] = {"chunk": t["chunk"], "cb": t["cb"], "leg": t["leg"], "ms": t["ms"],
+                                    "candidates": len(t["candidates"]), "columns": N_PROG * gc}
     s.synchronize()
     b.step(ss)
     s.synchronize()

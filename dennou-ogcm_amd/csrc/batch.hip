@@ -239,7 +239,7 @@ int dsht_set_tuning(dsht_plan* p, int key, int value) {
       p->leg_cb = value;
       return 0;
     case DSHT_TUNE_FFT:
-      if (value < 0 || value > 2) return fail(DSHT_EARG, "fft kernel must be 0, 1 or 2");
+      if (value < 0 || value > 3) return fail(DSHT_EARG, "fft kernel must be 0, 1, 2 or 3");
       p->fft_kind = value;
       return 0;
     case DSHT_TUNE_PIPELINE:

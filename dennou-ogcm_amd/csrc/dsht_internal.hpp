@@ -113,6 +113,7 @@ struct FftArgs {
   int rows_per_block;
   const double* rowscale;  // optional per-latitude scale applied after the inverse (nullable)
   int threads;             // workgroup size of the register kernels (128 or 256)
+  int fp32;                // FP32 arithmetic (FP32 Fourier path of the split forms)
 };
 
 // ---------------------------------------------------------------- workspace
@@ -184,8 +185,9 @@ struct dsht_plan {
   int chunk_cols = 0;  // column chunk of the device path (0 = auto)
   int leg_cb = 0;      // column block of the contractions (0 = auto, 16, 32)
   int pipeline = 0;    // 1 = two-stream chunk pipeline (FFT stage on a helper stream)
-  int fft_kind = 0;    // 0 = register two-pass FFT (256-thread blocks) where available, 1 = LDS
-                       // Stockham, 2 = register FFT in 128-thread blocks
+  int fft_kind = 0;    // 0 = register two-pass FFT (256-thread blocks; FP32 arithmetic in the
+                       // split forms' inverse FFT), 1 = LDS Stockham, 2 = register FFT in
+                       // 128-thread blocks, 3 = register FFT with FP64 arithmetic in every form
   int leg_kind = 0;    // contractions.  FP64: 0/1 = LDS-staged, 2 = per-wave fragment streams.
                        // Split forms: 0 = fragment streams over packed data panels, 1 = LDS-
                        // staged, 2 = fragment streams splitting data in place, 3 = split data

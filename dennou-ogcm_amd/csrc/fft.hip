@@ -451,6 +451,7 @@ static FftArgs fft_args(const dsht_plan* p, long ldg, int ncols, long gld) {
   a.rows_per_block = fft_rows_per_block(p->fft.M);
   a.rowscale = nullptr;
   a.threads = p->fft_kind == 2 ? 128 : 256;
+  a.fp32 = 0;
   return a;
 }
 
@@ -489,6 +490,9 @@ template <typename GT>
 static int fft_fwd_any(const dsht_plan* p, const double* xy, long ldg, int ncols, GT* G, long gld,
                        unsigned* colmax, hipStream_t s) {
   FftArgs a = fft_args(p, ldg, ncols, gld);
+  // forward: FP64 arithmetic always (a large zonal mean would cost the m > 0 coefficients
+  // their relative precision in FP32)
+  a.fp32 = 0;
   long nrows = (long)p->J * ncols;
   ProfScope ps("fft_fwd", s, (double)nrows * (8.0 * p->I + (double)sizeof(GT) * (p->N + 1)),
                (double)nrows * 2.5 * p->I * std::log2((double)p->I));
@@ -512,6 +516,9 @@ template <typename GT>
 static int fft_inv_any(const dsht_plan* p, const GT* G, long gld, int ncols, double* xy, long ldg,
                        hipStream_t s) {
   FftArgs a = fft_args(p, ldg, ncols, gld);
+  // inverse: FP32 arithmetic on the split forms' FP32 Fourier coefficients (each output point
+  // is a sum of already-FP32 terms; the error stays at FP32 rounding of the output scale)
+  a.fp32 = (sizeof(GT) == 8 && p->fft_kind == 0) ? 1 : 0;
   long nrows = (long)p->J * ncols;
   ProfScope ps("fft_inv", s, (double)nrows * (8.0 * p->I + (double)sizeof(GT) * (p->N + 1)),
                (double)nrows * 2.5 * p->I * std::log2((double)p->I));

@@ -24,17 +24,47 @@ namespace dsht {
 
 namespace {
 
-__device__ __forceinline__ double2 rcmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// complex values in FP64 (double2) or FP32 (float2) arithmetic
+template <typename V>
+struct Cx;
+template <>
+struct Cx<double2> {
+  using S = double;
+  __device__ static double2 mk(double x, double y) { return make_double2(x, y); }
+};
+template <>
+struct Cx<float2> {
+  using S = float;
+  __device__ static float2 mk(float x, float y) { return make_float2(x, y); }
+};
+template <typename V>
+__device__ __forceinline__ V mk(typename Cx<V>::S x, typename Cx<V>::S y) {
+  return Cx<V>::mk(x, y);
 }
-__device__ __forceinline__ double2 rcmulc(double2 a, double2 b) {  // a * conj(b)
-  return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+template <typename V, typename W>
+__device__ __forceinline__ V cvt(W a) {  // between double2 and float2
+  using S = typename Cx<V>::S;
+  return Cx<V>::mk((S)a.x, (S)a.y);
 }
-__device__ __forceinline__ double2 radd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 rsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-template <int SIGN>  // multiply by SIGN * i
-__device__ __forceinline__ double2 rmul_i(double2 a) {
-  return SIGN < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+template <typename V>
+__device__ __forceinline__ V rcmul(V a, V b) {
+  return mk<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename V>
+__device__ __forceinline__ V rcmulc(V a, V b) {  // a * conj(b)
+  return mk<V>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename V>
+__device__ __forceinline__ V radd(V a, V b) {
+  return mk<V>(a.x + b.x, a.y + b.y);
+}
+template <typename V>
+__device__ __forceinline__ V rsub(V a, V b) {
+  return mk<V>(a.x - b.x, a.y - b.y);
+}
+template <int SIGN, typename V>  // multiply by SIGN * i
+__device__ __forceinline__ V rmul_i(V a) {
+  return SIGN < 0 ? mk<V>(a.y, -a.x) : mk<V>(-a.y, a.x);
 }
 
 // ---- compile-time twiddles: cos and sin of 2 pi e / R by reduction to [0, pi/4] and
@@ -78,20 +108,21 @@ constexpr double ct_cs(int e, int R, bool want_sin) {
 }
 
 // x * exp(SIGN 2 pi i E / R), E a compile-time constant
-template <int E, int R, int SIGN>
-__device__ __forceinline__ double2 twc(double2 x) {
+template <int E, int R, int SIGN, typename V>
+__device__ __forceinline__ V twc(V x) {
+  using S = typename Cx<V>::S;
   constexpr int e = ((E % R) + R) % R;
   if constexpr (e == 0) {
     return x;
   } else if constexpr (2 * e == R) {
-    return make_double2(-x.x, -x.y);
+    return mk<V>(-x.x, -x.y);
   } else if constexpr (4 * e == R) {
     return rmul_i<SIGN>(x);
   } else if constexpr (4 * e == 3 * R) {
     return rmul_i<-SIGN>(x);
   } else {
     constexpr double c = ct_cs(e, R, false), s = SIGN * ct_cs(e, R, true);
-    return make_double2(x.x * c - x.y * s, x.x * s + x.y * c);
+    return mk<V>(x.x * (S)c - x.y * (S)s, x.x * (S)s + x.y * (S)c);
   }
 }
 
@@ -117,38 +148,39 @@ struct Split {
 };
 
 // in-register DFT y_q = sum_p x_p exp(SIGN 2 pi i p q / R), natural order in and out
-template <int R, int SIGN>
-__device__ __forceinline__ void dft(double2* x) {
+template <int R, int SIGN, typename V>
+__device__ __forceinline__ void dft(V* x) {
+  using S = typename Cx<V>::S;
   if constexpr (R == 1) {
     return;
   } else if constexpr (R == 2) {
-    const double2 a = x[0], b = x[1];
+    const V a = x[0], b = x[1];
     x[0] = radd(a, b);
     x[1] = rsub(a, b);
   } else if constexpr (R == 4) {
-    const double2 t0 = radd(x[0], x[2]), t1 = rsub(x[0], x[2]);
-    const double2 t2 = radd(x[1], x[3]), t3 = rmul_i<SIGN>(rsub(x[1], x[3]));
+    const V t0 = radd(x[0], x[2]), t1 = rsub(x[0], x[2]);
+    const V t2 = radd(x[1], x[3]), t3 = rmul_i<SIGN>(rsub(x[1], x[3]));
     x[0] = radd(t0, t2);
     x[1] = radd(t1, t3);
     x[2] = rsub(t0, t2);
     x[3] = rsub(t1, t3);
   } else if constexpr (R == 3) {
-    constexpr double s3 = 0.86602540378443864676;
-    const double2 t1 = radd(x[1], x[2]), t2 = rsub(x[1], x[2]);
-    const double2 a = make_double2(x[0].x - 0.5 * t1.x, x[0].y - 0.5 * t1.y);
-    const double2 b = rmul_i<SIGN>(make_double2(s3 * t2.x, s3 * t2.y));
+    const S s3 = (S)0.86602540378443864676, h = (S)0.5;
+    const V t1 = radd(x[1], x[2]), t2 = rsub(x[1], x[2]);
+    const V a = mk<V>(x[0].x - h * t1.x, x[0].y - h * t1.y);
+    const V b = rmul_i<SIGN>(mk<V>(s3 * t2.x, s3 * t2.y));
     x[0] = radd(x[0], t1);
     x[1] = radd(a, b);
     x[2] = rsub(a, b);
   } else if constexpr (R == 5) {
-    constexpr double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-    constexpr double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
-    const double2 t1 = radd(x[1], x[4]), t4 = rsub(x[1], x[4]);
-    const double2 t2 = radd(x[2], x[3]), t3 = rsub(x[2], x[3]);
-    const double2 a1 = make_double2(x[0].x + c1 * t1.x + c2 * t2.x, x[0].y + c1 * t1.y + c2 * t2.y);
-    const double2 a2 = make_double2(x[0].x + c2 * t1.x + c1 * t2.x, x[0].y + c2 * t1.y + c1 * t2.y);
-    const double2 b1 = rmul_i<SIGN>(make_double2(s1 * t4.x + s2 * t3.x, s1 * t4.y + s2 * t3.y));
-    const double2 b2 = rmul_i<SIGN>(make_double2(s2 * t4.x - s1 * t3.x, s2 * t4.y - s1 * t3.y));
+    const S c1 = (S)0.30901699437494742410, c2 = (S)-0.80901699437494742410;
+    const S s1 = (S)0.95105651629515357212, s2 = (S)0.58778525229247312917;
+    const V t1 = radd(x[1], x[4]), t4 = rsub(x[1], x[4]);
+    const V t2 = radd(x[2], x[3]), t3 = rsub(x[2], x[3]);
+    const V a1 = mk<V>(x[0].x + c1 * t1.x + c2 * t2.x, x[0].y + c1 * t1.y + c2 * t2.y);
+    const V a2 = mk<V>(x[0].x + c2 * t1.x + c1 * t2.x, x[0].y + c2 * t1.y + c1 * t2.y);
+    const V b1 = rmul_i<SIGN>(mk<V>(s1 * t4.x + s2 * t3.x, s1 * t4.y + s2 * t3.y));
+    const V b2 = rmul_i<SIGN>(mk<V>(s2 * t4.x - s1 * t3.x, s2 * t4.y - s1 * t3.y));
     x[0] = radd(x[0], radd(t1, t2));
     x[1] = radd(a1, b1);
     x[4] = rsub(a1, b1);
@@ -158,10 +190,10 @@ __device__ __forceinline__ void dft(double2* x) {
     // R = P * Q: n = p + P q, k = k1 + Q k2
     constexpr int P = Split<R>::P, Q = R / P;
     static_assert(P > 1, "radix must factor into 2, 3, 4, 5");
-    double2 y[R];
+    V y[R];
     sfor<P>([&](auto pc) {
       constexpr int p = decltype(pc)::value;
-      double2 t[Q];
+      V t[Q];
 #pragma unroll
       for (int q = 0; q < Q; q++) t[q] = x[p + P * q];
       dft<Q, SIGN>(t);
@@ -172,7 +204,7 @@ __device__ __forceinline__ void dft(double2* x) {
     });
 #pragma unroll
     for (int k1 = 0; k1 < Q; k1++) {
-      double2 s[P];
+      V s[P];
 #pragma unroll
       for (int p = 0; p < P; p++) s[p] = y[p * Q + k1];
       dft<P, SIGN>(s);
@@ -181,14 +213,6 @@ __device__ __forceinline__ void dft(double2* x) {
     }
   }
 }
-
-__device__ __forceinline__ double2 ld_g(const double2* p) { return *p; }
-__device__ __forceinline__ double2 ld_g(const float2* p) {
-  const float2 v = *p;
-  return make_double2((double)v.x, (double)v.y);
-}
-__device__ __forceinline__ void st_g(double2* p, double2 v) { *p = v; }
-__device__ __forceinline__ void st_g(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
 
 constexpr int odd_up(int x) { return x | 1; }
 
@@ -199,52 +223,55 @@ __device__ __forceinline__ double2 zero_of<double2>() { return make_double2(0.0,
 template <>
 __device__ __forceinline__ float2 zero_of<float2>() { return make_float2(0.f, 0.f); }
 
-template <int R1, int R2, int NT>
+template <int R1, int R2, int NT, typename V>
 struct RegFft {
   static constexpr int M = R1 * R2;
   static constexpr int THREADS = NT;
   static constexpr int ROWS = THREADS / R1;                      // rows per workgroup
   static constexpr int PY = (R2 % 2 == 0) ? R2 + 1 : R2 + 2;     // padded pass-1 stride (odd)
   static constexpr int SR = odd_up(R1 * PY > M + 1 ? R1 * PY : M + 1);  // padded row (odd)
-  static constexpr size_t LDS = (size_t)(ROWS * SR + 2 * M) * sizeof(double2);
+  static constexpr size_t LDS = (size_t)(ROWS * SR + 2 * M) * sizeof(V);
 };
 
 }  // namespace
 
-template <int R1, int R2, int NT, typename GT>
+// V: arithmetic type (double2, or float2 for the FP32 Fourier path of the split forms)
+template <int R1, int R2, int NT, typename GT, typename V>
 __global__ __launch_bounds__(NT) void k_fft_fwd_r(const double* __restrict__ xy, GT* __restrict__ G,
                                                   FftArgs a, unsigned* __restrict__ colmax) {
-  using C = RegFft<R1, R2, NT>;
+  using C = RegFft<R1, R2, NT, V>;
+  using S = typename Cx<V>::S;
   constexpr int M = C::M, ROWS = C::ROWS, PY = C::PY, SR = C::SR;
-  extern __shared__ double2 lds[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  V* const lds = reinterpret_cast<V*>(lds_raw);
   __shared__ unsigned smax[ROWS];
-  double2* buf = lds;              // ROWS * SR
-  double2* tw2 = lds + ROWS * SR;  // W_M^{n1 k2} at k2 * R1 + n1
-  double2* twi = tw2 + M;          // W_I^m
+  V* buf = lds;              // ROWS * SR
+  V* tw2 = lds + ROWS * SR;  // W_M^{n1 k2} at k2 * R1 + n1
+  V* twi = tw2 + M;          // W_I^m
   const int tid = threadIdx.x;
   for (int e = tid; e < M; e += C::THREADS) {
     const int k2 = e / R1, n1 = e % R1;
-    tw2[e] = a.twM[(n1 * k2) % M];
-    twi[e] = a.twI[e];
+    tw2[e] = cvt<V>(a.twM[(n1 * k2) % M]);
+    twi[e] = cvt<V>(a.twI[e]);
   }
   const long nrows = (long)a.J * a.ncols;
   const int N = a.N;
-  const double inv_i = 1.0 / (2 * M);
+  const S inv_i = (S)(1.0 / (2 * M)), h = (S)0.5;
   const int rr1 = tid / R1, n1 = tid % R1;  // pass-1 role
   const int rr2 = tid / R2, k2 = tid % R2;  // pass-2 role
   // pass-1 inputs straight from global memory; the next group's rows are loaded into the
   // same registers once the current group's pass-1 output is in LDS
-  double2 v[R2];
+  V v[R2];
   auto fetch = [&](long rb) {
     const long r = rb + rr1;
     if (rr1 < ROWS && r < nrows) {
       const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
       const double2* x = reinterpret_cast<const double2*>(xy + c * a.ldg + (long)j * (2 * M));
 #pragma unroll
-      for (int n2 = 0; n2 < R2; n2++) v[n2] = x[n1 + R1 * n2];
+      for (int n2 = 0; n2 < R2; n2++) v[n2] = cvt<V>(x[n1 + R1 * n2]);
     } else {
 #pragma unroll
-      for (int n2 = 0; n2 < R2; n2++) v[n2] = make_double2(0.0, 0.0);
+      for (int n2 = 0; n2 < R2; n2++) v[n2] = mk<V>(0, 0);
     }
   };
   const long rstep = (long)gridDim.x * ROWS;
@@ -254,23 +281,23 @@ __global__ __launch_bounds__(NT) void k_fft_fwd_r(const double* __restrict__ xy,
     if (colmax && tid < ROWS) smax[tid] = 0u;
     __syncthreads();  // the previous rows are out of buf; twiddles are in place
     if (rr1 < ROWS) {
-      double2* Y = buf + rr1 * SR + n1 * PY;
+      V* Y = buf + rr1 * SR + n1 * PY;
       Y[0] = v[0];
 #pragma unroll
       for (int q = 1; q < R2; q++) Y[q] = rcmul(v[q], tw2[q * R1 + n1]);
     }
     if (r0 + rstep < nrows) fetch(r0 + rstep);
     __syncthreads();
-    double2 u[R1];
+    V u[R1];
     if (rr2 < ROWS) {
-      const double2* Y = buf + rr2 * SR + k2;
+      const V* Y = buf + rr2 * SR + k2;
 #pragma unroll
       for (int p = 0; p < R1; p++) u[p] = Y[p * PY];
       dft<R1, -1>(u);
     }
     __syncthreads();
     if (rr2 < ROWS) {
-      double2* X = buf + rr2 * SR;
+      V* X = buf + rr2 * SR;
 #pragma unroll
       for (int k1 = 0; k1 < R1; k1++) X[k2 + R2 * k1] = u[k1];
     }
@@ -280,17 +307,17 @@ __global__ __launch_bounds__(NT) void k_fft_fwd_r(const double* __restrict__ xy,
       const int m = idx / ROWS, rr = idx % ROWS;
       const long r = r0 + rr;
       if (r >= nrows) continue;
-      const double2* X = buf + rr * SR;
-      const double2 zm = X[m];
-      double2 zc = X[m ? M - m : 0];
+      const V* X = buf + rr * SR;
+      const V zm = X[m];
+      V zc = X[m ? M - m : 0];
       zc.y = -zc.y;
-      const double2 E = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
-      const double2 O = make_double2(0.5 * (zm.y - zc.y), -0.5 * (zm.x - zc.x));
-      double2 g = radd(E, rcmul(twi[m], O));
-      g = make_double2(g.x * inv_i, g.y * inv_i);
+      const V E = mk<V>(h * (zm.x + zc.x), h * (zm.y + zc.y));
+      const V O = mk<V>(h * (zm.y - zc.y), -h * (zm.x - zc.x));
+      V g = radd(E, rcmul(twi[m], O));
+      g = mk<V>(g.x * inv_i, g.y * inv_i);
       const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
-      st_g(&G[(long)m * a.gmst + (long)j * a.gld + c], g);
-      if (colmax) atomicMax(&smax[rr], __float_as_uint((float)fmax(fabs(g.x), fabs(g.y))));
+      G[(long)m * a.gmst + (long)j * a.gld + c] = cvt<GT>(g);
+      if (colmax) atomicMax(&smax[rr], __float_as_uint(fmaxf(fabsf((float)g.x), fabsf((float)g.y))));
     }
     if (colmax) {  // per-column magnitude for the scaled FP16 contraction
       __syncthreads();
@@ -299,20 +326,22 @@ __global__ __launch_bounds__(NT) void k_fft_fwd_r(const double* __restrict__ xy,
   }
 }
 
-template <int R1, int R2, int NT, typename GT>
+template <int R1, int R2, int NT, typename GT, typename V>
 __global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, double* __restrict__ xy,
                                                   FftArgs a) {
-  using C = RegFft<R1, R2, NT>;
+  using C = RegFft<R1, R2, NT, V>;
+  using S = typename Cx<V>::S;
   constexpr int M = C::M, ROWS = C::ROWS, PY = C::PY, SR = C::SR;
-  extern __shared__ double2 lds[];
-  double2* buf = lds;
-  double2* tw2 = lds + ROWS * SR;
-  double2* twi = tw2 + M;
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  V* const lds = reinterpret_cast<V*>(lds_raw);
+  V* buf = lds;
+  V* tw2 = lds + ROWS * SR;
+  V* twi = tw2 + M;
   const int tid = threadIdx.x;
   for (int e = tid; e < M; e += C::THREADS) {
     const int k2 = e / R1, n1 = e % R1;
-    tw2[e] = a.twM[(n1 * k2) % M];
-    twi[e] = a.twI[e];
+    tw2[e] = cvt<V>(a.twM[(n1 * k2) % M]);
+    twi[e] = cvt<V>(a.twI[e]);
   }
   const long nrows = (long)a.J * a.ncols;
   const int N = a.N;
@@ -329,7 +358,7 @@ __global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, doub
       const int idx = tid + q * C::THREADS;
       const int m = idx / ROWS, rr = idx % ROWS;
       const long r = rb + rr;
-      gr[q] = zero_of<GT>();
+      gr[q] = cvt<GT>(make_float2(0.f, 0.f));
       if (idx < M * ROWS && m <= N && r < nrows) {
         const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
         gr[q] = G[(long)m * a.gmst + (long)j * a.gld + c];
@@ -348,8 +377,8 @@ __global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, doub
     for (int q = 0; q < NL; q++) {
       const int idx = tid + q * C::THREADS;
       if (idx < M * ROWS) {
-        double2 v = make_double2((double)gr[q].x, (double)gr[q].y);
-        if (idx < ROWS) v.y = 0.0;  // m = 0
+        V v = cvt<V>(gr[q]);
+        if (idx < ROWS) v.y = (S)0;  // m = 0
         buf[(idx % ROWS) * SR + idx / ROWS] = v;
       }
     }
@@ -358,38 +387,38 @@ __global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, doub
     // packed spectrum Z_m from the half-complex pair (X_m, X_{M-m}); X_M = 0 for m = 0
     for (int idx = tid; idx < ROWS * (M / 2 + 1); idx += C::THREADS) {
       const int m = idx / ROWS, rr = idx % ROWS;
-      double2* X = buf + rr * SR;
+      V* X = buf + rr * SR;
       const int mc = m ? M - m : 0;
-      const double2 xm = X[m], xo = X[mc];
-      const double2 c1 = m ? make_double2(xo.x, -xo.y) : make_double2(0.0, 0.0);
-      const double2 A1 = radd(xm, c1), B1 = rcmulc(rsub(xm, c1), twi[m]);
-      const double2 z1 = make_double2(A1.x - B1.y, A1.y + B1.x);
+      const V xm = X[m], xo = X[mc];
+      const V c1 = m ? mk<V>(xo.x, -xo.y) : mk<V>(0, 0);
+      const V A1 = radd(xm, c1), B1 = rcmulc(rsub(xm, c1), twi[m]);
+      const V z1 = mk<V>(A1.x - B1.y, A1.y + B1.x);
       if (m != 0 && 2 * m != M) {
-        const double2 c2 = make_double2(xm.x, -xm.y);
-        const double2 A2 = radd(xo, c2), B2 = rcmulc(rsub(xo, c2), twi[mc]);
-        X[mc] = make_double2(A2.x - B2.y, A2.y + B2.x);
+        const V c2 = mk<V>(xm.x, -xm.y);
+        const V A2 = radd(xo, c2), B2 = rcmulc(rsub(xo, c2), twi[mc]);
+        X[mc] = mk<V>(A2.x - B2.y, A2.y + B2.x);
       }
       X[m] = z1;
     }
     __syncthreads();
-    double2 v[R2];
+    V v[R2];
     if (rr1 < ROWS) {
-      const double2* Z = buf + rr1 * SR + n1;
+      const V* Z = buf + rr1 * SR + n1;
 #pragma unroll
       for (int n2 = 0; n2 < R2; n2++) v[n2] = Z[R1 * n2];
     }
     dft<R2, +1>(v);
     __syncthreads();
     if (rr1 < ROWS) {
-      double2* Y = buf + rr1 * SR + n1 * PY;
+      V* Y = buf + rr1 * SR + n1 * PY;
       Y[0] = v[0];
 #pragma unroll
       for (int q = 1; q < R2; q++) Y[q] = rcmulc(v[q], tw2[q * R1 + n1]);
     }
     __syncthreads();
     if (rr2 < ROWS) {
-      double2 u[R1];
-      const double2* Y = buf + rr2 * SR + k2;
+      V u[R1];
+      const V* Y = buf + rr2 * SR + k2;
 #pragma unroll
       for (int p = 0; p < R1; p++) u[p] = Y[p * PY];
       dft<R1, +1>(u);
@@ -399,7 +428,7 @@ __global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, doub
         const double s = a.rowscale ? a.rowscale[j] : 1.0;
         double2* x = reinterpret_cast<double2*>(xy + c * a.ldg + (long)j * (2 * M));
 #pragma unroll
-        for (int k1 = 0; k1 < R1; k1++) x[k2 + R2 * k1] = make_double2(u[k1].x * s, u[k1].y * s);
+        for (int k1 = 0; k1 < R1; k1++) x[k2 + R2 * k1] = make_double2((double)u[k1].x * s, (double)u[k1].y * s);
       }
     }
   }
@@ -420,40 +449,47 @@ static long fft_grid_cap() {
   return 4l * cus;
 }
 
-template <int R1, int R2, int NT, typename GT>
+template <int R1, int R2, int NT, typename GT, typename V>
 static int reg_fwd_nt(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
-  using C = RegFft<R1, R2, NT>;
+  using C = RegFft<R1, R2, NT, V>;
   const long nrows = (long)a.J * a.ncols;
   long nb = cdivl(nrows, C::ROWS);
   if (nb > fft_grid_cap()) nb = fft_grid_cap();
-  hipLaunchKernelGGL((k_fft_fwd_r<R1, R2, NT, GT>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, xy, G, a,
+  hipLaunchKernelGGL((k_fft_fwd_r<R1, R2, NT, GT, V>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, xy, G, a,
                      colmax);
   DSHT_HIP(hipGetLastError());
   return 0;
 }
 
-template <int R1, int R2, int NT, typename GT>
+template <int R1, int R2, int NT, typename GT, typename V>
 static int reg_inv_nt(const GT* G, const FftArgs& a, double* xy, hipStream_t s) {
-  using C = RegFft<R1, R2, NT>;
+  using C = RegFft<R1, R2, NT, V>;
   const long nrows = (long)a.J * a.ncols;
   long nb = cdivl(nrows, C::ROWS);
   if (nb > fft_grid_cap()) nb = fft_grid_cap();
-  hipLaunchKernelGGL((k_fft_inv_r<R1, R2, NT, GT>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, G, xy, a);
+  hipLaunchKernelGGL((k_fft_inv_r<R1, R2, NT, GT, V>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, G, xy, a);
   DSHT_HIP(hipGetLastError());
   return 0;
 }
 
-// workgroup size: 256 threads, or 128 (half the LDS per workgroup, more workgroups per CU)
+// workgroup size 256 threads (or 128: half the LDS per workgroup); arithmetic FP64, or
+// FP32 for the FP32 Fourier path of the split forms (a.fp32)
 template <int R1, int R2, typename GT>
 static int reg_fwd(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
-  if (a.threads == 128 && R1 <= 128) return reg_fwd_nt<R1, R2, 128, GT>(xy, a, G, colmax, s);
-  return reg_fwd_nt<R1, R2, 256, GT>(xy, a, G, colmax, s);
+  if constexpr (sizeof(GT) == 8) {
+    if (a.fp32) return reg_fwd_nt<R1, R2, 256, GT, float2>(xy, a, G, colmax, s);
+  }
+  if (a.threads == 128 && R1 <= 128) return reg_fwd_nt<R1, R2, 128, GT, double2>(xy, a, G, colmax, s);
+  return reg_fwd_nt<R1, R2, 256, GT, double2>(xy, a, G, colmax, s);
 }
 
 template <int R1, int R2, typename GT>
 static int reg_inv(const GT* G, const FftArgs& a, double* xy, hipStream_t s) {
-  if (a.threads == 128 && R1 <= 128) return reg_inv_nt<R1, R2, 128, GT>(G, a, xy, s);
-  return reg_inv_nt<R1, R2, 256, GT>(G, a, xy, s);
+  if constexpr (sizeof(GT) == 8) {
+    if (a.fp32) return reg_inv_nt<R1, R2, 256, GT, float2>(G, a, xy, s);
+  }
+  if (a.threads == 128 && R1 <= 128) return reg_inv_nt<R1, R2, 128, GT, double2>(G, a, xy, s);
+  return reg_inv_nt<R1, R2, 256, GT, double2>(G, a, xy, s);
 }
 
 // factor pairs (R1, R2) with M = R1 * R2 that have register kernels

@@ -1,17 +1,23 @@
 """Autotuner for the per-plan launch knobs (dsht_set_tuning).
 
 For one plan and a representative column count it times the dynamics-step transform set
-(3 inverse kinds over 4 fields, 1 forward over 6 tendencies) under each candidate
-(column chunk x contraction column block) and keeps the fastest.  Results are cached per
-(nmax, im, jm, precision, ncols bucket) in a JSON file so later runs reuse them.
+(3 inverse kinds over 4 fields, 1 forward over 6 tendencies) under each candidate and
+keeps the fastest.  The candidates are:
+  column chunk      TUNE_CHUNK_COLS: columns per Fourier scratch chunk (0 = ~48 MiB)
+  column block      TUNE_LEG_CB: column tiles per wave / workgroup of the contractions
+  contraction       TUNE_LEG: kernel family (split forms: packed data panels or
+                    LDS-shared split data; FP64: LDS-staged or per-wave fragment streams)
+Results are cached per (nmax, im, jm, precision, ncols bucket) in a JSON file so later
+runs reuse them.
 """
 import json
 import os
 import time
 
-from . import GRADLAT, GRADLON, TUNE_CHUNK_COLS, TUNE_LEG_CB, VAL
+from . import GRADLAT, GRADLON, PREC_FP64, TUNE_CHUNK_COLS, TUNE_LEG, TUNE_LEG_CB, VAL
 
 CACHE = os.environ.get("DSHT_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "dsht", "tuning.json"))
+KNOBS = (("chunk", TUNE_CHUNK_COLS), ("cb", TUNE_LEG_CB), ("leg", TUNE_LEG))
 
 
 def _bucket(ncols):
@@ -26,8 +32,15 @@ def _key(plan, ncols):
 
 
 def candidates(plan, ncols):
-    chunks = sorted({c for c in (32, 64, 128, 256, 512) if c < ncols} | {0})
-    return [(c, cb) for c in chunks for cb in (16, 32)]
+    """Candidate knob settings as dicts {chunk, cb, leg}."""
+    chunks = sorted({c for c in (64, 128, 256) if c < ncols} | {0})
+    legs = (0, 2) if plan.precision == PREC_FP64 else (0, 3)
+    return [{"chunk": c, "cb": cb, "leg": lg} for c in chunks for cb in (16, 32) for lg in legs]
+
+
+def apply(plan, setting):
+    for name, key in KNOBS:
+        plan.set_tuning(key, int(setting.get(name, 0)))
 
 
 def _time_step(plan, bufs, stream, reps=3):
@@ -48,7 +61,8 @@ def _time_step(plan, bufs, stream, reps=3):
 
 
 def tune(plan, ncols, dev=None, reps=3, use_cache=True, verbose=False):
-    """Pick (chunk, column block) for `plan` at `ncols` prognostic columns; applies and returns it."""
+    """Pick the knob setting for `plan` at `ncols` prognostic columns; applies and returns
+    {chunk, cb, leg, ms, candidates}."""
     import torch
 
     cache = {}
@@ -59,9 +73,7 @@ def tune(plan, ncols, dev=None, reps=3, use_cache=True, verbose=False):
             cache = {}
     key = _key(plan, ncols)
     if key in cache:
-        chunk, cb = cache[key]["chunk"], cache[key]["cb"]
-        plan.set_tuning(TUNE_CHUNK_COLS, chunk)
-        plan.set_tuning(TUNE_LEG_CB, cb)
+        apply(plan, cache[key])
         return cache[key]
     dev = dev or torch.device("cuda:0")
     f64 = dict(dtype=torch.float64, device=dev)
@@ -73,19 +85,16 @@ def tune(plan, ncols, dev=None, reps=3, use_cache=True, verbose=False):
     stream = torch.cuda.current_stream(dev).cuda_stream
     best = None
     results = []
-    for chunk, cb in candidates(plan, ncols):
-        plan.set_tuning(TUNE_CHUNK_COLS, chunk)
-        plan.set_tuning(TUNE_LEG_CB, cb)
+    for cand in candidates(plan, ncols):
+        apply(plan, cand)
         dt = _time_step(plan, (spec, grids, tend, spect), stream, reps)
-        results.append({"chunk": chunk, "cb": cb, "ms": 1e3 * dt})
+        results.append(dict(cand, ms=1e3 * dt))
         if verbose:
-            print(f"  tune {key}: chunk={chunk} cb={cb} {1e3 * dt:.3f} ms", flush=True)
+            print(f"  tune {key}: {cand} {1e3 * dt:.3f} ms", flush=True)
         if best is None or dt < best[0]:
-            best = (dt, chunk, cb)
-    _, chunk, cb = best
-    plan.set_tuning(TUNE_CHUNK_COLS, chunk)
-    plan.set_tuning(TUNE_LEG_CB, cb)
-    entry = {"chunk": chunk, "cb": cb, "ms": 1e3 * best[0], "candidates": results}
+            best = (dt, cand)
+    apply(plan, best[1])
+    entry = dict(best[1], ms=1e3 * best[0], candidates=results)
     if use_cache:
         cache[key] = entry
         try:

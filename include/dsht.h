@@ -163,7 +163,9 @@ int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, d
 #define DSHT_TUNE_CHUNK_COLS 0 /* 0 = automatic */
 #define DSHT_TUNE_LEG_CB 1     /* 0 = automatic, 16 or 32 */
 #define DSHT_TUNE_FFT 2        /* 0 = register two-pass kernel where the length has one (256-thread
-                                  blocks), 1 = LDS Stockham, 2 = register kernel, 128-thread blocks */
+                                  blocks; FP32 arithmetic in the inverse FFT of the split forms,
+                                  whose Fourier coefficients are FP32), 1 = LDS Stockham, 2 = register kernel, 128-thread blocks,
+                                  3 = register kernel, FP64 arithmetic in every form */
 #define DSHT_TUNE_LEG 3        /* Legendre contractions.  FP64: 0 or 1 = LDS-staged, 2 = per-wave
                                   fragment streams.  Split forms: 0 = per-wave fragment streams
                                   over pre-packed data panels, 1 = LDS-staged, 2 = fragment
