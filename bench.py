@@ -7,6 +7,10 @@ Workload (one "step" = one dynamics-step transform set at T170, I=512, J=256, KM
   6 nonlinear tendency fields grid -> spectral (w_xy): 6 forward units per layer.
   18 units = 9 transform pairs per layer, 540 pairs per step.  A unit is one single-field,
   single-layer transform in one direction; pairs/s = units / 2 / s.
+The value and both gradients of the 4 fields come from one fused inverse call
+(dsht_inv_vgrad_d: one Legendre pass per column chunk for all three); the same step with
+one call per kind (xy_w, xy_GradLon_w, xy_GradLat_w separately) is timed beside it and
+reported per form as `separate_calls`.
 Inputs are seeded red-spectrum fields (variance ~ (n+1)^-3) with depth-dependent zonal-mean
 T/S backgrounds, already resident in HBM when the timed region starts.  The precision form
 reported as `value` is the fastest Legendre form whose output stays within relative L2
@@ -60,10 +64,16 @@ def synth_fields(torch, dev, L):
     return spec.contiguous()
 
 
-def run_step(plan, dsht, bufs, stream):
+def run_step(plan, dsht, bufs, stream, fused=True):
+    """One step.  fused: the three inverse kinds of each field come from one
+    dsht_inv_vgrad_d call (one contraction pass for value and both gradients); otherwise
+    three dsht_inv_d calls, as the SpmlUtil function-per-kind API makes them."""
     spec, g_out, tend, spec_t = bufs
-    for k, kind in enumerate(INV_KINDS):
-        plan.inv_d(getattr(dsht, kind), spec, g_out[k], stream=stream)
+    if fused:
+        plan.inv_vgrad_d(spec, *g_out, stream=stream)
+    else:
+        for k, kind in enumerate(INV_KINDS):
+            plan.inv_d(getattr(dsht, kind), spec, g_out[k], stream=stream)
     plan.fwd_d(dsht.VAL, tend, spec_t, stream=stream)
 
 
@@ -73,14 +83,14 @@ def rel_l2(a, b):
     return d / n if n > 0 else d
 
 
-def time_steps(torch, plan, dsht, bufs, steps, warmup, min_warm_s=0.3):
+def time_steps(torch, plan, dsht, bufs, steps, warmup, min_warm_s=0.3, fused=True):
     """W warmup steps (continued until `min_warm_s` of GPU work so clocks have settled),
     then K timed steps back to back; per-step event pairs give the latency distribution."""
     stream = torch.cuda.current_stream().cuda_stream
     t0 = time.perf_counter()
     done = 0
     while done < warmup or time.perf_counter() - t0 < min_warm_s:
-        run_step(plan, dsht, bufs, stream)
+        run_step(plan, dsht, bufs, stream, fused)
         done += 1
         if done % 8 == 0:
             torch.cuda.synchronize()
@@ -89,7 +99,7 @@ def time_steps(torch, plan, dsht, bufs, steps, warmup, min_warm_s=0.3):
     t0 = time.perf_counter()
     for i in range(steps):
         ev[i][0].record()
-        run_step(plan, dsht, bufs, stream)
+        run_step(plan, dsht, bufs, stream, fused)
         ev[i][1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -210,12 +220,6 @@ def energy_drift(torch, plan, dsht, form, spec, trips=100):
     return (float((w * w).sum()) - e0) / e0
 
 
-def chunk_columns(form):
-    """Column chunk the library picks by default: ~48 MiB of Fourier scratch per chunk."""
-    per = (NMAX + 1) * JM * (16 if form == "fp64" else 8)
-    return max(32, (48 << 20) // per // 32 * 32)
-
-
 def traffic_table(path=os.path.join(ROOT, "profiles", "hbm_traffic_T170L60.md")):
     """Measured HBM bytes per full-chunk launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-
     calibrated) from the committed profile summary: {(form, scope): bytes}."""
@@ -231,20 +235,20 @@ def traffic_table(path=os.path.join(ROOT, "profiles", "hbm_traffic_T170L60.md"))
 
 
 def roofline_probe(torch, plan, dsht, form, scope, bufs, reps=10):
-    """Average duration and algorithmic bytes/flops of `scope` on full-chunk launches of the
+    """Average duration and algorithmic bytes/flops of `scope` on the launches of the
     step's own call (the launch shape the committed counter profile measured)."""
     spec, g_out, tend, spec_t = bufs
     stream = torch.cuda.current_stream().cuda_stream
-    nc = chunk_columns(form)
     pid = {v: k for k, v in dsht.PREC_NAMES.items()}[form]
     plan.set_precision(pid)
     inv = scope.endswith("inv")
+    nc = spec.shape[1] if inv else tend.shape[0]
 
     def call():
         if inv:
-            plan.inv_d(dsht.VAL, spec[:, :nc], g_out[0][:nc], stream=stream)
+            plan.inv_vgrad_d(spec, *g_out, stream=stream)
         else:
-            plan.fwd_d(dsht.VAL, tend[:nc], spec_t[:, :nc], stream=stream)
+            plan.fwd_d(dsht.VAL, tend, spec_t, stream=stream)
 
     for _ in range(3):
         call()
@@ -265,7 +269,8 @@ def roofline_probe(torch, plan, dsht, form, scope, bufs, reps=10):
     else:
         roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
     roof["traffic"] = traffic_table().get((form, scope))
-    roof.update({"kernel": scope, "columns_per_launch": nc, "avg_launch_us": 1e6 * t,
+    per_call = 3 if scope == "fft_inv" else 1  # the fused inverse runs one FFT per output grid
+    roof.update({"kernel": scope, "columns_per_launch": nc * reps * per_call / cnt, "avg_launch_us": 1e6 * t,
                  "algorithmic_bytes_per_launch": by / cnt, "algorithmic_flops_per_launch": fl / cnt,
                  "traffic_source": "profiles/hbm_traffic_T170L60.md" if roof["traffic"] else None})
     return roof
@@ -487,6 +492,9 @@ def main():
         print(f"[bench] form {form}: timing {args.steps} steps", file=sys.stderr, flush=True)
         wall, lat = time_steps(torch, plan, dsht, bufs, args.steps, args.warmup)
         prof = kernel_profile(torch, plan, dsht, bufs, max(2, min(args.steps, 5)))
+        # the same step with one dsht_inv_d call per inverse kind (function-per-kind API)
+        wall_s, _ = time_steps(torch, plan, dsht, bufs, args.steps, args.warmup, fused=False)
+        sep_err = max(rel_l2(a, b) for a, b in zip(g_out, outs[:3]))
         results[form] = {
             "pairs_per_s": PAIRS_PER_STEP * args.steps / wall,
             "ms_per_step": 1e3 * wall / args.steps,
@@ -494,6 +502,9 @@ def main():
             "p99_step_ms": float(np.percentile(lat, 99)),
             "errors": err,
             "kernels": prof,
+            "separate_calls": {"pairs_per_s": PAIRS_PER_STEP * args.steps / wall_s,
+                               "ms_per_step": 1e3 * wall_s / args.steps,
+                               "rel_l2_vs_fused": sep_err},
         }
         print(f"[bench] form {form}: {results[form]['pairs_per_s']:.0f} pairs/s "
               f"{results[form]['ms_per_step']:.3f} ms/step err {err}", file=sys.stderr, flush=True)
@@ -537,13 +548,15 @@ def main():
         "dtype": "f64" if best == "fp64" else best,
         "data": "synthetic",
         "config": {"workload": "T170L60 dynamics-step transform set (4 fields to grid with lon/lat "
-                               "derivatives, 6 tendencies to spectral)",
+                               "derivatives, 6 tendencies to spectral); inverse kinds of a field in one "
+                               "fused call (forms.*.separate_calls: one call per kind)",
                    "nmax": NMAX, "im": IM, "jm": JM, "km": KM, "units_per_step": UNITS_PER_STEP,
                    "pairs_per_step": PAIRS_PER_STEP, "parallelism": "single GPU"},
         "p50_step_ms": r["p50_step_ms"],
         "p99_step_ms": r["p99_step_ms"],
         "forms": {f: {"pairs_per_s": v["pairs_per_s"], "ms_per_step": v["ms_per_step"],
                       "p99_step_ms": v["p99_step_ms"], "errors": v["errors"],
+                      "separate_calls": v["separate_calls"],
                       "kernels": {kk: {"avg_us": kv["avg_us"], "GBps": kv["GBps"], "TFLOPs": kv["TFLOPs"]}
                                   for kk, kv in v["kernels"].items()}}
                   for f, v in results.items()},

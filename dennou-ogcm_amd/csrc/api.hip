@@ -150,15 +150,7 @@ int pipe_join(ChunkPipe& cp) {
   return pipe_after(cp.evX, cp.aux, cp.s);
 }
 
-// column chunk: keeps each Fourier scratch buffer near 48 MiB (MALL resident)
-static int chunk_cols(const dsht_plan* p, int ncols) {
-  if (p->chunk_cols > 0) return p->chunk_cols < ncols ? p->chunk_cols : ncols;
-  long per = (long)(p->N + 1) * p->J * (long)sizeof(double2);
-  long c = (48l << 20) / per;
-  c = c / 32 * 32;
-  if (c < 32) c = 32;
-  return c < ncols ? (int)c : ncols;
-}
+static int chunk_cols(const dsht_plan* p, int ncols) { return chunk_columns(p, ncols, sizeof(double2)); }
 
 static size_t fourier_bytes(const dsht_plan* p, int ch) {
   return (size_t)(p->N + 1) * p->J * ch * sizeof(double2);
@@ -225,9 +217,9 @@ static int run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, l
 // inverse family: spectra -> grids
 static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, long sl, long sc,
                    double* out0, double* out1, long ldg, int ncols, const double* pro, const double* rs,
-                   hipStream_t s) {
+                   hipStream_t s, double* out2 = nullptr) {
   if (ncols <= 0) return 0;
-  const int nout = (op == I_VORDIV2UV) ? 2 : 1;
+  const int nout = inv_nout(op);
   const int ch = chunk_cols(p, ncols);
   const int nchunks = cdiv(ncols, ch);
   const size_t slot_bytes = nout * fourier_bytes(p, ch);
@@ -241,14 +233,16 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
     const int slot = cp.on ? (k & 1) : 0;
     double2* G0 = (double2*)((char*)wl.buf + slot * slot_bytes);
     double2* G1 = G0 + (size_t)(p->N + 1) * p->J * ch;
+    double2* G2 = G1 + (size_t)(p->N + 1) * p->J * ch;
     if (cp.on && k >= 2) DSHT_HIP(hipStreamWaitEvent(s, cp.evF[slot], 0));  // slot free again
     LegInvArgs a;
     a.W0 = in0 + (long)c0 * sc;
     a.W1 = in1 ? in1 + (long)c0 * sc : nullptr;
     a.sl = sl;
     a.sc = sc;
-    a.G0 = G0;
-    a.G1 = nout > 1 ? G1 : nullptr;
+    a.G0 = out0 ? G0 : nullptr;
+    a.G1 = nout > 1 && out1 ? G1 : nullptr;
+    a.G2 = nout > 2 && out2 ? G2 : nullptr;
     a.ncols = nc;
     a.gld = nc;
     a.gmst = (long)p->J * nc;
@@ -267,8 +261,9 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
       fs = cp.aux;
       if ((rc = pipe_after(cp.evL[slot], s, fs))) return rc;
     }
-    if ((rc = launch_fft_inv(p, G0, nc, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
-    if (nout > 1 && (rc = launch_fft_inv(p, G1, nc, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (a.G0 && (rc = launch_fft_inv(p, G0, nc, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (a.G1 && (rc = launch_fft_inv(p, G1, nc, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (a.G2 && (rc = launch_fft_inv(p, G2, nc, nc, out2 + (long)c0 * ldg, ldg, fs))) return rc;
     if (cp.on) DSHT_HIP(hipEventRecord(cp.evF[slot], fs));
   }
   return pipe_join(cp);
@@ -570,10 +565,10 @@ static int dispatch_fwd(dsht_plan* p, int op, const double* in0, const double* i
 
 static int dispatch_inv(dsht_plan* p, int op, const double* in0, const double* in1, long sl, long sc,
                         double* out0, double* out1, long ldg, int ncols, const double* pro, const double* rs,
-                        hipStream_t s) {
+                        hipStream_t s, double* out2 = nullptr) {
   if (p->prec != DSHT_PREC_FP64)
-    return mixed_run_inv(p, op, in0, in1, sl, sc, out0, out1, ldg, ncols, pro, rs, s);
-  return run_inv(p, op, in0, in1, sl, sc, out0, out1, ldg, ncols, pro, rs, s);
+    return mixed_run_inv(p, op, in0, in1, sl, sc, out0, out1, ldg, ncols, pro, rs, s, out2);
+  return run_inv(p, op, in0, in1, sl, sc, out0, out1, ldg, ncols, pro, rs, s, out2);
 }
 
 int dsht_fwd_d(dsht_plan* p, int kind, const double* xy, long ldg, double* w, long sl, long sc, int ncols,
@@ -607,6 +602,35 @@ int dsht_inv_d(dsht_plan* p, int kind, const double* w, long sl, long sc, double
   if (int rc = check_grid_ptr(xy, ldg, p)) return rc;
   if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
   return dispatch_inv(p, op, w, nullptr, sl, sc, xy, nullptr, ldg, ncols, diag, rs, (hipStream_t)stream);
+}
+
+int dsht_inv_vgrad_d(dsht_plan* p, const double* w, long sl, long sc, double* xy, double* xy_lon, double* xy_lat,
+                     long ldg, int ncols, const double* diag, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (ncols < 0) return fail(DSHT_EARG, "ncols < 0");
+  if (!xy && !xy_lon && !xy_lat) return fail(DSHT_EARG, "dsht_inv_vgrad_d: every output is NULL");
+  if (ncols == 0) return 0;
+  double* outs[3] = {xy, xy_lon, xy_lat};
+  for (double* o : outs)
+    if (o)
+      if (int rc = check_grid_ptr(o, ldg, p)) return rc;
+  if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  // kernel families without the fused contraction (FP64 fragment streams, first split design)
+  // run the three transforms one after another
+  const bool fused = p->prec == DSHT_PREC_FP64 ? p->leg_kind != 2 : p->leg_kind != 1;
+  if (!fused) {
+    if (xy)
+      if (int rc = dispatch_inv(p, I_VAL, w, nullptr, sl, sc, xy, nullptr, ldg, ncols, diag, nullptr, s)) return rc;
+    if (xy_lon)
+      if (int rc = dispatch_inv(p, I_GRADLAMBDA, w, nullptr, sl, sc, xy_lon, nullptr, ldg, ncols, diag, p->d_rs_icos, s))
+        return rc;
+    if (xy_lat)
+      if (int rc = dispatch_inv(p, I_GRADMU, w, nullptr, sl, sc, xy_lat, nullptr, ldg, ncols, diag, p->d_rs_icos, s))
+        return rc;
+    return 0;
+  }
+  return dispatch_inv(p, I_VGRAD, w, nullptr, sl, sc, xy, xy_lon, ldg, ncols, diag, p->d_rs_icos, s, xy_lat);
 }
 
 int dsht_alpha_fwd_d(dsht_plan* p, const double* a, const double* b, long ldg, double* w, long sl, long sc,
@@ -753,6 +777,25 @@ int dsht_xy_w(dsht_plan* p, int kind, const double* w, double* xy, int ncols) {
   if ((rc = h2d(c, dw, w, LL * ncols))) return rc;
   if ((rc = dsht_inv_d(p, kind, dw, 1, (long)LL, dg, (long)IJ, ncols, nullptr, c->s))) return rc;
   return d2h_sync(c, xy, dg, IJ * ncols);
+}
+
+int dsht_xy_vgrad_w(dsht_plan* p, const double* w, double* xy, double* xy_lon, double* xy_lat, int ncols) {
+  HOST_PROLOGUE(p, ncols);
+  if (!w || (!xy && !xy_lon && !xy_lat)) return fail(DSHT_EARG, "NULL array");
+  double *dg0, *dg1, *dg2, *dw;
+  int rc;
+  if ((rc = ctx_buf(c, 0, IJ * ncols, &dg0)) || (rc = ctx_buf(c, 2, IJ * ncols, &dg1)) ||
+      (rc = ctx_buf(c, 3, IJ * ncols, &dg2)) || (rc = ctx_buf(c, 1, LL * ncols, &dw)))
+    return rc;
+  if ((rc = h2d(c, dw, w, LL * ncols))) return rc;
+  if ((rc = dsht_inv_vgrad_d(p, dw, 1, (long)LL, xy ? dg0 : nullptr, xy_lon ? dg1 : nullptr, xy_lat ? dg2 : nullptr,
+                             (long)IJ, ncols, nullptr, c->s)))
+    return rc;
+  if (xy) DSHT_HIP(hipMemcpyAsync(xy, dg0, sizeof(double) * IJ * ncols, hipMemcpyDeviceToHost, c->s));
+  if (xy_lon) DSHT_HIP(hipMemcpyAsync(xy_lon, dg1, sizeof(double) * IJ * ncols, hipMemcpyDeviceToHost, c->s));
+  if (xy_lat) DSHT_HIP(hipMemcpyAsync(xy_lat, dg2, sizeof(double) * IJ * ncols, hipMemcpyDeviceToHost, c->s));
+  DSHT_HIP(hipStreamSynchronize(c->s));
+  return 0;
 }
 
 int dsht_w_alpha_xy(dsht_plan* p, const double* a, const double* b, double* w, int ncols) {

@@ -7,7 +7,8 @@
 //
 // Per instance (KM layers) and step, the dynamics-step transform set:
 //   4 prognostic fields (vor, div, T, S) spectral -> grid with lon/lat derivatives
-//   (xy_w, xy_GradLon_w, xy_GradLat_w) and 6 tendency fields grid -> spectral (w_xy).
+//   (xy_w, xy_GradLon_w, xy_GradLat_w; one fused dsht_inv_vgrad_d call) and 6 tendency
+//   fields grid -> spectral (w_xy).
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -16,8 +17,7 @@
 
 namespace dsht {
 
-constexpr int NPROG = 4, NTEND = 6, NINV = 3;
-static const int INV_KINDS[NINV] = {DSHT_VAL, DSHT_GRADLON, DSHT_GRADLAT};
+constexpr int NPROG = 4, NTEND = 6, NINV = 3;  // NINV grids per prognostic column: value, GradLon, GradLat
 
 struct BatchInstance {
   dsht_plan* plan;
@@ -53,10 +53,10 @@ static int batch_group_step(BatchGroup& g, hipStream_t s) {
   dsht_plan* p = g.plan;
   const long IJ = (long)p->I * p->J;
   const int C = NPROG * g.cols;
-  for (int k = 0; k < NINV; k++) {
-    int rc = dsht_inv_d(p, INV_KINDS[k], g.spec, C, 1, g.grid + (size_t)k * C * IJ, IJ, C, nullptr, s);
-    if (rc) return rc;
-  }
+  // value and both gradients of every prognostic column in one fused inverse
+  double* gk[NINV];
+  for (int k = 0; k < NINV; k++) gk[k] = g.grid + (size_t)k * C * IJ;
+  if (int rc = dsht_inv_vgrad_d(p, g.spec, C, 1, gk[0], gk[1], gk[2], IJ, C, nullptr, s)) return rc;
   return dsht_fwd_d(p, DSHT_VAL, g.tend, IJ, g.spect, NTEND * g.cols, 1, NTEND * g.cols, nullptr, s);
 }
 
@@ -160,10 +160,9 @@ int dsht_batch_step_sequential(dsht_batch* b, void* stream) {
     const int C = NPROG * g.cols, CT = NTEND * g.cols;
     for (int f = 0; f < NPROG; f++) {
       const int c = f * g.cols + in.col0;
-      for (int k = 0; k < NINV; k++) {
-        double* out = g.grid + (size_t)k * C * IJ + (size_t)c * IJ;
-        if (int rc = dsht_inv_d(p, INV_KINDS[k], g.spec + c, C, 1, out, IJ, in.km, nullptr, s)) return rc;
-      }
+      double* out[NINV];
+      for (int k = 0; k < NINV; k++) out[k] = g.grid + (size_t)k * C * IJ + (size_t)c * IJ;
+      if (int rc = dsht_inv_vgrad_d(p, g.spec + c, C, 1, out[0], out[1], out[2], IJ, in.km, nullptr, s)) return rc;
     }
     for (int f = 0; f < NTEND; f++) {
       const int c = f * g.cols + in.col0;

@@ -33,7 +33,9 @@ __host__ __device__ inline int n_par(int N, int m, int p) {
 
 // ---------------------------------------------------------------- ops
 enum FwdOp { F_VAL = 0, F_DIVLAMBDA = 1, F_DIVMU = 2, F_ALPHA = 3, F_UV2VORDIV = 4 };
-enum InvOp { I_VAL = 0, I_GRADLAMBDA = 1, I_GRADMU = 2, I_ALPHA = 3, I_VORDIV2UV = 4 };
+// I_VGRAD: value, GradLon and GradLat of one field from one contraction pass (P and H
+// share the data; GradLon is i m times the P result)
+enum InvOp { I_VAL = 0, I_GRADLAMBDA = 1, I_GRADMU = 2, I_ALPHA = 3, I_VORDIV2UV = 4, I_VGRAD = 5 };
 
 // inputs, outputs and tables of each operator
 template <int OP>
@@ -47,10 +49,13 @@ struct FwdTraits {
 template <int OP>
 struct InvTraits {
   static constexpr int NIN = (OP == I_ALPHA || OP == I_VORDIV2UV) ? 2 : 1;
-  static constexpr int NOUT = (OP == I_VORDIV2UV) ? 2 : 1;
+  static constexpr int NOUT = (OP == I_VORDIV2UV) ? 2 : (OP == I_VGRAD ? 3 : 1);
   static constexpr bool USE_P = (OP != I_GRADMU);
-  static constexpr bool USE_H = (OP == I_GRADMU || OP == I_ALPHA || OP == I_VORDIV2UV);
+  static constexpr bool USE_H = (OP == I_GRADMU || OP == I_ALPHA || OP == I_VORDIV2UV || OP == I_VGRAD);
 };
+// runtime counterparts (host side)
+inline int inv_nin(int op) { return (op == I_ALPHA || op == I_VORDIV2UV) ? 2 : 1; }
+inline int inv_nout(int op) { return op == I_VORDIV2UV ? 2 : (op == I_VGRAD ? 3 : 1); }
 
 
 // Arguments of the forward Legendre contraction (Fourier -> spectral).
@@ -80,6 +85,7 @@ struct LegInvArgs {
   long sl, sc;
   double2* G0;       // Fourier output 0 [m][j*gld + c]
   double2* G1;       // Fourier output 1
+  double2* G2 = nullptr;  // Fourier output 2 (I_VGRAD); outputs of I_VGRAD may be NULL (skipped)
   int ncols;
   long gld, gmst;
   const double* tabP;  // inverse-swizzled P table
@@ -195,6 +201,23 @@ struct dsht_plan {
 };
 
 namespace dsht {
+// Columns per chunk of one device call.  The Fourier scratch of one output stays within
+// CHUNK_FOURIER_BYTES, and a call that needs several chunks gets equal ones (rounded up to
+// the 16-column MFMA tile).  Whole-call chunks measure faster than MALL-sized ones: at
+// T170L60 the FP64 step takes 0.84 ms with 512 MiB chunks against 1.0 ms with 48 MiB
+// chunks (fewer, larger FFT and contraction launches).
+constexpr long CHUNK_FOURIER_BYTES = 512l << 20;
+inline int chunk_columns(const dsht_plan* p, int ncols, long fourier_elem_bytes) {
+  if (p->chunk_cols > 0) return p->chunk_cols < ncols ? p->chunk_cols : ncols;
+  const long per = (long)(p->N + 1) * p->J * fourier_elem_bytes;
+  long cap = CHUNK_FOURIER_BYTES / per;
+  cap = cap < 16 ? 16 : cap / 16 * 16;
+  if (ncols <= cap) return ncols;
+  const long nch = (ncols + cap - 1) / cap;
+  const long c = ((ncols + nch - 1) / nch + 15) / 16 * 16;
+  return (int)(c < ncols ? c : ncols);
+}
+
 // error handling
 void set_error(const char* fmt, ...);
 int fail(int code, const char* fmt, ...);
@@ -262,7 +285,8 @@ void mixed_free(dsht_plan* p);
 int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, long ldg, double* out0,
                   double* out1, long sl, long sc, int ncols, const double* diag0, const double* diag1,
                   double scale, const double* cw, hipStream_t s);
+// outputs of I_VGRAD: out0 value, out1 GradLon, out2 GradLat (any may be NULL)
 int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, long sl, long sc,
                   double* out0, double* out1, long ldg, int ncols, const double* pro, const double* rs,
-                  hipStream_t s);
+                  hipStream_t s, double* out2 = nullptr);
 }  // namespace dsht

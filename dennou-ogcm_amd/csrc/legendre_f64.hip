@@ -23,6 +23,7 @@
 //   I_ALPHA      (i m P.sA + H.sB)/(R cos^2)  (xy_AlphaOptr_w)
 //   I_VORDIV2UV  psi, chi = -R^2/(n(n+1)) (vor, div);
 //                U = (i m P.chi - H.psi)/(R cos), V = (i m P.psi + H.chi)/(R cos)  (calc_VorDiv2UV)
+//   I_VGRAD      P.s, i m P.s / cos, H.s / cos from one pass (xy_w, xy_GradLon_w, xy_GradLat_w)
 // plus an optional per-l diagonal (filters, implicit diffusion, Helmholtz) on the spectral side.
 #include <type_traits>
 
@@ -49,6 +50,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_fwd(LegFwdArgs a) {
   using Tr = FwdTraits<OP>;
   constexpr int NT = WAVES * 64;
   constexpr int NIN = Tr::NIN, NOUT = Tr::NOUT, NCT = CB / TILE;
+  constexpr int NACC = OP == I_VGRAD ? 2 : NOUT;  // I_VGRAD: P and H products of one input
   constexpr int PITCH = CB + 1;  // LDS row pitch (doubles)
   constexpr int EPT = (KC * CB + NT - 1) / NT;  // staged elements per thread
   constexpr int NKS = KC / 4;
@@ -260,6 +262,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_inv(LegInvArgs a) {
   using Tr = InvTraits<OP>;
   constexpr int NT = WAVES * 64;
   constexpr int NIN = Tr::NIN, NOUT = Tr::NOUT, NCT = CB / TILE;
+  constexpr int NACC = OP == I_VGRAD ? 2 : NOUT;  // I_VGRAD: P and H products of one input
   constexpr int PITCH = CB + 1;
   constexpr int EPT = (KC * CB + NT - 1) / NT;
   constexpr int NKS = KC / 4;
@@ -281,9 +284,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_inv(LegInvArgs a) {
 
   for (int tbase = 0; tbase < RT; tbase += WAVES * MAXT) {
     // [output][tile][ct][re/im][sym/anti]
-    double4_t acc[NOUT][MAXT][NCT][2][2];
+    double4_t acc[NACC][MAXT][NCT][2][2];
 #pragma unroll
-    for (int o = 0; o < NOUT; o++)
+    for (int o = 0; o < NACC; o++)
 #pragma unroll
       for (int t = 0; t < MAXT; t++)
 #pragma unroll
@@ -380,6 +383,12 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_inv(LegInvArgs a) {
             } else if constexpr (OP == I_GRADMU) {
 #pragma unroll
               for (int r = 0; r < 2; r++) acc[0][tt][ct][r][sH] = mfma(xH, b[0][ct][r], acc[0][tt][ct][r][sH]);
+            } else if constexpr (OP == I_VGRAD) {
+#pragma unroll
+              for (int r = 0; r < 2; r++) {
+                acc[0][tt][ct][r][sP] = mfma(xP, b[0][ct][r], acc[0][tt][ct][r][sP]);
+                acc[1][tt][ct][r][sH] = mfma(xH, b[0][ct][r], acc[1][tt][ct][r][sH]);
+              }
             } else if constexpr (OP == I_ALPHA) {
               // i m P.A + H.B
               acc[0][tt][ct][0][sP] = mfma(xP, -dm * b[0][ct][1], acc[0][tt][ct][0][sP]);
@@ -452,11 +461,17 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_inv(LegInvArgs a) {
           const double sN = a.rs ? a.rs[jN] : 1.0, sS = a.rs ? a.rs[jS] : 1.0;
 #pragma unroll
           for (int o = 0; o < NOUT; o++) {
-            double nre = acc[o][tt][ct][0][0][i] + acc[o][tt][ct][0][1][i];
-            double nim = acc[o][tt][ct][1][0][i] + acc[o][tt][ct][1][1][i];
-            double sre = acc[o][tt][ct][0][0][i] - acc[o][tt][ct][0][1][i];
-            double sim = acc[o][tt][ct][1][0][i] - acc[o][tt][ct][1][1][i];
-            if constexpr (OP == I_GRADLAMBDA) {
+            // I_VGRAD: output 0 = P result (no row scale), 1 = i m P result, 2 = H result
+            const int ai = OP == I_VGRAD ? (o == 2 ? 1 : 0) : o;
+            const bool deriv = OP == I_GRADLAMBDA || (OP == I_VGRAD && o == 1);
+            const bool rowsc = !(OP == I_VGRAD && o == 0);
+            double2* Gb = o == 0 ? a.G0 : (o == 1 ? a.G1 : a.G2);
+            if (OP == I_VGRAD && !Gb) continue;
+            double nre = acc[ai][tt][ct][0][0][i] + acc[ai][tt][ct][0][1][i];
+            double nim = acc[ai][tt][ct][1][0][i] + acc[ai][tt][ct][1][1][i];
+            double sre = acc[ai][tt][ct][0][0][i] - acc[ai][tt][ct][0][1][i];
+            double sim = acc[ai][tt][ct][1][0][i] - acc[ai][tt][ct][1][1][i];
+            if (deriv) {
               double t1 = -dm * nim;
               nim = dm * nre;
               nre = t1;
@@ -464,9 +479,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_inv(LegInvArgs a) {
               sim = dm * sre;
               sre = t1;
             }
-            double2* G = (o == 0 ? a.G0 : a.G1) + (long)m * a.gmst;
-            G[(long)jN * a.gld + c] = make_double2(nre * sN, m ? nim * sN : 0.0);
-            G[(long)jS * a.gld + c] = make_double2(sre * sS, m ? sim * sS : 0.0);
+            const double fN = rowsc ? sN : 1.0, fS = rowsc ? sS : 1.0;
+            double2* G = Gb + (long)m * a.gmst;
+            G[(long)jN * a.gld + c] = make_double2(nre * fN, m ? nim * fN : 0.0);
+            G[(long)jS * a.gld + c] = make_double2(sre * fS, m ? sim * fS : 0.0);
           }
         }
       }
@@ -508,7 +524,9 @@ static int fwd_pick(const dsht_plan* p, const LegFwdArgs& a, hipStream_t s) {
 template <int OP, int CB>
 static int inv_pick(const dsht_plan* p, const LegInvArgs& a, hipStream_t s) {
   const int RT = a.J2pad / TILE;
-  constexpr int KC = InvTraits<OP>::NIN == 2 ? 32 : 64;
+  // both tables: half the K chunk; the fused value/gradient form with 32-column blocks holds
+  // 3 outputs' epilogue beside its accumulators, so its K chunk is halved again
+  constexpr int KC = (InvTraits<OP>::USE_P && InvTraits<OP>::USE_H) ? (OP == I_VGRAD && CB == 32 ? 16 : 32) : 64;
   constexpr int CB1 = InvTraits<OP>::NOUT == 2 ? 16 : CB;
   if (RT <= 2) return inv_launch<OP, CB1, 2, 1, KC>(p, a, s);
   if (RT <= 4) return inv_launch<OP, CB1, 4, 1, KC>(p, a, s);
@@ -547,7 +565,9 @@ int launch_leg_fwd(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t 
 int launch_leg_inv(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t s) {
   const bool wide = p->leg_cb ? p->leg_cb == 32 : a.ncols > 16;
   double by, fl;
-  static const int nin[] = {1, 1, 1, 2, 2}, nout[] = {1, 1, 1, 1, 2}, nt[] = {1, 1, 1, 2, 4}, ntab[] = {1, 1, 1, 2, 2};
+  static const int nin[] = {1, 1, 1, 2, 2, 1}, nout[] = {1, 1, 1, 1, 2, 3}, nt[] = {1, 1, 1, 2, 4, 2},
+                   ntab[] = {1, 1, 1, 2, 2, 2};
+  if (op < 0 || op > I_VGRAD) return fail(DSHT_EARG, "launch_leg_inv: bad op %d", op);
   leg_work(p, nin[op], nout[op], nt[op], ntab[op], false, a.ncols, &by, &fl);
   ProfScope ps("leg_inv", s, by, fl);
   if (p->leg_kind == 2) return launch_leg_inv_direct(p, op, a, s);
@@ -557,6 +577,7 @@ int launch_leg_inv(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t 
     case I_GRADMU: return wide ? inv_pick<I_GRADMU, 32>(p, a, s) : inv_pick<I_GRADMU, 16>(p, a, s);
     case I_ALPHA: return wide ? inv_pick<I_ALPHA, 32>(p, a, s) : inv_pick<I_ALPHA, 16>(p, a, s);
     case I_VORDIV2UV: return wide ? inv_pick<I_VORDIV2UV, 32>(p, a, s) : inv_pick<I_VORDIV2UV, 16>(p, a, s);
+    case I_VGRAD: return wide ? inv_pick<I_VGRAD, 32>(p, a, s) : inv_pick<I_VGRAD, 16>(p, a, s);
   }
   return fail(DSHT_EARG, "launch_leg_inv: bad op %d", op);
 }

@@ -796,14 +796,7 @@ static int mx_inv_op(const dsht_plan* p, int op, const MxInvArgs& a, hipStream_t
 }
 
 // ---------------------------------------------------------------- drivers
-int mixed_chunk_cols(const dsht_plan* p, int ncols) {
-  if (p->chunk_cols > 0) return p->chunk_cols < ncols ? p->chunk_cols : ncols;
-  long per = (long)(p->N + 1) * p->J * (long)sizeof(float2);
-  long c = (48l << 20) / per;
-  c = c / 32 * 32;
-  if (c < 32) c = 32;
-  return c < ncols ? (int)c : ncols;
-}
+int mixed_chunk_cols(const dsht_plan* p, int ncols) { return chunk_columns(p, ncols, sizeof(float2)); }
 
 static float weight_bound(const dsht_plan* p, bool div) {
   double mx = 0.0;
@@ -920,15 +913,16 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
 
 int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, long sl, long sc,
                   double* out0, double* out1, long ldg, int ncols, const double* pro, const double* rs,
-                  hipStream_t s) {
+                  hipStream_t s, double* out2) {
   if (ncols <= 0) return 0;
   const int form = form_of(p->prec);
   if (form < 0) return fail(DSHT_EUNSUPPORTED, "precision form %d not available", p->prec);
   const int family = MxFormFamily(form);
   if (int rc = build_mixed(p, family)) return rc;
   MixedTables& t = ((MixedState*)p->mixed)->tab[family];
-  const int nin = (op == I_ALPHA || op == I_VORDIV2UV) ? 2 : 1;
-  const int nout = (op == I_VORDIV2UV) ? 2 : 1;
+  const int nin = inv_nin(op);
+  const int nout = inv_nout(op);
+  if (op == I_VGRAD && p->leg_kind == 1) return fail(DSHT_EUNSUPPORTED, "fused value/gradient inverse needs DSHT_TUNE_LEG != 1");
   const bool scaled = form >= 2;
   const int ch = mixed_chunk_cols(p, ncols);
   const int nchunks = cdiv(ncols, ch);
@@ -949,6 +943,10 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
     const int slot = cp.on ? (k & 1) : 0;
     float2* G0 = (float2*)((char*)wh.buf + slot * slot_bytes);
     float2* G1 = G0 + gsz;
+    float2* G2 = G1 + gsz;
+    float2* O0 = out0 ? G0 : nullptr;
+    float2* O1 = nout > 1 && out1 ? G1 : nullptr;
+    float2* O2 = nout > 2 && out2 ? G2 : nullptr;
     const double* w0 = in0 + (long)c0 * sc;
     const double* w1 = in1 ? in1 + (long)c0 * sc : nullptr;
     if (cp.on && k >= 2) DSHT_HIP(hipStreamWaitEvent(s, cp.evF[slot], 0));  // slot free again
@@ -998,8 +996,9 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
         j.colmax = colmax;
         j.pack = p->leg_kind == 0 ? pack : nullptr;
         j.staged = p->leg_kind == 3;
-        j.G0 = G0;
-        j.G1 = nout > 1 ? G1 : nullptr;
+        j.G0 = O0;
+        j.G1 = O1;
+        j.G2 = O2;
         j.gld = nc;
         j.gmst = (long)p->J * nc;
         rc = mx2_inv(p, t, form, op, j, s);
@@ -1018,8 +1017,9 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
       fs = cp.aux;
       if ((rc = pipe_after(cp.evL[slot], s, fs))) return rc;
     }
-    if ((rc = launch_fft_inv32(p, G0, nc, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
-    if (nout > 1 && (rc = launch_fft_inv32(p, G1, nc, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (O0 && (rc = launch_fft_inv32(p, G0, nc, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (O1 && (rc = launch_fft_inv32(p, G1, nc, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (O2 && (rc = launch_fft_inv32(p, G2, nc, nc, out2 + (long)c0 * ldg, ldg, fs))) return rc;
     if (cp.on) DSHT_HIP(hipEventRecord(cp.evF[slot], fs));
   }
   return pipe_join(cp);

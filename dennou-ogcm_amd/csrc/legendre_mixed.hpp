@@ -89,7 +89,8 @@ static inline int MxFormFamily(int form) { return form <= 1 ? 0 : 1; }
 // algorithmic work of one split-precision contraction launch (FP64-equivalent flops;
 // Fourier arrays in FP32, spectral in FP64, one read of each 16-bit table term used)
 static inline void mixed_work(const dsht_plan* p, int op, bool fwd, int ncols, double* bytes, double* flops) {
-  static const int nin[] = {1, 1, 1, 2, 2}, nout[] = {1, 1, 1, 1, 2}, nt[] = {1, 1, 1, 2, 4};
+  // inverse I_VGRAD (index 5): one input, three outputs, two products
+  static const int nin[] = {1, 1, 1, 2, 2, 1}, nout[] = {1, 1, 1, 1, 2, 3}, nt[] = {1, 1, 1, 2, 4, 2};
   double T = 0.5 * (p->N + 1.0) * (p->N + 2.0);
   double four = (double)(p->N + 1) * p->J * 8.0 * ncols, spec = (double)p->L * 8.0 * ncols;
   *bytes = fwd ? nin[op] * four + nout[op] * spec : nin[op] * spec + nout[op] * four;
@@ -112,6 +113,7 @@ struct Mx2InvJob {
   bool staged;              // split the data once per block into LDS instead of packing
   float2* G0;
   float2* G1;
+  float2* G2 = nullptr;     // third output (I_VGRAD)
   long gld, gmst;
 };
 

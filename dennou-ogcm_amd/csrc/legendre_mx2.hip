@@ -62,9 +62,13 @@ __device__ __forceinline__ void bsplit(float x, typename MxForm<FORM>::T* out) {
 }
 
 // operator groups: each is one (table, input) product stream; table 0 = P, 1 = H
-__host__ __device__ constexpr int inv_ng(int op) { return op == I_ALPHA ? 2 : (op == I_VORDIV2UV ? 4 : 1); }
+// I_VGRAD: (P, s) and (H, s)
+__host__ __device__ constexpr int inv_ng(int op) {
+  return (op == I_ALPHA || op == I_VGRAD) ? 2 : (op == I_VORDIV2UV ? 4 : 1);
+}
 __host__ __device__ constexpr int inv_tab(int op, int g) {
-  return op == I_GRADMU ? 1 : (op == I_ALPHA ? (g == 1 ? 1 : 0) : (op == I_VORDIV2UV ? (g >= 2 ? 1 : 0) : 0));
+  return op == I_GRADMU ? 1
+                        : ((op == I_ALPHA || op == I_VGRAD) ? (g == 1 ? 1 : 0) : (op == I_VORDIV2UV ? (g >= 2 ? 1 : 0) : 0));
 }
 __host__ __device__ constexpr int inv_in(int op, int g) {
   return op == I_ALPHA ? g : (op == I_VORDIV2UV ? (g & 1) : 0);
@@ -108,6 +112,7 @@ struct Mx2InvArgs {
   uint16_t* pack;
   float2* G0;
   float2* G1;
+  float2* G2;  // I_VGRAD GradLat; outputs of I_VGRAD may be NULL (not stored)
   long gld, gmst;
   const uint16_t* tP[3];
   const uint16_t* tH[3];
@@ -228,6 +233,10 @@ __device__ __forceinline__ void inv_epilogue(const Mx2InvArgs& a,
           onr[0] = nr[0], oni[0] = ni[0], osr[0] = sr[0], osi[0] = si[0];
         } else if constexpr (OP == I_GRADLAMBDA) {  // i m X
           onr[0] = -dm * ni[0], oni[0] = dm * nr[0], osr[0] = -dm * si[0], osi[0] = dm * sr[0];
+        } else if constexpr (OP == I_VGRAD) {  // X, i m X, H-group
+          onr[0] = nr[0], oni[0] = ni[0], osr[0] = sr[0], osi[0] = si[0];
+          onr[1] = -dm * ni[0], oni[1] = dm * nr[0], osr[1] = -dm * si[0], osi[1] = dm * sr[0];
+          onr[2] = nr[1], oni[2] = ni[1], osr[2] = sr[1], osi[2] = si[1];
         } else if constexpr (OP == I_ALPHA) {  // i m P.A + H.B
           onr[0] = -dm * ni[0] + nr[1], oni[0] = dm * nr[0] + ni[1];
           osr[0] = -dm * si[0] + sr[1], osi[0] = dm * sr[0] + si[1];
@@ -240,9 +249,13 @@ __device__ __forceinline__ void inv_epilogue(const Mx2InvArgs& a,
         const double sN = u * (a.rs ? a.rs[jN] : 1.0), sS = u * (a.rs ? a.rs[jS] : 1.0);
 #pragma unroll
         for (int o = 0; o < NOUT; o++) {
-          float2* G = (o == 0 ? a.G0 : a.G1) + (long)m * a.gmst;
-          G[(long)jN * a.gld + c] = make_float2((float)(onr[o] * sN), m ? (float)(oni[o] * sN) : 0.f);
-          G[(long)jS * a.gld + c] = make_float2((float)(osr[o] * sS), m ? (float)(osi[o] * sS) : 0.f);
+          float2* Gb = o == 0 ? a.G0 : (o == 1 ? a.G1 : a.G2);
+          if (OP == I_VGRAD && !Gb) continue;
+          // the value output of I_VGRAD carries no row scale
+          const double fN = (OP == I_VGRAD && o == 0) ? u : sN, fS = (OP == I_VGRAD && o == 0) ? u : sS;
+          float2* G = Gb + (long)m * a.gmst;
+          G[(long)jN * a.gld + c] = make_float2((float)(onr[o] * fN), m ? (float)(oni[o] * fN) : 0.f);
+          G[(long)jS * a.gld + c] = make_float2((float)(osr[o] * fS), m ? (float)(osi[o] * fS) : 0.f);
         }
       }
     }
@@ -798,6 +811,7 @@ int inv_launch(const dsht_plan* p, const MixedTables& t, const Mx2InvJob& j, hip
   a.pack = j.pack;
   a.G0 = j.G0;
   a.G1 = j.G1;
+  a.G2 = j.G2;
   a.gld = j.gld;
   a.gmst = j.gmst;
   for (int k = 0; k < 3; k++) {
@@ -873,6 +887,7 @@ int inv_form_nct(const dsht_plan* p, const MixedTables& t, int op, const Mx2InvJ
     case I_GRADMU: return inv_launch<I_GRADMU, FORM, NCT>(p, t, j, s);
     case I_ALPHA: return inv_launch<I_ALPHA, FORM, NCT>(p, t, j, s);
     case I_VORDIV2UV: return inv_launch<I_VORDIV2UV, FORM, NCT>(p, t, j, s);
+    case I_VGRAD: return inv_launch<I_VGRAD, FORM, NCT>(p, t, j, s);
   }
   return fail(DSHT_EARG, "bad inverse op %d", op);
 }

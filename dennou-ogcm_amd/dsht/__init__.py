@@ -34,9 +34,9 @@ PREC_NAMES = {PREC_FP64: "fp64", PREC_BF16X3: "bf16x3", PREC_FP16: "fp16",
 EXPORTS = [
     "dsht_create", "dsht_destroy", "dsht_last_error", "dsht_get_dims", "dsht_set_precision",
     "dsht_get_precision", "dsht_grid_info", "dsht_nm_l", "dsht_l_nm", "dsht_rn", "dsht_coef",
-    "dsht_w_xy", "dsht_xy_w", "dsht_w_alpha_xy", "dsht_xy_alpha_w", "dsht_uvcoslat2vordiv",
+    "dsht_w_xy", "dsht_xy_w", "dsht_xy_vgrad_w", "dsht_w_alpha_xy", "dsht_xy_alpha_w", "dsht_uvcoslat2vordiv",
     "dsht_vordiv2uv", "dsht_w_diag_w", "dsht_xy_diag_xy", "dsht_intlonlat", "dsht_avrlonlat",
-    "dsht_ya_intlon", "dsht_ya_avrlon", "dsht_interpolate", "dsht_fwd_d", "dsht_inv_d",
+    "dsht_ya_intlon", "dsht_ya_avrlon", "dsht_interpolate", "dsht_fwd_d", "dsht_inv_d", "dsht_inv_vgrad_d",
     "dsht_alpha_fwd_d", "dsht_alpha_inv_d", "dsht_uv2vordiv_d", "dsht_vordiv2uv_d",
     "dsht_diag_d", "dsht_intlonlat_d", "dsht_avrlonlat_d", "dsht_ya_intlon_d",
     "dsht_interpolate_d", "dsht_coef_d", "dsht_prof_enable", "dsht_prof_reset", "dsht_prof_query",
@@ -81,6 +81,7 @@ def load():
         "dsht_coef": ([vp, i, vp, i, vp], i),
         "dsht_w_xy": ([vp, i, vp, vp, i], i),
         "dsht_xy_w": ([vp, i, vp, vp, i], i),
+        "dsht_xy_vgrad_w": ([vp, vp, vp, vp, vp, i], i),
         "dsht_w_alpha_xy": ([vp, vp, vp, vp, i], i),
         "dsht_xy_alpha_w": ([vp, vp, vp, vp, i], i),
         "dsht_uvcoslat2vordiv": ([vp, vp, vp, vp, vp, i], i),
@@ -94,6 +95,7 @@ def load():
         "dsht_interpolate": ([vp, vp, vp, vp, vp, i], i),
         "dsht_fwd_d": ([vp, i, vp, lg, vp, lg, lg, i, vp, vp], i),
         "dsht_inv_d": ([vp, i, vp, lg, lg, vp, lg, i, vp, vp], i),
+        "dsht_inv_vgrad_d": ([vp, vp, lg, lg, vp, vp, vp, lg, i, vp, vp], i),
         "dsht_alpha_fwd_d": ([vp, vp, vp, lg, vp, lg, lg, i, vp, vp], i),
         "dsht_alpha_inv_d": ([vp, vp, vp, lg, lg, vp, lg, i, vp], i),
         "dsht_uv2vordiv_d": ([vp, vp, vp, lg, vp, vp, lg, lg, i, vp, vp], i),
@@ -282,6 +284,18 @@ class Plan:
     def xy_GradLat_w(self, w):
         return self._inv(w, GRADLAT)
 
+    def xy_GradLonLat_w(self, w, value=True):
+        """(xy_w(w), xy_GradLon_w(w), xy_GradLat_w(w)) from one fused pass; the first item is
+        None when value is False."""
+        a, nc = self._spec_in(w)
+        shape = (nc, self.jm, self.im) if a.ndim == 2 else (self.jm, self.im)
+        xy = np.empty(shape) if value else None
+        lon, lat = np.empty(shape), np.empty(shape)
+        _check(load().dsht_xy_vgrad_w(self.h, _ptr(a), _ptr(xy) if value else None, _ptr(lon), _ptr(lat), nc))
+        return xy, lon, lat
+
+    xya_GradLonLat_wa = xy_GradLonLat_w
+
     xya_GradLambda_wa = xy_GradLambda_w
     xya_GradMu_wa = xy_GradMu_w
     xya_GradLon_wa = xy_GradLon_w
@@ -395,6 +409,21 @@ class Plan:
         _check(load().dsht_inv_d(self.h, kind, _tptr(w), sl, sc, _tptr(xy), xy.stride(0), nc,
                                  _tptr(diag) if diag is not None else None, _stream(stream)))
         return xy
+
+    def inv_vgrad_d(self, w, xy, xy_lon, xy_lat, layout="LC", diag=None, stream=None):
+        """value, GradLon and GradLat grids of the same spectra in one fused pass; any one or
+        two of the outputs may be None"""
+        outs = [t for t in (xy, xy_lon, xy_lat) if t is not None]
+        if not outs:
+            raise ValueError("inv_vgrad_d: every output is None")
+        nc = outs[0].shape[0]
+        if any(t.shape[0] != nc or t.stride(0) != outs[0].stride(0) for t in outs):
+            raise ValueError("inv_vgrad_d: outputs differ in columns or pitch")
+        sl, sc = self._spec_strides(w, nc, layout)
+        tp = lambda t: _tptr(t) if t is not None else None
+        _check(load().dsht_inv_vgrad_d(self.h, _tptr(w), sl, sc, tp(xy), tp(xy_lon), tp(xy_lat),
+                                       outs[0].stride(0), nc, tp(diag), _stream(stream)))
+        return xy, xy_lon, xy_lat
 
     def alpha_fwd_d(self, a, b, w, layout="LC", diag=None, stream=None):
         nc = a.shape[0]

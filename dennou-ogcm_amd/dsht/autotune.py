@@ -1,8 +1,8 @@
 """Autotuner for the per-plan launch knobs (dsht_set_tuning).
 
 For one plan and a representative column count it times the dynamics-step transform set
-(3 inverse kinds over 4 fields, 1 forward over 6 tendencies) under each candidate and
-keeps the fastest.  The candidates are:
+(value and both gradients of 4 fields in one fused inverse call, 1 forward over 6
+tendencies) under each candidate and keeps the fastest.  The candidates are:
   column chunk      TUNE_CHUNK_COLS: columns per Fourier scratch chunk (0 = ~48 MiB)
   column block      TUNE_LEG_CB: column tiles per wave / workgroup of the contractions
   contraction       TUNE_LEG: kernel family (split forms: packed data panels or
@@ -14,7 +14,7 @@ import json
 import os
 import time
 
-from . import GRADLAT, GRADLON, PREC_FP64, TUNE_CHUNK_COLS, TUNE_LEG, TUNE_LEG_CB, VAL
+from . import PREC_FP64, TUNE_CHUNK_COLS, TUNE_LEG, TUNE_LEG_CB, VAL
 
 CACHE = os.environ.get("DSHT_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "dsht", "tuning.json"))
 KNOBS = (("chunk", TUNE_CHUNK_COLS), ("cb", TUNE_LEG_CB), ("leg", TUNE_LEG))
@@ -47,14 +47,12 @@ def _time_step(plan, bufs, stream, reps=3):
     import torch
     spec, grids, tend, spect = bufs
     for _ in range(2):
-        for k, kind in enumerate((VAL, GRADLON, GRADLAT)):
-            plan.inv_d(kind, spec, grids[k], stream=stream)
+        plan.inv_vgrad_d(spec, *grids, stream=stream)
         plan.fwd_d(VAL, tend, spect, stream=stream)
     torch.cuda.synchronize()
     t = time.perf_counter()
     for _ in range(reps):
-        for k, kind in enumerate((VAL, GRADLON, GRADLAT)):
-            plan.inv_d(kind, spec, grids[k], stream=stream)
+        plan.inv_vgrad_d(spec, *grids, stream=stream)
         plan.fwd_d(VAL, tend, spect, stream=stream)
     torch.cuda.synchronize()
     return (time.perf_counter() - t) / reps

@@ -100,6 +100,12 @@ int dsht_coef(const dsht_plan* plan, int which, const double* params, int nparam
 int dsht_w_xy(dsht_plan* plan, int kind, const double* xy, double* w, int ncols);
 /* xy_w, xy_GradLambda_w, xy_GradMu_w, xy_GradLon_w, xy_GradLat_w, xya_wa          */
 int dsht_xy_w(dsht_plan* plan, int kind, const double* w, double* xy, int ncols);
+/* xy_w, xy_GradLon_w and xy_GradLat_w of the same field in one pass (SpmlUtil_mod.f90:414-428;
+   the reference callers take both gradients of one field, HBEBaroc_hspm_vfvm_mod.f90:403,416,
+   LPhys_RediGM_hspm_vfvm_mod.f90:432-433,437-438).  The contraction reads
+   the spectra once for the P and H products.  Any one or two outputs may be NULL.   */
+int dsht_xy_vgrad_w(dsht_plan* plan, const double* w, double* xy, double* xy_lon, double* xy_lat,
+                    int ncols);
 /* w_AlphaOptr_xy / wa_AlphaOptr_xya (SpmlUtil_mod.f90:398-412,453-473)           */
 int dsht_w_alpha_xy(dsht_plan* plan, const double* a, const double* b, double* w, int ncols);
 /* xy_AlphaOptr_w / xya_AlphaOptr_wa (SpmlUtil_mod.f90:414-451)                   */
@@ -132,6 +138,9 @@ int dsht_fwd_d(dsht_plan* plan, int kind, const double* xy, long ldg, double* w,
                int ncols, const double* diag, void* stream);
 int dsht_inv_d(dsht_plan* plan, int kind, const double* w, long sl, long sc, double* xy, long ldg,
                int ncols, const double* diag, void* stream);
+/* device form of dsht_xy_vgrad_w: value, GradLon and GradLat grids of each column */
+int dsht_inv_vgrad_d(dsht_plan* plan, const double* w, long sl, long sc, double* xy, double* xy_lon,
+                     double* xy_lat, long ldg, int ncols, const double* diag, void* stream);
 int dsht_alpha_fwd_d(dsht_plan* plan, const double* a, const double* b, long ldg, double* w,
                      long sl, long sc, int ncols, const double* diag, void* stream);
 int dsht_alpha_inv_d(dsht_plan* plan, const double* wa, const double* wb, long sl, long sc,

@@ -128,12 +128,16 @@ def test_device_api_layouts(gpu, layout):
     assert rel_l2(got, w) < TOL
 
 
-def test_roundtrip_t170_chunked(gpu):
-    """T170 (I=512, J=256) round trip over more columns than one scratch chunk."""
+@pytest.mark.parametrize("chunk,nc", [(32, 80), (0, 800)])
+def test_roundtrip_t170_chunked(gpu, chunk, nc):
+    """T170 (I=512, J=256) round trip over several scratch chunks: a forced 32-column chunk
+    with a ragged last chunk, and the automatic policy splitting 800 columns (over its
+    512 MiB Fourier cap at FP64) into equal chunks."""
     import torch
     import dsht
-    N, I, J, nc = 170, 512, 256, 80
+    N, I, J = 170, 512, 256
     p = dsht.Plan(N, I, J, radius=6.371e6)
+    p.set_tuning(dsht.TUNE_CHUNK_COLS, chunk)
     w = torch.tensor(red_spectrum(N, nc, 17).T.copy(), device=gpu)
     xy = torch.empty((nc, J, I), dtype=torch.float64, device=gpu)
     w2 = torch.zeros_like(w)
@@ -177,6 +181,7 @@ def test_chunk_pipeline_matches_serial(gpu, prec):
     N, I, J, nc = 170, 512, 256, 300
     p = dsht.Plan(N, I, J, radius=6.371e6)
     p.set_precision({v: k for k, v in dsht.PREC_NAMES.items()}[prec])
+    p.set_tuning(dsht.TUNE_CHUNK_COLS, 64)  # 5 chunks, the last one ragged
     w = torch.tensor(red_spectrum(N, nc, 23).T.copy(), device=gpu)
     out = {}
     for pipe in (0, 1):
