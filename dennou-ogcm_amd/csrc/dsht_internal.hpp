@@ -120,6 +120,7 @@ struct FftArgs {
   const double* rowscale;  // optional per-latitude scale applied after the inverse (nullable)
   int threads;             // workgroup size of the register kernels (128 or 256)
   int fp32;                // FP32 arithmetic (FP32 Fourier path of the split forms)
+  int order;               // row-group order of the register kernels (0 packed, 1 latitude-major)
 };
 
 // ---------------------------------------------------------------- workspace
@@ -194,6 +195,7 @@ struct dsht_plan {
   int fft_kind = 0;    // 0 = register two-pass FFT (256-thread blocks; FP32 arithmetic in the
                        // split forms' inverse FFT), 1 = LDS Stockham, 2 = register FFT in
                        // 128-thread blocks, 3 = register FFT with FP64 arithmetic in every form
+  int fft_order = 0;   // register FFT row-group order: 0 packed, 1 latitude-major
   int leg_kind = 0;    // contractions.  FP64: 0/1 = LDS-staged, 2 = per-wave fragment streams.
                        // Split forms: 0 = fragment streams over packed data panels, 1 = LDS-
                        // staged, 2 = fragment streams splitting data in place, 3 = split data

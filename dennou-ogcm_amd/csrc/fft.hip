@@ -452,6 +452,7 @@ static FftArgs fft_args(const dsht_plan* p, long ldg, int ncols, long gld) {
   a.rowscale = nullptr;
   a.threads = p->fft_kind == 2 ? 128 : 256;
   a.fp32 = 0;
+  a.order = p->fft_order;
   return a;
 }
 
@@ -490,9 +491,10 @@ template <typename GT>
 static int fft_fwd_any(const dsht_plan* p, const double* xy, long ldg, int ncols, GT* G, long gld,
                        unsigned* colmax, hipStream_t s) {
   FftArgs a = fft_args(p, ldg, ncols, gld);
-  // forward: FP64 arithmetic always (a large zonal mean would cost the m > 0 coefficients
-  // their relative precision in FP32)
-  a.fp32 = 0;
+  // forward: FP32 arithmetic for the split forms' FP32 Fourier coefficients; the register
+  // kernel transforms each row relative to its first sample (FP64), so a large zonal mean
+  // does not cost the m > 0 coefficients their relative precision
+  a.fp32 = (sizeof(GT) == 8 && p->fft_kind == 0) ? 1 : 0;
   long nrows = (long)p->J * ncols;
   ProfScope ps("fft_fwd", s, (double)nrows * (8.0 * p->I + (double)sizeof(GT) * (p->N + 1)),
                (double)nrows * 2.5 * p->I * std::log2((double)p->I));

@@ -233,11 +233,40 @@ struct RegFft {
   static constexpr size_t LDS = (size_t)(ROWS * SR + 2 * M) * sizeof(V);
 };
 
+// Row groups.  A group is ROWS rows (latitude j, column c) that share one workgroup pass;
+// its rows are consecutive columns of one latitude, so the Fourier side (columns fastest)
+// is stored and loaded in ROWS-column runs.  Group order (a.order):
+//   0  packed: row r = g*ROWS + rr of the latitude-major row list (j = r / ncols);
+//   1  latitude-major: group g takes latitude g mod J and columns (g / J)*ROWS + rr, so the
+//      groups in flight at once spread over all latitudes (grid rows at many offsets within
+//      the column pitch) instead of a few neighbouring latitudes of every column.
+// (32-bit index arithmetic: J * ncols < 2^31 rows per call)
+__device__ __forceinline__ bool map_row(const FftArgs& a, long g, int rr, int rows, int& j, int& c) {
+  const int gi = (int)g;
+  if (a.order) {
+    j = gi % a.J;
+    c = (gi / a.J) * rows + rr;
+    return c < a.ncols;
+  }
+  const int r = gi * rows + rr;
+  j = r / a.ncols;
+  c = r - j * a.ncols;
+  return j < a.J;
+}
+__host__ __device__ inline long row_groups(const FftArgs& a, int rows) {
+  return a.order ? (long)a.J * ((a.ncols + rows - 1) / rows) : ((long)a.J * a.ncols + rows - 1) / rows;
+}
+
 }  // namespace
+
+// waves per SIMD the register kernels are compiled for: FP32 arithmetic fits three
+// (<= 168 VGPRs, 4 workgroups of LDS per CU), FP64 two (its LDS rows allow two workgroups)
+template <typename V>
+constexpr int WPE = sizeof(typename Cx<V>::S) == 4 ? 3 : 2;
 
 // V: arithmetic type (double2, or float2 for the FP32 Fourier path of the split forms)
 template <int R1, int R2, int NT, typename GT, typename V>
-__global__ __launch_bounds__(NT) void k_fft_fwd_r(const double* __restrict__ xy, GT* __restrict__ G,
+__global__ __launch_bounds__(NT, WPE<V>) void k_fft_fwd_r(const double* __restrict__ xy, GT* __restrict__ G,
                                                   FftArgs a, unsigned* __restrict__ colmax) {
   using C = RegFft<R1, R2, NT, V>;
   using S = typename Cx<V>::S;
@@ -245,6 +274,11 @@ __global__ __launch_bounds__(NT) void k_fft_fwd_r(const double* __restrict__ xy,
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   V* const lds = reinterpret_cast<V*>(lds_raw);
   __shared__ unsigned smax[ROWS];
+  // FP32 arithmetic transforms x - x0 (x0 = the row's first sample, subtracted in FP64) and
+  // adds x0 back to m = 0 in FP64: rounding then scales with the row's zonal variation, not
+  // with a large zonal mean (temperature, salinity backgrounds)
+  constexpr bool REF = sizeof(typename Cx<V>::S) == 4;
+  __shared__ double xref[REF ? ROWS : 1];
   V* buf = lds;              // ROWS * SR
   V* tw2 = lds + ROWS * SR;  // W_M^{n1 k2} at k2 * R1 + n1
   V* twi = tw2 + M;          // W_I^m
@@ -254,29 +288,54 @@ __global__ __launch_bounds__(NT) void k_fft_fwd_r(const double* __restrict__ xy,
     tw2[e] = cvt<V>(a.twM[(n1 * k2) % M]);
     twi[e] = cvt<V>(a.twI[e]);
   }
-  const long nrows = (long)a.J * a.ncols;
+  const long ngroups = row_groups(a, ROWS);
   const int N = a.N;
   const S inv_i = (S)(1.0 / (2 * M)), h = (S)0.5;
   const int rr1 = tid / R1, n1 = tid % R1;  // pass-1 role
   const int rr2 = tid / R2, k2 = tid % R2;  // pass-2 role
+  // store role: with THREADS a multiple of ROWS every item of a thread is in row tid % ROWS
+  constexpr bool FIXR = C::THREADS % ROWS == 0;
+  const int rs = tid % ROWS;
   // pass-1 inputs straight from global memory; the next group's rows are loaded into the
   // same registers once the current group's pass-1 output is in LDS
+  // FP64 arithmetic loads straight into v; FP32 arithmetic (REF) keeps the raw FP64 samples
+  // and converts them at the top of the next group, so the loads stay in flight across the
+  // barrier (the conversion does not wait on them right after issue)
   V v[R2];
-  auto fetch = [&](long rb) {
-    const long r = rb + rr1;
-    if (rr1 < ROWS && r < nrows) {
-      const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
-      const double2* x = reinterpret_cast<const double2*>(xy + c * a.ldg + (long)j * (2 * M));
+  double2 raw[REF ? R2 : 1];
+  double x0 = 0.0, x0n = 0.0;  // reference sample of the thread's pass-1 row (REF): current, next
+  auto fetch = [&](long g) {
+    int j, c;
+    if (rr1 < ROWS && map_row(a, g, rr1, ROWS, j, c)) {
+      const double* xr = xy + c * a.ldg + (long)j * (2 * M);
+      const double2* x = reinterpret_cast<const double2*>(xr);
+      if constexpr (REF) {
+        x0n = xr[0];
 #pragma unroll
-      for (int n2 = 0; n2 < R2; n2++) v[n2] = cvt<V>(x[n1 + R1 * n2]);
+        for (int n2 = 0; n2 < R2; n2++) raw[n2] = x[n1 + R1 * n2];
+      } else {
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) v[n2] = cvt<V>(x[n1 + R1 * n2]);
+      }
     } else {
+      if constexpr (REF) {
+        x0n = 0.0;
 #pragma unroll
-      for (int n2 = 0; n2 < R2; n2++) v[n2] = mk<V>(0, 0);
+        for (int n2 = 0; n2 < R2; n2++) raw[n2] = make_double2(0.0, 0.0);
+      } else {
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) v[n2] = mk<V>(0, 0);
+      }
     }
   };
-  const long rstep = (long)gridDim.x * ROWS;
-  if ((long)blockIdx.x * ROWS < nrows) fetch((long)blockIdx.x * ROWS);
-  for (long r0 = (long)blockIdx.x * ROWS; r0 < nrows; r0 += rstep) {
+  const long gstep = gridDim.x;
+  if ((long)blockIdx.x < ngroups) fetch(blockIdx.x);
+  for (long g0 = blockIdx.x; g0 < ngroups; g0 += gstep) {
+    if constexpr (REF) {
+      x0 = x0n;
+#pragma unroll
+      for (int n2 = 0; n2 < R2; n2++) v[n2] = mk<V>(raw[n2].x - x0, raw[n2].y - x0);
+    }
     dft<R2, -1>(v);
     if (colmax && tid < ROWS) smax[tid] = 0u;
     __syncthreads();  // the previous rows are out of buf; twiddles are in place
@@ -285,8 +344,11 @@ __global__ __launch_bounds__(NT) void k_fft_fwd_r(const double* __restrict__ xy,
       Y[0] = v[0];
 #pragma unroll
       for (int q = 1; q < R2; q++) Y[q] = rcmul(v[q], tw2[q * R1 + n1]);
+      if constexpr (REF) {
+        if (n1 == 0) xref[rr1] = x0;
+      }
     }
-    if (r0 + rstep < nrows) fetch(r0 + rstep);
+    if (g0 + gstep < ngroups) fetch(g0 + gstep);
     __syncthreads();
     V u[R1];
     if (rr2 < ROWS) {
@@ -303,10 +365,12 @@ __global__ __launch_bounds__(NT) void k_fft_fwd_r(const double* __restrict__ xy,
     }
     __syncthreads();
     // half-complex split of the packed transform, 1/I, store; rows fastest across threads
+    int js = 0, cs = 0;
+    const bool oks = FIXR && map_row(a, g0, rs, ROWS, js, cs);
     for (int idx = tid; idx < (N + 1) * ROWS; idx += C::THREADS) {
-      const int m = idx / ROWS, rr = idx % ROWS;
-      const long r = r0 + rr;
-      if (r >= nrows) continue;
+      const int m = idx / ROWS, rr = FIXR ? rs : idx % ROWS;
+      int j = js, c = cs;
+      if (FIXR ? !oks : !map_row(a, g0, rr, ROWS, j, c)) continue;
       const V* X = buf + rr * SR;
       const V zm = X[m];
       V zc = X[m ? M - m : 0];
@@ -315,19 +379,22 @@ __global__ __launch_bounds__(NT) void k_fft_fwd_r(const double* __restrict__ xy,
       const V O = mk<V>(h * (zm.y - zc.y), -h * (zm.x - zc.x));
       V g = radd(E, rcmul(twi[m], O));
       g = mk<V>(g.x * inv_i, g.y * inv_i);
-      const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
+      if constexpr (REF) {
+        if (m == 0) g.x = (S)((double)g.x + xref[rr]);
+      }
       G[(long)m * a.gmst + (long)j * a.gld + c] = cvt<GT>(g);
       if (colmax) atomicMax(&smax[rr], __float_as_uint(fmaxf(fabsf((float)g.x), fabsf((float)g.y))));
     }
     if (colmax) {  // per-column magnitude for the scaled FP16 contraction
       __syncthreads();
-      if (tid < ROWS && r0 + tid < nrows) atomicMax(colmax + (int)((r0 + tid) % a.ncols), smax[tid]);
+      int j, c;
+      if (tid < ROWS && map_row(a, g0, tid, ROWS, j, c)) atomicMax(colmax + c, smax[tid]);
     }
   }
 }
 
 template <int R1, int R2, int NT, typename GT, typename V>
-__global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, double* __restrict__ xy,
+__global__ __launch_bounds__(NT, WPE<V>) void k_fft_inv_r(const GT* __restrict__ G, double* __restrict__ xy,
                                                   FftArgs a) {
   using C = RegFft<R1, R2, NT, V>;
   using S = typename Cx<V>::S;
@@ -343,7 +410,7 @@ __global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, doub
     tw2[e] = cvt<V>(a.twM[(n1 * k2) % M]);
     twi[e] = cvt<V>(a.twI[e]);
   }
-  const long nrows = (long)a.J * a.ncols;
+  const long ngroups = row_groups(a, ROWS);
   const int N = a.N;
   const int rr1 = tid / R1, n1 = tid % R1;
   const int rr2 = tid / R2, k2 = tid % R2;
@@ -352,26 +419,28 @@ __global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, doub
   // current group is in LDS, so they overlap its FFT passes and grid stores.
   constexpr int NL = (M * ROWS + C::THREADS - 1) / C::THREADS;
   GT gr[NL];
-  auto fetch = [&](long rb) {
+  // with THREADS a multiple of ROWS every item idx = tid + q * THREADS is in row tid % ROWS
+  constexpr bool FIXR = C::THREADS % ROWS == 0;
+  auto fetch = [&](long g) {
+    int j0 = 0, c0 = 0;
+    const bool ok0 = FIXR && map_row(a, g, tid % ROWS, ROWS, j0, c0);
 #pragma unroll
     for (int q = 0; q < NL; q++) {
       const int idx = tid + q * C::THREADS;
-      const int m = idx / ROWS, rr = idx % ROWS;
-      const long r = rb + rr;
+      const int m = idx / ROWS;
       gr[q] = cvt<GT>(make_float2(0.f, 0.f));
-      if (idx < M * ROWS && m <= N && r < nrows) {
-        const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
-        gr[q] = G[(long)m * a.gmst + (long)j * a.gld + c];
-      }
+      int j = j0, c = c0;
+      const bool ok = FIXR ? ok0 : map_row(a, g, idx % ROWS, ROWS, j, c);
+      if (ok && idx < M * ROWS && m <= N) gr[q] = G[(long)m * a.gmst + (long)j * a.gld + c];
     }
   };
   // prefetching holds the next group in registers through the whole pipeline: only where
   // that fits the register budget (FP32 coefficients, M <= 256)
   constexpr bool PF = sizeof(GT) == 8 && M <= 256;
-  const long rstep = (long)gridDim.x * ROWS;
-  if (PF && (long)blockIdx.x * ROWS < nrows) fetch((long)blockIdx.x * ROWS);
-  for (long r0 = (long)blockIdx.x * ROWS; r0 < nrows; r0 += rstep) {
-    if (!PF) fetch(r0);
+  const long gstep = gridDim.x;
+  if (PF && (long)blockIdx.x < ngroups) fetch(blockIdx.x);
+  for (long g0 = blockIdx.x; g0 < ngroups; g0 += gstep) {
+    if (!PF) fetch(g0);
     __syncthreads();  // the previous rows are out of buf
 #pragma unroll
     for (int q = 0; q < NL; q++) {
@@ -382,7 +451,7 @@ __global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, doub
         buf[(idx % ROWS) * SR + idx / ROWS] = v;
       }
     }
-    if (PF && r0 + rstep < nrows) fetch(r0 + rstep);
+    if (PF && g0 + gstep < ngroups) fetch(g0 + gstep);
     __syncthreads();
     // packed spectrum Z_m from the half-complex pair (X_m, X_{M-m}); X_M = 0 for m = 0
     for (int idx = tid; idx < ROWS * (M / 2 + 1); idx += C::THREADS) {
@@ -422,9 +491,8 @@ __global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, doub
 #pragma unroll
       for (int p = 0; p < R1; p++) u[p] = Y[p * PY];
       dft<R1, +1>(u);
-      const long r = r0 + rr2;
-      if (r < nrows) {
-        const int j = (int)(r / a.ncols), c = (int)(r % a.ncols);
+      int j, c;
+      if (map_row(a, g0, rr2, ROWS, j, c)) {
         const double s = a.rowscale ? a.rowscale[j] : 1.0;
         double2* x = reinterpret_cast<double2*>(xy + c * a.ldg + (long)j * (2 * M));
 #pragma unroll
@@ -435,10 +503,11 @@ __global__ __launch_bounds__(NT) void k_fft_inv_r(const GT* __restrict__ G, doub
 }
 
 // ---------------------------------------------------------------- dispatch
-// workgroups resident at once: the register kernels are held to two per CU by their LDS
-// rows, so the grid is capped at four per CU and each workgroup walks several row groups
-// (its loads for the next group overlap the current one)
-static long fft_grid_cap() {
+// The register kernels walk row groups in a persistent loop (each workgroup's loads for its
+// next group overlap the current one), so the grid is the number of workgroups that are
+// resident at once: every workgroup then runs from the start, and no second, partly empty
+// round of workgroups follows the first.
+static int cu_count() {
   static int cus = 0;
   if (!cus) {
     int dev = 0;
@@ -446,15 +515,25 @@ static long fft_grid_cap() {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
       cus = 256;
   }
-  return 4l * cus;
+  return cus;
+}
+
+template <typename K>
+static long resident_grid(K kernel, int threads, size_t lds) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess || per_cu <= 0) {
+    (void)hipGetLastError();
+    per_cu = 2;
+  }
+  return (long)per_cu * cu_count();
 }
 
 template <int R1, int R2, int NT, typename GT, typename V>
 static int reg_fwd_nt(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
   using C = RegFft<R1, R2, NT, V>;
-  const long nrows = (long)a.J * a.ncols;
-  long nb = cdivl(nrows, C::ROWS);
-  if (nb > fft_grid_cap()) nb = fft_grid_cap();
+  static const long cap = resident_grid(k_fft_fwd_r<R1, R2, NT, GT, V>, C::THREADS, C::LDS);
+  long nb = row_groups(a, C::ROWS);
+  if (nb > cap) nb = cap;
   hipLaunchKernelGGL((k_fft_fwd_r<R1, R2, NT, GT, V>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, xy, G, a,
                      colmax);
   DSHT_HIP(hipGetLastError());
@@ -464,9 +543,9 @@ static int reg_fwd_nt(const double* xy, const FftArgs& a, GT* G, unsigned* colma
 template <int R1, int R2, int NT, typename GT, typename V>
 static int reg_inv_nt(const GT* G, const FftArgs& a, double* xy, hipStream_t s) {
   using C = RegFft<R1, R2, NT, V>;
-  const long nrows = (long)a.J * a.ncols;
-  long nb = cdivl(nrows, C::ROWS);
-  if (nb > fft_grid_cap()) nb = fft_grid_cap();
+  static const long cap = resident_grid(k_fft_inv_r<R1, R2, NT, GT, V>, C::THREADS, C::LDS);
+  long nb = row_groups(a, C::ROWS);
+  if (nb > cap) nb = cap;
   hipLaunchKernelGGL((k_fft_inv_r<R1, R2, NT, GT, V>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, G, xy, a);
   DSHT_HIP(hipGetLastError());
   return 0;

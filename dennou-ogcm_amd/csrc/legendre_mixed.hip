@@ -819,6 +819,11 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
   MixedTables& t = ((MixedState*)p->mixed)->tab[family];
   const int nin = (op == F_ALPHA || op == F_UV2VORDIV) ? 2 : 1;
   const bool scaled = form >= 2;
+  // the first (LDS-staged) split design keeps the BF16 forms only: its FP16 product-order
+  // weighting loses up to ~2e-5 relative on zero-mean data (the fragment-stream kernels
+  // hold the FP16 forms to their bounds)
+  if (scaled && p->leg_kind == 1)
+    return fail(DSHT_EUNSUPPORTED, "DSHT_TUNE_LEG=1 (first split design) supports the BF16 forms only");
   const int ch = mixed_chunk_cols(p, ncols);
   const int nchunks = cdiv(ncols, ch);
   const size_t gsz = (size_t)(p->N + 1) * p->J * ch;
@@ -924,6 +929,11 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
   const int nout = inv_nout(op);
   if (op == I_VGRAD && p->leg_kind == 1) return fail(DSHT_EUNSUPPORTED, "fused value/gradient inverse needs DSHT_TUNE_LEG != 1");
   const bool scaled = form >= 2;
+  // the first (LDS-staged) split design keeps the BF16 forms only: its FP16 product-order
+  // weighting loses up to ~2e-5 relative on zero-mean data (the fragment-stream kernels
+  // hold the FP16 forms to their bounds)
+  if (scaled && p->leg_kind == 1)
+    return fail(DSHT_EUNSUPPORTED, "DSHT_TUNE_LEG=1 (first split design) supports the BF16 forms only");
   const int ch = mixed_chunk_cols(p, ncols);
   const int nchunks = cdiv(ncols, ch);
   const size_t gsz = (size_t)(p->N + 1) * p->J * ch;

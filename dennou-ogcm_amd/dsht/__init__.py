@@ -44,7 +44,7 @@ EXPORTS = [
     "dsht_batch_info", "dsht_batch_ngroups", "dsht_batch_buffers", "dsht_batch_step",
     "dsht_batch_step_sequential", "dsht_batch_capture", "dsht_batch_launch", "dsht_batch_destroy",
 ]
-TUNE_CHUNK_COLS, TUNE_LEG_CB, TUNE_FFT, TUNE_LEG, TUNE_PIPELINE = 0, 1, 2, 3, 4
+TUNE_CHUNK_COLS, TUNE_LEG_CB, TUNE_FFT, TUNE_LEG, TUNE_PIPELINE, TUNE_FFT_ORDER = 0, 1, 2, 3, 4, 5
 
 
 class DshtError(RuntimeError):

@@ -170,6 +170,12 @@ def test_fft_kernels(gpu, N, I, J):
         assert rel_l2(out[kind][1], ref_f) < TOL, (kind, rel_l2(out[kind][1], ref_f))
     assert rel_l2(out[0][0], out[1][0]) < 1e-14
     assert rel_l2(out[0][1], out[1][1]) < 1e-14
+    # register kernels in both row-group orders: the same rows, only their grouping differs
+    p.set_tuning(dsht.TUNE_FFT, 0)
+    for order in (0, 1):
+        p.set_tuning(dsht.TUNE_FFT_ORDER, order)
+        got = (p._inv(w, dsht.VAL), p._fwd(xy, dsht.VAL))
+        assert np.array_equal(got[0], out[0][0]) and np.array_equal(got[1], out[0][1]), order
 
 
 @pytest.mark.parametrize("prec", ["fp64", "bf16x6"])
