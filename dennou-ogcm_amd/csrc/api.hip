@@ -153,7 +153,7 @@ int pipe_join(ChunkPipe& cp) {
 static int chunk_cols(const dsht_plan* p, int ncols) { return chunk_columns(p, ncols, sizeof(double2)); }
 
 static size_t fourier_bytes(const dsht_plan* p, int ch) {
-  return (size_t)(p->N + 1) * p->J * ch * sizeof(double2);
+  return (size_t)(p->N + 1) * p->J * fourier_pitch(ch) * sizeof(double2);
 }
 
 static int check_plan(const dsht_plan* p) {
@@ -179,14 +179,15 @@ static int run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, l
     const int nc = ncols - c0 < ch ? ncols - c0 : ch;
     const int slot = cp.on ? (k & 1) : 0;
     double2* G0 = (double2*)((char*)wl.buf + slot * slot_bytes);
-    double2* G1 = G0 + (size_t)(p->N + 1) * p->J * ch;
+    double2* G1 = G0 + (size_t)(p->N + 1) * p->J * fourier_pitch(ch);
+    const long gp = fourier_pitch(nc);
     hipStream_t fs = s;
     if (cp.on) {
       fs = cp.aux;
       if (k >= 2) DSHT_HIP(hipStreamWaitEvent(fs, cp.evL[slot], 0));  // slot free again
     }
-    if ((rc = launch_fft_fwd(p, in0 + (long)c0 * ldg, ldg, nc, G0, nc, fs))) return rc;
-    if (nin > 1 && (rc = launch_fft_fwd(p, in1 + (long)c0 * ldg, ldg, nc, G1, nc, fs))) return rc;
+    if ((rc = launch_fft_fwd(p, in0 + (long)c0 * ldg, ldg, nc, G0, gp, fs))) return rc;
+    if (nin > 1 && (rc = launch_fft_fwd(p, in1 + (long)c0 * ldg, ldg, nc, G1, gp, fs))) return rc;
     if (cp.on && (rc = pipe_after(cp.evF[slot], fs, s))) return rc;
     LegFwdArgs a;
     a.G0 = G0;
@@ -196,8 +197,8 @@ static int run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, l
     a.sl = sl;
     a.sc = sc;
     a.ncols = nc;
-    a.gld = nc;
-    a.gmst = (long)p->J * nc;
+    a.gld = fourier_pitch(nc);
+    a.gmst = (long)p->J * a.gld;
     a.tabP = p->d_tabFP;
     a.tabH = p->d_tabFH;
     a.offF = p->d_offF;
@@ -232,8 +233,8 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
     const int nc = ncols - c0 < ch ? ncols - c0 : ch;
     const int slot = cp.on ? (k & 1) : 0;
     double2* G0 = (double2*)((char*)wl.buf + slot * slot_bytes);
-    double2* G1 = G0 + (size_t)(p->N + 1) * p->J * ch;
-    double2* G2 = G1 + (size_t)(p->N + 1) * p->J * ch;
+    double2* G1 = G0 + (size_t)(p->N + 1) * p->J * fourier_pitch(ch);
+    double2* G2 = G1 + (size_t)(p->N + 1) * p->J * fourier_pitch(ch);
     if (cp.on && k >= 2) DSHT_HIP(hipStreamWaitEvent(s, cp.evF[slot], 0));  // slot free again
     LegInvArgs a;
     a.W0 = in0 + (long)c0 * sc;
@@ -244,8 +245,8 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
     a.G1 = nout > 1 && out1 ? G1 : nullptr;
     a.G2 = nout > 2 && out2 ? G2 : nullptr;
     a.ncols = nc;
-    a.gld = nc;
-    a.gmst = (long)p->J * nc;
+    a.gld = fourier_pitch(nc);
+    a.gmst = (long)p->J * a.gld;
     a.tabP = p->d_tabIP;
     a.tabH = p->d_tabIH;
     a.offI = p->d_offI;
@@ -261,9 +262,9 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
       fs = cp.aux;
       if ((rc = pipe_after(cp.evL[slot], s, fs))) return rc;
     }
-    if (a.G0 && (rc = launch_fft_inv(p, G0, nc, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
-    if (a.G1 && (rc = launch_fft_inv(p, G1, nc, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
-    if (a.G2 && (rc = launch_fft_inv(p, G2, nc, nc, out2 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (a.G0 && (rc = launch_fft_inv(p, G0, a.gld, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (a.G1 && (rc = launch_fft_inv(p, G1, a.gld, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (a.G2 && (rc = launch_fft_inv(p, G2, a.gld, nc, out2 + (long)c0 * ldg, ldg, fs))) return rc;
     if (cp.on) DSHT_HIP(hipEventRecord(cp.evF[slot], fs));
   }
   return pipe_join(cp);

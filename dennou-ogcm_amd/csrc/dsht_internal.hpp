@@ -195,7 +195,7 @@ struct dsht_plan {
   int fft_kind = 0;    // 0 = register two-pass FFT (256-thread blocks; FP32 arithmetic in the
                        // split forms' inverse FFT), 1 = LDS Stockham, 2 = register FFT in
                        // 128-thread blocks, 3 = register FFT with FP64 arithmetic in every form
-  int fft_order = 0;   // register FFT row-group order: 0 packed, 1 latitude-major
+  int fft_order = 1;   // register FFT row-group order: 0 packed, 1 latitude-major (default)
   int leg_kind = 0;    // contractions.  FP64: 0/1 = LDS-staged, 2 = per-wave fragment streams.
                        // Split forms: 0 = fragment streams over packed data panels, 1 = LDS-
                        // staged, 2 = fragment streams splitting data in place, 3 = split data
@@ -209,6 +209,9 @@ namespace dsht {
 // T170L60 the FP64 step takes 0.84 ms with 512 MiB chunks against 1.0 ms with 48 MiB
 // chunks (fewer, larger FFT and contraction launches).
 constexpr long CHUNK_FOURIER_BYTES = 512l << 20;
+// Fourier row pitch of a chunk of nc columns: a multiple of the 16-column group, so every
+// 16-column run of a latitude row starts on a 128-byte line (FP32) or 256-byte pair of lines
+inline long fourier_pitch(int nc) { return ((long)nc + 15) / 16 * 16; }
 inline int chunk_columns(const dsht_plan* p, int ncols, long fourier_elem_bytes) {
   if (p->chunk_cols > 0) return p->chunk_cols < ncols ? p->chunk_cols : ncols;
   const long per = (long)(p->N + 1) * p->J * fourier_elem_bytes;

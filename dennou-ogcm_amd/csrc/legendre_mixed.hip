@@ -826,7 +826,7 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
     return fail(DSHT_EUNSUPPORTED, "DSHT_TUNE_LEG=1 (first split design) supports the BF16 forms only");
   const int ch = mixed_chunk_cols(p, ncols);
   const int nchunks = cdiv(ncols, ch);
-  const size_t gsz = (size_t)(p->N + 1) * p->J * ch;
+  const size_t gsz = (size_t)(p->N + 1) * p->J * fourier_pitch(ch);
   const size_t cmb = (2 * (size_t)ch * sizeof(unsigned) + 255) / 256 * 256;
   const size_t slot_bytes = nin * gsz * sizeof(float2) + cmb;  // Fourier inputs + column maxima
   WsHandle wh;
@@ -850,8 +850,9 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
       hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(2 * nc, 256)), dim3(256), 0, fs, colmax, 2 * nc);
       DSHT_HIP(hipGetLastError());
     }
-    int rc = launch_fft_fwd32(p, in0 + (long)c0 * ldg, ldg, nc, G0, nc, scaled ? colmax : nullptr, fs);
-    if (!rc && nin > 1) rc = launch_fft_fwd32(p, in1 + (long)c0 * ldg, ldg, nc, G1, nc, scaled ? colmax + nc : nullptr, fs);
+    const long gp = fourier_pitch(nc);
+    int rc = launch_fft_fwd32(p, in0 + (long)c0 * ldg, ldg, nc, G0, gp, scaled ? colmax : nullptr, fs);
+    if (!rc && nin > 1) rc = launch_fft_fwd32(p, in1 + (long)c0 * ldg, ldg, nc, G1, gp, scaled ? colmax + nc : nullptr, fs);
     if (rc) return rc;
     if (cp.on && (rc = pipe_after(cp.evF[slot], fs, s))) return rc;
     MxFwdArgs a;
@@ -862,8 +863,8 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
     a.sl = sl;
     a.sc = sc;
     a.ncols = nc;
-    a.gld = nc;
-    a.gmst = (long)p->J * nc;
+    a.gld = fourier_pitch(nc);
+    a.gmst = (long)p->J * a.gld;
     for (int q = 0; q < 3; q++) {
       a.tP[q] = t.fP[q];
       a.tH[q] = t.fH[q];
@@ -887,8 +888,8 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
         Mx2FwdJob j;
         j.G0 = G0;
         j.G1 = nin > 1 ? G1 : nullptr;
-        j.gld = nc;
-        j.gmst = (long)p->J * nc;
+        j.gld = fourier_pitch(nc);
+        j.gmst = (long)p->J * j.gld;
         j.ncols = nc;
         j.W0 = a.W0;
         j.W1 = a.W1;
@@ -936,7 +937,7 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
     return fail(DSHT_EUNSUPPORTED, "DSHT_TUNE_LEG=1 (first split design) supports the BF16 forms only");
   const int ch = mixed_chunk_cols(p, ncols);
   const int nchunks = cdiv(ncols, ch);
-  const size_t gsz = (size_t)(p->N + 1) * p->J * ch;
+  const size_t gsz = (size_t)(p->N + 1) * p->J * fourier_pitch(ch);
   const size_t slot_bytes = nout * gsz * sizeof(float2);  // Fourier outputs of one chunk
   const int nslot = p->pipeline && nchunks > 1 ? 2 : 1;
   const size_t cmb = (2 * (size_t)ch * sizeof(unsigned) + 255) / 256 * 256;
@@ -975,8 +976,8 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
     a.G0 = G0;
     a.G1 = nout > 1 ? G1 : nullptr;
     a.ncols = nc;
-    a.gld = nc;
-    a.gmst = (long)p->J * nc;
+    a.gld = fourier_pitch(nc);
+    a.gmst = (long)p->J * a.gld;
     for (int q = 0; q < 3; q++) {
       a.tP[q] = t.iP[q];
       a.tH[q] = t.iH[q];
@@ -1009,8 +1010,8 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
         j.G0 = O0;
         j.G1 = O1;
         j.G2 = O2;
-        j.gld = nc;
-        j.gmst = (long)p->J * nc;
+        j.gld = fourier_pitch(nc);
+        j.gmst = (long)p->J * j.gld;
         rc = mx2_inv(p, t, form, op, j, s);
       } else {
         switch (form) {
@@ -1027,9 +1028,10 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
       fs = cp.aux;
       if ((rc = pipe_after(cp.evL[slot], s, fs))) return rc;
     }
-    if (O0 && (rc = launch_fft_inv32(p, G0, nc, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
-    if (O1 && (rc = launch_fft_inv32(p, G1, nc, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
-    if (O2 && (rc = launch_fft_inv32(p, G2, nc, nc, out2 + (long)c0 * ldg, ldg, fs))) return rc;
+    const long gp = fourier_pitch(nc);
+    if (O0 && (rc = launch_fft_inv32(p, G0, gp, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (O1 && (rc = launch_fft_inv32(p, G1, gp, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (O2 && (rc = launch_fft_inv32(p, G2, gp, nc, out2 + (long)c0 * ldg, ldg, fs))) return rc;
     if (cp.on) DSHT_HIP(hipEventRecord(cp.evF[slot], fs));
   }
   return pipe_join(cp);

@@ -184,8 +184,10 @@ int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, d
 #define DSHT_TUNE_PIPELINE 4   /* 1 = run each call's column chunks as a two-stream pipeline (FFT of
                                   chunk k on a helper stream beside the contraction of chunk k+1) */
 #define DSHT_TUNE_FFT_ORDER 5  /* row-group order of the register FFTs: 0 = packed (groups walk
-                                  the columns of one latitude first), 1 = latitude-major (the
-                                  groups in flight spread over all latitudes) */
+                                  the columns of one latitude first and may straddle two
+                                  latitudes), 1 = latitude-major (default: each group is 16
+                                  columns of one latitude, the groups in flight spread over all
+                                  latitudes) */
 int dsht_set_tuning(dsht_plan* plan, int key, int value);
 int dsht_get_tuning(const dsht_plan* plan, int key);
 
