@@ -241,8 +241,11 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
     a.W1 = in1 ? in1 + (long)c0 * sc : nullptr;
     a.sl = sl;
     a.sc = sc;
-    a.G0 = out0 ? G0 : nullptr;
-    a.G1 = nout > 1 && out1 ? G1 : nullptr;
+    // fused value/gradient: GradLon comes from the value's Fourier coefficients through the
+    // derivative inverse FFT (i m X_m, 1/cos row scale), so the contraction writes two outputs
+    const bool dlon = op == I_VGRAD && out1 && fft_inv_deriv_ok(p);
+    a.G0 = out0 || dlon ? G0 : nullptr;
+    a.G1 = nout > 1 && out1 && !dlon ? G1 : nullptr;
     a.G2 = nout > 2 && out2 ? G2 : nullptr;
     a.ncols = nc;
     a.gld = fourier_pitch(nc);
@@ -262,7 +265,8 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
       fs = cp.aux;
       if ((rc = pipe_after(cp.evL[slot], s, fs))) return rc;
     }
-    if (a.G0 && (rc = launch_fft_inv(p, G0, a.gld, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (out0 && (rc = launch_fft_inv(p, G0, a.gld, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (dlon && (rc = launch_fft_inv(p, G0, a.gld, nc, out1 + (long)c0 * ldg, ldg, fs, p->d_rs_icos, 1))) return rc;
     if (a.G1 && (rc = launch_fft_inv(p, G1, a.gld, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
     if (a.G2 && (rc = launch_fft_inv(p, G2, a.gld, nc, out2 + (long)c0 * ldg, ldg, fs))) return rc;
     if (cp.on) DSHT_HIP(hipEventRecord(cp.evF[slot], fs));

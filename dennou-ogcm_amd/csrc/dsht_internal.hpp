@@ -121,6 +121,7 @@ struct FftArgs {
   int threads;             // workgroup size of the register kernels (128 or 256)
   int fp32;                // FP32 arithmetic (FP32 Fourier path of the split forms)
   int order;               // row-group order of the register kernels (0 packed, 1 latitude-major)
+  int deriv;               // inverse: transform i m X_m (longitude derivative) instead of X_m
 };
 
 // ---------------------------------------------------------------- workspace
@@ -246,8 +247,11 @@ int fft_plan_init(dsht_plan* p);
 void fft_plan_free(dsht_plan* p);
 int launch_fft_fwd(const dsht_plan* p, const double* xy, long ldg, int ncols, double2* G, long gld,
                    hipStream_t s);
+// rowscale: per-latitude factor on the output (nullable); deriv: transform i m X_m (the
+// longitude derivative of the field), register kernels only (fft_inv_deriv_ok)
+bool fft_inv_deriv_ok(const dsht_plan* p);
 int launch_fft_inv(const dsht_plan* p, const double2* G, long gld, int ncols, double* xy, long ldg,
-                   hipStream_t s);
+                   hipStream_t s, const double* rowscale = nullptr, int deriv = 0);
 // FP32 Fourier intermediates of the split-precision forms; colmax (nullable) receives the
 // per-column max |G| as float bits
 bool reg_fft_supported(int M);
@@ -261,7 +265,7 @@ int launch_leg_fwd_direct(const dsht_plan* p, int op, const LegFwdArgs& a, hipSt
 int launch_fft_fwd32(const dsht_plan* p, const double* xy, long ldg, int ncols, float2* G, long gld,
                      unsigned* colmax, hipStream_t s);
 int launch_fft_inv32(const dsht_plan* p, const float2* G, long gld, int ncols, double* xy, long ldg,
-                     hipStream_t s);
+                     hipStream_t s, const double* rowscale = nullptr, int deriv = 0);
 // legendre_f64.hip
 int launch_leg_fwd(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t s);
 int launch_leg_inv(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t s);

@@ -453,6 +453,7 @@ static FftArgs fft_args(const dsht_plan* p, long ldg, int ncols, long gld) {
   a.threads = p->fft_kind == 2 ? 128 : 256;
   a.fp32 = 0;
   a.order = p->fft_order;
+  a.deriv = 0;
   return a;
 }
 
@@ -514,10 +515,15 @@ static int fft_fwd_any(const dsht_plan* p, const double* xy, long ldg, int ncols
   return 0;
 }
 
+bool fft_inv_deriv_ok(const dsht_plan* p) { return p->fft_kind != 1 && reg_fft_supported(p->fft.M); }
+
 template <typename GT>
 static int fft_inv_any(const dsht_plan* p, const GT* G, long gld, int ncols, double* xy, long ldg,
-                       hipStream_t s) {
+                       hipStream_t s, const double* rowscale, int deriv) {
   FftArgs a = fft_args(p, ldg, ncols, gld);
+  a.rowscale = rowscale;
+  a.deriv = deriv;
+  if (deriv && !fft_inv_deriv_ok(p)) return fail(DSHT_EUNSUPPORTED, "derivative inverse FFT needs a register kernel");
   // inverse: FP32 arithmetic on the split forms' FP32 Fourier coefficients (each output point
   // is a sum of already-FP32 terms; the error stays at FP32 rounding of the output scale)
   a.fp32 = (sizeof(GT) == 8 && p->fft_kind == 0) ? 1 : 0;
@@ -546,8 +552,8 @@ int launch_fft_fwd(const dsht_plan* p, const double* xy, long ldg, int ncols, do
 }
 
 int launch_fft_inv(const dsht_plan* p, const double2* G, long gld, int ncols, double* xy, long ldg,
-                   hipStream_t s) {
-  return fft_inv_any<double2>(p, G, gld, ncols, xy, ldg, s);
+                   hipStream_t s, const double* rowscale, int deriv) {
+  return fft_inv_any<double2>(p, G, gld, ncols, xy, ldg, s, rowscale, deriv);
 }
 
 int launch_fft_fwd32(const dsht_plan* p, const double* xy, long ldg, int ncols, float2* G, long gld,
@@ -556,8 +562,8 @@ int launch_fft_fwd32(const dsht_plan* p, const double* xy, long ldg, int ncols, 
 }
 
 int launch_fft_inv32(const dsht_plan* p, const float2* G, long gld, int ncols, double* xy, long ldg,
-                     hipStream_t s) {
-  return fft_inv_any<float2>(p, G, gld, ncols, xy, ldg, s);
+                     hipStream_t s, const double* rowscale, int deriv) {
+  return fft_inv_any<float2>(p, G, gld, ncols, xy, ldg, s, rowscale, deriv);
 }
 
 }  // namespace dsht

@@ -395,7 +395,9 @@ __global__ __launch_bounds__(NT, WPE<V>) void k_fft_fwd_r(const double* __restri
   }
 }
 
-template <int R1, int R2, int NT, typename GT, typename V>
+// DERIV: transform i m X_m (longitude derivative); a separate instantiation keeps the plain
+// kernel's register allocation
+template <int R1, int R2, int NT, typename GT, typename V, bool DERIV>
 __global__ __launch_bounds__(NT, WPE<V>) void k_fft_inv_r(const GT* __restrict__ G, double* __restrict__ xy,
                                                   FftArgs a) {
   using C = RegFft<R1, R2, NT, V>;
@@ -460,7 +462,12 @@ __global__ __launch_bounds__(NT, WPE<V>) void k_fft_inv_r(const GT* __restrict__
       const int m = idx / ROWS, rr = idx % ROWS;
       V* X = buf + rr * SR;
       const int mc = m ? M - m : 0;
-      const V xm = X[m], xo = X[mc];
+      V xm = X[m], xo = X[mc];
+      if constexpr (DERIV) {  // longitude derivative: each X_k becomes i k X_k
+        const S fm = (S)m, fc = (S)mc;
+        xm = mk<V>(-fm * xm.y, fm * xm.x);
+        xo = mk<V>(-fc * xo.y, fc * xo.x);
+      }
       const V c1 = m ? mk<V>(xo.x, -xo.y) : mk<V>(0, 0);
       const V A1 = radd(xm, c1), B1 = rcmulc(rsub(xm, c1), twi[m]);
       const V z1 = mk<V>(A1.x - B1.y, A1.y + B1.x);
@@ -545,10 +552,16 @@ static int reg_fwd_nt(const double* xy, const FftArgs& a, GT* G, unsigned* colma
 template <int R1, int R2, int NT, typename GT, typename V>
 static int reg_inv_nt(const GT* G, const FftArgs& a, double* xy, hipStream_t s) {
   using C = RegFft<R1, R2, NT, V>;
-  static const long cap = resident_grid(k_fft_inv_r<R1, R2, NT, GT, V>, C::THREADS, C::LDS);
+  static const long cap = resident_grid(k_fft_inv_r<R1, R2, NT, GT, V, false>, C::THREADS, C::LDS);
+  static const long capd = resident_grid(k_fft_inv_r<R1, R2, NT, GT, V, true>, C::THREADS, C::LDS);
   long nb = row_groups(a, C::ROWS);
-  if (nb > cap) nb = cap;
-  hipLaunchKernelGGL((k_fft_inv_r<R1, R2, NT, GT, V>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, G, xy, a);
+  if (nb > (a.deriv ? capd : cap)) nb = a.deriv ? capd : cap;
+  if (a.deriv)
+    hipLaunchKernelGGL((k_fft_inv_r<R1, R2, NT, GT, V, true>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, G, xy,
+                       a);
+  else
+    hipLaunchKernelGGL((k_fft_inv_r<R1, R2, NT, GT, V, false>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, G,
+                       xy, a);
   DSHT_HIP(hipGetLastError());
   return 0;
 }

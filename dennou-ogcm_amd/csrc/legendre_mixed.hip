@@ -955,8 +955,11 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
     float2* G0 = (float2*)((char*)wh.buf + slot * slot_bytes);
     float2* G1 = G0 + gsz;
     float2* G2 = G1 + gsz;
-    float2* O0 = out0 ? G0 : nullptr;
-    float2* O1 = nout > 1 && out1 ? G1 : nullptr;
+    // fused value/gradient: GradLon through the derivative inverse FFT of the value's
+    // coefficients (i m X_m, 1/cos row scale); the contraction writes two outputs
+    const bool dlon = op == I_VGRAD && out1 && fft_inv_deriv_ok(p);
+    float2* O0 = out0 || dlon ? G0 : nullptr;
+    float2* O1 = nout > 1 && out1 && !dlon ? G1 : nullptr;
     float2* O2 = nout > 2 && out2 ? G2 : nullptr;
     const double* w0 = in0 + (long)c0 * sc;
     const double* w1 = in1 ? in1 + (long)c0 * sc : nullptr;
@@ -1029,7 +1032,8 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
       if ((rc = pipe_after(cp.evL[slot], s, fs))) return rc;
     }
     const long gp = fourier_pitch(nc);
-    if (O0 && (rc = launch_fft_inv32(p, G0, gp, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (out0 && (rc = launch_fft_inv32(p, G0, gp, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
+    if (dlon && (rc = launch_fft_inv32(p, G0, gp, nc, out1 + (long)c0 * ldg, ldg, fs, p->d_rs_icos, 1))) return rc;
     if (O1 && (rc = launch_fft_inv32(p, G1, gp, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
     if (O2 && (rc = launch_fft_inv32(p, G2, gp, nc, out2 + (long)c0 * ldg, ldg, fs))) return rc;
     if (cp.on) DSHT_HIP(hipEventRecord(cp.evF[slot], fs));

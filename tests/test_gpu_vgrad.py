@@ -4,8 +4,9 @@ One call returns xy_w(w), xy_GradLon_w(w) and xy_GradLat_w(w) (SpmlUtil_mod.f90:
 Checks:
   * against the FP64 oracle at <= 1e-12 (FP64 form) and at each split form's error bound;
   * against the three separate inverse calls of the same plan: <= 1e-13 relative L2 for
-    FP64 (the fused kernel sums the same products in different K chunks) and <= 1e-12 for
-    the split forms (same products and chunk order);
+    FP64, and <= 2e-7 for the split forms, whose fused GradLon multiplies the value's FP32
+    Fourier coefficients by i m inside the FP32 inverse FFT (one more FP32 rounding than the
+    separate call, which applies i m in the FP64 contraction epilogue);
   * every kernel family (DSHT_TUNE_LEG), NULL outputs, both column blocks, the chunked
     T170 path and the two-stream chunk pipeline.
 """
@@ -17,6 +18,7 @@ from conftest import red_spectrum, rel_l2
 pytestmark = pytest.mark.gpu
 
 BOUND = {"bf16x6": 1e-6, "fp16x3": 1e-6, "bf16x3": 6e-5, "fp16": 4e-3}
+SPLIT_SEP = 2e-7  # fused vs separate calls, split forms (see above)
 
 
 def _prec(name):
@@ -92,7 +94,7 @@ def test_vgrad_split_forms(gpu, form, N, I, J, leg):
     for name, k, g, s in zip(("VAL", "GRADLON", "GRADLAT"), (dsht.VAL, dsht.GRADLON, dsht.GRADLAT), got, sep):
         r = o.inv(w, k)
         assert rel_l2(g, r) < BOUND[form], (form, name, rel_l2(g, r))
-        assert rel_l2(g, s) < 1e-12, (form, name, rel_l2(g, s))
+        assert rel_l2(g, s) < SPLIT_SEP, (form, name, rel_l2(g, s))
 
 
 @pytest.mark.parametrize("prec", ["fp64", "bf16x6"])
@@ -115,7 +117,7 @@ def test_vgrad_null_outputs(gpu, prec):
         for i in range(3):
             h = outs[i].cpu().numpy()
             if i in keep:
-                assert rel_l2(h, sep[i]) < 1e-12, (keep, i)
+                assert rel_l2(h, sep[i]) < (1e-13 if prec == "fp64" else SPLIT_SEP), (keep, i)
             else:
                 assert np.isnan(h).all(), (keep, i)  # untouched
     # every output NULL is an argument error of the C entry point
@@ -142,7 +144,7 @@ def test_vgrad_t170_chunked(gpu, prec, pipeline):
     torch.cuda.synchronize()
     got = [t.cpu().numpy() for t in outs]
     sep = _separate(p, wt, outs[0])
-    tol = 1e-13 if prec == "fp64" else 1e-12
+    tol = 1e-13 if prec == "fp64" else SPLIT_SEP
     for i in range(3):
         assert rel_l2(got[i], sep[i]) < tol, (prec, pipeline, i, rel_l2(got[i], sep[i]))
     if prec == "fp64":
