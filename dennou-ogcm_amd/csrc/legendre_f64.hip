@@ -568,7 +568,10 @@ int launch_leg_inv(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t 
   static const int nin[] = {1, 1, 1, 2, 2, 1}, nout[] = {1, 1, 1, 1, 2, 3}, nt[] = {1, 1, 1, 2, 4, 2},
                    ntab[] = {1, 1, 1, 2, 2, 2};
   if (op < 0 || op > I_VGRAD) return fail(DSHT_EARG, "launch_leg_inv: bad op %d", op);
-  leg_work(p, nin[op], nout[op], nt[op], ntab[op], false, a.ncols, &by, &fl);
+  // the fused value/gradient form counts the Fourier arrays it writes (GradLon may come
+  // from the derivative FFT instead)
+  const int nw = op == I_VGRAD ? (a.G0 ? 1 : 0) + (a.G1 ? 1 : 0) + (a.G2 ? 1 : 0) : nout[op];
+  leg_work(p, nin[op], nw, nt[op], ntab[op], false, a.ncols, &by, &fl);
   ProfScope ps("leg_inv", s, by, fl);
   if (p->leg_kind == 2) return launch_leg_inv_direct(p, op, a, s);
   switch (op) {

@@ -996,7 +996,7 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
     int rc;
     {
       double by, fl;
-      mixed_work(p, op, false, nc, &by, &fl);
+      mixed_work(p, op, false, nc, &by, &fl, (O0 ? 1 : 0) + (O1 ? 1 : 0) + (O2 ? 1 : 0));
       ProfScope ps("leg_inv", s, by, fl);
       if (p->leg_kind != 1) {
         Mx2InvJob j;

@@ -88,12 +88,15 @@ static inline int MxFormFamily(int form) { return form <= 1 ? 0 : 1; }
 
 // algorithmic work of one split-precision contraction launch (FP64-equivalent flops;
 // Fourier arrays in FP32, spectral in FP64, one read of each 16-bit table term used)
-static inline void mixed_work(const dsht_plan* p, int op, bool fwd, int ncols, double* bytes, double* flops) {
+// nwritten: Fourier arrays an inverse writes when it differs from the operator's outputs
+// (the fused value/gradient form with GradLon from the derivative FFT); -1 = all
+static inline void mixed_work(const dsht_plan* p, int op, bool fwd, int ncols, double* bytes, double* flops,
+                              int nwritten = -1) {
   // inverse I_VGRAD (index 5): one input, three outputs, two products
   static const int nin[] = {1, 1, 1, 2, 2, 1}, nout[] = {1, 1, 1, 1, 2, 3}, nt[] = {1, 1, 1, 2, 4, 2};
   double T = 0.5 * (p->N + 1.0) * (p->N + 2.0);
   double four = (double)(p->N + 1) * p->J * 8.0 * ncols, spec = (double)p->L * 8.0 * ncols;
-  *bytes = fwd ? nin[op] * four + nout[op] * spec : nin[op] * spec + nout[op] * four;
+  *bytes = fwd ? nin[op] * four + nout[op] * spec : nin[op] * spec + (nwritten >= 0 ? nwritten : nout[op]) * four;
   *flops = 2.0 * p->J * T * ncols * nt[op];
 }
 
