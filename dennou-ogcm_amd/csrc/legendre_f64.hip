@@ -38,6 +38,21 @@ __device__ __forceinline__ double4_t mfma(double a, double b, double4_t c) {
 // result element i of lane l holds D[l/16 + 4i][l%16].
 __device__ __forceinline__ int acc_row(int lane, int i) { return (lane >> 4) + 4 * i; }
 
+// ------------------------------------------------------------------ block placement
+// 1-D grid of 8 * ceil((N+1)/8) * CB-column blocks.  Workgroups are dispatched to the 8 XCDs
+// round robin, so block b runs on XCD b mod 8: wavenumber m = b mod 8 + 8k takes the
+// consecutive blocks of XCD m mod 8, and every column block of one m shares that XCD's L2
+// copy of its table panel (the table is fetched from HBM/MALL once per m, not once per
+// column block).  Wavenumbers alternate over the XCDs, which spreads the uneven per-m work.
+__device__ __forceinline__ bool xcd_block(int N, int ncols, int CB, int& m, int& cb) {
+  const int nblk = (ncols + CB - 1) / CB;
+  const int b = blockIdx.x, idx = b >> 3;
+  m = (b & 7) + 8 * (idx / nblk);
+  cb = idx % nblk;
+  return m <= N;
+}
+static inline dim3 xcd_grid(int N, int ncols, int CB) { return dim3(8u * (unsigned)cdiv(N + 1, 8) * cdiv(ncols, CB)); }
+
 // ------------------------------------------------------------------ forward
 // WAVES waves per workgroup; wave w owns row tiles t = tbase + w + WAVES*tt (tt < MAXT).
 // The K loop (north-hemisphere latitudes) runs in chunks of KC and is software pipelined:
@@ -50,7 +65,6 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_fwd(LegFwdArgs a) {
   using Tr = FwdTraits<OP>;
   constexpr int NT = WAVES * 64;
   constexpr int NIN = Tr::NIN, NOUT = Tr::NOUT, NCT = CB / TILE;
-  constexpr int NACC = OP == I_VGRAD ? 2 : NOUT;  // I_VGRAD: P and H products of one input
   constexpr int PITCH = CB + 1;  // LDS row pitch (doubles)
   constexpr int EPT = (KC * CB + NT - 1) / NT;  // staged elements per thread
   constexpr int NKS = KC / 4;
@@ -58,8 +72,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_fwd(LegFwdArgs a) {
   // [input][S re, S im, A re, A im][KC][PITCH]
   __shared__ double sB[NIN][4][KC * PITCH];
 
-  const int m = blockIdx.y;
-  const int c0 = blockIdx.x * CB;
+  int m, cb;
+  if (!xcd_block(a.N, a.ncols, CB, m, cb)) return;  // uniform over the workgroup
+  const int c0 = cb * CB;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int N = a.N, J2 = a.J2;
   const int nE = n_par(N, m, 0), nO = n_par(N, m, 1);
@@ -269,8 +284,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_inv(LegInvArgs a) {
   constexpr bool UP = Tr::USE_P, UH = Tr::USE_H;
   __shared__ double sB[NIN][2][KC * PITCH];  // [input][re, im][KC][PITCH]
 
-  const int m = blockIdx.y;
-  const int c0 = blockIdx.x * CB;
+  int m, cb;
+  if (!xcd_block(a.N, a.ncols, CB, m, cb)) return;  // uniform over the workgroup
+  const int c0 = cb * CB;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int N = a.N, J2 = a.J2;
   const int nE = n_par(N, m, 0), nO = n_par(N, m, 1);
@@ -493,7 +509,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_inv(LegInvArgs a) {
 // ------------------------------------------------------------------ launchers
 template <int OP, int CB, int WAVES, int MAXT, int KC>
 static int fwd_launch(const dsht_plan* p, const LegFwdArgs& a, hipStream_t s) {
-  dim3 grid(cdiv(a.ncols, CB), p->N + 1);
+  const dim3 grid = xcd_grid(p->N, a.ncols, CB);
   hipLaunchKernelGGL((k_leg_fwd<OP, CB, WAVES, MAXT, KC>), grid, dim3(WAVES * 64), 0, s, a);
   DSHT_HIP(hipGetLastError());
   return 0;
@@ -501,7 +517,7 @@ static int fwd_launch(const dsht_plan* p, const LegFwdArgs& a, hipStream_t s) {
 
 template <int OP, int CB, int WAVES, int MAXT, int KC>
 static int inv_launch(const dsht_plan* p, const LegInvArgs& a, hipStream_t s) {
-  dim3 grid(cdiv(a.ncols, CB), p->N + 1);
+  const dim3 grid = xcd_grid(p->N, a.ncols, CB);
   hipLaunchKernelGGL((k_leg_inv<OP, CB, WAVES, MAXT, KC>), grid, dim3(WAVES * 64), 0, s, a);
   DSHT_HIP(hipGetLastError());
   return 0;
