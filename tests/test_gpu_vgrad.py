@@ -4,9 +4,10 @@ One call returns xy_w(w), xy_GradLon_w(w) and xy_GradLat_w(w) (SpmlUtil_mod.f90:
 Checks:
   * against the FP64 oracle at <= 1e-12 (FP64 form) and at each split form's error bound;
   * against the three separate inverse calls of the same plan: <= 1e-13 relative L2 for
-    FP64, and <= 2e-7 for the split forms, whose fused GradLon multiplies the value's FP32
-    Fourier coefficients by i m inside the FP32 inverse FFT (one more FP32 rounding than the
-    separate call, which applies i m in the FP64 contraction epilogue);
+    FP64, and <= 5e-7 for the split forms (1.4e-7 measured at T170), whose fused GradLon
+    multiplies the value's FP32 Fourier coefficients by i m inside the FP32 inverse FFT
+    (one more FP32 rounding than the separate call, which applies i m in the FP64
+    contraction epilogue);
   * every kernel family (DSHT_TUNE_LEG), NULL outputs, both column blocks, the chunked
     T170 path and the two-stream chunk pipeline.
 """
@@ -18,7 +19,7 @@ from conftest import red_spectrum, rel_l2
 pytestmark = pytest.mark.gpu
 
 BOUND = {"bf16x6": 1e-6, "fp16x3": 1e-6, "bf16x3": 6e-5, "fp16": 4e-3}
-SPLIT_SEP = 2e-7  # fused vs separate calls, split forms (see above)
+SPLIT_SEP = 5e-7  # fused vs separate calls, split forms (see above)
 
 
 def _prec(name):
