@@ -61,6 +61,14 @@ module SpmlUtil_mod
       real(c_double), intent(in) :: w(*)
       real(c_double) :: xy(*)
     end function
+    integer(c_int) function c_xy_vgrad_w(plan, w, xy, xy_lon, xy_lat, ncols) bind(C, name="dsht_xy_vgrad_w")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      real(c_double), intent(in) :: w(*)
+      type(c_ptr), value :: xy
+      real(c_double) :: xy_lon(*), xy_lat(*)
+      integer(c_int), value :: ncols
+    end function
     integer(c_int) function c_w_alpha_xy(plan, a, b, w, ncols) bind(C, name="dsht_w_alpha_xy")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: plan
@@ -144,6 +152,8 @@ module SpmlUtil_mod
   public :: l_nm, nm_l
   public :: IntLonLat_xy, ya_IntLon_xya, AvrLonLat_xy, ya_AvrLon_xya
   public :: a_Interpolate_wa, Interpolate_w
+  ! extension without a counterpart in the reference module (see calc_GradLonLat_xya_wa)
+  public :: calc_GradLonLat_xya_wa
 
   interface calc_VorDiv2UV
     module procedure calc_VorDiv2UV_w_xy
@@ -313,6 +323,16 @@ contains
     real(DP) :: xy_GradLat_w(0:im-1, jm)
     call chk(c_xy_w(plan, DSHT_GRADLAT, w, xy_GradLat_w, 1_c_int), 'xy_GradLat_w')
   end function xy_GradLat_w
+
+  ! Extension: xya_GradLon_wa(wa) and xya_GradLat_wa(wa) of the same field in one fused pass
+  ! (one contraction for both gradients).  For callers that take both gradients of one field:
+  ! HBEBaroc_hspm_vfvm_mod.f90:403,416, LPhys_RediGM_hspm_vfvm_mod.f90:432-438.
+  subroutine calc_GradLonLat_xya_wa(wa, xya_lon, xya_lat)
+    real(DP), intent(in) :: wa(:,:)
+    real(DP), intent(out) :: xya_lon(0:im-1, jm, size(wa, 2)), xya_lat(0:im-1, jm, size(wa, 2))
+    call chk(c_xy_vgrad_w(plan, wa, c_null_ptr, xya_lon, xya_lat, int(size(wa, 2), c_int)), &
+             'calc_GradLonLat_xya_wa')
+  end subroutine calc_GradLonLat_xya_wa
 
   function xya_GradLambda_wa(wa)
     real(DP), intent(in) :: wa(:,:)

@@ -16,6 +16,7 @@ program spml_dropin_test
   real(DP) :: w(( NM + 1)**2), w2((NM + 1)**2), vor((NM + 1)**2), div((NM + 1)**2)
   real(DP) :: xy(0:IM-1, JM), u(0:IM-1, JM), v(0:IM-1, JM)
   real(DP) :: wa((NM + 1)**2, KM), wa2((NM + 1)**2, KM), xya(0:IM-1, JM, KM)
+  real(DP) :: xya_lon(0:IM-1, JM, KM), xya_lat(0:IM-1, JM, KM)
   real(DP) :: err, ref
   integer :: l, n, m, k, nfail
   integer :: nmv(2)
@@ -83,6 +84,12 @@ program spml_dropin_test
     err = max(err, maxval(abs(xya(:, :, k) - xy_GradLat_w(wa(:, k)))))
   end do
   call check('xya_GradLat_wa per layer', err, 1d-12)
+  ! fused gradients against the separate calls
+  xya = xya_GradLat_wa(wa)
+  call calc_GradLonLat_xya_wa(wa, xya_lon, xya_lat)
+  call check('calc_GradLonLat_xya_wa lat', maxval(abs(xya_lat - xya)), 1d-12)
+  xya = xya_GradLon_wa(wa)
+  call check('calc_GradLonLat_xya_wa lon', maxval(abs(xya_lon - xya)), 1d-12)
 
   ! per-layer calls from concurrent threads (each OpenMP thread gets its own stream)
   !$omp parallel do private(k)
