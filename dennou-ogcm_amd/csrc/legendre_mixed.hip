@@ -901,6 +901,7 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
         j.diag0 = diag0;
         j.diag1 = diag1;
         j.scale = scale;
+        j.shared = p->leg_kind != 2;  // 2: the per-wave fragment-stream forward
         rc = mx2_fwd(p, t, form, op, j, s);
       } else {
         switch (form) {

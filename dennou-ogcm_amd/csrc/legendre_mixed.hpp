@@ -134,6 +134,7 @@ struct Mx2FwdJob {
   const double* diag0;
   const double* diag1;
   double scale;
+  bool shared = true;       // split the data once per block through LDS (k_mx2_fwd_sh)
 };
 
 size_t mx2_inv_pack_bytes(const MixedTables& t, int form, int op, int ncols);
