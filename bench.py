@@ -221,7 +221,7 @@ def energy_drift(torch, plan, dsht, form, spec, trips=100):
 
 
 def traffic_table(path=os.path.join(ROOT, "profiles", "hbm_traffic_T170L60.md")):
-    """Measured HBM bytes per full-chunk launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-
+    """Measured HBM bytes per launch of the step's calls (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-
     calibrated) from the committed profile summary: {(form, scope): bytes}."""
     out = {}
     try:
@@ -514,7 +514,7 @@ def main():
     best = max(ok, key=lambda f: results[f]["pairs_per_s"])
     r = results[best]
     drift = {f: energy_drift(torch, plan, dsht, f, spec) for f in results}
-    # roofline of the dominant kernel of the reported form, probed on full-chunk launches
+    # roofline of the dominant kernel of the reported form, probed on the step's own calls
     kname = max(r["kernels"].items(), key=lambda kv: kv[1]["ms_total"])[0]
     roof = roofline_probe(torch, plan, dsht, best, kname, bufs)
     batched = serving = None

@@ -261,6 +261,9 @@ __host__ __device__ inline long row_groups(const FftArgs& a, int rows) {
 
 }  // namespace
 
+// grid rows are streamed once per transform: non-temporal loads (forward) and stores (inverse)
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
 // waves per SIMD the register kernels are compiled for: FP32 arithmetic fits three
 // (<= 168 VGPRs, 4 workgroups of LDS per CU), FP64 two (its LDS rows allow two workgroups)
 template <typename V>
@@ -314,7 +317,10 @@ __global__ __launch_bounds__(NT, WPE<V>) void k_fft_fwd_r(const double* __restri
       if constexpr (REF) {
         x0n = xr[0];
 #pragma unroll
-        for (int n2 = 0; n2 < R2; n2++) raw[n2] = x[n1 + R1 * n2];
+        for (int n2 = 0; n2 < R2; n2++) {
+          const dvec2 t = __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(x + n1 + R1 * n2));
+          raw[n2] = make_double2(t.x, t.y);
+        }
       } else {
 #pragma unroll
         for (int n2 = 0; n2 < R2; n2++) v[n2] = cvt<V>(x[n1 + R1 * n2]);
@@ -505,7 +511,10 @@ __global__ __launch_bounds__(NT, WPE<V>) void k_fft_inv_r(const GT* __restrict__
         const double s = a.rowscale ? a.rowscale[j] : 1.0;
         double2* x = reinterpret_cast<double2*>(xy + c * a.ldg + (long)j * (2 * M));
 #pragma unroll
-        for (int k1 = 0; k1 < R1; k1++) x[k2 + R2 * k1] = make_double2((double)u[k1].x * s, (double)u[k1].y * s);
+        for (int k1 = 0; k1 < R1; k1++) {
+          const dvec2 t = {(double)u[k1].x * s, (double)u[k1].y * s};
+          __builtin_nontemporal_store(t, reinterpret_cast<dvec2*>(x + k2 + R2 * k1));
+        }
       }
     }
   }

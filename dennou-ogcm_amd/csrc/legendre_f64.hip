@@ -256,10 +256,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_fwd(LegFwdArgs a) {
             const double* dg = o == 0 ? a.diag0 : a.diag1;
             double* W = o == 0 ? a.W0 : a.W1;
             const double sre = a.scale * (dg ? dg[lre] : 1.0);
-            W[lre * a.sl + (long)c * a.sc] = re * sre;
+            __builtin_nontemporal_store(re * sre, W + lre * a.sl + (long)c * a.sc);  // written once
             if (m > 0) {
               const double sim = a.scale * (dg ? dg[lim] : 1.0);
-              W[lim * a.sl + (long)c * a.sc] = im * sim;
+              __builtin_nontemporal_store(im * sim, W + lim * a.sl + (long)c * a.sc);
             }
           }
         }
@@ -332,8 +332,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_inv(LegInvArgs a) {
             const int n = m + p + 2 * k;
             const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
             const double pr = a.pro ? a.pro[lre] : 1.0;
-            bre[i][q] = pr * Win[i][lre * a.sl + (long)c * a.sc];
-            if (m > 0) bim[i][q] = pr * Win[i][lim * a.sl + (long)c * a.sc];
+            // each block reads its spectra once: non-temporal
+            bre[i][q] = pr * __builtin_nontemporal_load(Win[i] + lre * a.sl + (long)c * a.sc);
+            if (m > 0) bim[i][q] = pr * __builtin_nontemporal_load(Win[i] + lim * a.sl + (long)c * a.sc);
           }
         }
     };

@@ -161,8 +161,9 @@ __global__ __launch_bounds__(256) void k_mx2_ipack(Mx2InvArgs a) {
       const int n = m + p + 2 * k;
       const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
       const double pr = a.pro ? a.pro[lre] : 1.0;
-      vre[e] = pr * W[lre * a.sl + (long)c * a.sc];
-      if (m > 0) vim[e] = pr * W[lim * a.sl + (long)c * a.sc];
+      // the spectra are read once here: non-temporal
+      vre[e] = pr * __builtin_nontemporal_load(W + lre * a.sl + (long)c * a.sc);
+      if (m > 0) vim[e] = pr * __builtin_nontemporal_load(W + lim * a.sl + (long)c * a.sc);
     }
   }
   T tre[NTB][8], tim[NTB][8];
@@ -936,8 +937,8 @@ __global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
           for (int o = 0; o < NOUT; o++) {
             const double* dg = o == 0 ? a.diag0 : a.diag1;
             double* W = o == 0 ? a.W0 : a.W1;
-            W[lre * a.sl + (long)c * a.sc] = orr[o] * u * (dg ? dg[lre] : 1.0);
-            if (m > 0) W[lim * a.sl + (long)c * a.sc] = oii[o] * u * (dg ? dg[lim] : 1.0);
+            __builtin_nontemporal_store(orr[o] * u * (dg ? dg[lre] : 1.0), W + lre * a.sl + (long)c * a.sc);
+            if (m > 0) __builtin_nontemporal_store(oii[o] * u * (dg ? dg[lim] : 1.0), W + lim * a.sl + (long)c * a.sc);
           }
         }
       }

@@ -3,7 +3,8 @@
 For one plan and a representative column count it times the dynamics-step transform set
 (value and both gradients of 4 fields in one fused inverse call, 1 forward over 6
 tendencies) under each candidate and keeps the fastest.  The candidates are:
-  column chunk      TUNE_CHUNK_COLS: columns per Fourier scratch chunk (0 = ~48 MiB)
+  column chunk      TUNE_CHUNK_COLS: columns per Fourier scratch chunk (0 = whole call up to
+                    512 MiB of Fourier scratch per output)
   column block      TUNE_LEG_CB: column tiles per wave / workgroup of the contractions
   contraction       TUNE_LEG: kernel family (split forms: packed data panels or
                     LDS-shared split data; FP64: LDS-staged or per-wave fragment streams)
