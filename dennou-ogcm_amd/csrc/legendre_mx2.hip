@@ -756,7 +756,7 @@ __global__ __launch_bounds__(256) void k_mx2_fwd(Mx2FwdArgs a) {
 // the accumulators of up to NSW slots (slot s = round*4*NSW + w + 4*i) and reads, per slot,
 // the combination its parity needs (P with sym at parity 0, with anti at parity 1; H the
 // opposite).  Same tables, slots and epilogue as k_mx2_fwd.
-template <int OP, int FORM, int MAXT, int NSW>
+template <int OP, int FORM, int MAXT, int NSW, int NCT>
 __global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
   using F = MxForm<FORM>;
   using X = Mx2<FORM>;
@@ -767,9 +767,9 @@ __global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
   constexpr bool UP = FwdTraits<OP>::USE_P, UH = FwdTraits<OP>::USE_H;
   constexpr int FRAG = 64 * 8;                    // elements of one B fragment (64 lanes x 8)
   constexpr int PANEL = NIN * 2 * 2 * NTB * FRAG;  // [input][sym, anti][re, im][term] fragments
-  __shared__ __attribute__((aligned(16))) T sB[2][PANEL];
-  int m, ct;
-  if (!xcd_decode(a.N, a.CT, m, ct)) return;  // uniform over the block
+  __shared__ __attribute__((aligned(16))) T sB[2][NCT * PANEL];  // [buffer][column tile][panel]
+  int m, cg;
+  if (!xcd_decode(a.N, cdiv(a.CT, NCT), m, cg)) return;  // uniform over the block
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int nE = n_par(a.N, m, 0), nO = n_par(a.N, m, 1);
   const int tE = cdiv(nE, TILE), tO = cdiv(nO, TILE);
@@ -778,80 +778,85 @@ __global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
   const long toff = a.off[m];
   const float2* Gm[2] = {a.G0 + (long)m * a.gmst, NIN > 1 ? a.G1 + (long)m * a.gmst : nullptr};
   const double dm = (double)m;
-  // staging role: column sc of the tile, latitudes 2*jp and 2*jp + 1 of the K step; they are
-  // elements e0, e0 + 1 of fragment lane fl
+  // staging role: column scol of each of the block's NCT tiles, latitudes 2*jp and 2*jp + 1
+  // of the K step; they are elements e0, e0 + 1 of fragment lane fl
   const int scol = tid & 15, jp = tid >> 4;
-  const int cs = ct * TILE + scol;
-  const bool csok = cs < a.ncols;
   const int fl = scol + 16 * (jp >> 2), e0 = 2 * (jp & 3);
-  float sdsc = 1.f;
-  if constexpr (F::SCALED) {
-    if (csok) sdsc = colscale<NIN>(a.colmax, a.ncols, cs, a.bound);
-  }
-  // epilogue role: column c of the tile
-  const int c = ct * TILE + (lane & 15);
-  const bool cok = c < a.ncols;
-  double u = a.minv[m] * X::DSCALE * a.scale;
-  if constexpr (F::SCALED) {
-    if (cok) u /= (double)colscale<NIN>(a.colmax, a.ncols, c, a.bound);
+  int cs[NCT];
+  bool csok[NCT];
+  float sdsc[NCT];
+#pragma unroll
+  for (int t = 0; t < NCT; t++) {
+    cs[t] = (cg * NCT + t) * TILE + scol;
+    csok[t] = cs[t] < a.ncols;
+    sdsc[t] = 1.f;
+    if constexpr (F::SCALED) {
+      if (csok[t]) sdsc[t] = colscale<NIN>(a.colmax, a.ncols, cs[t], a.bound);
+    }
   }
 
   // staging is split in two so that the next K step's loads are in flight during this
   // step's MFMAs: fetch() loads the north/south values, stage() combines, splits and stores
-  float2 gN[NIN][2], gS[NIN][2];
+  float2 gN[NCT][NIN][2], gS[NCT][NIN][2];
   float gw[2];
   auto fetch = [&](int ks) {
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int jn = ks * KM + 2 * jp + h;
-      gw[h] = a.cw[jn] * sdsc;
+      gw[h] = a.cw[jn];
 #pragma unroll
-      for (int i = 0; i < NIN; i++) {
-        gN[i][h] = gS[i][h] = make_float2(0.f, 0.f);
-        if (jn < a.J2 && csok) {
-          gN[i][h] = Gm[i][(long)(a.J2 + jn) * a.gld + cs];
-          gS[i][h] = Gm[i][(long)(a.J2 - 1 - jn) * a.gld + cs];
+      for (int t = 0; t < NCT; t++)
+#pragma unroll
+        for (int i = 0; i < NIN; i++) {
+          gN[t][i][h] = gS[t][i][h] = make_float2(0.f, 0.f);
+          if (jn < a.J2 && csok[t]) {
+            gN[t][i][h] = Gm[i][(long)(a.J2 + jn) * a.gld + cs[t]];
+            gS[t][i][h] = Gm[i][(long)(a.J2 - 1 - jn) * a.gld + cs[t]];
+          }
         }
-      }
     }
   };
   auto stage = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < NIN; i++) {
-      float v[2][4];  // [latitude][sym re, sym im, anti re, anti im]
+    for (int t = 0; t < NCT; t++)
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const float2 gn = gN[i][h], gs = gS[i][h];
-        const float cw = gw[h];
-        v[h][0] = cw * (gn.x + gs.x);
-        v[h][1] = cw * (gn.y + gs.y);
-        v[h][2] = cw * (gn.x - gs.x);
-        v[h][3] = cw * (gn.y - gs.y);
-      }
+      for (int i = 0; i < NIN; i++) {
+        float v[2][4];  // [latitude][sym re, sym im, anti re, anti im]
 #pragma unroll
-      for (int qr = 0; qr < 4; qr++) {  // qr = 2*q + r
-        T t0[NTB], t1[NTB];
-        bsplit<FORM>(v[0][qr], t0);
-        bsplit<FORM>(v[1][qr], t1);
+        for (int h = 0; h < 2; h++) {
+          const float2 gn = gN[t][i][h], gs = gS[t][i][h];
+          const float cw = gw[h] * sdsc[t];
+          v[h][0] = cw * (gn.x + gs.x);
+          v[h][1] = cw * (gn.y + gs.y);
+          v[h][2] = cw * (gn.x - gs.x);
+          v[h][3] = cw * (gn.y - gs.y);
+        }
 #pragma unroll
-        for (int t = 0; t < NTB; t++) {
-          T* dst = sB[buf] + ((i * 4 + qr) * NTB + t) * FRAG + fl * 8 + e0;
-          dst[0] = t0[t];
-          dst[1] = t1[t];
+        for (int qr = 0; qr < 4; qr++) {  // qr = 2*q + r
+          T t0[NTB], t1[NTB];
+          bsplit<FORM>(v[0][qr], t0);
+          bsplit<FORM>(v[1][qr], t1);
+#pragma unroll
+          for (int k = 0; k < NTB; k++) {
+            T* dst = sB[buf] + t * PANEL + ((i * 4 + qr) * NTB + k) * FRAG + fl * 8 + e0;
+            dst[0] = t0[k];
+            dst[1] = t1[k];
+          }
         }
       }
-    }
   };
 
   const int nslot = sE + sO;
   for (int rbase = 0; rbase < nslot; rbase += 4 * NSW) {
-    float4_t acc[NSW][NG][MAXT][2];
+    float4_t acc[NSW][NG][MAXT][NCT][2];
 #pragma unroll
     for (int q = 0; q < NSW; q++)
 #pragma unroll
       for (int g = 0; g < NG; g++)
 #pragma unroll
-        for (int tt = 0; tt < MAXT; tt++) acc[q][g][tt][0] = acc[q][g][tt][1] = float4_t{0, 0, 0, 0};
+        for (int tt = 0; tt < MAXT; tt++)
+#pragma unroll
+          for (int t = 0; t < NCT; t++) acc[q][g][tt][t][0] = acc[q][g][tt][t][1] = float4_t{0, 0, 0, 0};
     fetch(0);
     stage(0);
     __syncthreads();
@@ -881,64 +886,74 @@ __global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
           const int tb = fwd_tab(OP, gi), in = fwd_in(OP, gi);
           const int cq = tb == 0 ? PP : 1 - PP;  // 0 = sym, 1 = anti
 #pragma unroll
-          for (int r = 0; r < 2; r++) {
-            V8 B[NTB];
+          for (int t = 0; t < NCT; t++)
 #pragma unroll
-            for (int t = 0; t < NTB; t++)
-              B[t] = *reinterpret_cast<const V8*>(sB[buf] + ((in * 4 + 2 * cq + r) * NTB + t) * FRAG + lane * 8);
+            for (int r = 0; r < 2; r++) {
+              V8 B[NTB];
 #pragma unroll
-            for (int tt = 0; tt < MAXT; tt++)
+              for (int k = 0; k < NTB; k++)
+                B[k] = *reinterpret_cast<const V8*>(sB[buf] + t * PANEL + ((in * 4 + 2 * cq + r) * NTB + k) * FRAG +
+                                                    lane * 8);
 #pragma unroll
-              for (int pr = 0; pr < X::NP; pr++)
-                acc[q][gi][tt][r] = mfma16(A[tb][tt][X::pa(pr)], B[X::pb(pr)], acc[q][gi][tt][r]);
-          }
+              for (int tt = 0; tt < MAXT; tt++)
+#pragma unroll
+                for (int pr = 0; pr < X::NP; pr++)
+                  acc[q][gi][tt][t][r] = mfma16(A[tb][tt][X::pa(pr)], B[X::pb(pr)], acc[q][gi][tt][t][r]);
+            }
         }
       }
       if (ks + 1 < KS) stage(buf ^ 1);
       __syncthreads();  // buf ^ 1 is complete; buf is restaged only after this barrier
     }
-    // ---- epilogue (FP64), per slot: combine, unscale, diagonal, scatter to SPMODEL order
+    // ---- epilogue (FP64), per slot and column tile: combine, unscale, diagonal, scatter
 #pragma unroll
-    for (int q = 0; q < NSW; q++) {
-      const int sidx = rbase + w + 4 * q;
-      if (sidx >= nslot || !cok) continue;
-      const int PP = sidx >= sE ? 1 : 0;
-      const int tp0 = (PP ? sidx - sE : sidx) * MAXT, tpar = PP ? tO : tE;
+    for (int t = 0; t < NCT; t++) {
+      const int c = (cg * NCT + t) * TILE + (lane & 15);
+      if (c >= a.ncols) continue;
+      double u = a.minv[m] * X::DSCALE * a.scale;
+      if constexpr (F::SCALED) u /= (double)colscale<NIN>(a.colmax, a.ncols, c, a.bound);
 #pragma unroll
-      for (int tt = 0; tt < MAXT; tt++) {
-        const int tp = tp0 + tt;
-        if (tp >= tpar) continue;
+      for (int q = 0; q < NSW; q++) {
+        const int sidx = rbase + w + 4 * q;
+        if (sidx >= nslot) continue;
+        const int PP = sidx >= sE ? 1 : 0;
+        const int tp0 = (PP ? sidx - sE : sidx) * MAXT, tpar = PP ? tO : tE;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int k = tp * TILE + acc_row16(lane, i);
-          const int n = m + PP + 2 * k;
-          if (n > a.N) continue;
-          double xr[NG], xi[NG];
+        for (int tt = 0; tt < MAXT; tt++) {
+          const int tp = tp0 + tt;
+          if (tp >= tpar) continue;
 #pragma unroll
-          for (int gi = 0; gi < NG; gi++) {
-            xr[gi] = acc[q][gi][tt][0][i];
-            xi[gi] = acc[q][gi][tt][1][i];
-          }
-          double orr[NOUT], oii[NOUT];
-          if constexpr (OP == F_VAL) {
-            orr[0] = xr[0], oii[0] = xi[0];
-          } else if constexpr (OP == F_DIVLAMBDA) {  // i m P.G
-            orr[0] = -dm * xi[0], oii[0] = dm * xr[0];
-          } else if constexpr (OP == F_DIVMU) {  // -H.G
-            orr[0] = -xr[0], oii[0] = -xi[0];
-          } else if constexpr (OP == F_ALPHA) {  // i m P.A - H.B
-            orr[0] = -dm * xi[0] - xr[1], oii[0] = dm * xr[0] - xi[1];
-          } else {  // vor = i m P.V + H.U ; div = i m P.U - H.V
-            orr[0] = -dm * xi[0] + xr[1], oii[0] = dm * xr[0] + xi[1];
-            orr[1] = -dm * xi[2] - xr[3], oii[1] = dm * xr[2] - xi[3];
-          }
-          const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
+          for (int i = 0; i < 4; i++) {
+            const int k = tp * TILE + acc_row16(lane, i);
+            const int n = m + PP + 2 * k;
+            if (n > a.N) continue;
+            double xr[NG], xi[NG];
 #pragma unroll
-          for (int o = 0; o < NOUT; o++) {
-            const double* dg = o == 0 ? a.diag0 : a.diag1;
-            double* W = o == 0 ? a.W0 : a.W1;
-            __builtin_nontemporal_store(orr[o] * u * (dg ? dg[lre] : 1.0), W + lre * a.sl + (long)c * a.sc);
-            if (m > 0) __builtin_nontemporal_store(oii[o] * u * (dg ? dg[lim] : 1.0), W + lim * a.sl + (long)c * a.sc);
+            for (int gi = 0; gi < NG; gi++) {
+              xr[gi] = acc[q][gi][tt][t][0][i];
+              xi[gi] = acc[q][gi][tt][t][1][i];
+            }
+            double orr[NOUT], oii[NOUT];
+            if constexpr (OP == F_VAL) {
+              orr[0] = xr[0], oii[0] = xi[0];
+            } else if constexpr (OP == F_DIVLAMBDA) {  // i m P.G
+              orr[0] = -dm * xi[0], oii[0] = dm * xr[0];
+            } else if constexpr (OP == F_DIVMU) {  // -H.G
+              orr[0] = -xr[0], oii[0] = -xi[0];
+            } else if constexpr (OP == F_ALPHA) {  // i m P.A - H.B
+              orr[0] = -dm * xi[0] - xr[1], oii[0] = dm * xr[0] - xi[1];
+            } else {  // vor = i m P.V + H.U ; div = i m P.U - H.V
+              orr[0] = -dm * xi[0] + xr[1], oii[0] = dm * xr[0] + xi[1];
+              orr[1] = -dm * xi[2] - xr[3], oii[1] = dm * xr[2] - xi[3];
+            }
+            const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
+#pragma unroll
+            for (int o = 0; o < NOUT; o++) {
+              const double* dg = o == 0 ? a.diag0 : a.diag1;
+              double* W = o == 0 ? a.W0 : a.W1;
+              __builtin_nontemporal_store(orr[o] * u * (dg ? dg[lre] : 1.0), W + lre * a.sl + (long)c * a.sc);
+              if (m > 0) __builtin_nontemporal_store(oii[o] * u * (dg ? dg[lim] : 1.0), W + lim * a.sl + (long)c * a.sc);
+            }
           }
         }
       }
@@ -1075,12 +1090,23 @@ int fwd_launch(const dsht_plan* p, const MixedTables& t, const Mx2FwdJob& j, hip
   a.N = p->N;
   a.J2 = p->J2;
   a.J2p32 = t.J2p32;
-  const long nb = 8l * cdiv(p->N + 1, 8) * a.CT;
-  if (j.shared)
-    hipLaunchKernelGGL((k_mx2_fwd_sh<OP, FORM, FwdShape<OP>::MAXT, FwdShape<OP>::NSW>), dim3((unsigned)nb), dim3(256), 0,
-                       s, a);
-  else
+  // LDS-shared form: NCT column tiles per block.  Two tiles halve the table fetches per column
+  // but double the accumulators; measured at T170 they pay only for the single-group bf16x6
+  // form (6 products per data fragment: forward 101 -> 84 us).  The column-block knob forces
+  // 1 (16) or 2 (32) tiles.
+  const int fnct = !j.shared ? 1
+                             : (p->leg_cb == 16 ? 1 : (p->leg_cb == 32 ? 2 : (fwd_ng(OP) == 1 && Mx2<FORM>::NP == 6 ? 2 : 1)));
+  const long nb = 8l * cdiv(p->N + 1, 8) * cdiv(a.CT, fnct);
+  if (j.shared) {
+    if (fnct == 2)
+      hipLaunchKernelGGL((k_mx2_fwd_sh<OP, FORM, FwdShape<OP>::MAXT, FwdShape<OP>::NSW, 2>), dim3((unsigned)nb), dim3(256),
+                         0, s, a);
+    else
+      hipLaunchKernelGGL((k_mx2_fwd_sh<OP, FORM, FwdShape<OP>::MAXT, FwdShape<OP>::NSW, 1>), dim3((unsigned)nb), dim3(256),
+                         0, s, a);
+  } else {
     hipLaunchKernelGGL((k_mx2_fwd<OP, FORM, FwdShape<OP>::MAXT>), dim3((unsigned)nb), dim3(256), 0, s, a);
+  }
   DSHT_HIP(hipGetLastError());
   return 0;
 }

@@ -117,3 +117,28 @@ def test_mixed_forward_large_zonal_mean(gpu, form):
     assert rel_l2(q[zonal], r[zonal]) < 1e-6
     bound = 1e-6 if form == "bf16x6" else 3e-6
     assert rel_l2(q[~zonal], r[~zonal]) < bound, rel_l2(q[~zonal], r[~zonal])
+
+
+@pytest.mark.parametrize("cb", [16, 32])
+@pytest.mark.parametrize("form", list(BOUND))
+def test_mixed_forward_column_tiles(gpu, form, cb):
+    """The LDS-shared forward with one (16) and two (32) column tiles per block, over a
+    column count that leaves the last block half empty."""
+    import dsht
+    import oracle
+    N, I, J, R = 42, 128, 64, 6.371e6
+    p = dsht.Plan(N, I, J, radius=R)
+    o = oracle.Oracle(N, I, J, R)
+    p.set_precision(_prec(form))
+    p.set_tuning(dsht.TUNE_LEG_CB, cb)
+    tol = BOUND[form]
+    rng = np.random.default_rng(9)
+    xy = (rng.standard_normal((21, J, I)) + 20.0) * o.coslat[None, :, None]
+    for kind in ("VAL", "DIVLAMBDA", "DIVMU"):
+        k = getattr(dsht, kind)
+        assert rel_l2(p._fwd(xy, k), o.fwd(xy, k)) < tol, (form, cb, kind)
+    A, B = xy[:11], xy[10:] * 1e-3
+    assert rel_l2(p.w_AlphaOptr_xy(A, B), o.alpha_fwd(A, B)) < tol
+    vor, div = p.calc_UVCosLat2VorDiv(A, B)
+    rv, rd = o.uv2vordiv(A, B)
+    assert rel_l2(vor, rv) < tol and rel_l2(div, rd) < tol
