@@ -621,9 +621,9 @@ int dsht_inv_vgrad_d(dsht_plan* p, const double* w, long sl, long sc, double* xy
       if (int rc = check_grid_ptr(o, ldg, p)) return rc;
   if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
   hipStream_t s = (hipStream_t)stream;
-  // kernel families without the fused contraction (FP64 fragment streams, first split design)
-  // run the three transforms one after another
-  const bool fused = p->prec == DSHT_PREC_FP64 ? p->leg_kind != 2 : p->leg_kind != 1;
+  // the first split design has no fused contraction: it runs the three transforms one after
+  // another
+  const bool fused = p->prec == DSHT_PREC_FP64 || p->leg_kind != 1;
   if (!fused) {
     if (xy)
       if (int rc = dispatch_inv(p, I_VAL, w, nullptr, sl, sc, xy, nullptr, ldg, ncols, diag, nullptr, s)) return rc;

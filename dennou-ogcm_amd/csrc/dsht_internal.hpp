@@ -197,7 +197,8 @@ struct dsht_plan {
                        // split forms' inverse FFT), 1 = LDS Stockham, 2 = register FFT in
                        // 128-thread blocks, 3 = register FFT with FP64 arithmetic in every form
   int fft_order = 1;   // register FFT row-group order: 0 packed, 1 latitude-major (default)
-  int leg_kind = 0;    // contractions.  FP64: 0/1 = LDS-staged, 2 = per-wave fragment streams.
+  int leg_kind = 0;    // contractions.  FP64: 0 = LDS-staged inverse, fragment-stream forward;
+                       // 1/3 = LDS-staged; 2 = per-wave fragment streams.
                        // Split forms: 0 = fragment streams over packed data panels, 1 = LDS-
                        // staged, 2 = fragment streams splitting data in place, 3 = split data
                        // shared through LDS

@@ -568,7 +568,9 @@ int launch_leg_fwd(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t 
   static const int nin[] = {1, 1, 1, 2, 2}, nout[] = {1, 1, 1, 1, 2}, nt[] = {1, 1, 1, 2, 4}, ntab[] = {1, 1, 1, 2, 2};
   leg_work(p, nin[op], nout[op], nt[op], ntab[op], true, a.ncols, &by, &fl);
   ProfScope ps("leg_fwd", s, by, fl);
-  if (p->leg_kind == 2) return launch_leg_fwd_direct(p, op, a, s);
+  // forward: the per-wave fragment-stream kernel by default (measured 165 against 174 us at
+  // T170 x 360); DSHT_TUNE_LEG = 1 or 3 selects the LDS-staged kernel
+  if (p->leg_kind == 0 || p->leg_kind == 2) return launch_leg_fwd_direct(p, op, a, s);
   switch (op) {
     case F_VAL: return wide ? fwd_pick<F_VAL, 32>(p, a, s) : fwd_pick<F_VAL, 16>(p, a, s);
     case F_DIVLAMBDA: return wide ? fwd_pick<F_DIVLAMBDA, 32>(p, a, s) : fwd_pick<F_DIVLAMBDA, 16>(p, a, s);

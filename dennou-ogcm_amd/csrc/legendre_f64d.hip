@@ -44,6 +44,7 @@ template <int OP, int NCT, int MAXT, int U>
 __global__ __launch_bounds__(256) void k_f64_inv(LegInvArgs a, int CG, int RG) {
   using Tr = InvTraits<OP>;
   constexpr int NIN = Tr::NIN, NOUT = Tr::NOUT;
+  constexpr int NACC = OP == I_VGRAD ? 2 : NOUT;  // I_VGRAD: P and H products of one input
   constexpr bool UP = Tr::USE_P, UH = Tr::USE_H;
   int m, sub;
   if (!xcd_decode64(a.N, CG * RG, m, sub)) return;
@@ -60,9 +61,9 @@ __global__ __launch_bounds__(256) void k_f64_inv(LegInvArgs a, int CG, int RG) {
   const double dm = (double)m;
   const double* Win[2] = {a.W0, a.W1};
 
-  double4_t acc[NOUT][MAXT][NCT][2][2];
+  double4_t acc[NACC][MAXT][NCT][2][2];
 #pragma unroll
-  for (int o = 0; o < NOUT; o++)
+  for (int o = 0; o < NACC; o++)
 #pragma unroll
     for (int tt = 0; tt < MAXT; tt++)
 #pragma unroll
@@ -126,6 +127,12 @@ __global__ __launch_bounds__(256) void k_f64_inv(LegInvArgs a, int CG, int RG) {
             } else if constexpr (OP == I_GRADMU) {
 #pragma unroll
               for (int r = 0; r < 2; r++) acc[0][tt][ct][r][sH] = mfma64(xH, b0[r], acc[0][tt][ct][r][sH]);
+            } else if constexpr (OP == I_VGRAD) {
+#pragma unroll
+              for (int r = 0; r < 2; r++) {
+                acc[0][tt][ct][r][sP] = mfma64(xP, b0[r], acc[0][tt][ct][r][sP]);
+                acc[NACC - 1][tt][ct][r][sH] = mfma64(xH, b0[r], acc[NACC - 1][tt][ct][r][sH]);
+              }
             } else {
               const double(&b1)[2] = xb[slot][u][NIN - 1][ct];
               if constexpr (OP == I_ALPHA) {  // i m P.A + H.B
@@ -179,11 +186,17 @@ __global__ __launch_bounds__(256) void k_f64_inv(LegInvArgs a, int CG, int RG) {
         const double sN = a.rs ? a.rs[jN] : 1.0, sS = a.rs ? a.rs[jS] : 1.0;
 #pragma unroll
         for (int o = 0; o < NOUT; o++) {
-          double nre = acc[o][tt][ct][0][0][i] + acc[o][tt][ct][0][1][i];
-          double nim = acc[o][tt][ct][1][0][i] + acc[o][tt][ct][1][1][i];
-          double sre = acc[o][tt][ct][0][0][i] - acc[o][tt][ct][0][1][i];
-          double sim = acc[o][tt][ct][1][0][i] - acc[o][tt][ct][1][1][i];
-          if constexpr (OP == I_GRADLAMBDA) {
+          // I_VGRAD: output 0 = P result (no row scale), 1 = i m P result, 2 = H result
+          const int ai = OP == I_VGRAD ? (o == 2 ? 1 : 0) : o;
+          const bool deriv = OP == I_GRADLAMBDA || (OP == I_VGRAD && o == 1);
+          const bool rowsc = !(OP == I_VGRAD && o == 0);
+          double2* Gb = o == 0 ? a.G0 : (o == 1 ? a.G1 : a.G2);
+          if (OP == I_VGRAD && !Gb) continue;
+          double nre = acc[ai][tt][ct][0][0][i] + acc[ai][tt][ct][0][1][i];
+          double nim = acc[ai][tt][ct][1][0][i] + acc[ai][tt][ct][1][1][i];
+          double sre = acc[ai][tt][ct][0][0][i] - acc[ai][tt][ct][0][1][i];
+          double sim = acc[ai][tt][ct][1][0][i] - acc[ai][tt][ct][1][1][i];
+          if (deriv) {
             double t1 = -dm * nim;
             nim = dm * nre;
             nre = t1;
@@ -191,9 +204,10 @@ __global__ __launch_bounds__(256) void k_f64_inv(LegInvArgs a, int CG, int RG) {
             sim = dm * sre;
             sre = t1;
           }
-          double2* G = (o == 0 ? a.G0 : a.G1) + (long)m * a.gmst;
-          G[(long)jN * a.gld + c] = make_double2(nre * sN, m ? nim * sN : 0.0);
-          G[(long)jS * a.gld + c] = make_double2(sre * sS, m ? sim * sS : 0.0);
+          const double fN = rowsc ? sN : 1.0, fS = rowsc ? sS : 1.0;
+          double2* G = Gb + (long)m * a.gmst;
+          G[(long)jN * a.gld + c] = make_double2(nre * fN, m ? nim * fN : 0.0);
+          G[(long)jS * a.gld + c] = make_double2(sre * fS, m ? sim * fS : 0.0);
         }
       }
     }
@@ -372,6 +386,9 @@ int inv_go(const dsht_plan* p, const LegInvArgs& a, hipStream_t s, bool narrow) 
   };
   if constexpr (TWO) {
     return go(std::integral_constant<int, 1>(), std::integral_constant<int, 1>());
+  } else if constexpr (OP == I_VGRAD) {  // two accumulator groups: 1 row tile x 2 column tiles
+    if (narrow) return go(std::integral_constant<int, 1>(), std::integral_constant<int, 1>());
+    return go(std::integral_constant<int, 2>(), std::integral_constant<int, 1>());
   } else {
     if (narrow) return go(std::integral_constant<int, 1>(), std::integral_constant<int, 1>());
     return go(std::integral_constant<int, 2>(), std::integral_constant<int, 2>());
@@ -398,6 +415,7 @@ int launch_leg_inv_direct(const dsht_plan* p, int op, const LegInvArgs& a, hipSt
     case I_GRADMU: return inv_go<I_GRADMU>(p, a, s, narrow);
     case I_ALPHA: return inv_go<I_ALPHA>(p, a, s, narrow);
     case I_VORDIV2UV: return inv_go<I_VORDIV2UV>(p, a, s, narrow);
+    case I_VGRAD: return inv_go<I_VGRAD>(p, a, s, narrow);
   }
   return fail(DSHT_EARG, "launch_leg_inv_direct: bad op %d", op);
 }

@@ -35,7 +35,7 @@ def _key(plan, ncols):
 def candidates(plan, ncols):
     """Candidate knob settings as dicts {chunk, cb, leg}."""
     chunks = sorted({c for c in (64, 128, 256) if c < ncols} | {0})
-    legs = (0, 2) if plan.precision == PREC_FP64 else (0, 3)
+    legs = (0, 1, 2) if plan.precision == PREC_FP64 else (0, 3)
     return [{"chunk": c, "cb": cb, "leg": lg} for c in chunks for cb in (16, 32) for lg in legs]
 
 

@@ -175,7 +175,8 @@ int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, d
                                   blocks; FP32 arithmetic in the inverse FFT of the split forms,
                                   whose Fourier coefficients are FP32), 1 = LDS Stockham, 2 = register kernel, 128-thread blocks,
                                   3 = register kernel, FP64 arithmetic in every form */
-#define DSHT_TUNE_LEG 3        /* Legendre contractions.  FP64: 0 or 1 = LDS-staged, 2 = per-wave
+#define DSHT_TUNE_LEG 3        /* Legendre contractions.  FP64: 0 = LDS-staged inverse with the
+                                  fragment-stream forward (default), 1 or 3 = LDS-staged, 2 = per-wave
                                   fragment streams.  Split forms: 0 = per-wave fragment streams
                                   over pre-packed data panels, 1 = LDS-staged (BF16 forms
                                   only; FP16 forms return DSHT_EUNSUPPORTED), 2 = fragment

@@ -225,3 +225,29 @@ def test_fortran_dropin(gpu):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "PASS" in r.stdout
+
+
+@pytest.mark.parametrize("leg", [0, 1, 2])
+@pytest.mark.parametrize("N,I,J", SIZES[1:3])
+def test_fp64_families(gpu, N, I, J, leg):
+    """Every FP64 contraction family (DSHT_TUNE_LEG: 0 = LDS-staged inverse with the
+    fragment-stream forward, 1 = LDS-staged, 2 = fragment streams) against the oracle."""
+    import dsht
+    p, o = _pair(N, I, J)
+    p.set_tuning(dsht.TUNE_LEG, leg)
+    rng = np.random.default_rng(200 + N + leg)
+    xy = rng.standard_normal((19, J, I)) * o.coslat[None, :, None]
+    for kind in ("VAL", "DIVLAMBDA", "DIVMU"):
+        k = getattr(dsht, kind)
+        assert rel_l2(p._fwd(xy, k), o.fwd(xy, k)) < TOL, (leg, kind)
+    A, B = xy[:10], xy[9:]
+    assert rel_l2(p.w_AlphaOptr_xy(A, B), o.alpha_fwd(A, B)) < TOL
+    vor, div = p.calc_UVCosLat2VorDiv(A, B)
+    rv, rd = o.uv2vordiv(A, B)
+    assert rel_l2(vor, rv) < TOL and rel_l2(div, rd) < TOL
+    w = red_spectrum(N, 19, seed=N + leg)
+    for kind in ("VAL", "GRADLON", "GRADLAT"):
+        k = getattr(dsht, kind)
+        assert rel_l2(p._inv(w, k), o.inv(w, k)) < TOL, (leg, kind)
+    for k, got in zip((dsht.VAL, dsht.GRADLON, dsht.GRADLAT), p.xy_GradLonLat_w(w)):
+        assert rel_l2(got, o.inv(w, k)) < TOL, (leg, "fused", k)
