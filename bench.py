@@ -370,11 +370,16 @@ def batched_bench(torch, dsht, n_inst=64, steps=10, seq_steps=2, seed=0xB47C4, p
 
 
 # ---------------------------------------------------------------- serving form
-def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E, factor=2.0):
+def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E, factor=1.6, concurrent_factor=2.0):
     """Poisson arrivals of independent runs (truncation, KM, rotation rate drawn per run),
-    served with continuous batching (per-truncation pools stepping asynchronously on their
-    own streams; admission capped so a pool's step stays within `factor` x the step time
-    of one KM=60 run of that truncation), without the latency cap, and one run at a time."""
+    served four ways:
+      batched            continuous batching, one pool step on the GPU at a time (pools take
+                         turns), admission capped so a pool's step stays within `factor` x
+                         the step time of one KM=60 run of that truncation;
+      batched_concurrent every pool stepping on its own stream whenever its previous step has
+                         finished, capped at `concurrent_factor`;
+      batched_uncapped   concurrent, no latency cap;
+      one_at_a_time      the runs in arrival order, each alone."""
     from dsht import serve
 
     def stream():
@@ -382,18 +387,20 @@ def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E, fac
 
     out = {}
     caps = {}
-    for mode in ("batched", "batched_uncapped", "one_at_a_time"):
-        srv = serve.Server(latency_factor=factor if mode == "batched" else None)
-        if mode == "batched":  # calibrate every pool before the clock starts
+    for mode in ("batched", "batched_concurrent", "batched_uncapped", "one_at_a_time"):
+        capped = mode in ("batched", "batched_concurrent")
+        serial = mode == "batched"
+        srv = serve.Server(latency_factor=(factor if serial else concurrent_factor) if capped else None,
+                           serialize=serial)
+        if capped:  # calibrate every pool before the clock starts
             for n in serve.GRIDS:
                 p = srv.pool(n)
-                caps[f"T{n}"] = {"step_ms_km60": p.t_ref, "max_active_columns": p.max_cols}
+                caps.setdefault(mode, {})[f"T{n}"] = {"step_ms_km60": p.t_ref, "max_active_columns": p.max_cols}
         reqs = stream()
         out[mode] = srv.run(reqs, batched=(mode != "one_at_a_time"))
-        if mode == "batched":
-            out[mode]["p99_step_ms_by_truncation"] = {
-                f"T{n}": float(np.percentile(np.concatenate([r.step_ms for r in reqs if r.nmax == n]), 99))
-                for n in sorted({r.nmax for r in reqs})}
+        out[mode]["p99_step_ms_by_truncation"] = {
+            f"T{n}": float(np.percentile(np.concatenate([r.step_ms for r in reqs if r.nmax == n]), 99))
+            for n in sorted({r.nmax for r in reqs})}
         srv = None
         torch.cuda.synchronize()
     # each truncation's KM=60 run alone: the latency reference
@@ -403,19 +410,24 @@ def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E, fac
         serve.Server().run([r], batched=True)
         solo[f"T{n}"] = float(np.median(r.step_ms[5:]))
     b, u = out["batched"], out["one_at_a_time"]
+    bc = out["batched_concurrent"]
     big = max(solo, key=lambda k: int(k[1:]))
     return {
         "workload": f"{n_req} runs, Poisson arrivals at {rate:g}/s, {nsteps} steps each, truncation from "
                     f"{list(serve.GRIDS)}, KM from [20, 30, 60], rotation 0.25-4x Earth; FP64 transforms",
         "batched": b,
+        "batched_concurrent": bc,
         "batched_uncapped": out["batched_uncapped"],
         "one_at_a_time": u,
         "latency_caps": caps,
         "throughput_ratio": b["pairs_per_s"] / u["pairs_per_s"],
+        "throughput_ratio_concurrent": bc["pairs_per_s"] / u["pairs_per_s"],
         "throughput_ratio_uncapped": out["batched_uncapped"]["pairs_per_s"] / u["pairs_per_s"],
         "solo_step_ms_km60": solo,
         "p99_over_largest_alone": b["p99_step_ms"] / solo[big],
+        "p99_over_largest_alone_concurrent": bc["p99_step_ms"] / solo[big],
         "p99_over_alone_by_truncation": {k: v / solo[k] for k, v in b["p99_step_ms_by_truncation"].items()},
+        "p99_over_alone_by_truncation_concurrent": {k: v / solo[k] for k, v in bc["p99_step_ms_by_truncation"].items()},
     }
 
 

@@ -238,12 +238,19 @@ class Server:
     """Continuous-batching request loop over per-truncation slot pools, one stream each."""
 
     def __init__(self, capacity=None, budget_bytes=64 << 30, dt=600.0, precision=0, dev=None,
-                 latency_factor=None, ref_km=60):
+                 latency_factor=None, ref_km=60, serialize=True):
+        """serialize: at most one pool step on the GPU at a time (pools take turns round robin),
+        so a step never shares the GPU with another truncation's kernels; measured against the
+        concurrent form (serialize=False: every pool steps on its own stream whenever its
+        previous step has finished), it holds the p99 step latency near the latency cap at
+        the same throughput."""
         self.dev = dev
         self.capacity = capacity or {21: 240, 42: 240, 85: 240, 170: 240}
         self.budget = budget_bytes
         self.dt, self.precision = dt, precision
         self.latency_factor, self.ref_km = latency_factor, ref_km
+        self.serialize = serialize
+        self._turn = 0
         self.pools = {}
         self.used = 0
 
@@ -332,10 +339,19 @@ class Server:
                 active.append(r)
                 progressed = True
             max_active = max(max_active, len(active))
-            # launches: every pool with runs and no step in flight
-            for nmax, pool in self.pools.items():
+            # launches: every pool with runs and no step in flight (serialized: the next pool
+            # in turn, once nothing is in flight)
+            order = list(self.pools.items())
+            if self.serialize and order:
+                k = self._turn % len(order)
+                order = order[k:] + order[:k]
+            for nmax, pool in order:
+                if self.serialize and inflight:
+                    break
                 if nmax in inflight or not pool.runs:
                     continue
+                if self.serialize:
+                    self._turn = (list(self.pools).index(nmax) + 1) % len(self.pools)
                 runs = list(pool.runs)
                 hi = pool.active_hi()
                 tl = time.perf_counter()

@@ -16,7 +16,8 @@ def _reqs(nsteps=6):
     return [Request(i, n, km, om, nsteps, t, 1000 + i) for i, (n, km, om, t) in enumerate(spec)]
 
 
-def test_results_same_alone_and_in_mix(gpu):
+@pytest.mark.parametrize("serialize", [True, False])
+def test_results_same_alone_and_in_mix(gpu, serialize):
     from dsht.serve import Server
     final = {}
 
@@ -26,7 +27,7 @@ def test_results_same_alone_and_in_mix(gpu):
         return f
 
     cap = {21: 16, 42: 16, 85: 16, 170: 16}
-    srv = Server(capacity=cap, dt=600.0)
+    srv = Server(capacity=cap, dt=600.0, serialize=serialize)
     reqs = _reqs()
     stats = srv.run(reqs, batched=True, on_finish=keep("mix"))
     # the mix really overlapped: several runs were on the GPU in the same iteration
