@@ -370,6 +370,52 @@ def batched_bench(torch, dsht, n_inst=64, steps=10, seq_steps=2, seed=0xB47C4, p
 
 
 # ---------------------------------------------------------------- serving form
+def column_bench(torch, dsht, reps=21, nsets=3):
+    """Vertical column diagnostics (HBEDiagnose_OMG, _HydPres, _UVBarot) on T170L60 device
+    grids (60 layers of 512 x 256 points).  Calls rotate over `nsets` buffer sets (each
+    call touches about 190 MB, so a set is out of the 256 MB last-level cache again when it
+    comes back round).  Each kernel's average duration comes from events on the stream it
+    is launched on; its algorithmic bytes are every input layer read once and every output
+    level written once (8 bytes a value)."""
+    plan = dsht.Plan(NMAX, IM, JM, kmax=KM, radius=6.371e6)
+    g = torch.Generator(device="cuda").manual_seed(0xC01)
+    shape = (KM, JM, IM)
+    cdk = torch.full((KM,), 1.0 / KM, dtype=torch.float64, device="cuda")
+
+    def rnd(scale=1.0, shift=0.0):
+        return shift + scale * torch.randn(shape, dtype=torch.float64, device="cuda", generator=g)
+
+    sets = []
+    for _ in range(nsets):
+        sets.append({"div": rnd(1e-6), "h": rnd(0.05, 1.0), "dens": rnd(), "u": rnd(), "v": rnd(),
+                     "omg": torch.empty((KM + 1, JM, IM), dtype=torch.float64, device="cuda"),
+                     "hp": torch.empty(shape, dtype=torch.float64, device="cuda"),
+                     "ub": torch.empty((JM, IM), dtype=torch.float64, device="cuda"),
+                     "vb": torch.empty((JM, IM), dtype=torch.float64, device="cuda")})
+    npts = IM * JM
+    calls = {
+        "omega": (lambda b: plan.col_omega_d(b["div"], b["h"], cdk, out=b["omg"]), (2 * KM + KM + 1) * npts * 8),
+        "hydpres": (lambda b: plan.col_hydpres_d(b["dens"], b["h"], cdk, 9.8, out=b["hp"]), 3 * KM * npts * 8),
+        "uvbarot": (lambda b: plan.col_uvbarot_d(b["u"], b["v"], b["h"], cdk, b["ub"], b["vb"]),
+                    (3 * KM + 2) * npts * 8),
+    }
+    out = {"workload": f"T{NMAX}L{KM} grids ({IM} x {JM} points, {KM} layers), FP64; calls rotate over "
+                       f"{nsets} buffer sets"}
+    for name, (fn, nbytes) in calls.items():
+        for k in range(nsets):
+            fn(sets[k])
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for k in range(reps):
+            fn(sets[k % nsets])
+        e1.record()
+        e1.synchronize()
+        us = 1e3 * e0.elapsed_time(e1) / reps
+        gbps = nbytes / (us * 1e-6) / 1e9
+        out[name] = {"avg_us": us, "algorithmic_bytes": nbytes, "GBps": gbps, "frac_hbm_peak": gbps / 8000.0}
+    return out
+
+
 def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E, factor=1.6, concurrent_factor=2.0):
     """Poisson arrivals of independent runs (truncation, KM, rotation rate drawn per run),
     served four ways:
@@ -443,6 +489,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy 3 times (HBM counter calibration) and exit")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--column-only", action="store_true",
+                    help="only run the vertical column diagnostics section and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: the transform path has no "
@@ -460,6 +508,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8, "launches": 3}))
+        return
+    if args.column_only:
+        import dsht
+        print(json.dumps({"column_diagnostics": column_bench(torch, dsht)}))
         return
 
     import dsht
@@ -529,6 +581,10 @@ def main():
     # roofline of the dominant kernel of the reported form, probed on the step's own calls
     kname = max(r["kernels"].items(), key=lambda kv: kv[1]["ms_total"])[0]
     roof = roofline_probe(torch, plan, dsht, best, kname, bufs)
+    print("[bench] column diagnostics", file=sys.stderr, flush=True)
+    column = column_bench(torch, dsht)
+    print("[bench] column: " + ", ".join(f"{k} {column[k]['avg_us']:.1f} us {column[k]['GBps']:.0f} GB/s"
+                                         for k in ("omega", "hydpres", "uvbarot")), file=sys.stderr, flush=True)
     batched = serving = None
     if not args.no_batch:
         print("[bench] batched form (64 instances)", file=sys.stderr, flush=True)
@@ -579,6 +635,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,
+        "column_diagnostics": column,
     }
     print(json.dumps(line))
 

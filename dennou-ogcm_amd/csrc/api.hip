@@ -730,6 +730,39 @@ int dsht_interpolate_d(dsht_plan* p, const double* w, long sl, const double* lon
   return launch_interpolate(p, w, sl, lon, lat, out, npts, (hipStream_t)stream);
 }
 
+// ---------------------------------------------------------------- vertical column diagnostics
+static int check_col(const dsht_plan* p, int km, const double* cdk, long ldg) {
+  if (int rc = check_plan(p)) return rc;
+  if (km < 1) return fail(DSHT_EARG, "km must be >= 1");
+  if (!cdk) return fail(DSHT_EARG, "layer thickness pointer is NULL");
+  if (ldg < (long)p->I * p->J) return fail(DSHT_EARG, "grid column pitch %ld must be >= I*J", ldg);
+  return 0;
+}
+
+int dsht_col_omega_d(dsht_plan* p, const double* div, const double* h, long ldg, int km, const double* cdk,
+                     double* omg, long ldo, void* stream) {
+  if (int rc = check_col(p, km, cdk, ldg)) return rc;
+  if (!div || !h || !omg) return fail(DSHT_EARG, "grid pointer is NULL");
+  if (ldo < (long)p->I * p->J) return fail(DSHT_EARG, "output column pitch %ld must be >= I*J", ldo);
+  return launch_col_omega(div, h, ldg, km, cdk, omg, ldo, p->I * p->J, (hipStream_t)stream);
+}
+
+int dsht_col_hydpres_d(dsht_plan* p, const double* dens, const double* h, long ldg, int km, const double* cdk,
+                       double grav, double* out, long ldo, void* stream) {
+  if (int rc = check_col(p, km, cdk, ldg)) return rc;
+  if (!dens || !h || !out) return fail(DSHT_EARG, "grid pointer is NULL");
+  if (ldo < (long)p->I * p->J) return fail(DSHT_EARG, "output column pitch %ld must be >= I*J", ldo);
+  return launch_col_hydpres(dens, h, ldg, km, cdk, grav, out, ldo, p->I * p->J, (hipStream_t)stream);
+}
+
+int dsht_col_uvbarot_d(dsht_plan* p, const double* u, const double* v, const double* h, long ldg, int km,
+                       const double* cdk, double* ub, double* vb, void* stream) {
+  if (int rc = check_col(p, km, cdk, ldg)) return rc;
+  if (!u || !v || !h) return fail(DSHT_EARG, "grid pointer is NULL");
+  if (!ub && !vb) return fail(DSHT_EARG, "both outputs are NULL");
+  return launch_col_uvbarot(u, v, h, ldg, km, cdk, ub, vb, p->I * p->J, (hipStream_t)stream);
+}
+
 int dsht_coef_d(dsht_plan* p, int which, const double* params, int nparams, double* out, void* stream) {
   if (int rc = check_plan(p)) return rc;
   if (which < DSHT_COEF_LAPLA || which > DSHT_COEF_BEDIFF) return fail(DSHT_EARG, "unknown coefficient table %d", which);

@@ -273,6 +273,13 @@ int launch_leg_inv(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t 
 // ops.hip
 int launch_diag(const double* d, const double* w, double* out, long sl, long sc, int L, int ncols,
                 hipStream_t s);
+// vertical column diagnostics (column.hip)
+int launch_col_omega(const double* div, const double* h, long ldg, int km, const double* cdk, double* omg, long ldo,
+                     int npts, hipStream_t s);
+int launch_col_hydpres(const double* dens, const double* h, long ldg, int km, const double* cdk, double grav,
+                       double* p, long ldo, int npts, hipStream_t s);
+int launch_col_uvbarot(const double* u, const double* v, const double* h, long ldg, int km, const double* cdk,
+                       double* ub, double* vb, int npts, hipStream_t s);
 int launch_lonlat_sum(const dsht_plan* p, const double* xy, long ldg, int ncols, double* out,
                       double factor, hipStream_t s);
 int launch_lon_sum(const dsht_plan* p, const double* xy, long ldg, int ncols, double* out,

@@ -39,7 +39,8 @@ EXPORTS = [
     "dsht_ya_intlon", "dsht_ya_avrlon", "dsht_interpolate", "dsht_fwd_d", "dsht_inv_d", "dsht_inv_vgrad_d",
     "dsht_alpha_fwd_d", "dsht_alpha_inv_d", "dsht_uv2vordiv_d", "dsht_vordiv2uv_d",
     "dsht_diag_d", "dsht_intlonlat_d", "dsht_avrlonlat_d", "dsht_ya_intlon_d",
-    "dsht_interpolate_d", "dsht_coef_d", "dsht_prof_enable", "dsht_prof_reset", "dsht_prof_query",
+    "dsht_interpolate_d", "dsht_coef_d", "dsht_col_omega_d", "dsht_col_hydpres_d", "dsht_col_uvbarot_d",
+    "dsht_prof_enable", "dsht_prof_reset", "dsht_prof_query",
     "dsht_set_tuning", "dsht_get_tuning", "dsht_batch_create", "dsht_batch_add", "dsht_batch_finalize",
     "dsht_batch_info", "dsht_batch_ngroups", "dsht_batch_buffers", "dsht_batch_step",
     "dsht_batch_step_sequential", "dsht_batch_capture", "dsht_batch_launch", "dsht_batch_destroy",
@@ -106,6 +107,9 @@ def load():
         "dsht_ya_intlon_d": ([vp, vp, lg, vp, i, vp], i),
         "dsht_interpolate_d": ([vp, vp, lg, vp, vp, vp, i, vp], i),
         "dsht_coef_d": ([vp, i, vp, i, vp, vp], i),
+        "dsht_col_omega_d": ([vp, vp, vp, lg, i, vp, vp, lg, vp], i),
+        "dsht_col_hydpres_d": ([vp, vp, vp, lg, i, vp, ctypes.c_double, vp, lg, vp], i),
+        "dsht_col_uvbarot_d": ([vp, vp, vp, vp, lg, i, vp, vp, vp, vp], i),
         "dsht_prof_enable": ([i], i),
         "dsht_prof_reset": ([], i),
         "dsht_prof_query": ([ctypes.c_char_p, vp, vp, vp, vp], i),
@@ -469,6 +473,67 @@ class Plan:
         _check(load().dsht_intlonlat_d(self.h, _tptr(xy), xy.stride(0), _tptr(out), xy.shape[0],
                                        _stream(stream)))
         return out
+
+    # ------------------------------------------------ vertical column diagnostics (device)
+    def _col_in(self, cdk, *ts):
+        """(km, ldg) of same-shape (km, J, I) float64 device grids (layer 0 on top) and a
+        float64 device vector cdk of km layer thicknesses."""
+        import torch
+        km = ts[0].shape[0]
+        if cdk.dtype != torch.float64 or not cdk.is_cuda or cdk.dim() != 1 or cdk.numel() != km \
+                or cdk.stride(0) != 1:
+            raise ValueError(f"cdk must be a contiguous float64 device vector of {km} layer thicknesses")
+        for t in ts:
+            if t.dtype != torch.float64 or not t.is_cuda:
+                raise TypeError("column diagnostics take float64 device tensors")
+            if tuple(t.shape) != (km, self.jm, self.im) or t.stride(2) != 1 or t.stride(1) != self.im:
+                raise ValueError(f"grids must be (km, {self.jm}, {self.im}) with contiguous layers")
+            if t.stride(0) != ts[0].stride(0):
+                raise ValueError("grids differ in layer pitch")
+        return km, ts[0].stride(0)
+
+    def _col_out(self, out, nlev, like):
+        import torch
+        if out is None:
+            return torch.empty((nlev, self.jm, self.im), dtype=torch.float64, device=like.device)
+        if tuple(out.shape) != (nlev, self.jm, self.im) or out.stride(2) != 1 or out.stride(1) != self.im:
+            raise ValueError(f"output must be ({nlev}, {self.jm}, {self.im}) with contiguous levels")
+        return out
+
+    def col_omega_d(self, div, h, cdk, out=None, stream=None):
+        """HBEDiagnose_OMG (HBEDiagnose_hspm_vfvm_mod.f90:90-143): the (km+1, J, I) half-level
+        omega from the layer divergence and thickness factor h; half level k is on top of
+        layer k and out[km] = 0."""
+        km, ldg = self._col_in(cdk, div, h)
+        out = self._col_out(out, km + 1, div)
+        _check(load().dsht_col_omega_d(self.h, _tptr(div), _tptr(h), ldg, km, _tptr(cdk), _tptr(out),
+                                       out.stride(0), _stream(stream)))
+        return out
+
+    def col_hydpres_d(self, dens, h, cdk, grav, out=None, stream=None):
+        """HBEDiagnose_HydPres (HBEDiagnose_hspm_vfvm_mod.f90:290-325): (km, J, I) hydrostatic
+        pressure at the layer centres from the density anomaly."""
+        km, ldg = self._col_in(cdk, dens, h)
+        out = self._col_out(out, km, dens)
+        _check(load().dsht_col_hydpres_d(self.h, _tptr(dens), _tptr(h), ldg, km, _tptr(cdk), float(grav),
+                                         _tptr(out), out.stride(0), _stream(stream)))
+        return out
+
+    def col_uvbarot_d(self, u, v, h, cdk, ub=None, vb=None, stream=None):
+        """HBEDiagnose_UVBarot (HBEDiagnose_hspm_vfvm_mod.f90:221-255): the thickness-weighted
+        layer means of u and v, (J, I) each."""
+        import torch
+        km, ldg = self._col_in(cdk, u, v, h)
+        res = []
+        for o in (ub, vb):
+            if o is None:
+                o = torch.empty((self.jm, self.im), dtype=torch.float64, device=u.device)
+            elif tuple(o.shape) != (self.jm, self.im) or not o.is_contiguous():
+                raise ValueError(f"outputs must be contiguous ({self.jm}, {self.im}) grids")
+            res.append(o)
+        _check(load().dsht_col_uvbarot_d(self.h, _tptr(u), _tptr(v), _tptr(h), ldg, km, _tptr(cdk),
+                                         _tptr(res[0]), _tptr(res[1]), _stream(stream)))
+        return res[0], res[1]
 
     def set_tuning(self, key, value):
         _check(load().dsht_set_tuning(self.h, int(key), int(value)))

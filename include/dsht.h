@@ -161,6 +161,25 @@ int dsht_ya_intlon_d(dsht_plan* plan, const double* xy, long ldg, double* out, i
                      void* stream);
 int dsht_interpolate_d(dsht_plan* plan, const double* w, long sl, const double* lon,
                        const double* lat, double* out, int npts, void* stream);
+/* Vertical column diagnostics between the inverse and the forward transforms, on device
+ * grids.  The km layers of one field are km consecutive grid columns with pitch ldg
+ * (layer 0 on top: the reference's KS), each an I x J grid; cdk holds the km layer
+ * thicknesses in sigma (z_CDK(KS:KE), DOGCM_GaussSpmVFvmGrid_mod.f90:400-402), device
+ * memory.  Each result equals the reference loop evaluated in FP64, in its order.
+ *
+ * HBEDiagnose_OMG (HBEDiagnose_hspm_vfvm_mod.f90:90-143): omg holds the km+1 half levels
+ * KS-1..KE (pitch ldo), half level k on top of layer k: omg[km] = 0,
+ * omg[k] = omg[k+1] - div[k] * (h[k] * cdk[k]).                                          */
+int dsht_col_omega_d(dsht_plan* plan, const double* div, const double* h, long ldg, int km,
+                     const double* cdk, double* omg, long ldo, void* stream);
+/* HBEDiagnose_HydPres (:290-325): r = 0; per layer r' = r + grav*h*dens*cdk,
+ * out[k] = 0.5*(r + r') (km columns, pitch ldo).                                        */
+int dsht_col_hydpres_d(dsht_plan* plan, const double* dens, const double* h, long ldg, int km,
+                       const double* cdk, double grav, double* out, long ldo, void* stream);
+/* HBEDiagnose_UVBarot (:221-255): thickness-weighted layer means of u and v, one I x J
+ * grid each; either output may be NULL.                                                 */
+int dsht_col_uvbarot_d(dsht_plan* plan, const double* u, const double* v, const double* h,
+                       long ldg, int km, const double* cdk, double* ub, double* vb, void* stream);
 /* coefficient table written to device memory */
 int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, double* out,
                 void* stream);

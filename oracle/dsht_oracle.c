@@ -634,3 +634,52 @@ void orc_diag(const orc_t* h, const double* d, double* w, int ncols) {
   for (int c = 0; c < ncols; c++)
     for (int l = 0; l < h->L; l++) w[(long)c * h->L + l] *= d[l];
 }
+
+/* ---------------------------------------------------------------- vertical column diagnostics
+ * HBEDiagnose_OMG, _HydPres, _UVBarot (model/dogcm/src/dynamics/hspm_vfvm/
+ * HBEDiagnose_hspm_vfvm_mod.f90:118-137, 312-322, 244-252): per grid point, the reference's
+ * loop over layers with its products and sums in its order.  fp-contract is off so that no
+ * product is fused into an add (the Fortran statements round every product). */
+#define ORC_NOFMA __attribute__((optimize("fp-contract=off")))
+
+ORC_NOFMA void orc_col_omega(const double* div, const double* h, int km, const double* cdk, double* omg,
+                             long npts) {
+  for (long i = 0; i < npts; i++) {
+    double r = 0.0;
+    omg[(long)km * npts + i] = r; /* r_OMG(KE) = 0 */
+    for (int k = km - 1; k >= 0; k--) {
+      const double dz = h[(long)k * npts + i] * cdk[k];   /* z_DZ = H * CDK */
+      const double dom = div[(long)k * npts + i] * dz;    /* z_DOMG = Div * z_DZ */
+      r = r - dom;                                        /* r_OMG(k) = r_OMG(k+1) - z_DOMG(k+1) */
+      omg[(long)k * npts + i] = r;
+    }
+  }
+}
+
+ORC_NOFMA void orc_col_hydpres(const double* dens, const double* h, int km, const double* cdk, double grav,
+                               double* p, long npts) {
+  for (long i = 0; i < npts; i++) {
+    double r = 0.0; /* r_HydPres(KS-1) = 0 */
+    for (int k = 0; k < km; k++) {
+      const double d = ((grav * h[(long)k * npts + i]) * dens[(long)k * npts + i]) * cdk[k];
+      const double rn = r + d;
+      p[(long)k * npts + i] = 0.5 * (r + rn);
+      r = rn;
+    }
+  }
+}
+
+ORC_NOFMA void orc_col_uvbarot(const double* u, const double* v, const double* h, int km, const double* cdk,
+                               double* ub, double* vb, long npts) {
+  for (long i = 0; i < npts; i++) {
+    double dep = 0.0, su = 0.0, sv = 0.0;
+    for (int k = 0; k < km; k++) {
+      const double f = h[(long)k * npts + i] * cdk[k]; /* z_Fac */
+      dep = dep + f;
+      su = su + u[(long)k * npts + i] * f;
+      sv = sv + v[(long)k * npts + i] * f;
+    }
+    ub[i] = su / dep;
+    vb[i] = sv / dep;
+  }
+}

@@ -81,6 +81,19 @@ void orc_coef_helmholtz(const orc_t* h, double diag, double c0, double* c);
 /* w <- d[l] * w  (column-wise) */
 void orc_diag(const orc_t* h, const double* d, double* w, int ncols);
 
+/* Vertical column diagnostics (model/dogcm/src/dynamics/hspm_vfvm/HBEDiagnose_hspm_vfvm_mod.f90).
+ * Layer k (0 = top, the reference's KS) of a field at x[k*npts + i], i < npts grid points;
+ * cdk[k] the layer thickness in sigma.  The reference loops, in their order, with no
+ * fused multiply-adds.
+ *   OMG     (:90-143)  omg (km+1 half levels): omg[km] = 0, omg[k] = omg[k+1] - div[k]*(h[k]*cdk[k])
+ *   HydPres (:290-325) r = 0; r' = r + ((grav*h[k])*dens[k])*cdk[k]; p[k] = 0.5*(r + r')
+ *   UVBarot (:221-255) ub = (sum_k u[k]*(h[k]*cdk[k])) / (sum_k h[k]*cdk[k]), vb likewise */
+void orc_col_omega(const double* div, const double* h, int km, const double* cdk, double* omg, long npts);
+void orc_col_hydpres(const double* dens, const double* h, int km, const double* cdk, double grav,
+                     double* p, long npts);
+void orc_col_uvbarot(const double* u, const double* v, const double* h, int km, const double* cdk,
+                     double* ub, double* vb, long npts);
+
 #ifdef __cplusplus
 }
 #endif

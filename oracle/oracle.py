@@ -64,6 +64,10 @@ def lib():
         L.orc_coef_advfilter.argtypes = [vp, dp]
         L.orc_coef_helmholtz.argtypes = [vp, d, d, dp]
         L.orc_diag.argtypes = [vp, dp, dp, i]
+        lg = ctypes.c_long
+        L.orc_col_omega.argtypes = [dp, dp, i, dp, dp, lg]
+        L.orc_col_hydpres.argtypes = [dp, dp, i, dp, d, dp, lg]
+        L.orc_col_uvbarot.argtypes = [dp, dp, dp, i, dp, dp, dp, lg]
         _lib = L
     return _lib
 
@@ -251,3 +255,41 @@ class Oracle:
         d = _c(d)
         lib().orc_diag(self.h, _p(d), _p(w), nc)
         return w
+
+
+# ---------------------------------------------------------------- vertical column diagnostics
+# Fields are (km, ...) arrays, layer 0 on top; cdk the km layer thicknesses in sigma.
+
+def _col(*arrs):
+    a = [_c(x) for x in arrs]
+    km = a[0].shape[0]
+    if any(x.shape != a[0].shape for x in a):
+        raise ValueError("column fields differ in shape")
+    return a, km, a[0].size // km
+
+
+def col_omega(div, h, cdk):
+    """HBEDiagnose_OMG: the (km+1, ...) half-level omega; half level k on top of layer k."""
+    (div, h), km, npts = _col(div, h)
+    cdk = _c(cdk)
+    omg = np.empty((km + 1,) + div.shape[1:])
+    lib().orc_col_omega(_p(div), _p(h), km, _p(cdk), _p(omg), npts)
+    return omg
+
+
+def col_hydpres(dens, h, cdk, grav):
+    """HBEDiagnose_HydPres: (km, ...) hydrostatic pressure at the layer centres."""
+    (dens, h), km, npts = _col(dens, h)
+    cdk = _c(cdk)
+    p = np.empty_like(dens)
+    lib().orc_col_hydpres(_p(dens), _p(h), km, _p(cdk), float(grav), _p(p), npts)
+    return p
+
+
+def col_uvbarot(u, v, h, cdk):
+    """HBEDiagnose_UVBarot: thickness-weighted layer means of u and v."""
+    (u, v, h), km, npts = _col(u, v, h)
+    cdk = _c(cdk)
+    ub, vb = np.empty(u.shape[1:]), np.empty(u.shape[1:])
+    lib().orc_col_uvbarot(_p(u), _p(v), _p(h), km, _p(cdk), _p(ub), _p(vb), npts)
+    return ub, vb

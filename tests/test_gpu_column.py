@@ -1,0 +1,128 @@
+"""Vertical column diagnostics on the GPU (dsht_col_omega_d, dsht_col_hydpres_d,
+dsht_col_uvbarot_d) against the FP64 oracle: HBEDiagnose_OMG, _HydPres and _UVBarot
+(model/dogcm/src/dynamics/hspm_vfvm/HBEDiagnose_hspm_vfvm_mod.f90:90-143, 290-325, 221-255).
+
+Both sides evaluate the reference loops in the reference's order with no fused
+multiply-adds and correctly rounded division, so the results must agree bit for bit.
+Cases: the T42L20, T85L60 and T170L60 grids; layer counts that do and do not fill the
+kernels' 8-layer load groups (1, 7, 9); padded column pitches on input and output; the
+argument errors of the C entry points.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GRAV = 9.8
+
+
+def _inputs(km, J, I, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.2, 1.0, km)
+    return {
+        "cdk": c / c.sum(),
+        "div": rng.standard_normal((km, J, I)) * 1e-6,
+        "h": 1.0 + 0.05 * rng.standard_normal((km, J, I)),
+        "dens": rng.standard_normal((km, J, I)) * 0.5,
+        "u": rng.standard_normal((km, J, I)) * 0.1,
+        "v": rng.standard_normal((km, J, I)) * 0.1,
+    }
+
+
+def _dev(a, gpu, pad=0):
+    """float64 device copy; with pad > 0 the layers sit at a pitch of I*J + pad doubles."""
+    import torch
+    a = np.ascontiguousarray(a)
+    if pad == 0:
+        return torch.tensor(a, device=gpu)
+    km, J, I = a.shape
+    base = torch.full((km, J * I + pad), np.nan, dtype=torch.float64, device=gpu)
+    base[:, :J * I] = torch.tensor(a.reshape(km, -1), device=gpu)
+    return base[:, :J * I].view(km, J, I)
+
+
+def _padded_out(nlev, J, I, pad, gpu):
+    import torch
+    base = torch.full((nlev, J * I + pad), np.nan, dtype=torch.float64, device=gpu)
+    return base[:, :J * I].view(nlev, J, I)
+
+
+@pytest.mark.parametrize("N,I,J,km", [(42, 128, 64, 20), (85, 256, 128, 60), (170, 512, 256, 60),
+                                      (21, 64, 32, 1), (21, 64, 32, 7), (42, 128, 64, 9)])
+def test_column_diagnostics_bitwise(gpu, N, I, J, km):
+    import torch
+    import dsht
+    import oracle
+    p = dsht.Plan(N, I, J, radius=6.371e6)
+    x = _inputs(km, J, I, seed=N + km)
+    d = {k: _dev(v, gpu) for k, v in x.items()}
+    omg = p.col_omega_d(d["div"], d["h"], d["cdk"])
+    hp = p.col_hydpres_d(d["dens"], d["h"], d["cdk"], GRAV)
+    ub, vb = p.col_uvbarot_d(d["u"], d["v"], d["h"], d["cdk"])
+    torch.cuda.synchronize()
+    r_omg = oracle.col_omega(x["div"], x["h"], x["cdk"])
+    r_hp = oracle.col_hydpres(x["dens"], x["h"], x["cdk"], GRAV)
+    r_ub, r_vb = oracle.col_uvbarot(x["u"], x["v"], x["h"], x["cdk"])
+    for name, g, r in (("omega", omg, r_omg), ("hydpres", hp, r_hp), ("ubarot", ub, r_ub), ("vbarot", vb, r_vb)):
+        g = g.cpu().numpy()
+        assert g.shape == r.shape, name
+        nbad = int((g != r).sum())
+        assert nbad == 0, (name, nbad, float(np.abs(g - r).max()))
+
+
+def test_column_padded_pitch(gpu):
+    """Inputs and outputs at padded layer pitches; the padding is never written."""
+    import torch
+    import dsht
+    import oracle
+    N, I, J, km = 42, 128, 64, 20
+    p = dsht.Plan(N, I, J, radius=6.371e6)
+    x = _inputs(km, J, I, seed=5)
+    div, h, dens = _dev(x["div"], gpu, 96), _dev(x["h"], gpu, 96), _dev(x["dens"], gpu, 96)
+    cdk = torch.tensor(x["cdk"], device=gpu)
+    omg = _padded_out(km + 1, J, I, 40, gpu)
+    hp = _padded_out(km, J, I, 40, gpu)
+    p.col_omega_d(div, h, cdk, out=omg)
+    p.col_hydpres_d(dens, h, cdk, GRAV, out=hp)
+    torch.cuda.synchronize()
+    assert np.array_equal(omg.cpu().numpy(), oracle.col_omega(x["div"], x["h"], x["cdk"]))
+    assert np.array_equal(hp.cpu().numpy(), oracle.col_hydpres(x["dens"], x["h"], x["cdk"], GRAV))
+    # the pitch padding after each level is untouched
+    pad = omg.as_strided((km + 1, 40), (omg.stride(0), 1), J * I).cpu().numpy()
+    assert np.isnan(pad).all()
+
+
+def test_column_one_output_and_errors(gpu):
+    import torch
+    import dsht
+    import oracle
+    N, I, J, km = 21, 64, 32, 5
+    p = dsht.Plan(N, I, J, radius=6.371e6)
+    x = _inputs(km, J, I, seed=9)
+    d = {k: _dev(v, gpu) for k, v in x.items()}
+    lib = dsht.load()
+    tp = dsht._tptr
+    vb = torch.full((J, I), np.nan, dtype=torch.float64, device=gpu)
+    rc = lib.dsht_col_uvbarot_d(p.h, tp(d["u"]), tp(d["v"]), tp(d["h"]), I * J, km, tp(d["cdk"]), None,
+                                tp(vb), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(vb.cpu().numpy(), oracle.col_uvbarot(x["u"], x["v"], x["h"], x["cdk"])[1])
+    # argument errors are reported, nothing is launched
+    out = torch.empty((km + 1, J, I), dtype=torch.float64, device=gpu)
+    cases = [
+        lib.dsht_col_uvbarot_d(p.h, tp(d["u"]), tp(d["v"]), tp(d["h"]), I * J, km, tp(d["cdk"]), None, None, None),
+        lib.dsht_col_omega_d(p.h, tp(d["div"]), tp(d["h"]), I * J, 0, tp(d["cdk"]), tp(out), I * J, None),
+        lib.dsht_col_omega_d(p.h, tp(d["div"]), tp(d["h"]), I * J, km, None, tp(out), I * J, None),
+        lib.dsht_col_omega_d(p.h, tp(d["div"]), tp(d["h"]), I * J - 1, km, tp(d["cdk"]), tp(out), I * J, None),
+        lib.dsht_col_hydpres_d(p.h, tp(d["dens"]), None, I * J, km, tp(d["cdk"]), ctypes.c_double(GRAV),
+                               tp(out), I * J, None),
+        lib.dsht_col_hydpres_d(p.h, tp(d["dens"]), tp(d["h"]), I * J, km, tp(d["cdk"]), ctypes.c_double(GRAV),
+                               tp(out), I * J - 2, None),
+    ]
+    assert all(rc != 0 for rc in cases), cases
+    with pytest.raises(ValueError):
+        p.col_omega_d(d["div"], d["h"][:, :, :I // 2], d["cdk"])
+    with pytest.raises(TypeError):
+        p.col_omega_d(d["div"].float(), d["h"].float(), d["cdk"])

@@ -1,0 +1,110 @@
+"""Pins the oracle's vertical column diagnostics (oracle/dsht_oracle.c: orc_col_*), the
+checker of HBEDiagnose_OMG, _HydPres and _UVBarot
+(model/dogcm/src/dynamics/hspm_vfvm/HBEDiagnose_hspm_vfvm_mod.f90:90-143, 221-255, 290-325).
+
+Each check is one that a plausible mistake fails:
+  - closed forms on uniform layers: omega falls linearly to 0 at the bottom half level,
+    the pressure sits at the layer centres (k + 1/2), the mean of a linear profile is its
+    middle value;
+  - independent library routines on random columns: numpy's cumulative sums from the
+    bottom (omega) and from the top (pressure), numpy's weighted average (barotropic mean);
+  - invariants: omega at the top half level is minus the column integral of the
+    divergence, the mean of a constant is that constant, the mean is odd in (u, v).
+A scan in the wrong direction, an off-by-one half level, the pressure at the interfaces
+instead of the centres, or a mean weighted without the thickness fails one of these.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+GRAV = 9.8
+
+
+def _cdk(km, seed):
+    """Random positive layer thicknesses in sigma summing to 1 (the reference's z_CDK)."""
+    c = np.random.default_rng(seed).uniform(0.2, 1.0, km)
+    return c / c.sum()
+
+
+@pytest.mark.parametrize("km", [1, 7, 8, 20, 60])
+def test_omega_closed_form_and_cumsum(km):
+    npts = 33
+    cdk = np.full(km, 1.0 / km)
+    D, H0 = 3e-6, 1.2
+    omg = O.col_omega(np.full((km, npts), D), np.full((km, npts), H0), cdk)
+    assert omg.shape == (km + 1, npts)
+    assert np.all(omg[km] == 0.0)
+    k = np.arange(km + 1)[:, None]
+    np.testing.assert_allclose(omg, -D * H0 * (km - k) / km * np.ones((1, npts)), rtol=1e-14, atol=1e-22)
+    # random columns against numpy's reversed cumulative sum
+    rng = np.random.default_rng(km)
+    div = rng.standard_normal((km, npts)) * 1e-6
+    h = 1.0 + 0.1 * rng.standard_normal((km, npts))
+    cdk = _cdk(km, km + 1)
+    omg = O.col_omega(div, h, cdk)
+    ref = -np.cumsum((div * (h * cdk[:, None]))[::-1], axis=0)[::-1]
+    assert np.all(omg[km] == 0.0)
+    np.testing.assert_allclose(omg[:km], ref, rtol=1e-12, atol=1e-20)
+    # the top half level is minus the column integral of the divergence
+    np.testing.assert_allclose(omg[0], -(div * h * cdk[:, None]).sum(0), rtol=1e-12, atol=1e-20)
+
+
+@pytest.mark.parametrize("km", [1, 7, 8, 20, 60])
+def test_hydpres_closed_form_and_cumsum(km):
+    npts = 29
+    cdk = np.full(km, 1.0 / km)
+    rho, H0 = 1.5, 0.9
+    p = O.col_hydpres(np.full((km, npts), rho), np.full((km, npts), H0), cdk, GRAV)
+    k = np.arange(km)[:, None]
+    np.testing.assert_allclose(p, GRAV * H0 * rho * (k + 0.5) / km * np.ones((1, npts)), rtol=1e-14)
+    rng = np.random.default_rng(100 + km)
+    dens = rng.standard_normal((km, npts))
+    h = 1.0 + 0.1 * rng.standard_normal((km, npts))
+    cdk = _cdk(km, km + 2)
+    p = O.col_hydpres(dens, h, cdk, GRAV)
+    d = GRAV * h * dens * cdk[:, None]
+    ref = np.cumsum(d, axis=0) - 0.5 * d
+    scale = np.abs(d).sum(0)
+    assert np.all(np.abs(p - ref) <= 1e-13 * scale)
+
+
+@pytest.mark.parametrize("km", [1, 7, 8, 20, 60])
+def test_uvbarot_invariants(km):
+    npts = 31
+    rng = np.random.default_rng(200 + km)
+    h = 1.0 + 0.1 * rng.standard_normal((km, npts))
+    cdk = _cdk(km, km + 3)
+    # a constant is its own mean (to rounding)
+    ub, vb = O.col_uvbarot(np.full((km, npts), 0.25), np.full((km, npts), -2.0), h, cdk)
+    np.testing.assert_allclose(ub, 0.25, rtol=4e-16 * km)
+    np.testing.assert_allclose(vb, -2.0, rtol=4e-16 * km)
+    # a linear profile on uniform layers averages to its middle value
+    lin = np.arange(km, dtype=np.float64)[:, None] * np.ones((1, npts))
+    ub, _ = O.col_uvbarot(lin, lin, np.ones((km, npts)), np.full(km, 1.0 / km))
+    np.testing.assert_allclose(ub, (km - 1) / 2.0, rtol=1e-14, atol=1e-14)
+    # random columns against numpy's weighted average; odd in (u, v)
+    u = rng.standard_normal((km, npts))
+    v = rng.standard_normal((km, npts))
+    ub, vb = O.col_uvbarot(u, v, h, cdk)
+    wts = h * cdk[:, None]
+    np.testing.assert_allclose(ub, np.average(u, axis=0, weights=wts), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(vb, np.average(v, axis=0, weights=wts), rtol=1e-12, atol=1e-15)
+    ub2, vb2 = O.col_uvbarot(-u, -v, h, cdk)
+    assert np.array_equal(ub2, -ub) and np.array_equal(vb2, -vb)
+
+
+def test_planted_mistake_is_caught():
+    """A top-down omega scan (a plausible mistake, planted here in a copy of the oracle's
+    loop) fails the bottom-boundary and cumulative-sum checks above."""
+    km, npts = 6, 5
+    rng = np.random.default_rng(7)
+    div = rng.standard_normal((km, npts))
+    h = np.ones((km, npts))
+    cdk = _cdk(km, 8)
+    wrong = np.zeros((km + 1, npts))
+    for k in range(km):  # integrates from the top instead of from the bottom
+        wrong[k + 1] = wrong[k] - div[k] * (h[k] * cdk[k])
+    good = O.col_omega(div, h, cdk)
+    assert not np.all(wrong[km] == 0.0)
+    assert not np.allclose(wrong, good)
