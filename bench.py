@@ -371,7 +371,8 @@ def batched_bench(torch, dsht, n_inst=64, steps=10, seq_steps=2, seed=0xB47C4, p
 
 # ---------------------------------------------------------------- serving form
 def column_bench(torch, dsht, reps=21, nsets=3):
-    """Vertical column diagnostics (HBEDiagnose_OMG, _HydPres, _UVBarot) on T170L60 device
+    """Vertical column diagnostics (HBEDiagnose_OMG, _HydPres, _UVBarot and the QUICK vertical
+    advective flux of calc_ADVFlxZ_QUICK) on T170L60 device
     grids (60 layers of 512 x 256 points).  Calls rotate over `nsets` buffer sets (each
     call touches about 190 MB, so a set is out of the 256 MB last-level cache again when it
     comes back round).  Each kernel's average duration comes from events on the stream it
@@ -381,6 +382,7 @@ def column_bench(torch, dsht, reps=21, nsets=3):
     g = torch.Generator(device="cuda").manual_seed(0xC01)
     shape = (KM, JM, IM)
     cdk = torch.full((KM,), 1.0 / KM, dtype=torch.float64, device="cuda")
+    rfdk = torch.full((KM + 1,), float(KM), dtype=torch.float64, device="cuda")
 
     def rnd(scale=1.0, shift=0.0):
         return shift + scale * torch.randn(shape, dtype=torch.float64, device="cuda", generator=g)
@@ -390,6 +392,7 @@ def column_bench(torch, dsht, reps=21, nsets=3):
         sets.append({"div": rnd(1e-6), "h": rnd(0.05, 1.0), "dens": rnd(), "u": rnd(), "v": rnd(),
                      "omg": torch.empty((KM + 1, JM, IM), dtype=torch.float64, device="cuda"),
                      "hp": torch.empty(shape, dtype=torch.float64, device="cuda"),
+                     "flx": torch.empty((KM + 1, JM, IM), dtype=torch.float64, device="cuda"),
                      "ub": torch.empty((JM, IM), dtype=torch.float64, device="cuda"),
                      "vb": torch.empty((JM, IM), dtype=torch.float64, device="cuda")})
     npts = IM * JM
@@ -398,6 +401,8 @@ def column_bench(torch, dsht, reps=21, nsets=3):
         "hydpres": (lambda b: plan.col_hydpres_d(b["dens"], b["h"], cdk, 9.8, out=b["hp"]), 3 * KM * npts * 8),
         "uvbarot": (lambda b: plan.col_uvbarot_d(b["u"], b["v"], b["h"], cdk, b["ub"], b["vb"]),
                     (3 * KM + 2) * npts * 8),
+        "quick": (lambda b: plan.col_quick_d(b["omg"], b["dens"], cdk, rfdk, out=b["flx"]),
+                  (2 * KM + 1 + KM + 1) * npts * 8),
     }
     out = {"workload": f"T{NMAX}L{KM} grids ({IM} x {JM} points, {KM} layers), FP64; calls rotate over "
                        f"{nsets} buffer sets"}
@@ -584,7 +589,7 @@ def main():
     print("[bench] column diagnostics", file=sys.stderr, flush=True)
     column = column_bench(torch, dsht)
     print("[bench] column: " + ", ".join(f"{k} {column[k]['avg_us']:.1f} us {column[k]['GBps']:.0f} GB/s"
-                                         for k in ("omega", "hydpres", "uvbarot")), file=sys.stderr, flush=True)
+                                         for k in ("omega", "hydpres", "uvbarot", "quick")), file=sys.stderr, flush=True)
     batched = serving = None
     if not args.no_batch:
         print("[bench] batched form (64 instances)", file=sys.stderr, flush=True)

@@ -763,6 +763,16 @@ int dsht_col_uvbarot_d(dsht_plan* p, const double* u, const double* v, const dou
   return launch_col_uvbarot(u, v, h, ldg, km, cdk, ub, vb, p->I * p->J, (hipStream_t)stream);
 }
 
+int dsht_col_quick_d(dsht_plan* p, const double* omg, long ldo, const double* trc, long ldg, int km,
+                     const double* cdk, const double* rfdk, int linfree, double* flx, long ldf, void* stream) {
+  if (int rc = check_col(p, km, cdk, ldg)) return rc;
+  if (km < 2) return fail(DSHT_EARG, "the QUICK flux needs km >= 2");
+  if (!omg || !trc || !flx || !rfdk) return fail(DSHT_EARG, "pointer is NULL");
+  if (ldo < (long)p->I * p->J || ldf < (long)p->I * p->J) return fail(DSHT_EARG, "half-level pitch must be >= I*J");
+  return launch_col_quick(omg, ldo, trc, ldg, km, cdk, rfdk, linfree ? 1 : 0, flx, ldf, p->I * p->J,
+                          (hipStream_t)stream);
+}
+
 int dsht_coef_d(dsht_plan* p, int which, const double* params, int nparams, double* out, void* stream) {
   if (int rc = check_plan(p)) return rc;
   if (which < DSHT_COEF_LAPLA || which > DSHT_COEF_BEDIFF) return fail(DSHT_EARG, "unknown coefficient table %d", which);

@@ -3,6 +3,7 @@
 //   HBEDiagnose_OMG     (model/dogcm/src/dynamics/hspm_vfvm/HBEDiagnose_hspm_vfvm_mod.f90:90-143)
 //   HBEDiagnose_UVBarot (same file:221-255)
 //   HBEDiagnose_HydPres (same file:290-325)
+//   calc_ADVFlxZ_QUICK  (model/dogcm/src/dynamics/hspm_vfvm/HBEBaroc_hspm_vfvm_mod.f90:233-317)
 //
 // Layout: the km layers of one field are km consecutive device grid columns (column pitch
 // ldg doubles, layer 0 on top = the reference's KS), each an I x J grid with longitude
@@ -136,6 +137,97 @@ __global__ __launch_bounds__(256) void k_col_uvbarot(const double* __restrict__ 
   if (vb) nts(vb + i, sv / dep);
 }
 
+// QUICK curvature term of layer l from its neighbours (HBEBaroc_hspm_vfvm_mod.f90:274-277):
+// ((T[l-1] - T[l]) rfdk[l] - (T[l] - T[l+1]) rfdk[l+1]) (rfdk[l] + rfdk[l+1])
+__device__ __forceinline__ double quick_diff(double ta, double tb, double tc, double ra, double rb) {
+#pragma clang fp contract(off)
+  const double x = (ta - tb) * ra;
+  const double y = (tb - tc) * rb;
+  return (x - y) * (ra + rb);
+}
+
+// Vertical advective flux at the km+1 half levels (half level h on top of layer h):
+//   h = 0      0, or omg[0] T[0] with a linear free surface                (:298, 307-312)
+//   h = 1, km-1  omg[h] (m1 T[h-1] + m2 T[h])                                (:300-301)
+//   2 <= h <= km-2  omg[h] ((m1 T[h-1] + m2 T[h]) - dfac (D[h-1] + D[h]))
+//                   + |omg[h]| dfac (D[h-1] - D[h])                          (:286-291)
+//   h = km     0
+// with m1 = cdk[h]/(cdk[h-1]+cdk[h]), m2 = cdk[h-1]/(cdk[h-1]+cdk[h]), dfac = 0.125 cdk[h] cdk[h-1]
+// (:264-266) and D the curvature term above.  The block computes m1, m2 and dfac of every
+// half level once into LDS (3 (km+1) doubles of dynamic LDS); each thread keeps a
+// three-layer window of the tracer and the previous curvature term in registers, so each
+// grid value is loaded once.
+__global__ __launch_bounds__(256) void k_col_quick(const double* __restrict__ omg, long ldo,
+                                                   const double* __restrict__ trc, long ldg, int km,
+                                                   const double* __restrict__ cdk, const double* __restrict__ rfdk,
+                                                   int linfree, double* __restrict__ flx, long ldf, int npts) {
+#pragma clang fp contract(off)
+  extern __shared__ double wts[];
+  double* sm1 = wts;
+  double* sm2 = wts + (km + 1);
+  double* sdf = wts + 2 * (km + 1);
+  for (int h = 1 + (int)threadIdx.x; h < km; h += 256) {
+    const double s = cdk[h - 1] + cdk[h];
+    sm1[h] = cdk[h] / s;
+    sm2[h] = cdk[h - 1] / s;
+    sdf[h] = (0.125 * cdk[h]) * cdk[h - 1];
+  }
+  __syncthreads();
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= npts) return;
+  const double* T = trc + i;
+  const double* W = omg + i;
+  double* F = flx + i;
+  double b = ntl(T), c = ntl(T + ldg);  // T[h-1], T[h] entering h = 1
+  nts(F, linfree ? ntl(W) * b : 0.0);
+  {
+    const double x = sm1[1] * b, y = sm2[1] * c;
+    nts(F + ldf, ntl(W + ldo) * (x + y));
+  }
+  if (km >= 4) {
+    double d = ntl(T + 2 * ldg);  // T[2]
+    double dprev = quick_diff(b, c, d, rfdk[1], rfdk[2]);
+    b = c;
+    c = d;  // entering h = 2: b = T[1], c = T[2]
+    // interior half levels h = 2 .. km-2: groups of CU with their loads issued together
+    auto step = [&](int h, double tn, double w) {
+      const double dcur = quick_diff(b, c, tn, rfdk[h], rfdk[h + 1]);
+      const double df = sdf[h];
+      const double x = sm1[h] * b, y = sm2[h] * c;
+      const double q = dprev + dcur;
+      const double e = df * q;
+      const double g = w * ((x + y) - e);
+      const double aw = fabs(w) * df;
+      const double j = aw * (dprev - dcur);
+      nts(F + (long)h * ldf, g + j);
+      dprev = dcur;
+      b = c;
+      c = tn;
+    };
+    int h = 2;
+    for (; h + CU - 1 <= km - 2; h += CU) {
+      double tn[CU], w[CU];
+#pragma unroll
+      for (int u = 0; u < CU; u++) {
+        tn[u] = ntl(T + (long)(h + u + 1) * ldg);
+        w[u] = ntl(W + (long)(h + u) * ldo);
+      }
+#pragma unroll
+      for (int u = 0; u < CU; u++) step(h + u, tn[u], w[u]);
+    }
+    for (; h <= km - 2; h++) step(h, ntl(T + (long)(h + 1) * ldg), ntl(W + (long)h * ldo));
+  } else if (km == 3) {
+    b = c;
+    c = ntl(T + 2 * ldg);
+  }
+  if (km >= 3) {  // h = km - 1 with b = T[km-2], c = T[km-1]
+    const int h = km - 1;
+    const double x = sm1[h] * b, y = sm2[h] * c;
+    nts(F + (long)h * ldf, ntl(W + (long)h * ldo) * (x + y));
+  }
+  nts(F + (long)km * ldf, 0.0);
+}
+
 int col_grid(int npts) { return (npts + 255) / 256; }
 
 }  // namespace
@@ -158,6 +250,16 @@ int launch_col_hydpres(const double* dens, const double* h, long ldg, int km, co
 int launch_col_uvbarot(const double* u, const double* v, const double* h, long ldg, int km, const double* cdk,
                        double* ub, double* vb, int npts, hipStream_t s) {
   hipLaunchKernelGGL(k_col_uvbarot, dim3(col_grid(npts)), dim3(256), 0, s, u, v, h, ldg, km, cdk, ub, vb, npts);
+  DSHT_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_col_quick(const double* omg, long ldo, const double* trc, long ldg, int km, const double* cdk,
+                     const double* rfdk, int linfree, double* flx, long ldf, int npts, hipStream_t s) {
+  const size_t lds = 3 * (size_t)(km + 1) * sizeof(double);
+  if (lds > 64 * 1024) return fail(DSHT_EARG, "the QUICK flux supports at most 2729 layers");
+  hipLaunchKernelGGL(k_col_quick, dim3(col_grid(npts)), dim3(256), lds, s, omg, ldo, trc, ldg, km, cdk, rfdk,
+                     linfree, flx, ldf, npts);
   DSHT_HIP(hipGetLastError());
   return 0;
 }

@@ -280,6 +280,8 @@ int launch_col_hydpres(const double* dens, const double* h, long ldg, int km, co
                        double* p, long ldo, int npts, hipStream_t s);
 int launch_col_uvbarot(const double* u, const double* v, const double* h, long ldg, int km, const double* cdk,
                        double* ub, double* vb, int npts, hipStream_t s);
+int launch_col_quick(const double* omg, long ldo, const double* trc, long ldg, int km, const double* cdk,
+                     const double* rfdk, int linfree, double* flx, long ldf, int npts, hipStream_t s);
 int launch_lonlat_sum(const dsht_plan* p, const double* xy, long ldg, int ncols, double* out,
                       double factor, hipStream_t s);
 int launch_lon_sum(const dsht_plan* p, const double* xy, long ldg, int ncols, double* out,

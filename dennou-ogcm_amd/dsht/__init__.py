@@ -40,7 +40,7 @@ EXPORTS = [
     "dsht_alpha_fwd_d", "dsht_alpha_inv_d", "dsht_uv2vordiv_d", "dsht_vordiv2uv_d",
     "dsht_diag_d", "dsht_intlonlat_d", "dsht_avrlonlat_d", "dsht_ya_intlon_d",
     "dsht_interpolate_d", "dsht_coef_d", "dsht_col_omega_d", "dsht_col_hydpres_d", "dsht_col_uvbarot_d",
-    "dsht_prof_enable", "dsht_prof_reset", "dsht_prof_query",
+    "dsht_col_quick_d", "dsht_prof_enable", "dsht_prof_reset", "dsht_prof_query",
     "dsht_set_tuning", "dsht_get_tuning", "dsht_batch_create", "dsht_batch_add", "dsht_batch_finalize",
     "dsht_batch_info", "dsht_batch_ngroups", "dsht_batch_buffers", "dsht_batch_step",
     "dsht_batch_step_sequential", "dsht_batch_capture", "dsht_batch_launch", "dsht_batch_destroy",
@@ -110,6 +110,7 @@ def load():
         "dsht_col_omega_d": ([vp, vp, vp, lg, i, vp, vp, lg, vp], i),
         "dsht_col_hydpres_d": ([vp, vp, vp, lg, i, vp, ctypes.c_double, vp, lg, vp], i),
         "dsht_col_uvbarot_d": ([vp, vp, vp, vp, lg, i, vp, vp, vp, vp], i),
+        "dsht_col_quick_d": ([vp, vp, lg, vp, lg, i, vp, vp, i, vp, lg, vp], i),
         "dsht_prof_enable": ([i], i),
         "dsht_prof_reset": ([], i),
         "dsht_prof_query": ([ctypes.c_char_p, vp, vp, vp, vp], i),
@@ -534,6 +535,24 @@ class Plan:
         _check(load().dsht_col_uvbarot_d(self.h, _tptr(u), _tptr(v), _tptr(h), ldg, km, _tptr(cdk),
                                          _tptr(res[0]), _tptr(res[1]), _stream(stream)))
         return res[0], res[1]
+
+    def col_quick_d(self, omg, trc, cdk, rfdk, linfree=False, out=None, stream=None):
+        """calc_ADVFlxZ_QUICK (HBEBaroc_hspm_vfvm_mod.f90:233-317): the (km+1, J, I) vertical
+        advective flux at the half levels from the (km+1, J, I) half-level omega and the
+        (km, J, I) tracer; rfdk the km+1 reciprocal half-level spacings."""
+        import torch
+        km, ldg = self._col_in(cdk, trc)
+        if tuple(omg.shape) != (km + 1, self.jm, self.im) or omg.stride(2) != 1 or omg.stride(1) != self.im \
+                or omg.dtype != torch.float64 or not omg.is_cuda:
+            raise ValueError(f"omega must be a float64 device grid ({km + 1}, {self.jm}, {self.im})")
+        if rfdk.dtype != torch.float64 or not rfdk.is_cuda or rfdk.dim() != 1 or rfdk.numel() != km + 1 \
+                or rfdk.stride(0) != 1:
+            raise ValueError(f"rfdk must be a contiguous float64 device vector of {km + 1} values")
+        out = self._col_out(out, km + 1, trc)
+        _check(load().dsht_col_quick_d(self.h, _tptr(omg), omg.stride(0), _tptr(trc), ldg, km, _tptr(cdk),
+                                       _tptr(rfdk), int(bool(linfree)), _tptr(out), out.stride(0),
+                                       _stream(stream)))
+        return out
 
     def set_tuning(self, key, value):
         _check(load().dsht_set_tuning(self.h, int(key), int(value)))

@@ -180,6 +180,14 @@ int dsht_col_hydpres_d(dsht_plan* plan, const double* dens, const double* h, lon
  * grid each; either output may be NULL.                                                 */
 int dsht_col_uvbarot_d(dsht_plan* plan, const double* u, const double* v, const double* h,
                        long ldg, int km, const double* cdk, double* ub, double* vb, void* stream);
+/* calc_ADVFlxZ_QUICK (HBEBaroc_hspm_vfvm_mod.f90:233-317): the vertical advective flux of a
+ * tracer at the km+1 half levels (pitch ldf) from the half-level omg (pitch ldo, as written
+ * by dsht_col_omega_d) and the km tracer layers trc (pitch ldg); rfdk holds the km+1
+ * reciprocal half-level spacings z_RFDK(KS-1:KE) (device memory).  flx[0] is 0, or
+ * omg[0]*trc[0] when linfree is nonzero (KinBCTYPE_LinFreeSurf); flx[km] is 0.  km >= 2. */
+int dsht_col_quick_d(dsht_plan* plan, const double* omg, long ldo, const double* trc, long ldg,
+                     int km, const double* cdk, const double* rfdk, int linfree, double* flx,
+                     long ldf, void* stream);
 /* coefficient table written to device memory */
 int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, double* out,
                 void* stream);

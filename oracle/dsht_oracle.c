@@ -683,3 +683,40 @@ ORC_NOFMA void orc_col_uvbarot(const double* u, const double* v, const double* h
     vb[i] = sv / dep;
   }
 }
+
+/* calc_ADVFlxZ_QUICK, HBEBaroc_hspm_vfvm_mod.f90:233-317, in the reference's structure: the
+ * interpolation weights per half level (:263-267), the curvature array of the interior layers
+ * (:271-279), the interior fluxes (:283-292), then the boundary half levels (:296-302,
+ * 307-312).  0-based: layer l = k - KS, half level h = k - (KS-1). */
+ORC_NOFMA void orc_col_quick(const double* omg, const double* trc, int km, const double* cdk,
+                             const double* rfdk, int linfree, double* flx, long npts) {
+  double* m1 = calloc((size_t)km + 1, sizeof(double));
+  double* m2 = calloc((size_t)km + 1, sizeof(double));
+  double* dfac = calloc((size_t)km + 1, sizeof(double));
+  double* diff = calloc((size_t)km, sizeof(double));
+  for (int h = 1; h <= km - 1; h++) { /* k = KS..KE-1 */
+    m1[h] = cdk[h] / (cdk[h - 1] + cdk[h]);
+    m2[h] = cdk[h - 1] / (cdk[h - 1] + cdk[h]);
+    dfac[h] = 0.125 * cdk[h] * cdk[h - 1];
+  }
+  for (long i = 0; i < npts; i++) {
+#define TR(l) trc[(long)(l) * npts + i]
+#define OM(h) omg[(long)(h) * npts + i]
+    for (int l = 1; l <= km - 2; l++) /* k = KS+1..KE-1 */
+      diff[l] = ((TR(l - 1) - TR(l)) * rfdk[l] - (TR(l) - TR(l + 1)) * rfdk[l + 1]) * (rfdk[l] + rfdk[l + 1]);
+    for (int h = 2; h <= km - 2; h++) /* k = KS+1..KE-2 */
+      flx[(long)h * npts + i] = OM(h) * ((m1[h] * TR(h - 1) + m2[h] * TR(h)) - dfac[h] * (diff[h - 1] + diff[h])) +
+                                fabs(OM(h)) * dfac[h] * (diff[h - 1] - diff[h]);
+    flx[i] = 0.0;
+    flx[(long)km * npts + i] = 0.0;
+    flx[npts + i] = OM(1) * (m1[1] * TR(0) + m2[1] * TR(1));
+    flx[(long)(km - 1) * npts + i] = OM(km - 1) * (m1[km - 1] * TR(km - 2) + m2[km - 1] * TR(km - 1));
+    if (linfree) flx[i] = OM(0) * TR(0);
+#undef TR
+#undef OM
+  }
+  free(m1);
+  free(m2);
+  free(dfac);
+  free(diff);
+}

@@ -93,6 +93,12 @@ void orc_col_hydpres(const double* dens, const double* h, int km, const double* 
                      double* p, long npts);
 void orc_col_uvbarot(const double* u, const double* v, const double* h, int km, const double* cdk,
                      double* ub, double* vb, long npts);
+/* calc_ADVFlxZ_QUICK (model/dogcm/src/dynamics/hspm_vfvm/HBEBaroc_hspm_vfvm_mod.f90:233-317):
+ * flux at the km+1 half levels from omg (km+1 half levels), trc (km layers), cdk (km) and
+ * rfdk (km+1 reciprocal half-level spacings); linfree selects the linear free surface
+ * boundary at the top.  km >= 2. */
+void orc_col_quick(const double* omg, const double* trc, int km, const double* cdk, const double* rfdk,
+                   int linfree, double* flx, long npts);
 
 #ifdef __cplusplus
 }

@@ -68,6 +68,7 @@ def lib():
         L.orc_col_omega.argtypes = [dp, dp, i, dp, dp, lg]
         L.orc_col_hydpres.argtypes = [dp, dp, i, dp, d, dp, lg]
         L.orc_col_uvbarot.argtypes = [dp, dp, dp, i, dp, dp, dp, lg]
+        L.orc_col_quick.argtypes = [dp, dp, i, dp, dp, i, dp, lg]
         _lib = L
     return _lib
 
@@ -293,3 +294,15 @@ def col_uvbarot(u, v, h, cdk):
     ub, vb = np.empty(u.shape[1:]), np.empty(u.shape[1:])
     lib().orc_col_uvbarot(_p(u), _p(v), _p(h), km, _p(cdk), _p(ub), _p(vb), npts)
     return ub, vb
+
+
+def col_quick(omg, trc, cdk, rfdk, linfree=False):
+    """calc_ADVFlxZ_QUICK: the (km+1, ...) vertical advective flux at the half levels."""
+    (trc,), km, npts = _col(trc)
+    omg = _c(omg)
+    if omg.shape != (km + 1,) + trc.shape[1:]:
+        raise ValueError("omega must have km+1 half levels")
+    cdk, rfdk = _c(cdk), _c(rfdk)
+    flx = np.empty_like(omg)
+    lib().orc_col_quick(_p(omg), _p(trc), km, _p(cdk), _p(rfdk), int(bool(linfree)), _p(flx), npts)
+    return flx

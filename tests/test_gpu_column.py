@@ -1,6 +1,7 @@
 """Vertical column diagnostics on the GPU (dsht_col_omega_d, dsht_col_hydpres_d,
-dsht_col_uvbarot_d) against the FP64 oracle: HBEDiagnose_OMG, _HydPres and _UVBarot
-(model/dogcm/src/dynamics/hspm_vfvm/HBEDiagnose_hspm_vfvm_mod.f90:90-143, 290-325, 221-255).
+dsht_col_uvbarot_d, dsht_col_quick_d) against the FP64 oracle: HBEDiagnose_OMG, _HydPres and
+_UVBarot (model/dogcm/src/dynamics/hspm_vfvm/HBEDiagnose_hspm_vfvm_mod.f90:90-143, 290-325,
+221-255) and calc_ADVFlxZ_QUICK (HBEBaroc_hspm_vfvm_mod.f90:233-317).
 
 Both sides evaluate the reference loops in the reference's order with no fused
 multiply-adds and correctly rounded division, so the results must agree bit for bit.
@@ -126,3 +127,59 @@ def test_column_one_output_and_errors(gpu):
         p.col_omega_d(d["div"], d["h"][:, :, :I // 2], d["cdk"])
     with pytest.raises(TypeError):
         p.col_omega_d(d["div"].float(), d["h"].float(), d["cdk"])
+
+
+@pytest.mark.parametrize("N,I,J,km", [(42, 128, 64, 20), (85, 256, 128, 60), (170, 512, 256, 60),
+                                      (21, 64, 32, 2), (21, 64, 32, 3), (21, 64, 32, 4), (21, 64, 32, 5)])
+@pytest.mark.parametrize("linfree", [False, True])
+def test_quick_flux_bitwise(gpu, N, I, J, km, linfree):
+    """The QUICK flux of the omega the omega kernel wrote, bit for bit against the oracle."""
+    import torch
+    import dsht
+    import oracle
+    p = dsht.Plan(N, I, J, radius=6.371e6)
+    x = _inputs(km, J, I, seed=3 * N + km)
+    rng = np.random.default_rng(km)
+    rfdk = rng.uniform(0.5 * km, 2.0 * km, km + 1)
+    trc = 10.0 + rng.standard_normal((km, J, I))
+    d = {k: _dev(v, gpu) for k, v in x.items()}
+    omg = p.col_omega_d(d["div"], d["h"], d["cdk"])
+    flx = p.col_quick_d(omg, _dev(trc, gpu), d["cdk"], _dev(rfdk, gpu), linfree=linfree)
+    torch.cuda.synchronize()
+    r_omg = oracle.col_omega(x["div"], x["h"], x["cdk"])
+    ref = oracle.col_quick(r_omg, trc, x["cdk"], rfdk, linfree=linfree)
+    got = flx.cpu().numpy()
+    nbad = int((got != ref).sum())
+    assert nbad == 0, (km, linfree, nbad, float(np.abs(got - ref).max()))
+
+
+def test_quick_padded_and_errors(gpu):
+    import torch
+    import dsht
+    import oracle
+    N, I, J, km = 42, 128, 64, 20
+    p = dsht.Plan(N, I, J, radius=6.371e6)
+    rng = np.random.default_rng(1)
+    omg = rng.standard_normal((km + 1, J, I))
+    trc = rng.standard_normal((km, J, I))
+    c = rng.uniform(0.2, 1.0, km)
+    cdk, rfdk = c / c.sum(), rng.uniform(1.0, 40.0, km + 1)
+    out = _padded_out(km + 1, J, I, 24, gpu)
+    dc, dr = _dev(cdk, gpu), _dev(rfdk, gpu)
+    p.col_quick_d(_dev(omg, gpu, 8), _dev(trc, gpu, 56), dc, dr, out=out)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), oracle.col_quick(omg, trc, cdk, rfdk))
+    pad = out.as_strided((km + 1, 24), (out.stride(0), 1), J * I).cpu().numpy()
+    assert np.isnan(pad).all()
+    lib = dsht.load()
+    tp = dsht._tptr
+    dom, dtr, dfl = _dev(omg, gpu), _dev(trc, gpu), torch.empty((km + 1, J, I), dtype=torch.float64, device=gpu)
+    cases = [
+        lib.dsht_col_quick_d(p.h, tp(dom), I * J, tp(dtr), I * J, 1, tp(dc), tp(dr), 0, tp(dfl), I * J, None),
+        lib.dsht_col_quick_d(p.h, tp(dom), I * J, tp(dtr), I * J, km, tp(dc), None, 0, tp(dfl), I * J, None),
+        lib.dsht_col_quick_d(p.h, tp(dom), I * J - 1, tp(dtr), I * J, km, tp(dc), tp(dr), 0, tp(dfl), I * J, None),
+        lib.dsht_col_quick_d(p.h, None, I * J, tp(dtr), I * J, km, tp(dc), tp(dr), 0, tp(dfl), I * J, None),
+    ]
+    assert all(rc != 0 for rc in cases), cases
+    with pytest.raises(ValueError):
+        p.col_quick_d(dom[:km], dtr, dc, dr)

@@ -1,6 +1,8 @@
 """Pins the oracle's vertical column diagnostics (oracle/dsht_oracle.c: orc_col_*), the
 checker of HBEDiagnose_OMG, _HydPres and _UVBarot
-(model/dogcm/src/dynamics/hspm_vfvm/HBEDiagnose_hspm_vfvm_mod.f90:90-143, 221-255, 290-325).
+(model/dogcm/src/dynamics/hspm_vfvm/HBEDiagnose_hspm_vfvm_mod.f90:90-143, 221-255, 290-325)
+and of the QUICK vertical advective flux (HBEBaroc_hspm_vfvm_mod.f90:233-317, pinned by the
+polynomial closed forms further down).
 
 Each check is one that a plausible mistake fails:
   - closed forms on uniform layers: omega falls linearly to 0 at the bottom half level,
@@ -108,3 +110,65 @@ def test_planted_mistake_is_caught():
     good = O.col_omega(div, h, cdk)
     assert not np.all(wrong[km] == 0.0)
     assert not np.allclose(wrong, good)
+
+
+# ---------------------------------------------------------------- QUICK vertical flux
+# calc_ADVFlxZ_QUICK (HBEBaroc_hspm_vfvm_mod.f90:233-317).  On uniform layers (cdk = 1/km,
+# reciprocal spacing rfdk = km) the reference's curvature term of layer l is
+# 2 km^2 (T[l-1] - 2 T[l] + T[l+1]) and dfac = 1/(8 km^2), which fixes the flux of
+# polynomial profiles in closed form:
+#   constant or linear T: flux = omg (T[h-1] + T[h]) / 2 (the curvature term vanishes);
+#   T = l^2: flux = omg ((T[h-1] + T[h])/2 - 1);
+#   T = l^3: flux = omg ((T[h-1] + T[h])/2 - 1.5 (2h - 1)) - 1.5 |omg|  (the upwind term).
+# Boundaries: flux[0] = 0 (omg[0] T[0] with a linear free surface), flux[km] = 0, and the
+# centred interpolation at h = 1 and h = km - 1.
+
+def _uniform(km):
+    return np.full(km, 1.0 / km), np.full(km + 1, float(km))
+
+
+@pytest.mark.parametrize("km", [5, 8, 20, 60])
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_quick_polynomial_profiles(km, sign):
+    npts = 7
+    rng = np.random.default_rng(300 + km)
+    omg = sign * rng.uniform(0.5, 2.0, (km + 1, npts))
+    cdk, rfdk = _uniform(km)
+    l = np.arange(km, dtype=np.float64)[:, None] * np.ones((1, npts))
+    h = np.arange(1, km)[:, None]
+    for T, corr in ((np.full((km, npts), 3.0), None), (2.0 - 0.5 * l, None), (l ** 2, "quad"), (l ** 3, "cubic")):
+        f = O.col_quick(omg, T, cdk, rfdk)
+        mid = 0.5 * (T[:-1] + T[1:])  # half levels 1..km-1
+        ref = omg[1:km] * mid
+        if corr == "quad":
+            ref[1:-1] = omg[2:km - 1] * (mid[1:-1] - 1.0)
+        elif corr == "cubic":
+            hh = h[1:-1]
+            ref[1:-1] = omg[2:km - 1] * (mid[1:-1] - 1.5 * (2 * hh - 1)) - 1.5 * np.abs(omg[2:km - 1])
+        scale = np.abs(T).max() * np.abs(omg).max()
+        assert np.abs(f[1:km] - ref).max() <= 1e-12 * scale, (km, corr)
+        assert np.all(f[0] == 0.0) and np.all(f[km] == 0.0)
+    # linear free surface: the top half level carries omg[0] T[0]
+    T = rng.standard_normal((km, npts))
+    f = O.col_quick(omg, T, cdk, rfdk, linfree=True)
+    assert np.array_equal(f[0], omg[0] * T[0])
+
+
+@pytest.mark.parametrize("km", [2, 3, 4])
+def test_quick_short_columns(km):
+    """Columns too short for the interior formula: only the centred interpolation."""
+    npts = 4
+    rng = np.random.default_rng(km)
+    omg = rng.standard_normal((km + 1, npts))
+    T = rng.standard_normal((km, npts))
+    c = rng.uniform(0.2, 1.0, km)
+    cdk = c / c.sum()
+    rfdk = rng.uniform(1.0, 5.0, km + 1)
+    f = O.col_quick(omg, T, cdk, rfdk)
+    for hh in range(1, km):
+        if 2 <= hh <= km - 2:
+            continue
+        m1 = cdk[hh] / (cdk[hh - 1] + cdk[hh])
+        m2 = cdk[hh - 1] / (cdk[hh - 1] + cdk[hh])
+        np.testing.assert_allclose(f[hh], omg[hh] * (m1 * T[hh - 1] + m2 * T[hh]), rtol=1e-15, atol=1e-15)
+    assert np.all(f[0] == 0.0) and np.all(f[km] == 0.0)
