@@ -401,8 +401,9 @@ def column_bench(torch, dsht, reps=21, nsets=3):
         "hydpres": (lambda b: plan.col_hydpres_d(b["dens"], b["h"], cdk, 9.8, out=b["hp"]), 3 * KM * npts * 8),
         "uvbarot": (lambda b: plan.col_uvbarot_d(b["u"], b["v"], b["h"], cdk, b["ub"], b["vb"]),
                     (3 * KM + 2) * npts * 8),
+        # the flux reads the tracer layers and omega at half levels 1..km-1 (no free surface)
         "quick": (lambda b: plan.col_quick_d(b["omg"], b["dens"], cdk, rfdk, out=b["flx"]),
-                  (2 * KM + 1 + KM + 1) * npts * 8),
+                  (KM + (KM - 1) + (KM + 1)) * npts * 8),
     }
     out = {"workload": f"T{NMAX}L{KM} grids ({IM} x {JM} points, {KM} layers), FP64; calls rotate over "
                        f"{nsets} buffer sets"}
