@@ -500,7 +500,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: the transform path has no "
-                                   "multi-GPU form here (see DESIGN.md, sharding)", "n_gpus": args.gpus}))
+                                   "multi-GPU form here; layers, fields and instances are independent columns, so "
+                                   "it would shard by column blocks, one process per GPU", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch
