@@ -393,9 +393,14 @@ def column_bench(torch, dsht, reps=21, nsets=3):
                      "omg": torch.empty((KM + 1, JM, IM), dtype=torch.float64, device="cuda"),
                      "hp": torch.empty(shape, dtype=torch.float64, device="cuda"),
                      "flx": torch.empty((KM + 1, JM, IM), dtype=torch.float64, device="cuda"),
+                     "kv": rnd(1e-4, 1e-3), "e3": rnd(200.0, 4000.0),
+                     "qa": torch.empty(shape, dtype=torch.float64, device="cuda"),
                      "ub": torch.empty((JM, IM), dtype=torch.float64, device="cuda"),
                      "vb": torch.empty((JM, IM), dtype=torch.float64, device="cuda")})
     npts = IM * JM
+    fdk = torch.full((KM + 1,), 1.0 / KM, dtype=torch.float64, device="cuda")
+    fdk[0] = fdk[KM] = 0.5 / KM
+    work = torch.empty(3 * KM * npts, dtype=torch.float64, device="cuda")
     calls = {
         "omega": (lambda b: plan.col_omega_d(b["div"], b["h"], cdk, out=b["omg"]), (2 * KM + KM + 1) * npts * 8),
         "hydpres": (lambda b: plan.col_hydpres_d(b["dens"], b["h"], cdk, 9.8, out=b["hp"]), 3 * KM * npts * 8),
@@ -404,6 +409,10 @@ def column_bench(torch, dsht, reps=21, nsets=3):
         # the flux reads the tracer layers and omega at half levels 1..km-1 (no free surface)
         "quick": (lambda b: plan.col_quick_d(b["omg"], b["dens"], cdk, rfdk, out=b["flx"]),
                   (KM + (KM - 1) + (KM + 1)) * npts * 8),
+        # minimum: q0, rhs, kv, e3 read and qa written once (the elimination's rows go
+        # through `work` and qa in addition)
+        "vdiff": (lambda b: plan.col_vdiff_d(b["u"], b["v"], b["kv"], b["e3"], cdk, fdk, 3600.0, 1.0, "D", "N",
+                                             out=b["qa"], work=work), 5 * KM * npts * 8),
     }
     out = {"workload": f"T{NMAX}L{KM} grids ({IM} x {JM} points, {KM} layers), FP64; calls rotate over "
                        f"{nsets} buffer sets"}
@@ -591,7 +600,7 @@ def main():
     print("[bench] column diagnostics", file=sys.stderr, flush=True)
     column = column_bench(torch, dsht)
     print("[bench] column: " + ", ".join(f"{k} {column[k]['avg_us']:.1f} us {column[k]['GBps']:.0f} GB/s"
-                                         for k in ("omega", "hydpres", "uvbarot", "quick")), file=sys.stderr, flush=True)
+                                         for k in ("omega", "hydpres", "uvbarot", "quick", "vdiff")), file=sys.stderr, flush=True)
     batched = serving = None
     if not args.no_batch:
         print("[bench] batched form (64 instances)", file=sys.stderr, flush=True)

@@ -773,6 +773,21 @@ int dsht_col_quick_d(dsht_plan* p, const double* omg, long ldo, const double* tr
                           (hipStream_t)stream);
 }
 
+int dsht_col_vdiff_d(dsht_plan* p, const double* q0, const double* rhs, const double* kv, const double* e3, long ldg,
+                     int km, const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc, int btm_bc,
+                     double* qa, long ldo, double* work, long lwork, void* stream) {
+  if (int rc = check_col(p, km, cdk, ldg)) return rc;
+  if (km < 2) return fail(DSHT_EARG, "the vertical diffusion solve needs km >= 2");
+  if (!q0 || !rhs || !kv || !e3 || !qa || !fdk || !work) return fail(DSHT_EARG, "pointer is NULL");
+  if (ldo < (long)p->I * p->J) return fail(DSHT_EARG, "output column pitch %ld must be >= I*J", ldo);
+  if ((sfc_bc != 'D' && sfc_bc != 'N') || (btm_bc != 'D' && btm_bc != 'N'))
+    return fail(DSHT_EARG, "boundary types must be 'D' or 'N'");
+  const long need = 3l * km * p->I * p->J;
+  if (lwork < need) return fail(DSHT_EARG, "workspace of %ld doubles is below 3*km*I*J = %ld", lwork, need);
+  return launch_col_vdiff(q0, rhs, kv, e3, ldg, km, cdk, fdk, dt, alpha, sfc_bc == 'D', btm_bc == 'D', qa, ldo, work,
+                          p->I * p->J, (hipStream_t)stream);
+}
+
 int dsht_coef_d(dsht_plan* p, int which, const double* params, int nparams, double* out, void* stream) {
   if (int rc = check_plan(p)) return rc;
   if (which < DSHT_COEF_LAPLA || which > DSHT_COEF_BEDIFF) return fail(DSHT_EARG, "unknown coefficient table %d", which);

@@ -4,6 +4,7 @@
 //   HBEDiagnose_UVBarot (same file:221-255)
 //   HBEDiagnose_HydPres (same file:290-325)
 //   calc_ADVFlxZ_QUICK  (model/dogcm/src/dynamics/hspm_vfvm/HBEBaroc_hspm_vfvm_mod.f90:233-317)
+//   calc_VDiffTRCEq     (model/dogcm/src/physics/DOGCM_Phys_hspm_vfvm_mod.f90:752-861)
 //
 // Layout: the km layers of one field are km consecutive device grid columns (column pitch
 // ldg doubles, layer 0 on top = the reference's KS), each an I x J grid with longitude
@@ -228,6 +229,152 @@ __global__ __launch_bounds__(256) void k_col_quick(const double* __restrict__ om
   nts(F + (long)km * ldf, 0.0);
 }
 
+// Implicit vertical diffusion of a tracer increment, one column per thread
+// (calc_VDiffTRCEq, DOGCM_Phys_hspm_vfvm_mod.f90:800-858): the tridiagonal system
+//   Al x[k-1] + Ad x[k] + Au x[k+1] = dt rhs[k] / e3[k],   qa[k] = q0[k] + x[k]
+// with coef = dt/(e3 cdk), the half-level coefficients r (:809-823)
+//   r[0] = kv[0]/(fdk[0] e3[0]),  r[km] = kv[km-1]/(fdk[km] e3[km-1]),
+//   r[h] = (m1 kv[h-1] + m2 kv[h]) / (fdk[h] (m1 e3[h-1] + m2 e3[h])),  0 < h < km,
+// rows Al = -alpha coef r[k], Ad = 1 + alpha coef (r[k] + r[k+1]), Au = -alpha coef r[k+1]
+// (:825-830), and Dirichlet ('D', the boundary r doubled) or Neumann ('N', dropped) rows at
+// the surface and the bottom (:836-852).  The solve is LAPACK DGTSV (Gaussian elimination
+// with partial pivoting by row interchange, then back substitution; LAPACK 3.x
+// SRC/dgtsv.f, the NRHS = 1 path), in its order of operations.  The forward sweep leaves
+// each finished row's d, du and second superdiagonal in `work` (3 km values per point,
+// pitch npts) and its right-hand side in qa; the backward sweep reads them back in
+// reverse and writes qa = q0 + x.  A singular pivot (impossible with kv >= 0 and positive
+// thicknesses) is not reported.
+struct VdiffRow {
+  double al, ad, au, b;
+};
+
+__global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0, const double* __restrict__ rhs,
+                                                   const double* __restrict__ kv, const double* __restrict__ e3,
+                                                   long ldg, int km, const double* __restrict__ cdk,
+                                                   const double* __restrict__ fdk, double dt, double alpha,
+                                                   int sfc_dirichlet, int btm_dirichlet, double* __restrict__ qa,
+                                                   long ldo, double* __restrict__ work, int npts) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= npts) return;
+  const long nw = npts;
+  double* wd = work + i;                   // d of finished rows
+  double* wu = work + (long)km * nw + i;   // du
+  double* w2 = work + 2l * km * nw + i;    // second superdiagonal
+  // half-level coefficient r[h] from the layers above (h-1) and below (h)
+  auto rhalf = [&](int h, double kva, double e3a, double kvb, double e3b) {
+    const double s = cdk[h - 1] + cdk[h];
+    const double m1 = cdk[h] / s, m2 = cdk[h - 1] / s;
+    const double num = m1 * kva + m2 * kvb;
+    const double den = fdk[h] * (m1 * e3a + m2 * e3b);
+    return num / den;
+  };
+  // layer k's values and its lower half-level coefficient: carried one layer ahead
+  double kv_c = ntl(kv + i), e3_c = ntl(e3 + i);
+  double r_up = kv_c / (fdk[0] * e3_c);  // r[0]
+  auto row = [&](int k, double r_top, double r_bot, double e3k, double rk) {
+    VdiffRow o;
+    const double coef = dt / (e3k * cdk[k]);
+    const double ac = alpha * coef;
+    const double bt = dt * rk;
+    o.b = bt / e3k;
+    if (k == 0) {
+      o.al = 0.0;
+      o.ad = sfc_dirichlet ? 1.0 + ac * (r_bot + 2.0 * r_top) : 1.0 + ac * r_bot;
+      o.au = -(ac * r_bot);
+    } else if (k == km - 1) {
+      o.al = -(ac * r_top);
+      o.ad = btm_dirichlet ? 1.0 + ac * (2.0 * r_bot + r_top) : 1.0 + ac * r_top;
+      o.au = 0.0;
+    } else {
+      o.al = -(ac * r_top);
+      o.ad = 1.0 + ac * (r_top + r_bot);
+      o.au = -(ac * r_bot);
+    }
+    return o;
+  };
+  // builds row k (needs layer k+1 for the lower half-level coefficient)
+  auto next_row = [&](int k) {
+    const double rk = ntl(rhs + (long)k * ldg + i);
+    double r_bot;
+    if (k + 1 < km) {
+      const double kv_n = ntl(kv + (long)(k + 1) * ldg + i), e3_n = ntl(e3 + (long)(k + 1) * ldg + i);
+      r_bot = rhalf(k + 1, kv_c, e3_c, kv_n, e3_n);
+      const VdiffRow o = row(k, r_up, r_bot, e3_c, rk);
+      kv_c = kv_n;
+      e3_c = e3_n;
+      r_up = r_bot;
+      return o;
+    }
+    r_bot = kv_c / (fdk[km] * e3_c);
+    return row(k, r_up, r_bot, e3_c, rk);
+  };
+
+  // ---- forward elimination (DGTSV): row j current (d, du, b), row j+1 pending (dl = its Al)
+  VdiffRow r0 = next_row(0);
+  double d = r0.ad, du = r0.au, b = r0.b;
+  VdiffRow r1 = next_row(1);
+  double dl = r1.al, dn = r1.ad, dun = r1.au, bn = r1.b;
+  for (int j = 0; j <= km - 3; j++) {
+    const VdiffRow r2 = next_row(j + 2);  // row j+2, whose Al is dl of the next step
+    double du2;
+    if (fabs(d) >= fabs(dl)) {
+      const double fact = dl / d;
+      dn = dn - fact * du;
+      bn = bn - fact * b;
+      du2 = 0.0;
+    } else {
+      const double fact = d / dl;
+      d = dl;
+      const double temp = dn;
+      dn = du - fact * temp;
+      du2 = dun;
+      dun = -fact * du2;
+      du = temp;
+      const double tb = b;
+      b = bn;
+      bn = tb - fact * bn;
+    }
+    nts(wd + (long)j * nw, d);
+    nts(wu + (long)j * nw, du);
+    nts(w2 + (long)j * nw, du2);
+    nts(qa + (long)j * ldo + i, b);
+    d = dn, du = dun, b = bn;
+    dl = r2.al, dn = r2.ad, dun = r2.au, bn = r2.b;
+  }
+  {  // last elimination step, rows km-2 and km-1
+    if (fabs(d) >= fabs(dl)) {
+      const double fact = dl / d;
+      dn = dn - fact * du;
+      bn = bn - fact * b;
+    } else {
+      const double fact = d / dl;
+      d = dl;
+      const double temp = dn;
+      dn = du - fact * temp;
+      du = temp;
+      const double tb = b;
+      b = bn;
+      bn = tb - fact * bn;
+    }
+  }
+  // ---- back substitution
+  const double x1 = bn / dn;  // row km-1
+  const double x0 = (b - du * x1) / d;  // row km-2
+  nts(qa + (long)(km - 1) * ldo + i, ntl(q0 + (long)(km - 1) * ldg + i) + x1);
+  nts(qa + (long)(km - 2) * ldo + i, ntl(q0 + (long)(km - 2) * ldg + i) + x0);
+  double xa = x0, xb = x1;  // x[j+1], x[j+2]
+  for (int j = km - 3; j >= 0; j--) {
+    const double dj = ntl(wd + (long)j * nw), uj = ntl(wu + (long)j * nw), vj = ntl(w2 + (long)j * nw);
+    const double bj = ntl(qa + (long)j * ldo + i);
+    const double t = bj - uj * xa;
+    const double x = (t - vj * xb) / dj;
+    nts(qa + (long)j * ldo + i, ntl(q0 + (long)j * ldg + i) + x);
+    xb = xa;
+    xa = x;
+  }
+}
+
 int col_grid(int npts) { return (npts + 255) / 256; }
 
 }  // namespace
@@ -260,6 +407,15 @@ int launch_col_quick(const double* omg, long ldo, const double* trc, long ldg, i
   if (lds > 64 * 1024) return fail(DSHT_EARG, "the QUICK flux supports at most 2729 layers");
   hipLaunchKernelGGL(k_col_quick, dim3(col_grid(npts)), dim3(256), lds, s, omg, ldo, trc, ldg, km, cdk, rfdk,
                      linfree, flx, ldf, npts);
+  DSHT_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_col_vdiff(const double* q0, const double* rhs, const double* kv, const double* e3, long ldg, int km,
+                     const double* cdk, const double* fdk, double dt, double alpha, int sfc_dirichlet,
+                     int btm_dirichlet, double* qa, long ldo, double* work, int npts, hipStream_t s) {
+  hipLaunchKernelGGL(k_col_vdiff, dim3(col_grid(npts)), dim3(256), 0, s, q0, rhs, kv, e3, ldg, km, cdk, fdk, dt, alpha,
+                     sfc_dirichlet, btm_dirichlet, qa, ldo, work, npts);
   DSHT_HIP(hipGetLastError());
   return 0;
 }

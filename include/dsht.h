@@ -188,6 +188,16 @@ int dsht_col_uvbarot_d(dsht_plan* plan, const double* u, const double* v, const 
 int dsht_col_quick_d(dsht_plan* plan, const double* omg, long ldo, const double* trc, long ldg,
                      int km, const double* cdk, const double* rfdk, int linfree, double* flx,
                      long ldf, void* stream);
+/* calc_VDiffTRCEq (DOGCM_Phys_hspm_vfvm_mod.f90:752-861): implicit vertical diffusion of a
+ * tracer.  Per grid point, solves the tridiagonal system of the reference (coefficients from
+ * kv, e3 = xyz_H, z_CDK and the km+1 half-level spacings fdk = z_FDK(KS-1:KE); right-hand
+ * side dt*rhs/e3; 'D' or 'N' surface and bottom rows) with LAPACK DGTSV's elimination and
+ * back substitution, and writes qa = q0 + x (km layers, pitch ldo).  work: device scratch
+ * of lwork >= 3*km*I*J doubles (contents undefined on return).  km >= 2.              */
+int dsht_col_vdiff_d(dsht_plan* plan, const double* q0, const double* rhs, const double* kv,
+                     const double* e3, long ldg, int km, const double* cdk, const double* fdk,
+                     double dt, double alpha, int sfc_bc, int btm_bc, double* qa, long ldo,
+                     double* work, long lwork, void* stream);
 /* coefficient table written to device memory */
 int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, double* out,
                 void* stream);

@@ -720,3 +720,117 @@ ORC_NOFMA void orc_col_quick(const double* omg, const double* trc, int km, const
   free(dfac);
   free(diff);
 }
+
+/* LAPACK DGTSV (LAPACK 3.x, SRC/dgtsv.f), the NRHS = 1 path: Gaussian elimination with
+ * partial pivoting by row interchange (the interchange fills the second superdiagonal, kept
+ * in dl), then back substitution.  0-based restatement of the published routine. */
+ORC_NOFMA int orc_dgtsv(int n, double* dl, double* d, double* du, double* b) {
+  int swaps = 0;
+  for (int i = 0; i <= n - 3; i++) {
+    if (fabs(d[i]) >= fabs(dl[i])) {
+      if (d[i] == 0.0) return -(i + 1);
+      const double fact = dl[i] / d[i];
+      d[i + 1] = d[i + 1] - fact * du[i];
+      b[i + 1] = b[i + 1] - fact * b[i];
+      dl[i] = 0.0;
+    } else {
+      const double fact = d[i] / dl[i];
+      d[i] = dl[i];
+      double temp = d[i + 1];
+      d[i + 1] = du[i] - fact * temp;
+      dl[i] = du[i + 1];
+      du[i + 1] = -fact * dl[i];
+      du[i] = temp;
+      temp = b[i];
+      b[i] = b[i + 1];
+      b[i + 1] = temp - fact * b[i + 1];
+      swaps++;
+    }
+  }
+  if (n > 1) {
+    const int i = n - 2;
+    if (fabs(d[i]) >= fabs(dl[i])) {
+      if (d[i] == 0.0) return -(i + 1);
+      const double fact = dl[i] / d[i];
+      d[i + 1] = d[i + 1] - fact * du[i];
+      b[i + 1] = b[i + 1] - fact * b[i];
+    } else {
+      const double fact = d[i] / dl[i];
+      d[i] = dl[i];
+      const double temp = d[i + 1];
+      d[i + 1] = du[i] - fact * temp;
+      du[i] = temp;
+      const double tb = b[i];
+      b[i] = b[i + 1];
+      b[i + 1] = tb - fact * b[i + 1];
+      swaps++;
+    }
+  }
+  if (d[n - 1] == 0.0) return -n;
+  b[n - 1] = b[n - 1] / d[n - 1];
+  if (n > 1) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / d[n - 2];
+  for (int i = n - 3; i >= 0; i--) b[i] = (b[i] - du[i] * b[i + 1] - dl[i] * b[i + 2]) / d[i];
+  return swaps;
+}
+
+/* calc_VDiffTRCEq, DOGCM_Phys_hspm_vfvm_mod.f90:800-858, per grid point in the reference's
+ * order: coef (:809), half-level coefficients r (:811-823), interior rows (:825-830),
+ * right-hand side (:831), boundary rows (:836-852), DGTSV (:855), q0 + x (:857).  Layer k
+ * (0-based) is the reference's KS+k; half level h is KS-1+h. */
+ORC_NOFMA long orc_col_vdiff(const double* q0, const double* rhs, const double* kv, const double* e3, int km,
+                             const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc,
+                             int btm_bc, double* qa, long npts) {
+  double* coef = malloc(sizeof(double) * (size_t)km);
+  double* r = malloc(sizeof(double) * (size_t)(km + 1));
+  double* al = malloc(sizeof(double) * (size_t)km);
+  double* ad = malloc(sizeof(double) * (size_t)km);
+  double* au = malloc(sizeof(double) * (size_t)km);
+  double* b = malloc(sizeof(double) * (size_t)km);
+  long swaps = 0;
+  for (long i = 0; i < npts; i++) {
+#define KV(k) kv[(long)(k) * npts + i]
+#define E3(k) e3[(long)(k) * npts + i]
+    for (int k = 0; k < km; k++) coef[k] = dt / (E3(k) * cdk[k]);
+    for (int k = 0; k < km - 1; k++) { /* reference k = KS..KE-1: half level k+1 */
+      const double m1 = cdk[k + 1] / (cdk[k] + cdk[k + 1]);
+      const double m2 = cdk[k] / (cdk[k] + cdk[k + 1]);
+      r[k + 1] = (m1 * KV(k) + m2 * KV(k + 1)) / (fdk[k + 1] * (m1 * E3(k) + m2 * E3(k + 1)));
+    }
+    r[0] = KV(0) / (fdk[0] * E3(0));
+    r[km] = KV(km - 1) / (fdk[km] * E3(km - 1));
+    for (int k = 1; k <= km - 2; k++) { /* r(k-1) -> r[k], r(k) -> r[k+1] */
+      al[k] = -(alpha * coef[k] * r[k]);
+      ad[k] = 1.0 + alpha * coef[k] * (r[k] + r[k + 1]);
+      au[k] = -(alpha * coef[k] * r[k + 1]);
+    }
+    for (int k = 0; k < km; k++) b[k] = dt * rhs[(long)k * npts + i] / E3(k);
+    if (sfc_bc == 'D') {
+      ad[0] = 1.0 + alpha * coef[0] * (r[1] + 2.0 * r[0]);
+      au[0] = -(alpha * coef[0] * r[1]);
+    } else {
+      ad[0] = 1.0 + alpha * coef[0] * (r[1]);
+      au[0] = -(alpha * coef[0] * r[1]);
+    }
+    const int n = km - 1;
+    if (btm_bc == 'D') {
+      al[n] = -(alpha * coef[n] * r[n]);
+      ad[n] = 1.0 + alpha * coef[n] * (2.0 * r[n + 1] + r[n]);
+    } else {
+      al[n] = -(alpha * coef[n] * r[n]);
+      ad[n] = 1.0 + alpha * coef[n] * (+r[n]);
+    }
+    /* DGTSV(N, 1, Al(KS+1:KE), Ad(KS:KE), Au(KS:KE-1), b, N, info) */
+    const int sw = orc_dgtsv(km, al + 1, ad, au, b);
+    if (sw > 0) swaps += sw;
+    for (int k = 0; k < km; k++) qa[(long)k * npts + i] = q0[(long)k * npts + i] + b[k];
+#undef KV
+#undef E3
+  }
+  free(coef);
+  free(r);
+  free(al);
+  free(ad);
+  free(au);
+  free(b);
+  return swaps;
+}

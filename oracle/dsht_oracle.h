@@ -99,6 +99,18 @@ void orc_col_uvbarot(const double* u, const double* v, const double* h, int km, 
  * boundary at the top.  km >= 2. */
 void orc_col_quick(const double* omg, const double* trc, int km, const double* cdk, const double* rfdk,
                    int linfree, double* flx, long npts);
+/* LAPACK DGTSV, NRHS = 1 (LAPACK 3.x SRC/dgtsv.f): solves the tridiagonal system with
+ * subdiagonal dl[n-1], diagonal d[n], superdiagonal du[n-1] in place (b <- x; dl, d, du
+ * overwritten as DGTSV does).  Returns the number of row interchanges, or -(i+1) for a
+ * zero pivot at row i (DGTSV's INFO). */
+int orc_dgtsv(int n, double* dl, double* d, double* du, double* b);
+/* calc_VDiffTRCEq (model/dogcm/src/physics/DOGCM_Phys_hspm_vfvm_mod.f90:752-861): qa = q0 + x
+ * per grid point, x from the reference's tridiagonal system solved by orc_dgtsv; fdk holds the
+ * km+1 half-level spacings, sfc_bc / btm_bc are 'D' or 'N'.  Returns the total number of
+ * row interchanges over all points. */
+long orc_col_vdiff(const double* q0, const double* rhs, const double* kv, const double* e3, int km,
+                   const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc,
+                   int btm_bc, double* qa, long npts);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,10 @@ def lib():
         L.orc_col_hydpres.argtypes = [dp, dp, i, dp, d, dp, lg]
         L.orc_col_uvbarot.argtypes = [dp, dp, dp, i, dp, dp, dp, lg]
         L.orc_col_quick.argtypes = [dp, dp, i, dp, dp, i, dp, lg]
+        L.orc_dgtsv.argtypes = [i, dp, dp, dp, dp]
+        L.orc_dgtsv.restype = i
+        L.orc_col_vdiff.argtypes = [dp, dp, dp, dp, i, dp, dp, d, d, i, i, dp, lg]
+        L.orc_col_vdiff.restype = lg
         _lib = L
     return _lib
 
@@ -306,3 +310,22 @@ def col_quick(omg, trc, cdk, rfdk, linfree=False):
     flx = np.empty_like(omg)
     lib().orc_col_quick(_p(omg), _p(trc), km, _p(cdk), _p(rfdk), int(bool(linfree)), _p(flx), npts)
     return flx
+
+
+def dgtsv(dl, d, du, b):
+    """LAPACK DGTSV restated (NRHS = 1): (x, number of row interchanges)."""
+    dl, d, du, b = (np.array(a, dtype=np.float64, copy=True) for a in (dl, d, du, b))
+    sw = lib().orc_dgtsv(d.size, _p(dl), _p(d), _p(du), _p(b))
+    if sw < 0:
+        raise ZeroDivisionError(f"zero pivot at row {-sw - 1}")
+    return b, sw
+
+
+def col_vdiff(q0, rhs, kv, e3, cdk, fdk, dt, alpha, sfc_bc="D", btm_bc="N"):
+    """calc_VDiffTRCEq: (q0 + the implicit vertical diffusion increment, row interchanges)."""
+    (q0, rhs, kv, e3), km, npts = _col(q0, rhs, kv, e3)
+    cdk, fdk = _c(cdk), _c(fdk)
+    qa = np.empty_like(q0)
+    sw = lib().orc_col_vdiff(_p(q0), _p(rhs), _p(kv), _p(e3), km, _p(cdk), _p(fdk), float(dt), float(alpha),
+                             ord(sfc_bc), ord(btm_bc), _p(qa), npts)
+    return qa, int(sw)

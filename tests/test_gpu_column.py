@@ -183,3 +183,73 @@ def test_quick_padded_and_errors(gpu):
     assert all(rc != 0 for rc in cases), cases
     with pytest.raises(ValueError):
         p.col_quick_d(dom[:km], dtr, dc, dr)
+
+
+def _vdiff_case(km, J, I, seed, pivot):
+    """Physical inputs; with pivot, thin layers under thick ones and a large diffusivity make
+    DGTSV interchange rows."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.2, 1.0, km)
+    if pivot:
+        c[1::2] *= 1e-3
+    cdk = c / c.sum()
+    fdk = np.concatenate([[0.5 * cdk[0]], 0.5 * (cdk[:-1] + cdk[1:]), [0.5 * cdk[-1]]])
+    kv = rng.uniform(1e-4, 1e-2, (km, J, I)) * (30.0 if pivot else 1.0)
+    e3 = 4000.0 * (1.0 + 0.05 * rng.standard_normal((km, J, I)))
+    q0 = 10.0 + rng.standard_normal((km, J, I))
+    rhs = rng.standard_normal((km, J, I)) * 1e-5
+    return q0, rhs, kv, e3, cdk, fdk
+
+
+@pytest.mark.parametrize("N,I,J,km", [(42, 128, 64, 20), (85, 256, 128, 60), (170, 512, 256, 60),
+                                      (21, 64, 32, 2), (21, 64, 32, 3), (21, 64, 32, 4)])
+@pytest.mark.parametrize("pivot", [False, True])
+def test_vdiff_bitwise(gpu, N, I, J, km, pivot):
+    """calc_VDiffTRCEq with DGTSV's elimination, bit for bit against the oracle, with and
+    without row interchanges."""
+    import torch
+    import dsht
+    import oracle
+    p = dsht.Plan(N, I, J, radius=6.371e6)
+    q0, rhs, kv, e3, cdk, fdk = _vdiff_case(km, J, I, N + km, pivot)
+    dt, alpha = 3600.0, 1.0
+    out = p.col_vdiff_d(_dev(q0, gpu), _dev(rhs, gpu), _dev(kv, gpu), _dev(e3, gpu), _dev(cdk, gpu), _dev(fdk, gpu),
+                        dt, alpha, "D", "N")
+    torch.cuda.synchronize()
+    ref, swaps = oracle.col_vdiff(q0, rhs, kv, e3, cdk, fdk, dt, alpha, "D", "N")
+    if pivot and km >= 20:  # enough thin layers for row interchanges (short columns may have none)
+        assert swaps > 0
+    got = out.cpu().numpy()
+    nbad = int((got != ref).sum())
+    assert nbad == 0, (km, pivot, nbad, float(np.abs(got - ref).max()))
+
+
+@pytest.mark.parametrize("bcs", [("N", "N"), ("D", "D"), ("N", "D")])
+def test_vdiff_boundaries_pitch_and_errors(gpu, bcs):
+    import torch
+    import dsht
+    import oracle
+    N, I, J, km = 21, 64, 32, 9
+    p = dsht.Plan(N, I, J, radius=6.371e6)
+    q0, rhs, kv, e3, cdk, fdk = _vdiff_case(km, J, I, 77, True)
+    dq0, drhs, dkv, de3 = (_dev(a, gpu, 16) for a in (q0, rhs, kv, e3))
+    dc, df = _dev(cdk, gpu), _dev(fdk, gpu)
+    out = _padded_out(km, J, I, 32, gpu)
+    work = torch.full((3 * km * J * I + 5,), np.nan, dtype=torch.float64, device=gpu)
+    p.col_vdiff_d(dq0, drhs, dkv, de3, dc, df, 600.0, 0.5, *bcs, out=out, work=work)
+    torch.cuda.synchronize()
+    ref, _ = oracle.col_vdiff(q0, rhs, kv, e3, cdk, fdk, 600.0, 0.5, *bcs)
+    assert np.array_equal(out.cpu().numpy(), ref)
+    pad = out.as_strided((km, 32), (out.stride(0), 1), J * I).cpu().numpy()
+    assert np.isnan(pad).all()
+    lib = dsht.load()
+    tp = dsht._tptr
+    d = ctypes.c_double
+    small = lib.dsht_col_vdiff_d(p.h, tp(dq0), tp(drhs), tp(dkv), tp(de3), dq0.stride(0), km, tp(dc), tp(df), d(1.0),
+                                 d(1.0), ord("D"), ord("N"), tp(out), out.stride(0), tp(work), 3 * km * J * I - 1,
+                                 None)
+    badbc = lib.dsht_col_vdiff_d(p.h, tp(dq0), tp(drhs), tp(dkv), tp(de3), dq0.stride(0), km, tp(dc), tp(df), d(1.0),
+                                 d(1.0), ord("X"), ord("N"), tp(out), out.stride(0), tp(work), work.numel(), None)
+    short = lib.dsht_col_vdiff_d(p.h, tp(dq0), tp(drhs), tp(dkv), tp(de3), dq0.stride(0), 1, tp(dc), tp(df), d(1.0),
+                                 d(1.0), ord("D"), ord("N"), tp(out), out.stride(0), tp(work), work.numel(), None)
+    assert small != 0 and badbc != 0 and short != 0

@@ -172,3 +172,76 @@ def test_quick_short_columns(km):
         m2 = cdk[hh - 1] / (cdk[hh - 1] + cdk[hh])
         np.testing.assert_allclose(f[hh], omg[hh] * (m1 * T[hh - 1] + m2 * T[hh]), rtol=1e-15, atol=1e-15)
     assert np.all(f[0] == 0.0) and np.all(f[km] == 0.0)
+
+
+# ---------------------------------------------------------------- implicit vertical diffusion
+# calc_VDiffTRCEq (DOGCM_Phys_hspm_vfvm_mod.f90:752-861) solves its tridiagonal systems with
+# LAPACK DGTSV, which is absent here; the oracle restates the published routine (LAPACK 3.x
+# SRC/dgtsv.f).  The restatement is pinned against scipy's LAPACK dgtsv, bit for bit, on
+# diagonally dominant systems and on systems that force row interchanges.  The assembly is
+# pinned by closed forms: with kv = 0 the matrix is the identity (qa = q0 + dt rhs/e3), and
+# multiplying row k by e3 cdk/dt and summing telescopes the flux terms, so
+#   sum_k e3 cdk x = dt sum_k cdk rhs - dt alpha (2 r0 x0 [surface 'D'] + 2 rN xN [bottom 'D'])
+# with r0 = kv0/(fdk0 e30) and rN = kv_{km-1}/(fdk_km e3_{km-1}); a wrong sign, index or
+# boundary factor breaks it.
+
+@pytest.mark.parametrize("n", [2, 3, 5, 60])
+@pytest.mark.parametrize("kind", ["dominant", "general"])
+def test_dgtsv_matches_lapack(n, kind):
+    from scipy.linalg import lapack
+    rng = np.random.default_rng(n + (0 if kind == "dominant" else 1000))
+    for _ in range(5):
+        dl, du = rng.standard_normal(n - 1), rng.standard_normal(n - 1)
+        d = rng.standard_normal(n)
+        if kind == "dominant":  # |d| > 7 >= |dl| + |du| + 1 after every elimination step
+            dl, du = np.clip(dl, -3, 3), np.clip(du, -3, 3)
+            d = np.sign(d) * (np.abs(d) + 7.0)
+        b = rng.standard_normal(n)
+        x, sw = O.dgtsv(dl, d, du, b)
+        _, _, _, xl, info = lapack.dgtsv(dl.copy(), d.copy(), du.copy(), b.copy())
+        assert info == 0
+        assert np.array_equal(x, xl), (n, kind, np.abs(x - xl).max())
+        if kind == "dominant":
+            assert sw == 0
+
+
+def _vdiff_inputs(km, npts, seed, pivot=False):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.2, 1.0, km)
+    if pivot:  # thin layers under thick ones: DGTSV interchanges rows
+        c[1::2] *= 1e-3
+    cdk = c / c.sum()
+    fdk = np.concatenate([[0.5 * cdk[0]], 0.5 * (cdk[:-1] + cdk[1:]), [0.5 * cdk[-1]]])
+    kv = rng.uniform(1e-4, 1e-2, (km, npts)) * (30.0 if pivot else 1.0)
+    e3 = 4000.0 * (1.0 + 0.05 * rng.standard_normal((km, npts)))
+    q0 = 10.0 + rng.standard_normal((km, npts))
+    rhs = rng.standard_normal((km, npts)) * 1e-5
+    return q0, rhs, kv, e3, cdk, fdk
+
+
+@pytest.mark.parametrize("km", [2, 3, 20, 60])
+@pytest.mark.parametrize("bcs", [("N", "N"), ("D", "N"), ("N", "D"), ("D", "D")])
+def test_vdiff_column_integral(km, bcs):
+    npts = 6
+    dt, alpha = 3600.0, 1.0
+    for pivot in (False, True):
+        q0, rhs, kv, e3, cdk, fdk = _vdiff_inputs(km, npts, 10 * km + len(bcs[0]), pivot)
+        qa, sw = O.col_vdiff(q0, rhs, kv, e3, cdk, fdk, dt, alpha, *bcs)
+        x = qa - q0
+        lhs = (e3 * cdk[:, None] * x).sum(0)
+        rhs_int = dt * (cdk[:, None] * rhs).sum(0)
+        # tolerance relative to the magnitude of every term (the telescoped interior fluxes
+        # cancel only to rounding of their own size)
+        scale = (e3 * cdk[:, None] * np.abs(x)).sum(0) + dt * (cdk[:, None] * np.abs(rhs)).sum(0)
+        for on, k, h in ((bcs[0] == "D", 0, 0), (bcs[1] == "D", -1, -1)):
+            if on:
+                t = dt * alpha * 2.0 * kv[k] / (fdk[h] * e3[k]) * x[k]
+                rhs_int -= t
+                scale += np.abs(t)
+        assert np.all(np.abs(lhs - rhs_int) <= 1e-8 * scale), (km, bcs, pivot)
+        if pivot and km >= 3:
+            assert sw > 0  # the row-interchange path ran
+    # no diffusivity: the identity system
+    q0, rhs, kv, e3, cdk, fdk = _vdiff_inputs(km, npts, 5)
+    qa, sw = O.col_vdiff(q0, rhs, np.zeros_like(kv), e3, cdk, fdk, dt, alpha, *bcs)
+    assert np.array_equal(qa, q0 + dt * rhs / e3) and sw == 0
