@@ -773,9 +773,9 @@ int dsht_col_quick_d(dsht_plan* p, const double* omg, long ldo, const double* tr
                           (hipStream_t)stream);
 }
 
-int dsht_col_vdiff_d(dsht_plan* p, const double* q0, const double* rhs, const double* kv, const double* e3, long ldg,
+static int col_vdiff(dsht_plan* p, const double* q0, const double* rhs, const double* kv, const double* e3, long ldg,
                      int km, const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc, int btm_bc,
-                     double* qa, long ldo, double* work, long lwork, void* stream) {
+                     const double* bfric, double* qa, long ldo, double* work, long lwork, void* stream) {
   if (int rc = check_col(p, km, cdk, ldg)) return rc;
   if (km < 2) return fail(DSHT_EARG, "the vertical diffusion solve needs km >= 2");
   if (!q0 || !rhs || !kv || !e3 || !qa || !fdk || !work) return fail(DSHT_EARG, "pointer is NULL");
@@ -784,8 +784,24 @@ int dsht_col_vdiff_d(dsht_plan* p, const double* q0, const double* rhs, const do
     return fail(DSHT_EARG, "boundary types must be 'D' or 'N'");
   const long need = 3l * km * p->I * p->J;
   if (lwork < need) return fail(DSHT_EARG, "workspace of %ld doubles is below 3*km*I*J = %ld", lwork, need);
-  return launch_col_vdiff(q0, rhs, kv, e3, ldg, km, cdk, fdk, dt, alpha, sfc_bc == 'D', btm_bc == 'D', qa, ldo, work,
-                          p->I * p->J, (hipStream_t)stream);
+  return launch_col_vdiff(q0, rhs, kv, e3, ldg, km, cdk, fdk, dt, alpha, sfc_bc == 'D', btm_bc == 'D', bfric, qa, ldo,
+                          work, p->I * p->J, (hipStream_t)stream);
+}
+
+int dsht_col_vdiff_d(dsht_plan* p, const double* q0, const double* rhs, const double* kv, const double* e3, long ldg,
+                     int km, const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc, int btm_bc,
+                     double* qa, long ldo, double* work, long lwork, void* stream) {
+  return col_vdiff(p, q0, rhs, kv, e3, ldg, km, cdk, fdk, dt, alpha, sfc_bc, btm_bc, nullptr, qa, ldo, work, lwork,
+                   stream);
+}
+
+int dsht_col_vdiff_mom_d(dsht_plan* p, const double* q0, const double* rhs, const double* kv, const double* e3,
+                         long ldg, int km, const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc,
+                         int btm_bc, const double* bfric, double* qa, long ldo, double* work, long lwork,
+                         void* stream) {
+  if (!bfric) return fail(DSHT_EARG, "bottom friction coefficient pointer is NULL");
+  return col_vdiff(p, q0, rhs, kv, e3, ldg, km, cdk, fdk, dt, alpha, sfc_bc, btm_bc, bfric, qa, ldo, work, lwork,
+                   stream);
 }
 
 int dsht_coef_d(dsht_plan* p, int which, const double* params, int nparams, double* out, void* stream) {

@@ -5,6 +5,7 @@
 //   HBEDiagnose_HydPres (same file:290-325)
 //   calc_ADVFlxZ_QUICK  (model/dogcm/src/dynamics/hspm_vfvm/HBEBaroc_hspm_vfvm_mod.f90:233-317)
 //   calc_VDiffTRCEq     (model/dogcm/src/physics/DOGCM_Phys_hspm_vfvm_mod.f90:752-861)
+//   calc_VDiffMOMEq     (same file:865-975)
 //
 // Layout: the km layers of one field are km consecutive device grid columns (column pitch
 // ldg doubles, layer 0 on top = the reference's KS), each an I x J grid with longitude
@@ -244,19 +245,25 @@ __global__ __launch_bounds__(256) void k_col_quick(const double* __restrict__ om
 // pitch npts) and its right-hand side in qa; the backward sweep reads them back in
 // reverse and writes qa = q0 + x.  A singular pivot (impossible with kv >= 0 and positive
 // thicknesses) is not reported.
+// MOM selects calc_VDiffMOMEq (:865-975): right-hand side dt rhs[k] (no division by e3) and,
+// for a Dirichlet bottom, Ad = 1 + alpha coef (bfric + r[km-1]) with the bottom friction
+// coefficient bfric of the grid point (:963).
 struct VdiffRow {
   double al, ad, au, b;
 };
 
+template <bool MOM>
 __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0, const double* __restrict__ rhs,
                                                    const double* __restrict__ kv, const double* __restrict__ e3,
                                                    long ldg, int km, const double* __restrict__ cdk,
                                                    const double* __restrict__ fdk, double dt, double alpha,
-                                                   int sfc_dirichlet, int btm_dirichlet, double* __restrict__ qa,
+                                                   int sfc_dirichlet, int btm_dirichlet,
+                                                   const double* __restrict__ bfric, double* __restrict__ qa,
                                                    long ldo, double* __restrict__ work, int npts) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= npts) return;
+  const double bf = (MOM && btm_dirichlet) ? ntl(bfric + i) : 0.0;
   const long nw = npts;
   double* wd = work + i;                   // d of finished rows
   double* wu = work + (long)km * nw + i;   // du
@@ -277,14 +284,17 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
     const double coef = dt / (e3k * cdk[k]);
     const double ac = alpha * coef;
     const double bt = dt * rk;
-    o.b = bt / e3k;
+    o.b = MOM ? bt : bt / e3k;
     if (k == 0) {
       o.al = 0.0;
       o.ad = sfc_dirichlet ? 1.0 + ac * (r_bot + 2.0 * r_top) : 1.0 + ac * r_bot;
       o.au = -(ac * r_bot);
     } else if (k == km - 1) {
       o.al = -(ac * r_top);
-      o.ad = btm_dirichlet ? 1.0 + ac * (2.0 * r_bot + r_top) : 1.0 + ac * r_top;
+      if (MOM)
+        o.ad = btm_dirichlet ? 1.0 + ac * (bf + r_top) : 1.0 + ac * r_top;
+      else
+        o.ad = btm_dirichlet ? 1.0 + ac * (2.0 * r_bot + r_top) : 1.0 + ac * r_top;
       o.au = 0.0;
     } else {
       o.al = -(ac * r_top);
@@ -413,9 +423,14 @@ int launch_col_quick(const double* omg, long ldo, const double* trc, long ldg, i
 
 int launch_col_vdiff(const double* q0, const double* rhs, const double* kv, const double* e3, long ldg, int km,
                      const double* cdk, const double* fdk, double dt, double alpha, int sfc_dirichlet,
-                     int btm_dirichlet, double* qa, long ldo, double* work, int npts, hipStream_t s) {
-  hipLaunchKernelGGL(k_col_vdiff, dim3(col_grid(npts)), dim3(256), 0, s, q0, rhs, kv, e3, ldg, km, cdk, fdk, dt, alpha,
-                     sfc_dirichlet, btm_dirichlet, qa, ldo, work, npts);
+                     int btm_dirichlet, const double* bfric, double* qa, long ldo, double* work, int npts,
+                     hipStream_t s) {
+  if (bfric)
+    hipLaunchKernelGGL(k_col_vdiff<true>, dim3(col_grid(npts)), dim3(256), 0, s, q0, rhs, kv, e3, ldg, km, cdk, fdk,
+                       dt, alpha, sfc_dirichlet, btm_dirichlet, bfric, qa, ldo, work, npts);
+  else
+    hipLaunchKernelGGL(k_col_vdiff<false>, dim3(col_grid(npts)), dim3(256), 0, s, q0, rhs, kv, e3, ldg, km, cdk, fdk,
+                       dt, alpha, sfc_dirichlet, btm_dirichlet, bfric, qa, ldo, work, npts);
   DSHT_HIP(hipGetLastError());
   return 0;
 }

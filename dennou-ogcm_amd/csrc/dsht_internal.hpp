@@ -284,7 +284,8 @@ int launch_col_quick(const double* omg, long ldo, const double* trc, long ldg, i
                      const double* rfdk, int linfree, double* flx, long ldf, int npts, hipStream_t s);
 int launch_col_vdiff(const double* q0, const double* rhs, const double* kv, const double* e3, long ldg, int km,
                      const double* cdk, const double* fdk, double dt, double alpha, int sfc_dirichlet,
-                     int btm_dirichlet, double* qa, long ldo, double* work, int npts, hipStream_t s);
+                     int btm_dirichlet, const double* bfric, double* qa, long ldo, double* work, int npts,
+                     hipStream_t s);
 int launch_lonlat_sum(const dsht_plan* p, const double* xy, long ldg, int ncols, double* out,
                       double factor, hipStream_t s);
 int launch_lon_sum(const dsht_plan* p, const double* xy, long ldg, int ncols, double* out,

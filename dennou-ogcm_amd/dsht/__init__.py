@@ -40,7 +40,7 @@ EXPORTS = [
     "dsht_alpha_fwd_d", "dsht_alpha_inv_d", "dsht_uv2vordiv_d", "dsht_vordiv2uv_d",
     "dsht_diag_d", "dsht_intlonlat_d", "dsht_avrlonlat_d", "dsht_ya_intlon_d",
     "dsht_interpolate_d", "dsht_coef_d", "dsht_col_omega_d", "dsht_col_hydpres_d", "dsht_col_uvbarot_d",
-    "dsht_col_quick_d", "dsht_col_vdiff_d", "dsht_prof_enable", "dsht_prof_reset", "dsht_prof_query",
+    "dsht_col_quick_d", "dsht_col_vdiff_d", "dsht_col_vdiff_mom_d", "dsht_prof_enable", "dsht_prof_reset", "dsht_prof_query",
     "dsht_set_tuning", "dsht_get_tuning", "dsht_batch_create", "dsht_batch_add", "dsht_batch_finalize",
     "dsht_batch_info", "dsht_batch_ngroups", "dsht_batch_buffers", "dsht_batch_step",
     "dsht_batch_step_sequential", "dsht_batch_capture", "dsht_batch_launch", "dsht_batch_destroy",
@@ -113,6 +113,8 @@ def load():
         "dsht_col_quick_d": ([vp, vp, lg, vp, lg, i, vp, vp, i, vp, lg, vp], i),
         "dsht_col_vdiff_d": ([vp, vp, vp, vp, vp, lg, i, vp, vp, ctypes.c_double, ctypes.c_double, i, i, vp, lg,
                               vp, lg, vp], i),
+        "dsht_col_vdiff_mom_d": ([vp, vp, vp, vp, vp, lg, i, vp, vp, ctypes.c_double, ctypes.c_double, i, i, vp, vp,
+                                  lg, vp, lg, vp], i),
         "dsht_prof_enable": ([i], i),
         "dsht_prof_reset": ([], i),
         "dsht_prof_query": ([ctypes.c_char_p, vp, vp, vp, vp], i),
@@ -557,10 +559,11 @@ class Plan:
         return out
 
     def col_vdiff_d(self, q0, rhs, kv, e3, cdk, fdk, dt, alpha, sfc_bc="D", btm_bc="N", out=None, work=None,
-                    stream=None):
+                    stream=None, btm_frict=None):
         """calc_VDiffTRCEq (DOGCM_Phys_hspm_vfvm_mod.f90:752-861): q0 + the implicit vertical
         diffusion increment, (km, J, I); fdk the km+1 half-level spacings, sfc_bc/btm_bc 'D' or
-        'N'.  work: optional float64 device scratch of >= 3*km*J*I values."""
+        'N'.  work: optional float64 device scratch of >= 3*km*J*I values.  With btm_frict (a
+        (J, I) device grid of xy_BtmFrictCoef) the momentum form calc_VDiffMOMEq (:865-975)."""
         import torch
         km, ldg = self._col_in(cdk, q0, rhs, kv, e3)
         if fdk.dtype != torch.float64 or not fdk.is_cuda or fdk.dim() != 1 or fdk.numel() != km + 1 \
@@ -572,9 +575,18 @@ class Plan:
             work = torch.empty(need, dtype=torch.float64, device=q0.device)
         elif work.dtype != torch.float64 or not work.is_cuda or not work.is_contiguous():
             raise ValueError("work must be a contiguous float64 device tensor")
-        _check(load().dsht_col_vdiff_d(self.h, _tptr(q0), _tptr(rhs), _tptr(kv), _tptr(e3), ldg, km, _tptr(cdk),
-                                       _tptr(fdk), float(dt), float(alpha), ord(sfc_bc), ord(btm_bc), _tptr(out),
-                                       out.stride(0), _tptr(work), work.numel(), _stream(stream)))
+        if btm_frict is None:
+            _check(load().dsht_col_vdiff_d(self.h, _tptr(q0), _tptr(rhs), _tptr(kv), _tptr(e3), ldg, km, _tptr(cdk),
+                                           _tptr(fdk), float(dt), float(alpha), ord(sfc_bc), ord(btm_bc), _tptr(out),
+                                           out.stride(0), _tptr(work), work.numel(), _stream(stream)))
+            return out
+        if tuple(btm_frict.shape) != (self.jm, self.im) or not btm_frict.is_contiguous() \
+                or btm_frict.dtype != torch.float64 or not btm_frict.is_cuda:
+            raise ValueError(f"btm_frict must be a contiguous float64 device grid ({self.jm}, {self.im})")
+        _check(load().dsht_col_vdiff_mom_d(self.h, _tptr(q0), _tptr(rhs), _tptr(kv), _tptr(e3), ldg, km, _tptr(cdk),
+                                           _tptr(fdk), float(dt), float(alpha), ord(sfc_bc), ord(btm_bc),
+                                           _tptr(btm_frict), _tptr(out), out.stride(0), _tptr(work), work.numel(),
+                                           _stream(stream)))
         return out
 
     def set_tuning(self, key, value):

@@ -198,6 +198,13 @@ int dsht_col_vdiff_d(dsht_plan* plan, const double* q0, const double* rhs, const
                      const double* e3, long ldg, int km, const double* cdk, const double* fdk,
                      double dt, double alpha, int sfc_bc, int btm_bc, double* qa, long ldo,
                      double* work, long lwork, void* stream);
+/* calc_VDiffMOMEq (DOGCM_Phys_hspm_vfvm_mod.f90:865-975): the momentum form of the same solve:
+ * right-hand side dt*rhs (no division by e3) and, with a Dirichlet bottom, the bottom row
+ * 1 + alpha*coef*(bfric + r) with bfric = xy_BtmFrictCoef, one I x J device grid.      */
+int dsht_col_vdiff_mom_d(dsht_plan* plan, const double* q0, const double* rhs, const double* kv,
+                         const double* e3, long ldg, int km, const double* cdk, const double* fdk,
+                         double dt, double alpha, int sfc_bc, int btm_bc, const double* bfric,
+                         double* qa, long ldo, double* work, long lwork, void* stream);
 /* coefficient table written to device memory */
 int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, double* out,
                 void* stream);

@@ -777,9 +777,27 @@ ORC_NOFMA int orc_dgtsv(int n, double* dl, double* d, double* du, double* b) {
  * order: coef (:809), half-level coefficients r (:811-823), interior rows (:825-830),
  * right-hand side (:831), boundary rows (:836-852), DGTSV (:855), q0 + x (:857).  Layer k
  * (0-based) is the reference's KS+k; half level h is KS-1+h. */
-ORC_NOFMA long orc_col_vdiff(const double* q0, const double* rhs, const double* kv, const double* e3, int km,
-                             const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc,
-                             int btm_bc, double* qa, long npts) {
+static long col_vdiff_any(const double* q0, const double* rhs, const double* kv, const double* e3, int km,
+                          const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc, int btm_bc,
+                          const double* bfric, double* qa, long npts);
+
+long orc_col_vdiff(const double* q0, const double* rhs, const double* kv, const double* e3, int km,
+                   const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc, int btm_bc,
+                   double* qa, long npts) {
+  return col_vdiff_any(q0, rhs, kv, e3, km, cdk, fdk, dt, alpha, sfc_bc, btm_bc, NULL, qa, npts);
+}
+
+/* calc_VDiffMOMEq (DOGCM_Phys_hspm_vfvm_mod.f90:865-975): the same system with b = dt*rhs (:944)
+ * and the Dirichlet bottom row 1 + alpha*coef*(xy_BtmFrictCoef + r(KE-1)) (:963). */
+long orc_col_vdiff_mom(const double* q0, const double* rhs, const double* kv, const double* e3, int km,
+                       const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc, int btm_bc,
+                       const double* bfric, double* qa, long npts) {
+  return col_vdiff_any(q0, rhs, kv, e3, km, cdk, fdk, dt, alpha, sfc_bc, btm_bc, bfric, qa, npts);
+}
+
+ORC_NOFMA static long col_vdiff_any(const double* q0, const double* rhs, const double* kv, const double* e3, int km,
+                                    const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc,
+                                    int btm_bc, const double* bfric, double* qa, long npts) {
   double* coef = malloc(sizeof(double) * (size_t)km);
   double* r = malloc(sizeof(double) * (size_t)(km + 1));
   double* al = malloc(sizeof(double) * (size_t)km);
@@ -803,7 +821,7 @@ ORC_NOFMA long orc_col_vdiff(const double* q0, const double* rhs, const double* 
       ad[k] = 1.0 + alpha * coef[k] * (r[k] + r[k + 1]);
       au[k] = -(alpha * coef[k] * r[k + 1]);
     }
-    for (int k = 0; k < km; k++) b[k] = dt * rhs[(long)k * npts + i] / E3(k);
+    for (int k = 0; k < km; k++) b[k] = bfric ? dt * rhs[(long)k * npts + i] : dt * rhs[(long)k * npts + i] / E3(k);
     if (sfc_bc == 'D') {
       ad[0] = 1.0 + alpha * coef[0] * (r[1] + 2.0 * r[0]);
       au[0] = -(alpha * coef[0] * r[1]);
@@ -814,7 +832,7 @@ ORC_NOFMA long orc_col_vdiff(const double* q0, const double* rhs, const double* 
     const int n = km - 1;
     if (btm_bc == 'D') {
       al[n] = -(alpha * coef[n] * r[n]);
-      ad[n] = 1.0 + alpha * coef[n] * (2.0 * r[n + 1] + r[n]);
+      ad[n] = bfric ? 1.0 + alpha * coef[n] * (bfric[i] + r[n]) : 1.0 + alpha * coef[n] * (2.0 * r[n + 1] + r[n]);
     } else {
       al[n] = -(alpha * coef[n] * r[n]);
       ad[n] = 1.0 + alpha * coef[n] * (+r[n]);

@@ -111,6 +111,10 @@ int orc_dgtsv(int n, double* dl, double* d, double* du, double* b);
 long orc_col_vdiff(const double* q0, const double* rhs, const double* kv, const double* e3, int km,
                    const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc,
                    int btm_bc, double* qa, long npts);
+/* calc_VDiffMOMEq (same file:865-975): b = dt*rhs; Dirichlet bottom row 1 + alpha*coef*(bfric + r). */
+long orc_col_vdiff_mom(const double* q0, const double* rhs, const double* kv, const double* e3, int km,
+                       const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc,
+                       int btm_bc, const double* bfric, double* qa, long npts);
 
 #ifdef __cplusplus
 }

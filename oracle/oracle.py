@@ -73,6 +73,8 @@ def lib():
         L.orc_dgtsv.restype = i
         L.orc_col_vdiff.argtypes = [dp, dp, dp, dp, i, dp, dp, d, d, i, i, dp, lg]
         L.orc_col_vdiff.restype = lg
+        L.orc_col_vdiff_mom.argtypes = [dp, dp, dp, dp, i, dp, dp, d, d, i, i, dp, dp, lg]
+        L.orc_col_vdiff_mom.restype = lg
         _lib = L
     return _lib
 
@@ -321,11 +323,19 @@ def dgtsv(dl, d, du, b):
     return b, sw
 
 
-def col_vdiff(q0, rhs, kv, e3, cdk, fdk, dt, alpha, sfc_bc="D", btm_bc="N"):
-    """calc_VDiffTRCEq: (q0 + the implicit vertical diffusion increment, row interchanges)."""
+def col_vdiff(q0, rhs, kv, e3, cdk, fdk, dt, alpha, sfc_bc="D", btm_bc="N", btm_frict=None):
+    """calc_VDiffTRCEq, or with btm_frict calc_VDiffMOMEq: (q0 + the implicit vertical
+    diffusion increment, row interchanges)."""
     (q0, rhs, kv, e3), km, npts = _col(q0, rhs, kv, e3)
     cdk, fdk = _c(cdk), _c(fdk)
     qa = np.empty_like(q0)
-    sw = lib().orc_col_vdiff(_p(q0), _p(rhs), _p(kv), _p(e3), km, _p(cdk), _p(fdk), float(dt), float(alpha),
-                             ord(sfc_bc), ord(btm_bc), _p(qa), npts)
+    if btm_frict is None:
+        sw = lib().orc_col_vdiff(_p(q0), _p(rhs), _p(kv), _p(e3), km, _p(cdk), _p(fdk), float(dt), float(alpha),
+                                 ord(sfc_bc), ord(btm_bc), _p(qa), npts)
+    else:
+        bf = _c(btm_frict).reshape(-1)
+        if bf.size != npts:
+            raise ValueError("btm_frict must have one value per grid point")
+        sw = lib().orc_col_vdiff_mom(_p(q0), _p(rhs), _p(kv), _p(e3), km, _p(cdk), _p(fdk), float(dt),
+                                     float(alpha), ord(sfc_bc), ord(btm_bc), _p(bf), _p(qa), npts)
     return qa, int(sw)

@@ -253,3 +253,22 @@ def test_vdiff_boundaries_pitch_and_errors(gpu, bcs):
     short = lib.dsht_col_vdiff_d(p.h, tp(dq0), tp(drhs), tp(dkv), tp(de3), dq0.stride(0), 1, tp(dc), tp(df), d(1.0),
                                  d(1.0), ord("D"), ord("N"), tp(out), out.stride(0), tp(work), work.numel(), None)
     assert small != 0 and badbc != 0 and short != 0
+
+
+@pytest.mark.parametrize("N,I,J,km", [(85, 256, 128, 60), (21, 64, 32, 2), (21, 64, 32, 7)])
+@pytest.mark.parametrize("btm", ["D", "N"])
+def test_vdiff_momentum_bitwise(gpu, N, I, J, km, btm):
+    """calc_VDiffMOMEq (bottom friction in the Dirichlet bottom row), bit for bit."""
+    import torch
+    import dsht
+    import oracle
+    p = dsht.Plan(N, I, J, radius=6.371e6)
+    q0, rhs, kv, e3, cdk, fdk = _vdiff_case(km, J, I, 5 * N + km, True)
+    bf = np.random.default_rng(km).uniform(1e-4, 1e-2, (J, I))
+    out = p.col_vdiff_d(_dev(q0, gpu), _dev(rhs, gpu), _dev(kv, gpu), _dev(e3, gpu), _dev(cdk, gpu), _dev(fdk, gpu),
+                        1800.0, 1.0, "D", btm, btm_frict=torch.tensor(bf, device=gpu))
+    torch.cuda.synchronize()
+    ref, _ = oracle.col_vdiff(q0, rhs, kv, e3, cdk, fdk, 1800.0, 1.0, "D", btm, btm_frict=bf)
+    got = out.cpu().numpy()
+    nbad = int((got != ref).sum())
+    assert nbad == 0, (km, btm, nbad, float(np.abs(got - ref).max()))

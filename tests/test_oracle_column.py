@@ -245,3 +245,31 @@ def test_vdiff_column_integral(km, bcs):
     q0, rhs, kv, e3, cdk, fdk = _vdiff_inputs(km, npts, 5)
     qa, sw = O.col_vdiff(q0, rhs, np.zeros_like(kv), e3, cdk, fdk, dt, alpha, *bcs)
     assert np.array_equal(qa, q0 + dt * rhs / e3) and sw == 0
+
+
+@pytest.mark.parametrize("km", [2, 20, 60])
+@pytest.mark.parametrize("btm", ["N", "D"])
+def test_vdiff_momentum_column_integral(km, btm):
+    """calc_VDiffMOMEq: b = dt rhs, so sum_k e3 cdk x = dt sum_k e3 cdk rhs - dt alpha
+    (2 r0 x0 [surface 'D'] + bfric x_bottom [bottom 'D']); bfric = 0 with a Dirichlet bottom
+    is the no-flux bottom."""
+    npts = 5
+    dt, alpha = 1800.0, 1.0
+    q0, rhs, kv, e3, cdk, fdk = _vdiff_inputs(km, npts, 40 + km)
+    bf = np.random.default_rng(km).uniform(1e-4, 1e-2, npts)
+    qa, _ = O.col_vdiff(q0, rhs, kv, e3, cdk, fdk, dt, alpha, "D", btm, btm_frict=bf)
+    x = qa - q0
+    lhs = (e3 * cdk[:, None] * x).sum(0)
+    want = dt * (e3 * cdk[:, None] * rhs).sum(0)
+    scale = (e3 * cdk[:, None] * np.abs(x)).sum(0) + dt * (e3 * cdk[:, None] * np.abs(rhs)).sum(0)
+    t = dt * alpha * 2.0 * kv[0] / (fdk[0] * e3[0]) * x[0]
+    want -= t
+    scale += np.abs(t)
+    if btm == "D":
+        t = dt * alpha * bf * x[-1]
+        want -= t
+        scale += np.abs(t)
+    assert np.all(np.abs(lhs - want) <= 1e-9 * scale), (km, btm)
+    # the tracer form differs (rhs divided by e3)
+    qt, _ = O.col_vdiff(q0, rhs, kv, e3, cdk, fdk, dt, alpha, "D", btm)
+    assert not np.allclose(qt, qa)
