@@ -991,6 +991,116 @@ int dsht_interpolate(dsht_plan* p, const double* w, const double* lon, const dou
   return d2h_sync(c, out, dl + 2 * npts, npts);
 }
 
+// ---- vertical column kernels on host arrays: the km layers of a field are km consecutive
+// I x J grids (Fortran xyz(IM, JM, KS:KE)); staged through the calling thread's device buffers
+static int col_stage(HostCtx* c, int i, const double* const* src, const size_t* n, int nsrc, double** dst) {
+  size_t tot = 0;
+  for (int k = 0; k < nsrc; k++) tot += n[k];
+  if (int rc = ctx_buf(c, i, tot, dst)) return rc;
+  size_t off = 0;
+  for (int k = 0; k < nsrc; k++) {
+    if (src[k])
+      if (int rc = h2d(c, *dst + off, src[k], n[k])) return rc;
+    off += n[k];
+  }
+  return 0;
+}
+
+int dsht_col_omega(dsht_plan* p, const double* div, const double* h, int km, const double* cdk, double* omg) {
+  HOST_PROLOGUE(p, km);
+  if (!div || !h || !cdk || !omg) return fail(DSHT_EARG, "NULL array");
+  double *din, *dout, *dk;
+  const double* in[] = {div, h};
+  const size_t nin[] = {IJ * km, IJ * km};
+  const size_t nk[] = {(size_t)km};
+  int rc;
+  if ((rc = col_stage(c, 0, in, nin, 2, &din)) || (rc = col_stage(c, 2, &cdk, nk, 1, &dk)) ||
+      (rc = ctx_buf(c, 1, IJ * (km + 1), &dout)))
+    return rc;
+  if ((rc = dsht_col_omega_d(p, din, din + IJ * km, (long)IJ, km, dk, dout, (long)IJ, c->s))) return rc;
+  return d2h_sync(c, omg, dout, IJ * (km + 1));
+}
+
+int dsht_col_hydpres(dsht_plan* p, const double* dens, const double* h, int km, const double* cdk, double grav,
+                     double* out) {
+  HOST_PROLOGUE(p, km);
+  if (!dens || !h || !cdk || !out) return fail(DSHT_EARG, "NULL array");
+  double *din, *dout, *dk;
+  const double* in[] = {dens, h};
+  const size_t nin[] = {IJ * km, IJ * km};
+  const size_t nk[] = {(size_t)km};
+  int rc;
+  if ((rc = col_stage(c, 0, in, nin, 2, &din)) || (rc = col_stage(c, 2, &cdk, nk, 1, &dk)) ||
+      (rc = ctx_buf(c, 1, IJ * km, &dout)))
+    return rc;
+  if ((rc = dsht_col_hydpres_d(p, din, din + IJ * km, (long)IJ, km, dk, grav, dout, (long)IJ, c->s))) return rc;
+  return d2h_sync(c, out, dout, IJ * km);
+}
+
+int dsht_col_uvbarot(dsht_plan* p, const double* u, const double* v, const double* h, int km, const double* cdk,
+                     double* ub, double* vb) {
+  HOST_PROLOGUE(p, km);
+  if (!u || !v || !h || !cdk || !ub || !vb) return fail(DSHT_EARG, "NULL array");
+  double *din, *dout, *dk;
+  const double* in[] = {u, v, h};
+  const size_t nin[] = {IJ * km, IJ * km, IJ * km};
+  const size_t nk[] = {(size_t)km};
+  int rc;
+  if ((rc = col_stage(c, 0, in, nin, 3, &din)) || (rc = col_stage(c, 2, &cdk, nk, 1, &dk)) ||
+      (rc = ctx_buf(c, 1, 2 * IJ, &dout)))
+    return rc;
+  if ((rc = dsht_col_uvbarot_d(p, din, din + IJ * km, din + 2 * IJ * km, (long)IJ, km, dk, dout, dout + IJ, c->s)))
+    return rc;
+  DSHT_HIP(hipMemcpyAsync(ub, dout, sizeof(double) * IJ, hipMemcpyDeviceToHost, c->s));
+  return d2h_sync(c, vb, dout + IJ, IJ);
+}
+
+int dsht_col_quick(dsht_plan* p, const double* omg, const double* trc, int km, const double* cdk, const double* rfdk,
+                   int linfree, double* flx) {
+  HOST_PROLOGUE(p, km);
+  if (!omg || !trc || !cdk || !rfdk || !flx) return fail(DSHT_EARG, "NULL array");
+  double *din, *dout, *dk;
+  const double* in[] = {omg, trc};
+  const size_t nin[] = {IJ * (km + 1), IJ * km};
+  const double* kin[] = {cdk, rfdk};
+  const size_t nk[] = {(size_t)km, (size_t)km + 1};
+  int rc;
+  if ((rc = col_stage(c, 0, in, nin, 2, &din)) || (rc = col_stage(c, 2, kin, nk, 2, &dk)) ||
+      (rc = ctx_buf(c, 1, IJ * (km + 1), &dout)))
+    return rc;
+  if ((rc = dsht_col_quick_d(p, din, (long)IJ, din + IJ * (km + 1), (long)IJ, km, dk, dk + km, linfree, dout, (long)IJ,
+                             c->s)))
+    return rc;
+  return d2h_sync(c, flx, dout, IJ * (km + 1));
+}
+
+int dsht_col_vdiff(dsht_plan* p, const double* q0, const double* rhs, const double* kv, const double* e3, int km,
+                   const double* cdk, const double* fdk, double dt, double alpha, int sfc_bc, int btm_bc,
+                   const double* bfric, double* qa) {
+  HOST_PROLOGUE(p, km);
+  if (!q0 || !rhs || !kv || !e3 || !cdk || !fdk || !qa) return fail(DSHT_EARG, "NULL array");
+  double *din, *dout, *dk, *dw;
+  const double* in[] = {q0, rhs, kv, e3};
+  const size_t nin[] = {IJ * km, IJ * km, IJ * km, IJ * km};
+  const double* kin[] = {cdk, fdk, bfric};
+  const size_t nk[] = {(size_t)km, (size_t)km + 1, bfric ? IJ : 0};
+  int rc;
+  if ((rc = col_stage(c, 0, in, nin, 4, &din)) || (rc = col_stage(c, 2, kin, nk, 3, &dk)) ||
+      (rc = ctx_buf(c, 1, IJ * km, &dout)) || (rc = ctx_buf(c, 3, 3 * IJ * km, &dw)))
+    return rc;
+  const double* dq = din;
+  const double* dr = din + IJ * km;
+  const double* dkv = din + 2 * IJ * km;
+  const double* de = din + 3 * IJ * km;
+  const long lw = (long)(3 * IJ * km);
+  rc = bfric ? dsht_col_vdiff_mom_d(p, dq, dr, dkv, de, (long)IJ, km, dk, dk + km, dt, alpha, sfc_bc, btm_bc,
+                                    dk + 2 * km + 1, dout, (long)IJ, dw, lw, c->s)
+             : dsht_col_vdiff_d(p, dq, dr, dkv, de, (long)IJ, km, dk, dk + km, dt, alpha, sfc_bc, btm_bc, dout,
+                                (long)IJ, dw, lw, c->s);
+  if (rc) return rc;
+  return d2h_sync(c, qa, dout, IJ * km);
+}
+
 int dsht_coef(const dsht_plan* pc, int which, const double* params, int nparams, double* out) {
   dsht_plan* p = const_cast<dsht_plan*>(pc);
   HOST_PROLOGUE(p, 1);

@@ -40,7 +40,8 @@ EXPORTS = [
     "dsht_alpha_fwd_d", "dsht_alpha_inv_d", "dsht_uv2vordiv_d", "dsht_vordiv2uv_d",
     "dsht_diag_d", "dsht_intlonlat_d", "dsht_avrlonlat_d", "dsht_ya_intlon_d",
     "dsht_interpolate_d", "dsht_coef_d", "dsht_col_omega_d", "dsht_col_hydpres_d", "dsht_col_uvbarot_d",
-    "dsht_col_quick_d", "dsht_col_vdiff_d", "dsht_col_vdiff_mom_d", "dsht_prof_enable", "dsht_prof_reset", "dsht_prof_query",
+    "dsht_col_quick_d", "dsht_col_vdiff_d", "dsht_col_vdiff_mom_d", "dsht_col_omega", "dsht_col_hydpres",
+    "dsht_col_uvbarot", "dsht_col_quick", "dsht_col_vdiff", "dsht_prof_enable", "dsht_prof_reset", "dsht_prof_query",
     "dsht_set_tuning", "dsht_get_tuning", "dsht_batch_create", "dsht_batch_add", "dsht_batch_finalize",
     "dsht_batch_info", "dsht_batch_ngroups", "dsht_batch_buffers", "dsht_batch_step",
     "dsht_batch_step_sequential", "dsht_batch_capture", "dsht_batch_launch", "dsht_batch_destroy",
@@ -115,6 +116,11 @@ def load():
                               vp, lg, vp], i),
         "dsht_col_vdiff_mom_d": ([vp, vp, vp, vp, vp, lg, i, vp, vp, ctypes.c_double, ctypes.c_double, i, i, vp, vp,
                                   lg, vp, lg, vp], i),
+        "dsht_col_omega": ([vp, vp, vp, i, vp, vp], i),
+        "dsht_col_hydpres": ([vp, vp, vp, i, vp, ctypes.c_double, vp], i),
+        "dsht_col_uvbarot": ([vp, vp, vp, vp, i, vp, vp, vp], i),
+        "dsht_col_quick": ([vp, vp, vp, i, vp, vp, i, vp], i),
+        "dsht_col_vdiff": ([vp, vp, vp, vp, vp, i, vp, vp, ctypes.c_double, ctypes.c_double, i, i, vp, vp], i),
         "dsht_prof_enable": ([i], i),
         "dsht_prof_reset": ([], i),
         "dsht_prof_query": ([ctypes.c_char_p, vp, vp, vp, vp], i),
@@ -587,6 +593,64 @@ class Plan:
                                            _tptr(fdk), float(dt), float(alpha), ord(sfc_bc), ord(btm_bc),
                                            _tptr(btm_frict), _tptr(out), out.stride(0), _tptr(work), work.numel(),
                                            _stream(stream)))
+        return out
+
+    # column kernels on host arrays (numpy (km, J, I) fields, layer 0 on top)
+    def _col_host(self, *arrs):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in arrs]
+        km = a[0].shape[0]
+        for x in a:
+            if x.shape != (km, self.jm, self.im):
+                raise ValueError(f"fields must be (km, {self.jm}, {self.im})")
+        return a, km
+
+    def xyr_OMG(self, xyz_Div, xyz_H, z_CDK):
+        """HBEDiagnose_OMG on host arrays: (km+1, J, I) half-level omega."""
+        (div, h), km = self._col_host(xyz_Div, xyz_H)
+        cdk = _f64(np.asarray(z_CDK, dtype=np.float64).reshape(-1))
+        out = np.empty((km + 1, self.jm, self.im))
+        _check(load().dsht_col_omega(self.h, _ptr(div), _ptr(h), km, _ptr(cdk), _ptr(out)))
+        return out
+
+    def xyz_HydPres(self, xyz_DensEdd, xyz_H, z_CDK, grav):
+        (dens, h), km = self._col_host(xyz_DensEdd, xyz_H)
+        cdk = _f64(np.asarray(z_CDK, dtype=np.float64).reshape(-1))
+        out = np.empty((km, self.jm, self.im))
+        _check(load().dsht_col_hydpres(self.h, _ptr(dens), _ptr(h), km, _ptr(cdk), float(grav), _ptr(out)))
+        return out
+
+    def xy_UVBarot(self, xyz_U, xyz_V, xyz_H, z_CDK):
+        (u, v, h), km = self._col_host(xyz_U, xyz_V, xyz_H)
+        cdk = _f64(np.asarray(z_CDK, dtype=np.float64).reshape(-1))
+        ub, vb = np.empty((self.jm, self.im)), np.empty((self.jm, self.im))
+        _check(load().dsht_col_uvbarot(self.h, _ptr(u), _ptr(v), _ptr(h), km, _ptr(cdk), _ptr(ub), _ptr(vb)))
+        return ub, vb
+
+    def xyr_ADVFlxZ_QUICK(self, xyr_OMG, xyz_TRC, z_CDK, z_RFDK, linfree=False):
+        (trc,), km = self._col_host(xyz_TRC)
+        omg = np.ascontiguousarray(xyr_OMG, dtype=np.float64)
+        if omg.shape != (km + 1, self.jm, self.im):
+            raise ValueError("omega must have km+1 half levels")
+        cdk = _f64(np.asarray(z_CDK, dtype=np.float64).reshape(-1))
+        rfdk = _f64(np.asarray(z_RFDK, dtype=np.float64).reshape(-1))
+        out = np.empty((km + 1, self.jm, self.im))
+        _check(load().dsht_col_quick(self.h, _ptr(omg), _ptr(trc), km, _ptr(cdk), _ptr(rfdk), int(bool(linfree)),
+                                     _ptr(out)))
+        return out
+
+    def xyz_VDiff(self, xyz_q0, xyz_RHS, xyz_Kv, xyz_H, z_CDK, z_FDK, dt, alpha, sfc_bc="D", btm_bc="N",
+                  xy_BtmFrictCoef=None):
+        """calc_VDiffTRCEq on host arrays, or calc_VDiffMOMEq with xy_BtmFrictCoef."""
+        (q0, rhs, kv, e3), km = self._col_host(xyz_q0, xyz_RHS, xyz_Kv, xyz_H)
+        cdk = _f64(np.asarray(z_CDK, dtype=np.float64).reshape(-1))
+        fdk = _f64(np.asarray(z_FDK, dtype=np.float64).reshape(-1))
+        bf = None if xy_BtmFrictCoef is None else np.ascontiguousarray(xy_BtmFrictCoef, dtype=np.float64)
+        if bf is not None and bf.shape != (self.jm, self.im):
+            raise ValueError(f"xy_BtmFrictCoef must be ({self.jm}, {self.im})")
+        out = np.empty((km, self.jm, self.im))
+        _check(load().dsht_col_vdiff(self.h, _ptr(q0), _ptr(rhs), _ptr(kv), _ptr(e3), km, _ptr(cdk), _ptr(fdk),
+                                     float(dt), float(alpha), ord(sfc_bc), ord(btm_bc),
+                                     _ptr(bf) if bf is not None else None, _ptr(out)))
         return out
 
     def set_tuning(self, key, value):

@@ -205,6 +205,22 @@ int dsht_col_vdiff_mom_d(dsht_plan* plan, const double* q0, const double* rhs, c
                          const double* e3, long ldg, int km, const double* cdk, const double* fdk,
                          double dt, double alpha, int sfc_bc, int btm_bc, const double* bfric,
                          double* qa, long ldo, double* work, long lwork, void* stream);
+/* The column kernels on host arrays (the reference's own calling form): each field is
+ * xyz(IM, JM, KS:KE), km consecutive I x J grids; half-level arrays have km+1 grids
+ * (xyr(:, :, KS-1:KE)); cdk, rfdk, fdk are host vectors.  Staged through the calling
+ * thread's device buffers; results are those of the device entry points above.
+ * dsht_col_vdiff runs the momentum form when bfric (an I x J grid) is not NULL.     */
+int dsht_col_omega(dsht_plan* plan, const double* div, const double* h, int km,
+                   const double* cdk, double* omg);
+int dsht_col_hydpres(dsht_plan* plan, const double* dens, const double* h, int km,
+                     const double* cdk, double grav, double* out);
+int dsht_col_uvbarot(dsht_plan* plan, const double* u, const double* v, const double* h, int km,
+                     const double* cdk, double* ub, double* vb);
+int dsht_col_quick(dsht_plan* plan, const double* omg, const double* trc, int km,
+                   const double* cdk, const double* rfdk, int linfree, double* flx);
+int dsht_col_vdiff(dsht_plan* plan, const double* q0, const double* rhs, const double* kv,
+                   const double* e3, int km, const double* cdk, const double* fdk, double dt,
+                   double alpha, int sfc_bc, int btm_bc, const double* bfric, double* qa);
 /* coefficient table written to device memory */
 int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, double* out,
                 void* stream);

@@ -272,3 +272,35 @@ def test_vdiff_momentum_bitwise(gpu, N, I, J, km, btm):
     got = out.cpu().numpy()
     nbad = int((got != ref).sum())
     assert nbad == 0, (km, btm, nbad, float(np.abs(got - ref).max()))
+
+
+def test_column_host_entries(gpu):
+    """The host-array forms (dsht_col_omega, _hydpres, _uvbarot, _quick, _vdiff), the calling
+    form of the reference's Fortran routines, bit for bit against the oracle."""
+    import dsht
+    import oracle
+    N, I, J, km = 42, 128, 64, 20
+    p = dsht.Plan(N, I, J, radius=6.371e6)
+    x = _inputs(km, J, I, seed=21)
+    omg = p.xyr_OMG(x["div"], x["h"], x["cdk"])
+    assert np.array_equal(omg, oracle.col_omega(x["div"], x["h"], x["cdk"]))
+    assert np.array_equal(p.xyz_HydPres(x["dens"], x["h"], x["cdk"], GRAV),
+                          oracle.col_hydpres(x["dens"], x["h"], x["cdk"], GRAV))
+    ub, vb = p.xy_UVBarot(x["u"], x["v"], x["h"], x["cdk"])
+    rub, rvb = oracle.col_uvbarot(x["u"], x["v"], x["h"], x["cdk"])
+    assert np.array_equal(ub, rub) and np.array_equal(vb, rvb)
+    rfdk = np.random.default_rng(2).uniform(10.0, 40.0, km + 1)
+    trc = x["u"] * 10.0
+    for lin in (False, True):
+        assert np.array_equal(p.xyr_ADVFlxZ_QUICK(omg, trc, x["cdk"], rfdk, linfree=lin),
+                              oracle.col_quick(omg, trc, x["cdk"], rfdk, linfree=lin))
+    q0, rhs, kv, e3, cdk, fdk = _vdiff_case(km, J, I, 9, True)
+    got = p.xyz_VDiff(q0, rhs, kv, e3, cdk, fdk, 3600.0, 1.0, "D", "D")
+    ref, sw = oracle.col_vdiff(q0, rhs, kv, e3, cdk, fdk, 3600.0, 1.0, "D", "D")
+    assert sw > 0 and np.array_equal(got, ref)
+    bf = np.full((J, I), 2e-3)
+    got = p.xyz_VDiff(q0, rhs, kv, e3, cdk, fdk, 3600.0, 1.0, "D", "D", xy_BtmFrictCoef=bf)
+    ref, _ = oracle.col_vdiff(q0, rhs, kv, e3, cdk, fdk, 3600.0, 1.0, "D", "D", btm_frict=bf)
+    assert np.array_equal(got, ref)
+    with pytest.raises(dsht.DshtError):
+        p.xyr_ADVFlxZ_QUICK(omg[:2], trc[:1], x["cdk"][:1], rfdk[:2])  # km = 1
