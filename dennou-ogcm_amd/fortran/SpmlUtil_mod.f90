@@ -139,6 +139,46 @@ module SpmlUtil_mod
       real(c_double), intent(in) :: w(*), lon(*), lat(*)
       real(c_double) :: out(*)
     end function
+    ! vertical column kernels (host-array forms)
+    integer(c_int) function c_col_omega(plan, div, h, nk, cdk, omg) bind(C, name="dsht_col_omega")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: nk
+      real(c_double), intent(in) :: div(*), h(*), cdk(*)
+      real(c_double) :: omg(*)
+    end function
+    integer(c_int) function c_col_hydpres(plan, dens, h, nk, cdk, grav, out) bind(C, name="dsht_col_hydpres")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: nk
+      real(c_double), intent(in) :: dens(*), h(*), cdk(*)
+      real(c_double), value :: grav
+      real(c_double) :: out(*)
+    end function
+    integer(c_int) function c_col_uvbarot(plan, u, v, h, nk, cdk, ub, vb) bind(C, name="dsht_col_uvbarot")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: nk
+      real(c_double), intent(in) :: u(*), v(*), h(*), cdk(*)
+      real(c_double) :: ub(*), vb(*)
+    end function
+    integer(c_int) function c_col_quick(plan, omg, trc, nk, cdk, rfdk, linfree, flx) bind(C, name="dsht_col_quick")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: nk, linfree
+      real(c_double), intent(in) :: omg(*), trc(*), cdk(*), rfdk(*)
+      real(c_double) :: flx(*)
+    end function
+    integer(c_int) function c_col_vdiff(plan, q0, rhs, kv, e3, nk, cdk, fdk, dt, alpha, sfc, btm, bfric, qa) &
+         bind(C, name="dsht_col_vdiff")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      integer(c_int), value :: nk, sfc, btm
+      real(c_double), intent(in) :: q0(*), rhs(*), kv(*), e3(*), cdk(*), fdk(*)
+      real(c_double), value :: dt, alpha
+      type(c_ptr), value :: bfric
+      real(c_double) :: qa(*)
+    end function
   end interface
 
   ! public procedures (SpmlUtil_mod.f90:71-135, horizontal part)
@@ -154,6 +194,9 @@ module SpmlUtil_mod
   public :: a_Interpolate_wa, Interpolate_w
   ! extension without a counterpart in the reference module (see calc_GradLonLat_xya_wa)
   public :: calc_GradLonLat_xya_wa
+  ! extensions: the grid-space column kernels of the dynamics and physics modules
+  ! (HBEDiagnose_OMG / _HydPres / _UVBarot, calc_ADVFlxZ_QUICK, calc_VDiffTRCEq / MOMEq)
+  public :: calc_OMG_xyr, calc_HydPres_xyz, calc_UVBarot_xy, calc_ADVFlxZ_QUICK_xyr, calc_VDiff_xyz
 
   interface calc_VorDiv2UV
     module procedure calc_VorDiv2UV_w_xy
@@ -470,5 +513,61 @@ contains
       a_Interpolate_wa(k) = Interpolate_w(wa(:, k), alon, alat)
     end do
   end function a_Interpolate_wa
+
+  ! ---------------------------------------------------------------- vertical column kernels
+  ! Fields are xyz(0:im-1, jm, nk) (layer 1 on top = KS), half-level fields xyr(0:im-1, jm, nk+1)
+  ! (level 1 = KS-1); z_CDK(nk), z_RFDK(nk+1), z_FDK(nk+1) as in the reference's grid module.
+
+  ! HBEDiagnose_OMG (HBEDiagnose_hspm_vfvm_mod.f90:90-143)
+  subroutine calc_OMG_xyr(xyr_OMG, xyz_Div, xyz_H, z_CDK)
+    real(DP), intent(in) :: xyz_Div(:,:,:), xyz_H(:,:,:), z_CDK(:)
+    real(DP), intent(out) :: xyr_OMG(0:im-1, jm, size(xyz_Div, 3) + 1)
+    call chk(c_col_omega(plan, xyz_Div, xyz_H, int(size(xyz_Div, 3), c_int), z_CDK, xyr_OMG), 'calc_OMG_xyr')
+  end subroutine calc_OMG_xyr
+
+  ! HBEDiagnose_HydPres (HBEDiagnose_hspm_vfvm_mod.f90:290-325)
+  subroutine calc_HydPres_xyz(xyz_HydPres, xyz_DensEdd, xyz_H, z_CDK, Grav)
+    real(DP), intent(in) :: xyz_DensEdd(:,:,:), xyz_H(:,:,:), z_CDK(:), Grav
+    real(DP), intent(out) :: xyz_HydPres(0:im-1, jm, size(xyz_DensEdd, 3))
+    call chk(c_col_hydpres(plan, xyz_DensEdd, xyz_H, int(size(xyz_DensEdd, 3), c_int), z_CDK, Grav, xyz_HydPres), &
+             'calc_HydPres_xyz')
+  end subroutine calc_HydPres_xyz
+
+  ! HBEDiagnose_UVBarot (HBEDiagnose_hspm_vfvm_mod.f90:221-255)
+  subroutine calc_UVBarot_xy(xy_UBarot, xy_VBarot, xyz_U, xyz_V, xyz_H, z_CDK)
+    real(DP), intent(in) :: xyz_U(:,:,:), xyz_V(:,:,:), xyz_H(:,:,:), z_CDK(:)
+    real(DP), intent(out) :: xy_UBarot(0:im-1, jm), xy_VBarot(0:im-1, jm)
+    call chk(c_col_uvbarot(plan, xyz_U, xyz_V, xyz_H, int(size(xyz_U, 3), c_int), z_CDK, xy_UBarot, xy_VBarot), &
+             'calc_UVBarot_xy')
+  end subroutine calc_UVBarot_xy
+
+  ! calc_ADVFlxZ_QUICK (HBEBaroc_hspm_vfvm_mod.f90:233-317)
+  subroutine calc_ADVFlxZ_QUICK_xyr(xyr_ADVFlxZ, xyr_OMG, xyz_TRC, z_CDK, z_RFDK, LinFreeSurf)
+    real(DP), intent(in) :: xyr_OMG(:,:,:), xyz_TRC(:,:,:), z_CDK(:), z_RFDK(:)
+    logical, intent(in) :: LinFreeSurf
+    real(DP), intent(out) :: xyr_ADVFlxZ(0:im-1, jm, size(xyz_TRC, 3) + 1)
+    integer(c_int) :: lin
+    lin = 0
+    if (LinFreeSurf) lin = 1
+    call chk(c_col_quick(plan, xyr_OMG, xyz_TRC, int(size(xyz_TRC, 3), c_int), z_CDK, z_RFDK, lin, xyr_ADVFlxZ), &
+             'calc_ADVFlxZ_QUICK_xyr')
+  end subroutine calc_ADVFlxZ_QUICK_xyr
+
+  ! calc_VDiffTRCEq (DOGCM_Phys_hspm_vfvm_mod.f90:752-861), or calc_VDiffMOMEq (:865-975) when
+  ! xy_BtmFrictCoef is present
+  subroutine calc_VDiff_xyz(xyz_qA, xyz_q0, xyz_RHS, xyz_Kv, xyz_H, z_CDK, z_FDK, dt, alpha, &
+                            SeaSfcBCType, SeaBtmBCType, xy_BtmFrictCoef)
+    real(DP), intent(in) :: xyz_q0(:,:,:), xyz_RHS(:,:,:), xyz_Kv(:,:,:), xyz_H(:,:,:), z_CDK(:), z_FDK(:)
+    real(DP), intent(in) :: dt, alpha
+    character, intent(in) :: SeaSfcBCType, SeaBtmBCType
+    real(DP), intent(in), target, optional :: xy_BtmFrictCoef(0:im-1, jm)
+    real(DP), intent(out) :: xyz_qA(0:im-1, jm, size(xyz_q0, 3))
+    type(c_ptr) :: bf
+    bf = c_null_ptr
+    if (present(xy_BtmFrictCoef)) bf = c_loc(xy_BtmFrictCoef)
+    call chk(c_col_vdiff(plan, xyz_q0, xyz_RHS, xyz_Kv, xyz_H, int(size(xyz_q0, 3), c_int), z_CDK, z_FDK, dt, &
+                         alpha, int(ichar(SeaSfcBCType), c_int), int(ichar(SeaBtmBCType), c_int), bf, xyz_qA), &
+             'calc_VDiff_xyz')
+  end subroutine calc_VDiff_xyz
 
 end module SpmlUtil_mod

@@ -103,6 +103,8 @@ program spml_dropin_test
   end do
   call check('per-layer calls', err, 1d-12)
 
+  call column_checks()
+
   call SpmlUtil_Final()
   if (nfail > 0) then
     write(*, '(a,i0,a)') 'FAIL (', nfail, ' checks)'
@@ -122,6 +124,59 @@ contains
       nfail = nfail + 1
     end if
   end subroutine check
+
+  ! the column kernels through the shim's extensions, on closed forms (uniform layers):
+  ! omega of a constant divergence falls linearly to 0 at the bottom; the pressure of a
+  ! constant density sits at the layer centres; the barotropic mean of a constant is that
+  ! constant; the QUICK flux of a constant tracer is omega*T; without diffusivity the solve
+  ! adds dt*rhs/H (tracer form) or dt*rhs/(1 + alpha*dt/(H*cdk)*bfric) at a Dirichlet
+  ! bottom (momentum form)
+  subroutine column_checks()
+    integer, parameter :: NK = 5
+    real(DP), parameter :: D0 = 2d-6, H0 = 1.5d0, RHO = 1.2d0, G = 9.8d0, T0 = 4d0, DT = 600d0
+    real(DP) :: div(0:IM-1, JM, NK), h(0:IM-1, JM, NK), omg(0:IM-1, JM, NK+1), flx(0:IM-1, JM, NK+1)
+    real(DP) :: p(0:IM-1, JM, NK), q0(0:IM-1, JM, NK), rhs(0:IM-1, JM, NK), kv(0:IM-1, JM, NK), qa(0:IM-1, JM, NK)
+    real(DP) :: ub(0:IM-1, JM), vb(0:IM-1, JM), bf(0:IM-1, JM)
+    real(DP) :: cdk(NK), rfdk(NK+1), fdk(NK+1), e1, e2, want
+    integer :: kk
+    cdk = 1d0 / NK
+    rfdk = dble(NK)
+    fdk = 1d0 / NK
+    div = D0
+    h = H0
+    call calc_OMG_xyr(omg, div, h, cdk)
+    e1 = 0d0
+    do kk = 1, NK + 1
+      e1 = max(e1, maxval(abs(omg(:, :, kk) + D0 * H0 * dble(NK + 1 - kk) / NK)))
+    end do
+    call check('calc_OMG_xyr (linear omega)', e1 / (D0 * H0), 1d-14)
+    p = RHO
+    call calc_HydPres_xyz(p, p * 0d0 + RHO, h, cdk, G)
+    e1 = 0d0
+    do kk = 1, NK
+      e1 = max(e1, maxval(abs(p(:, :, kk) - G * H0 * RHO * (kk - 0.5d0) / NK)))
+    end do
+    call check('calc_HydPres_xyz (layer centres)', e1 / (G * H0 * RHO), 1d-14)
+    q0 = 0.3d0
+    rhs = -0.7d0
+    call calc_UVBarot_xy(ub, vb, q0, rhs, h, cdk)
+    call check('calc_UVBarot_xy (constant)', max(maxval(abs(ub - 0.3d0)), maxval(abs(vb + 0.7d0))), 1d-15)
+    q0 = T0
+    call calc_ADVFlxZ_QUICK_xyr(flx, omg, q0, cdk, rfdk, .false.)
+    e1 = max(maxval(abs(flx(:, :, 1))), maxval(abs(flx(:, :, NK + 1))))
+    e1 = max(e1, maxval(abs(flx(:, :, 2:NK) - T0 * omg(:, :, 2:NK))))
+    call check('calc_ADVFlxZ_QUICK_xyr (constant tracer)', e1 / (D0 * H0 * T0), 1d-14)
+    kv = 0d0
+    rhs = 1d-4
+    call calc_VDiff_xyz(qa, q0, rhs, kv, h, cdk, fdk, DT, 1d0, 'D', 'N')
+    call check('calc_VDiff_xyz (no diffusivity)', maxval(abs(qa - (q0 + DT * rhs / h))), 1d-15)
+    bf = 3d-3
+    call calc_VDiff_xyz(qa, q0, rhs, kv, h, cdk, fdk, DT, 1d0, 'D', 'D', bf)
+    want = T0 + DT * 1d-4 / (1d0 + 1d0 * (DT / (H0 * cdk(NK))) * 3d-3)
+    e2 = maxval(abs(qa(:, :, NK) - want))
+    e1 = maxval(abs(qa(:, :, 1:NK-1) - (T0 + DT * 1d-4)))
+    call check('calc_VDiff_xyz momentum form (bottom friction)', max(e1, e2), 1d-14)
+  end subroutine column_checks
 
   subroutine fail(what)
     character(*), intent(in) :: what
