@@ -453,10 +453,13 @@ def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E, fac
         serial = mode == "batched"
         srv = serve.Server(latency_factor=(factor if serial else concurrent_factor) if capped else None,
                            serialize=serial)
-        if capped:  # calibrate every pool before the clock starts
-            for n in serve.GRIDS:
-                p = srv.pool(n)
+        # every mode builds (and, capped, calibrates) its pools and captures their step graphs
+        # before the clock starts
+        for n in serve.GRIDS:
+            p = srv.pool(n)
+            if capped:
                 caps.setdefault(mode, {})[f"T{n}"] = {"step_ms_km60": p.t_ref, "max_active_columns": p.max_cols}
+        torch.cuda.synchronize()
         reqs = stream()
         out[mode] = srv.run(reqs, batched=(mode != "one_at_a_time"))
         out[mode]["p99_step_ms_by_truncation"] = {
@@ -468,7 +471,10 @@ def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E, fac
     solo = {}
     for n in sorted({r.nmax for r in stream()}):
         r = serve.Request(0, n, 60, 1.0, 30, 0.0, 1)
-        serve.Server().run([r], batched=True)
+        srv = serve.Server()
+        srv.pool(n)
+        torch.cuda.synchronize()
+        srv.run([r], batched=True)
         solo[f"T{n}"] = float(np.median(r.step_ms[5:]))
     b, u = out["batched"], out["one_at_a_time"]
     bc = out["batched_concurrent"]

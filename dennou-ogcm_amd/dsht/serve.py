@@ -202,6 +202,23 @@ class Pool:
                 self.graphs[hi] = g
             g.replay()
 
+    def warm(self):
+        """Capture the step graph of every active width up front, so that no capture lands
+        inside a served step.  Replays nothing: the state is untouched."""
+        import torch
+        for hi in range(GRAPH_COLS, self.cap + GRAPH_COLS, GRAPH_COLS):
+            hi = min(self.cap, hi)
+            if hi in self.graphs:
+                continue
+            g = torch.cuda.CUDAGraph()
+            if self.mempool is None:
+                self.mempool = torch.cuda.graph_pool_handle()
+            with torch.cuda.stream(self.stream):
+                with torch.cuda.graph(g, pool=self.mempool, stream=self.stream):
+                    self.step(hi, self.stream.cuda_stream)
+            self.graphs[hi] = g
+        self.stream.synchronize()
+
     def step(self, hi, stream):
         """One step for columns [0, hi) of the pool."""
         p, dt = self.plan, self.dt
@@ -262,6 +279,7 @@ class Server:
             p = Pool(nmax, self.capacity[nmax], self.dt, dev, self.precision)
             if self.latency_factor:
                 p.calibrate(self.ref_km, self.latency_factor)
+            p.warm()
             self.pools[nmax] = p
         return self.pools[nmax]
 
