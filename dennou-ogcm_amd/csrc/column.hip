@@ -304,12 +304,10 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
     }
     return o;
   };
-  // builds row k (needs layer k+1 for the lower half-level coefficient)
-  auto next_row = [&](int k) {
-    const double rk = ntl(rhs + (long)k * ldg + i);
+  // builds row k from rhs[k] and layer k+1 (kv_n, e3_n: unused for the last row)
+  auto build_row = [&](int k, double rk, double kv_n, double e3_n) {
     double r_bot;
     if (k + 1 < km) {
-      const double kv_n = ntl(kv + (long)(k + 1) * ldg + i), e3_n = ntl(e3 + (long)(k + 1) * ldg + i);
       r_bot = rhalf(k + 1, kv_c, e3_c, kv_n, e3_n);
       const VdiffRow o = row(k, r_up, r_bot, e3_c, rk);
       kv_c = kv_n;
@@ -320,14 +318,17 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
     r_bot = kv_c / (fdk[km] * e3_c);
     return row(k, r_up, r_bot, e3_c, rk);
   };
+  auto next_row = [&](int k) {
+    const int kn = k + 1 < km ? k + 1 : km - 1;
+    return build_row(k, ntl(rhs + (long)k * ldg + i), ntl(kv + (long)kn * ldg + i), ntl(e3 + (long)kn * ldg + i));
+  };
 
   // ---- forward elimination (DGTSV): row j current (d, du, b), row j+1 pending (dl = its Al)
   VdiffRow r0 = next_row(0);
   double d = r0.ad, du = r0.au, b = r0.b;
   VdiffRow r1 = next_row(1);
   double dl = r1.al, dn = r1.ad, dun = r1.au, bn = r1.b;
-  for (int j = 0; j <= km - 3; j++) {
-    const VdiffRow r2 = next_row(j + 2);  // row j+2, whose Al is dl of the next step
+  auto eliminate = [&](int j, const VdiffRow& r2) {  // step j; r2 = row j+2
     double du2;
     if (fabs(d) >= fabs(dl)) {
       const double fact = dl / d;
@@ -352,7 +353,21 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
     qa[(long)j * ldo + i] = b;
     d = dn, du = dun, b = bn;
     dl = r2.al, dn = r2.ad, dun = r2.au, bn = r2.b;
+  };
+  int j = 0;
+  for (; j + CU - 1 <= km - 3; j += CU) {  // rows j+2 .. j+CU+1 loaded together
+    double pr[CU], pk[CU], pe[CU];
+#pragma unroll
+    for (int u = 0; u < CU; u++) {
+      const int k = j + u + 2, kn = k + 1 < km ? k + 1 : km - 1;
+      pr[u] = ntl(rhs + (long)k * ldg + i);
+      pk[u] = ntl(kv + (long)kn * ldg + i);
+      pe[u] = ntl(e3 + (long)kn * ldg + i);
+    }
+#pragma unroll
+    for (int u = 0; u < CU; u++) eliminate(j + u, build_row(j + u + 2, pr[u], pk[u], pe[u]));
   }
+  for (; j <= km - 3; j++) eliminate(j, next_row(j + 2));
   {  // last elimination step, rows km-2 and km-1
     if (fabs(d) >= fabs(dl)) {
       const double fact = dl / d;
@@ -375,15 +390,31 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
   nts(qa + (long)(km - 1) * ldo + i, ntl(q0 + (long)(km - 1) * ldg + i) + x1);
   nts(qa + (long)(km - 2) * ldo + i, ntl(q0 + (long)(km - 2) * ldg + i) + x0);
   double xa = x0, xb = x1;  // x[j+1], x[j+2]
-  for (int j = km - 3; j >= 0; j--) {
-    const double dj = wd[(long)j * nw], uj = wu[(long)j * nw], vj = w2[(long)j * nw];
-    const double bj = qa[(long)j * ldo + i];
+  auto back = [&](int jj, double dj, double uj, double vj, double bj, double qj) {
     const double t = bj - uj * xa;
     const double x = (t - vj * xb) / dj;
-    nts(qa + (long)j * ldo + i, ntl(q0 + (long)j * ldg + i) + x);
+    nts(qa + (long)jj * ldo + i, qj + x);
     xb = xa;
     xa = x;
+  };
+  int jb = km - 3;
+  for (; jb - (CU - 1) >= 0; jb -= CU) {  // rows jb .. jb-CU+1 loaded together
+    double pd[CU], pu[CU], pv[CU], pb[CU], pq[CU];
+#pragma unroll
+    for (int u = 0; u < CU; u++) {
+      const long r = jb - u;
+      pd[u] = wd[r * nw];
+      pu[u] = wu[r * nw];
+      pv[u] = w2[r * nw];
+      pb[u] = qa[r * ldo + i];
+      pq[u] = ntl(q0 + r * ldg + i);
+    }
+#pragma unroll
+    for (int u = 0; u < CU; u++) back(jb - u, pd[u], pu[u], pv[u], pb[u], pq[u]);
   }
+  for (; jb >= 0; jb--)
+    back(jb, wd[(long)jb * nw], wu[(long)jb * nw], w2[(long)jb * nw], qa[(long)jb * ldo + i],
+         ntl(q0 + (long)jb * ldg + i));
 }
 
 int col_grid(int npts) { return (npts + 255) / 256; }
