@@ -58,11 +58,13 @@ def build_fortran(verbose=True):
     mod = os.path.join(FDIR, "SpmlUtil_mod.f90")
     test = os.path.join(FDIR, "spml_dropin_test.f90")
     exe = os.path.join(HERE, "spml_dropin_test")
-    subprocess.run([fc, "-O2", "-c", mod, "-module-dir", OBJ, "-o", os.path.join(OBJ, "SpmlUtil_mod.o")],
+    # -fopenmp: the test program drives the shim from concurrent OpenMP threads, as the
+    # reference's !$omp parallel do over layers does
+    subprocess.run([fc, "-O2", "-fopenmp", "-c", mod, "-module-dir", OBJ, "-o", os.path.join(OBJ, "SpmlUtil_mod.o")],
                    check=True)
-    subprocess.run([fc, "-O2", "-c", test, "-module-dir", OBJ, "-o", os.path.join(OBJ, "spml_dropin_test.o")],
-                   check=True)
-    subprocess.run([fc, "-O2", "-o", exe, os.path.join(OBJ, "spml_dropin_test.o"),
+    subprocess.run([fc, "-O2", "-fopenmp", "-c", test, "-module-dir", OBJ, "-o",
+                    os.path.join(OBJ, "spml_dropin_test.o")], check=True)
+    subprocess.run([fc, "-O2", "-fopenmp", "-o", exe, os.path.join(OBJ, "spml_dropin_test.o"),
                     os.path.join(OBJ, "SpmlUtil_mod.o"), "-L" + HERE, "-ldsht",
                     "-Wl,-rpath," + HERE], check=True)
     if verbose:

@@ -526,6 +526,29 @@ int dsht_grid_info(const dsht_plan* p, double* x_lon, double* y_lat, double* x_l
   return 0;
 }
 
+int dsht_sig_grid(int kmax, int tmax, double sig_min, double sig_max, double* sig, double* sig_weight) {
+  if (kmax <= 0 || tmax <= 0) return fail(DSHT_EARG, "dsht_sig_grid: kmax = %d and tmax = %d must be positive", kmax, tmax);
+  if (kmax % 2) return fail(DSHT_EARG, "dsht_sig_grid: kmax = %d must be even", kmax);
+  if (tmax > kmax) return fail(DSHT_EARG, "dsht_sig_grid: tmax = %d must be <= kmax = %d", tmax, kmax);
+  const double half = 0.5 * (sig_max - sig_min);
+  for (int i = 0; i <= kmax; i++) {
+    const double th = M_PI * i / kmax;
+    if (sig) sig[i] = 0.5 * (sig_max + sig_min) + half * std::cos(th);
+    if (!sig_weight) continue;
+    // Clenshaw-Curtis: integrate the degree-tmax Chebyshev interpolant exactly; the even
+    // modes k integrate to 2/(1-k^2), the last one counted half when tmax = kmax
+    double acc = 1.0;
+    for (int k = 2; k <= tmax; k += 2) {
+      const double c = 2.0 / (1.0 - (double)k * k) * std::cos(k * th);
+      acc += (k == kmax) ? 0.5 * c : c;
+    }
+    double wgt = 2.0 / kmax * acc * half;
+    if (i == 0 || i == kmax) wgt *= 0.5;
+    sig_weight[i] = wgt;
+  }
+  return 0;
+}
+
 int dsht_nm_l(const dsht_plan* p, int* n_of_l, int* m_of_l) {
   if (int rc = check_plan(p)) return rc;
   for (int n = 0; n <= p->N; n++)
@@ -731,6 +754,25 @@ int dsht_interpolate_d(dsht_plan* p, const double* w, long sl, const double* lon
 }
 
 // ---------------------------------------------------------------- vertical column diagnostics
+// address ranges of device arrays, for the no-aliasing checks of the column kernels
+// (their outputs are written while inputs are still read; the kernels take restrict pointers)
+struct Span {
+  uintptr_t lo, hi;
+};
+static Span span_of(const double* x, long n) {
+  const uintptr_t a = (uintptr_t)x;
+  return Span{a, a + (uintptr_t)(x && n > 0 ? n : 0) * sizeof(double)};
+}
+static bool overlaps(const Span& a, const Span& b) { return a.lo < b.hi && b.lo < a.hi && a.hi > a.lo && b.hi > b.lo; }
+
+// fails unless the output span is disjoint from every input span
+static int check_disjoint(const char* what, Span out, const Span* ins, const char* const* names, int n) {
+  for (int i = 0; i < n; i++)
+    if (overlaps(out, ins[i]))
+      return fail(DSHT_EARG, "the output %s overlaps the input %s (in-place calls are not supported)", what, names[i]);
+  return 0;
+}
+
 static int check_col(const dsht_plan* p, int km, const double* cdk, long ldg) {
   if (int rc = check_plan(p)) return rc;
   if (km < 1) return fail(DSHT_EARG, "km must be >= 1");
@@ -744,6 +786,10 @@ int dsht_col_omega_d(dsht_plan* p, const double* div, const double* h, long ldg,
   if (int rc = check_col(p, km, cdk, ldg)) return rc;
   if (!div || !h || !omg) return fail(DSHT_EARG, "grid pointer is NULL");
   if (ldo < (long)p->I * p->J) return fail(DSHT_EARG, "output column pitch %ld must be >= I*J", ldo);
+  const long IJ = (long)p->I * p->J;
+  const Span ins[] = {span_of(div, (km - 1) * ldg + IJ), span_of(h, (km - 1) * ldg + IJ), span_of(cdk, km)};
+  static const char* const names[] = {"div", "h", "cdk"};
+  if (int rc = check_disjoint("omega", span_of(omg, km * ldo + IJ), ins, names, 3)) return rc;
   return launch_col_omega(div, h, ldg, km, cdk, omg, ldo, p->I * p->J, (hipStream_t)stream);
 }
 
@@ -752,6 +798,10 @@ int dsht_col_hydpres_d(dsht_plan* p, const double* dens, const double* h, long l
   if (int rc = check_col(p, km, cdk, ldg)) return rc;
   if (!dens || !h || !out) return fail(DSHT_EARG, "grid pointer is NULL");
   if (ldo < (long)p->I * p->J) return fail(DSHT_EARG, "output column pitch %ld must be >= I*J", ldo);
+  const long IJ = (long)p->I * p->J;
+  const Span ins[] = {span_of(dens, (km - 1) * ldg + IJ), span_of(h, (km - 1) * ldg + IJ), span_of(cdk, km)};
+  static const char* const names[] = {"dens", "h", "cdk"};
+  if (int rc = check_disjoint("pressure", span_of(out, (km - 1) * ldo + IJ), ins, names, 3)) return rc;
   return launch_col_hydpres(dens, h, ldg, km, cdk, grav, out, ldo, p->I * p->J, (hipStream_t)stream);
 }
 
@@ -760,6 +810,13 @@ int dsht_col_uvbarot_d(dsht_plan* p, const double* u, const double* v, const dou
   if (int rc = check_col(p, km, cdk, ldg)) return rc;
   if (!u || !v || !h) return fail(DSHT_EARG, "grid pointer is NULL");
   if (!ub && !vb) return fail(DSHT_EARG, "both outputs are NULL");
+  const long IJ = (long)p->I * p->J;
+  const Span ins[] = {span_of(u, (km - 1) * ldg + IJ), span_of(v, (km - 1) * ldg + IJ),
+                      span_of(h, (km - 1) * ldg + IJ), span_of(cdk, km)};
+  static const char* const names[] = {"u", "v", "h", "cdk"};
+  if (int rc = check_disjoint("ub", span_of(ub, IJ), ins, names, 4)) return rc;
+  if (int rc = check_disjoint("vb", span_of(vb, IJ), ins, names, 4)) return rc;
+  if (overlaps(span_of(ub, IJ), span_of(vb, IJ))) return fail(DSHT_EARG, "the outputs ub and vb overlap");
   return launch_col_uvbarot(u, v, h, ldg, km, cdk, ub, vb, p->I * p->J, (hipStream_t)stream);
 }
 
@@ -769,6 +826,11 @@ int dsht_col_quick_d(dsht_plan* p, const double* omg, long ldo, const double* tr
   if (km < 2) return fail(DSHT_EARG, "the QUICK flux needs km >= 2");
   if (!omg || !trc || !flx || !rfdk) return fail(DSHT_EARG, "pointer is NULL");
   if (ldo < (long)p->I * p->J || ldf < (long)p->I * p->J) return fail(DSHT_EARG, "half-level pitch must be >= I*J");
+  const long IJ = (long)p->I * p->J;
+  const Span ins[] = {span_of(omg, km * ldo + IJ), span_of(trc, (km - 1) * ldg + IJ), span_of(cdk, km),
+                      span_of(rfdk, km + 1)};
+  static const char* const names[] = {"omega", "tracer", "cdk", "rfdk"};
+  if (int rc = check_disjoint("flux", span_of(flx, km * ldf + IJ), ins, names, 4)) return rc;
   return launch_col_quick(omg, ldo, trc, ldg, km, cdk, rfdk, linfree ? 1 : 0, flx, ldf, p->I * p->J,
                           (hipStream_t)stream);
 }
@@ -784,6 +846,17 @@ static int col_vdiff(dsht_plan* p, const double* q0, const double* rhs, const do
     return fail(DSHT_EARG, "boundary types must be 'D' or 'N'");
   const long need = 3l * km * p->I * p->J;
   if (lwork < need) return fail(DSHT_EARG, "workspace of %ld doubles is below 3*km*I*J = %ld", lwork, need);
+  // the solve writes qa (and the workspace) while it still reads the inputs: no output may
+  // share memory with an input
+  const long IJ = (long)p->I * p->J;
+  const Span out_qa = span_of(qa, (km - 1) * ldo + IJ), out_w = span_of(work, lwork);
+  const Span ins[] = {span_of(q0, (km - 1) * ldg + IJ), span_of(rhs, (km - 1) * ldg + IJ),
+                      span_of(kv, (km - 1) * ldg + IJ), span_of(e3, (km - 1) * ldg + IJ),
+                      span_of(bfric, bfric ? IJ : 0), span_of(cdk, km), span_of(fdk, km + 1)};
+  static const char* const names[] = {"q0", "rhs", "kv", "e3", "bfric", "cdk", "fdk"};
+  if (int rc = check_disjoint("qa", out_qa, ins, names, 7)) return rc;
+  if (int rc = check_disjoint("workspace", out_w, ins, names, 7)) return rc;
+  if (overlaps(out_qa, out_w)) return fail(DSHT_EARG, "the output qa overlaps the workspace");
   return launch_col_vdiff(q0, rhs, kv, e3, ldg, km, cdk, fdk, dt, alpha, sfc_bc == 'D', btm_bc == 'D', bfric, qa, ldo,
                           work, p->I * p->J, (hipStream_t)stream);
 }

@@ -33,7 +33,7 @@ PREC_NAMES = {PREC_FP64: "fp64", PREC_BF16X3: "bf16x3", PREC_FP16: "fp16",
 # every entry point declared in include/dsht.h
 EXPORTS = [
     "dsht_create", "dsht_destroy", "dsht_last_error", "dsht_get_dims", "dsht_set_precision",
-    "dsht_get_precision", "dsht_grid_info", "dsht_nm_l", "dsht_l_nm", "dsht_rn", "dsht_coef",
+    "dsht_get_precision", "dsht_grid_info", "dsht_sig_grid", "dsht_nm_l", "dsht_l_nm", "dsht_rn", "dsht_coef",
     "dsht_w_xy", "dsht_xy_w", "dsht_xy_vgrad_w", "dsht_w_alpha_xy", "dsht_xy_alpha_w", "dsht_uvcoslat2vordiv",
     "dsht_vordiv2uv", "dsht_w_diag_w", "dsht_xy_diag_xy", "dsht_intlonlat", "dsht_avrlonlat",
     "dsht_ya_intlon", "dsht_ya_avrlon", "dsht_interpolate", "dsht_fwd_d", "dsht_inv_d", "dsht_inv_vgrad_d",
@@ -77,6 +77,7 @@ def load():
         "dsht_set_precision": ([vp, i], i),
         "dsht_get_precision": ([vp], i),
         "dsht_grid_info": ([vp, vp, vp, vp, vp, vp, vp], i),
+        "dsht_sig_grid": ([i, i, d, d, vp, vp], i),
         "dsht_nm_l": ([vp, vp, vp], i),
         "dsht_l_nm": ([vp, i, i], i),
         "dsht_rn": ([vp, vp], i),
@@ -185,6 +186,15 @@ def prof_query(name):
     _check(load().dsht_prof_query(name.encode(), ctypes.byref(ms), ctypes.byref(n),
                                   ctypes.byref(by), ctypes.byref(fl)))
     return ms.value, n.value, by.value, fl.value
+
+
+def sig_grid(kmax, tmax, sig_min=-1.0, sig_max=0.0):
+    """(g_Sig, g_Sig_WEIGHT): the kmax+1 Chebyshev-Lobatto levels of [sig_min, sig_max] and
+    their Clenshaw-Curtis weights, the vertical half of get_SpmlGridInfo
+    (SpmlUtil_mod.f90:288,291; at_module_omp.f90:598-619).  Host-only."""
+    sig, wgt = np.zeros(kmax + 1), np.zeros(kmax + 1)
+    _check(load().dsht_sig_grid(int(kmax), int(tmax), float(sig_min), float(sig_max), _ptr(sig), _ptr(wgt)))
+    return sig, wgt
 
 
 class Plan:
@@ -579,6 +589,10 @@ class Plan:
         need = 3 * km * self.jm * self.im
         if work is None:
             work = torch.empty(need, dtype=torch.float64, device=q0.device)
+            if stream is not None:
+                # the scratch is used on the caller's stream: keep the caching allocator from
+                # handing its block out again before that stream's work is done
+                work.record_stream(torch.cuda.ExternalStream(stream))
         elif work.dtype != torch.float64 or not work.is_cuda or not work.is_contiguous():
             raise ValueError("work must be a contiguous float64 device tensor")
         if btm_frict is None:
@@ -604,24 +618,33 @@ class Plan:
                 raise ValueError(f"fields must be (km, {self.jm}, {self.im})")
         return a, km
 
+    @staticmethod
+    def _col_vec(name, v, n):
+        """A host layer vector of exactly n values (the C entry points copy n values; a
+        halo-sized KA vector would be read shifted)."""
+        v = _f64(np.asarray(v, dtype=np.float64).reshape(-1))
+        if v.size != n:
+            raise ValueError(f"{name} has {v.size} values, expected {n} (pass the KS:KE section, not a halo array)")
+        return v
+
     def xyr_OMG(self, xyz_Div, xyz_H, z_CDK):
         """HBEDiagnose_OMG on host arrays: (km+1, J, I) half-level omega."""
         (div, h), km = self._col_host(xyz_Div, xyz_H)
-        cdk = _f64(np.asarray(z_CDK, dtype=np.float64).reshape(-1))
+        cdk = self._col_vec("z_CDK", z_CDK, km)
         out = np.empty((km + 1, self.jm, self.im))
         _check(load().dsht_col_omega(self.h, _ptr(div), _ptr(h), km, _ptr(cdk), _ptr(out)))
         return out
 
     def xyz_HydPres(self, xyz_DensEdd, xyz_H, z_CDK, grav):
         (dens, h), km = self._col_host(xyz_DensEdd, xyz_H)
-        cdk = _f64(np.asarray(z_CDK, dtype=np.float64).reshape(-1))
+        cdk = self._col_vec("z_CDK", z_CDK, km)
         out = np.empty((km, self.jm, self.im))
         _check(load().dsht_col_hydpres(self.h, _ptr(dens), _ptr(h), km, _ptr(cdk), float(grav), _ptr(out)))
         return out
 
     def xy_UVBarot(self, xyz_U, xyz_V, xyz_H, z_CDK):
         (u, v, h), km = self._col_host(xyz_U, xyz_V, xyz_H)
-        cdk = _f64(np.asarray(z_CDK, dtype=np.float64).reshape(-1))
+        cdk = self._col_vec("z_CDK", z_CDK, km)
         ub, vb = np.empty((self.jm, self.im)), np.empty((self.jm, self.im))
         _check(load().dsht_col_uvbarot(self.h, _ptr(u), _ptr(v), _ptr(h), km, _ptr(cdk), _ptr(ub), _ptr(vb)))
         return ub, vb
@@ -631,8 +654,8 @@ class Plan:
         omg = np.ascontiguousarray(xyr_OMG, dtype=np.float64)
         if omg.shape != (km + 1, self.jm, self.im):
             raise ValueError("omega must have km+1 half levels")
-        cdk = _f64(np.asarray(z_CDK, dtype=np.float64).reshape(-1))
-        rfdk = _f64(np.asarray(z_RFDK, dtype=np.float64).reshape(-1))
+        cdk = self._col_vec("z_CDK", z_CDK, km)
+        rfdk = self._col_vec("z_RFDK", z_RFDK, km + 1)
         out = np.empty((km + 1, self.jm, self.im))
         _check(load().dsht_col_quick(self.h, _ptr(omg), _ptr(trc), km, _ptr(cdk), _ptr(rfdk), int(bool(linfree)),
                                      _ptr(out)))
@@ -642,8 +665,8 @@ class Plan:
                   xy_BtmFrictCoef=None):
         """calc_VDiffTRCEq on host arrays, or calc_VDiffMOMEq with xy_BtmFrictCoef."""
         (q0, rhs, kv, e3), km = self._col_host(xyz_q0, xyz_RHS, xyz_Kv, xyz_H)
-        cdk = _f64(np.asarray(z_CDK, dtype=np.float64).reshape(-1))
-        fdk = _f64(np.asarray(z_FDK, dtype=np.float64).reshape(-1))
+        cdk = self._col_vec("z_CDK", z_CDK, km)
+        fdk = self._col_vec("z_FDK", z_FDK, km + 1)
         bf = None if xy_BtmFrictCoef is None else np.ascontiguousarray(xy_BtmFrictCoef, dtype=np.float64)
         if bf is not None and bf.shape != (self.jm, self.im):
             raise ValueError(f"xy_BtmFrictCoef must be ({self.jm}, {self.im})")

@@ -6,8 +6,11 @@
 ! Same procedure names, argument shapes and meanings as the reference:
 !   grid     xy(0:im-1, jm), xya(0:im-1, jm, :)     (longitude fastest, mu ascending)
 !   spectral w(lm), wa(lm, :)  with lm = (nm+1)**2, SPMODEL order l = n*(n+1)+m+1
-! Vertical (Chebyshev) operators of the reference module (xyz_DSig_xyz, IntSig_*, t_g,
-! g_t, ...) are not part of this module; they stay with the caller's vertical code.
+! The vertical grid (g_Sig, g_Sig_WEIGHT: Chebyshev-Lobatto levels of [-1, 0] and their
+! Clenshaw-Curtis weights, at_module_omp.f90:598-619) is built at SpmlUtil_Init and returned
+! by get_SpmlGridInfo.  The vertical (Chebyshev) operators of the reference module
+! (xyz_DSig_xyz, IntSig_*, t_g, g_t, ...) are not part of this module; they stay with the
+! caller's vertical code.
 ! Errors from the library stop the program with the library's message, as the
 ! reference's MessageNotify('E', ...) does.  Calls are thread safe (OpenMP loops).
 !-------------------------------------------------------------
@@ -40,6 +43,12 @@ module SpmlUtil_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: plan
       real(c_double) :: x_lon(*), y_lat(*), x_w(*), y_w(*), xy_lon(*), xy_lat(*)
+    end function
+    integer(c_int) function c_sig_grid(kmax, tmax, sig_min, sig_max, sig, sig_weight) bind(C, name="dsht_sig_grid")
+      import :: c_int, c_double
+      integer(c_int), value :: kmax, tmax
+      real(c_double), value :: sig_min, sig_max
+      real(c_double) :: sig(*), sig_weight(*)
     end function
     integer(c_int) function c_coef(plan, which, params, np, out) bind(C, name="dsht_coef")
       import :: c_int, c_ptr, c_double
@@ -212,9 +221,13 @@ module SpmlUtil_mod
   real(DP), public, save, allocatable :: x_Lon(:), y_Lat(:), x_Lon_Weight(:), y_Lat_Weight(:)
   real(DP), public, save, allocatable :: xy_Lon(:,:), xy_Lat(:,:)
   real(DP), public, save, allocatable :: rn(:,:)   ! rn(:,1) = -n(n+1); rn(:,2) = n
+  ! vertical grid (SpmlUtil_mod.f90:50,107): g_Sig(0:km) from the top (0) to the bottom (-1)
+  real(DP), public, save, allocatable :: g_Sig_WEIGHT(:)
+  real(DP), save, allocatable :: g_Sig(:)
+  real(DP), parameter :: SigMin = -1d0, SigMax = 0d0
 
   type(c_ptr), save :: plan = c_null_ptr
-  integer, save :: im, jm, km, nm, lm
+  integer, save :: im, jm, km, nm, lm, tm
   real(DP), save :: Radius
   logical, save :: initalizedFlag = .false.
 
@@ -237,12 +250,19 @@ contains
     error stop 1
   end subroutine chk
 
+  ! error stop with a message, for argument checks made on the Fortran side
+  subroutine arg_error(where, what)
+    character(*), intent(in) :: where, what
+    write(*, '(a)') 'SpmlUtil_mod: ' // where // ': ' // what
+    error stop 1
+  end subroutine arg_error
+
   subroutine SpmlUtil_Init(iMax, jMax, kMax, nMax, tMax, RPlanet, np)
     integer, intent(in) :: iMax, jMax, kMax, nMax, tMax
     real(DP), intent(in) :: RPlanet
     integer, intent(in), optional :: np
     integer :: n, m, l
-    im = iMax; jm = jMax; km = kMax; nm = nMax
+    im = iMax; jm = jMax; km = kMax; nm = nMax; tm = tMax
     lm = (nm + 1)**2
     Radius = RPlanet
     call chk(c_create(int(nMax, c_int), int(iMax, c_int), int(jMax, c_int), int(kMax + 1, c_int), &
@@ -250,6 +270,10 @@ contains
     allocate(x_Lon(0:im-1), x_Lon_Weight(0:im-1), y_Lat(jm), y_Lat_Weight(jm))
     allocate(xy_Lon(0:im-1, jm), xy_Lat(0:im-1, jm), xy_CosLat(0:im-1, jm), rn(lm, 2))
     call chk(c_grid_info(plan, x_Lon, y_Lat, x_Lon_Weight, y_Lat_Weight, xy_Lon, xy_Lat), 'get grid')
+    ! at_Initial(kMax, tMax, SigMin, SigMax) (SpmlUtil_mod.f90:226)
+    allocate(g_Sig(0:km), g_Sig_WEIGHT(0:km))
+    call chk(c_sig_grid(int(kMax, c_int), int(tMax, c_int), SigMin, SigMax, g_Sig, g_Sig_WEIGHT), &
+         & 'SpmlUtil_Init (vertical grid)')
     xy_CosLat = cos(xy_Lat)
     do n = 0, nm
       do m = -n, n
@@ -259,7 +283,6 @@ contains
       end do
     end do
     if (present(np)) continue
-    if (tMax < 0) continue
     initalizedFlag = .true.
   end subroutine SpmlUtil_Init
 
@@ -268,7 +291,7 @@ contains
     if (initalizedFlag) then
       rc = c_destroy(plan)
       plan = c_null_ptr
-      deallocate(x_Lon, x_Lon_Weight, y_Lat, y_Lat_Weight, xy_Lon, xy_Lat, xy_CosLat, rn)
+      deallocate(x_Lon, x_Lon_Weight, y_Lat, y_Lat_Weight, xy_Lon, xy_Lat, xy_CosLat, rn, g_Sig, g_Sig_WEIGHT)
       initalizedFlag = .false.
     end if
   end subroutine SpmlUtil_Final
@@ -288,9 +311,8 @@ contains
     if (present(y_Lat_Weight_)) y_Lat_Weight_ = y_Lat_Weight
     if (present(xy_Lon_)) xy_Lon_ = xy_Lon
     if (present(xy_Lat_)) xy_Lat_ = xy_Lat
-    ! the vertical (sigma) grid belongs to the caller's vertical discretisation
-    if (present(z_Sig_)) z_Sig_ = 0d0
-    if (present(z_Sig_Weight_)) z_Sig_Weight_ = 0d0
+    if (present(z_Sig_)) z_Sig_ = g_Sig
+    if (present(z_Sig_Weight_)) z_Sig_Weight_ = g_Sig_WEIGHT
   end subroutine get_SpmlGridInfo
 
   ! ---------------------------------------------------- index maps
@@ -518,10 +540,42 @@ contains
   ! Fields are xyz(0:im-1, jm, nk) (layer 1 on top = KS), half-level fields xyr(0:im-1, jm, nk+1)
   ! (level 1 = KS-1); z_CDK(nk), z_RFDK(nk+1), z_FDK(nk+1) as in the reference's grid module.
 
+  ! Shape checks of the column wrappers: every field is (0:im-1, jm, nk) without halos, and
+  ! the layer vectors have nk (z_CDK) or nk+1 (z_FDK, z_RFDK) entries.  The reference's
+  ! KA-sized halo arrays must be passed as their KS:KE (KS-1:KE for half levels) sections.
+  subroutine col_shape(where, nk, f, nlev)
+    character(*), intent(in) :: where
+    integer, intent(in) :: nk, nlev
+    real(DP), intent(in) :: f(:,:,:)
+    character(len=160) :: msg
+    if (size(f, 1) /= im .or. size(f, 2) /= jm .or. size(f, 3) /= nlev) then
+      write(msg, '(a,3(i0,1x),a,3(i0,1x))') 'field shape ', size(f, 1), size(f, 2), size(f, 3), &
+           & '/= im jm levels ', im, jm, nlev
+      call arg_error(where, trim(msg))
+    end if
+    if (nk < 1) call arg_error(where, 'no layers')
+  end subroutine col_shape
+
+  subroutine col_vec(where, name, v, n)
+    character(*), intent(in) :: where, name
+    real(DP), intent(in) :: v(:)
+    integer, intent(in) :: n
+    character(len=160) :: msg
+    if (size(v) /= n) then
+      write(msg, '(a,a,i0,a,i0)') name, ' has ', size(v), ' entries, expected ', n
+      call arg_error(where, trim(msg))
+    end if
+  end subroutine col_vec
+
   ! HBEDiagnose_OMG (HBEDiagnose_hspm_vfvm_mod.f90:90-143)
   subroutine calc_OMG_xyr(xyr_OMG, xyz_Div, xyz_H, z_CDK)
     real(DP), intent(in) :: xyz_Div(:,:,:), xyz_H(:,:,:), z_CDK(:)
     real(DP), intent(out) :: xyr_OMG(0:im-1, jm, size(xyz_Div, 3) + 1)
+    integer :: nk
+    nk = size(xyz_Div, 3)
+    call col_shape('calc_OMG_xyr', nk, xyz_Div, nk)
+    call col_shape('calc_OMG_xyr', nk, xyz_H, nk)
+    call col_vec('calc_OMG_xyr', 'z_CDK', z_CDK, nk)
     call chk(c_col_omega(plan, xyz_Div, xyz_H, int(size(xyz_Div, 3), c_int), z_CDK, xyr_OMG), 'calc_OMG_xyr')
   end subroutine calc_OMG_xyr
 
@@ -529,6 +583,11 @@ contains
   subroutine calc_HydPres_xyz(xyz_HydPres, xyz_DensEdd, xyz_H, z_CDK, Grav)
     real(DP), intent(in) :: xyz_DensEdd(:,:,:), xyz_H(:,:,:), z_CDK(:), Grav
     real(DP), intent(out) :: xyz_HydPres(0:im-1, jm, size(xyz_DensEdd, 3))
+    integer :: nk
+    nk = size(xyz_DensEdd, 3)
+    call col_shape('calc_HydPres_xyz', nk, xyz_DensEdd, nk)
+    call col_shape('calc_HydPres_xyz', nk, xyz_H, nk)
+    call col_vec('calc_HydPres_xyz', 'z_CDK', z_CDK, nk)
     call chk(c_col_hydpres(plan, xyz_DensEdd, xyz_H, int(size(xyz_DensEdd, 3), c_int), z_CDK, Grav, xyz_HydPres), &
              'calc_HydPres_xyz')
   end subroutine calc_HydPres_xyz
@@ -537,6 +596,12 @@ contains
   subroutine calc_UVBarot_xy(xy_UBarot, xy_VBarot, xyz_U, xyz_V, xyz_H, z_CDK)
     real(DP), intent(in) :: xyz_U(:,:,:), xyz_V(:,:,:), xyz_H(:,:,:), z_CDK(:)
     real(DP), intent(out) :: xy_UBarot(0:im-1, jm), xy_VBarot(0:im-1, jm)
+    integer :: nk
+    nk = size(xyz_U, 3)
+    call col_shape('calc_UVBarot_xy', nk, xyz_U, nk)
+    call col_shape('calc_UVBarot_xy', nk, xyz_V, nk)
+    call col_shape('calc_UVBarot_xy', nk, xyz_H, nk)
+    call col_vec('calc_UVBarot_xy', 'z_CDK', z_CDK, nk)
     call chk(c_col_uvbarot(plan, xyz_U, xyz_V, xyz_H, int(size(xyz_U, 3), c_int), z_CDK, xy_UBarot, xy_VBarot), &
              'calc_UVBarot_xy')
   end subroutine calc_UVBarot_xy
@@ -547,6 +612,12 @@ contains
     logical, intent(in) :: LinFreeSurf
     real(DP), intent(out) :: xyr_ADVFlxZ(0:im-1, jm, size(xyz_TRC, 3) + 1)
     integer(c_int) :: lin
+    integer :: nk
+    nk = size(xyz_TRC, 3)
+    call col_shape('calc_ADVFlxZ_QUICK_xyr', nk, xyz_TRC, nk)
+    call col_shape('calc_ADVFlxZ_QUICK_xyr', nk, xyr_OMG, nk + 1)
+    call col_vec('calc_ADVFlxZ_QUICK_xyr', 'z_CDK', z_CDK, nk)
+    call col_vec('calc_ADVFlxZ_QUICK_xyr', 'z_RFDK', z_RFDK, nk + 1)
     lin = 0
     if (LinFreeSurf) lin = 1
     call chk(c_col_quick(plan, xyr_OMG, xyz_TRC, int(size(xyz_TRC, 3), c_int), z_CDK, z_RFDK, lin, xyr_ADVFlxZ), &
@@ -563,6 +634,14 @@ contains
     real(DP), intent(in), target, optional :: xy_BtmFrictCoef(0:im-1, jm)
     real(DP), intent(out) :: xyz_qA(0:im-1, jm, size(xyz_q0, 3))
     type(c_ptr) :: bf
+    integer :: nk
+    nk = size(xyz_q0, 3)
+    call col_shape('calc_VDiff_xyz', nk, xyz_q0, nk)
+    call col_shape('calc_VDiff_xyz', nk, xyz_RHS, nk)
+    call col_shape('calc_VDiff_xyz', nk, xyz_Kv, nk)
+    call col_shape('calc_VDiff_xyz', nk, xyz_H, nk)
+    call col_vec('calc_VDiff_xyz', 'z_CDK', z_CDK, nk)
+    call col_vec('calc_VDiff_xyz', 'z_FDK', z_FDK, nk + 1)
     bf = c_null_ptr
     if (present(xy_BtmFrictCoef)) bf = c_loc(xy_BtmFrictCoef)
     call chk(c_col_vdiff(plan, xyz_q0, xyz_RHS, xyz_Kv, xyz_H, int(size(xyz_q0, 3), c_int), z_CDK, z_FDK, dt, &

@@ -3,12 +3,17 @@
 ! Fortran through ISO_C_BINDING onto the GPU library.  Checks closed forms and
 ! invariants at T21 (im = 64, jm = 32) with an Earth radius:
 !   round trips, Laplacian eigenvalues, solid-body rotation, velocity <-> vorticity and
-!   divergence, global mean = s_0^0, multi-layer forms, and concurrent OpenMP-style
-!   per-layer calls (as in SpmlUtil_mod.f90:360-366).
+!   divergence, global mean = s_0^0, multi-layer forms, the vertical grid of
+!   get_SpmlGridInfo (Clenshaw-Curtis exactness on [-1, 0]), and per-layer calls from 4
+!   concurrent OpenMP threads (as in SpmlUtil_mod.f90:360-366), bit for bit equal to the
+!   serial calls.
 ! Prints PASS and exits 0, or stops with status 1.
+! With the argument "badsize" it passes a halo-sized (KA) z_CDK to a column wrapper after
+! the checks, which must stop with the shim's shape error (status 1).
 !-------------------------------------------------------------
 program spml_dropin_test
   use SpmlUtil_mod
+  !$ use omp_lib
   implicit none
   integer, parameter :: DP = kind(1d0)
   integer, parameter :: IM = 64, JM = 32, KM = 4, NM = 21
@@ -18,12 +23,30 @@ program spml_dropin_test
   real(DP) :: wa((NM + 1)**2, KM), wa2((NM + 1)**2, KM), xya(0:IM-1, JM, KM)
   real(DP) :: xya_lon(0:IM-1, JM, KM), xya_lat(0:IM-1, JM, KM)
   real(DP) :: err, ref
-  integer :: l, n, m, k, nfail
+  real(DP) :: z_sig(0:KM), z_sigw(0:KM), par_xy(0:IM-1, JM, 2*KM), par_w((NM + 1)**2, 2*KM)
+  real(DP) :: ser_xy(0:IM-1, JM, 2*KM), ser_w((NM + 1)**2, 2*KM), wl((NM + 1)**2, 2*KM)
+  integer :: l, n, m, k, nfail, tid(2*KM), nthr
   integer :: nmv(2)
+  character(len=16) :: arg
 
   nfail = 0
-  call SpmlUtil_Init(IM, JM, KM, NM, 0, R)
+  call SpmlUtil_Init(IM, JM, KM, NM, KM, R)
   if (.not. isInitialzed()) call fail('init')
+
+  ! vertical grid: Chebyshev-Lobatto levels of [-1, 0] from the top, Clenshaw-Curtis weights
+  ! exact for polynomials of degree <= tMax = KM
+  call get_SpmlGridInfo(z_Sig_=z_sig, z_Sig_Weight_=z_sigw)
+  err = 0d0
+  do k = 0, KM
+    err = max(err, abs(z_sig(k) - (-0.5d0 + 0.5d0 * cos(acos(-1d0) * k / KM))))
+  end do
+  call check('g_Sig (Chebyshev-Lobatto levels)', err, 1d-15)
+  err = 0d0
+  do n = 0, KM
+    err = max(err, abs(sum(z_sigw * z_sig**n) - (0d0 - (-1d0)**(n + 1)) / (n + 1)))
+  end do
+  call check('g_Sig_WEIGHT integrates sigma**n exactly', err, 1d-15)
+  call check('g_Sig_WEIGHT public array', maxval(abs(g_Sig_WEIGHT - z_sigw)), 0d0)
 
   ! red-spectrum coefficients (deterministic)
   do l = 1, (NM + 1)**2
@@ -91,8 +114,29 @@ program spml_dropin_test
   xya = xya_GradLon_wa(wa)
   call check('calc_GradLonLat_xya_wa lon', maxval(abs(xya_lon - xya)), 1d-12)
 
-  ! per-layer calls from concurrent threads (each OpenMP thread gets its own stream)
-  !$omp parallel do private(k)
+  ! per-layer calls from 4 concurrent threads (each OpenMP thread gets its own stream), as
+  ! the reference's !$omp parallel do over layers makes them; every result must equal the
+  ! serial call bit for bit
+  do k = 1, 2*KM
+    wl(:, k) = cshift(w, k) * (1d0 + 0.1d0 * k)
+    ser_xy(:, :, k) = xy_GradLat_w(wl(:, k))
+    ser_w(:, k) = w_DivMu_xy(ser_xy(:, :, k))
+  end do
+  tid = -1
+  nthr = 1
+  !$omp parallel do private(k) num_threads(4) schedule(static, 1)
+  do k = 1, 2*KM
+    !$ tid(k) = omp_get_thread_num()
+    par_xy(:, :, k) = xy_GradLat_w(wl(:, k))
+    par_w(:, k) = w_DivMu_xy(par_xy(:, :, k))
+  end do
+  !$omp end parallel do
+  !$ nthr = count([(any(tid == k), k = 0, 3)])
+  write(*, '(a,i0)') 'threads that made per-layer calls: ', nthr
+  if (nthr < 4) call fail('per-layer calls did not run on 4 threads')
+  call check('per-layer calls from 4 threads = serial (grid)', maxval(abs(par_xy - ser_xy)), 0d0)
+  call check('per-layer calls from 4 threads = serial (spectral)', maxval(abs(par_w - ser_w)), 0d0)
+  !$omp parallel do private(k) num_threads(4)
   do k = 1, KM
     xya(:, :, k) = xy_w(wa(:, k))
   end do
@@ -104,6 +148,11 @@ program spml_dropin_test
   call check('per-layer calls', err, 1d-12)
 
   call column_checks()
+
+  if (command_argument_count() > 0) then
+    call get_command_argument(1, arg)
+    if (trim(arg) == 'badsize') call bad_size()
+  end if
 
   call SpmlUtil_Final()
   if (nfail > 0) then
@@ -177,6 +226,18 @@ contains
     e1 = maxval(abs(qa(:, :, 1:NK-1) - (T0 + DT * 1d-4)))
     call check('calc_VDiff_xyz momentum form (bottom friction)', max(e1, e2), 1d-14)
   end subroutine column_checks
+
+  ! a halo-sized (KA = nk + 2) layer-thickness vector: the shim must refuse it
+  subroutine bad_size()
+    integer, parameter :: NK = 3
+    real(DP) :: f(0:IM-1, JM, NK), qa(0:IM-1, JM, NK), cdk(NK + 2), fdk(NK + 1)
+    f = 1d0
+    cdk = 1d0
+    fdk = 1d0
+    write(*, '(a)') 'passing a KA-sized z_CDK'
+    call calc_VDiff_xyz(qa, f, f, f, f, cdk, fdk, 1d0, 1d0, 'D', 'N')
+    write(*, '(a)') 'BADSIZE NOT REFUSED'
+  end subroutine bad_size
 
   subroutine fail(what)
     character(*), intent(in) :: what

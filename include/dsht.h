@@ -86,6 +86,14 @@ int dsht_get_precision(const dsht_plan* plan);
    x_Lon_Weight, y_Lat_Weight (sum 2), xy_Lon, xy_Lat; any pointer may be NULL.   */
 int dsht_grid_info(const dsht_plan* plan, double* x_lon, double* y_lat, double* x_lon_weight,
                    double* y_lat_weight, double* xy_lon, double* xy_lat);
+/* The vertical half of get_SpmlGridInfo (SpmlUtil_mod.f90:288,291): g_Sig and
+   g_Sig_WEIGHT, the kmax+1 Chebyshev-Lobatto points of [sig_min, sig_max] and their
+   Clenshaw-Curtis weights for tmax Chebyshev modes, as at_Initial(kMax, tMax, SigMin,
+   SigMax) builds them (at_module_omp.f90:598-619; SpmlUtil_init passes SigMin = -1,
+   SigMax = 0, SpmlUtil_mod.f90:173-174,226).  Host-only; needs no plan or device.
+   Errors as at_Initial's: kmax, tmax > 0, kmax even, tmax <= kmax.  sig, sig_weight
+   hold kmax+1 values each (either may be NULL); sig[0] = sig_max.                 */
+int dsht_sig_grid(int kmax, int tmax, double sig_min, double sig_max, double* sig, double* sig_weight);
 /* nm_l / l_nm (cascaded, SpmlUtil_mod.f90:120): 1-based l; returns 0 for bad (n,m) */
 int dsht_nm_l(const dsht_plan* plan, int* n_of_l, int* m_of_l);
 int dsht_l_nm(const dsht_plan* plan, int n, int m);
@@ -166,6 +174,8 @@ int dsht_interpolate_d(dsht_plan* plan, const double* w, long sl, const double* 
  * (layer 0 on top: the reference's KS), each an I x J grid; cdk holds the km layer
  * thicknesses in sigma (z_CDK(KS:KE), DOGCM_GaussSpmVFvmGrid_mod.f90:400-402), device
  * memory.  Each result equals the reference loop evaluated in FP64, in its order.
+ * Outputs must not overlap any input (the kernels write outputs while inputs are still
+ * being read); overlapping calls are refused with DSHT_EARG.
  *
  * HBEDiagnose_OMG (HBEDiagnose_hspm_vfvm_mod.f90:90-143): omg holds the km+1 half levels
  * KS-1..KE (pitch ldo), half level k on top of layer k: omg[km] = 0,
@@ -193,7 +203,9 @@ int dsht_col_quick_d(dsht_plan* plan, const double* omg, long ldo, const double*
  * kv, e3 = xyz_H, z_CDK and the km+1 half-level spacings fdk = z_FDK(KS-1:KE); right-hand
  * side dt*rhs/e3; 'D' or 'N' surface and bottom rows) with LAPACK DGTSV's elimination and
  * back substitution, and writes qa = q0 + x (km layers, pitch ldo).  work: device scratch
- * of lwork >= 3*km*I*J doubles (contents undefined on return).  km >= 2.              */
+ * of lwork >= 3*km*I*J doubles (contents undefined on return).  km >= 2.
+ * No aliasing: qa and work must not overlap each other or any input (q0, rhs, kv, e3,
+ * bfric, cdk, fdk); an in-place call (qa = q0) is refused with DSHT_EARG.            */
 int dsht_col_vdiff_d(dsht_plan* plan, const double* q0, const double* rhs, const double* kv,
                      const double* e3, long ldg, int km, const double* cdk, const double* fdk,
                      double dt, double alpha, int sfc_bc, int btm_bc, double* qa, long ldo,

@@ -59,7 +59,8 @@ def gauss(J):
 
 
 def assoc(n, m, x, polys):
-    """P_n^m(x) and H_n^m(x) (normalised)."""
+    """P_n^m(x), H_n^m(x) = (1-x^2) dP_n^m/dx and cos(phi) dP_n^m/dx = dP_n^m/dphi
+    (normalised; x = sin(phi), so cos(phi) = (1-x^2)^{1/2})."""
     Dm = polys[(n, m)]
     Dm1 = polys[(n, m + 1)]
     s2 = 1 - x * x
@@ -67,7 +68,9 @@ def assoc(n, m, x, polys):
     P = norm * s2 ** (mp.mpf(m) / 2) * peval(Dm, x)
     # d/dx [(1-x^2)^{m/2} D_m] = (1-x^2)^{m/2} D_{m+1} - m x (1-x^2)^{m/2-1} D_m
     H = norm * s2 ** (mp.mpf(m) / 2) * (s2 * peval(Dm1, x) - m * x * peval(Dm, x))
-    return P, H
+    # the latitude derivative, evaluated with its own powers of (1-x^2) rather than from H
+    dphi = norm * (s2 ** (mp.mpf(m + 1) / 2) * peval(Dm1, x) - m * x * s2 ** (mp.mpf(m - 1) / 2) * peval(Dm, x))
+    return P, H, dphi
 
 
 def make(N, I, J, seed):
@@ -84,10 +87,13 @@ def make(N, I, J, seed):
     lam = [2 * mp.pi * i / I for i in range(I)]
     PT = {}
     HT = {}
+    DT = {}
     for n in range(N + 1):
         for m in range(n + 1):
             for j in range(J):
-                PT[n, m, j], HT[n, m, j] = assoc(n, m, mu[j], polys)
+                PT[n, m, j], HT[n, m, j], DT[n, m, j] = assoc(n, m, mu[j], polys)
+    # physical longitude and latitude of each node: phi = asin(mu), cos(phi) from phi itself
+    coslat = [mp.cos(mp.asin(x)) for x in mu]
     nn = np.floor(np.sqrt(np.arange(L))).astype(int)
     w_in = rng.standard_normal(L) * (nn + 1.0) ** -1.5
     xy_in = rng.standard_normal((J, I))
@@ -96,22 +102,28 @@ def make(N, I, J, seed):
         return n * (n + 1) + m
 
     def inverse(kind):
+        """Direct sums.  gradlon = (1/cos phi) d/dlambda and gradlat = d/dphi: the unit-sphere
+        gradient components that the reference's callers divide by the radius
+        (HBEBaroc_hspm_vfvm_mod.f90:403,416; LPhys_RediGM_hspm_vfvm_mod.f90:433-434)."""
+        table = {"gradmu": HT, "gradlat": DT}.get(kind, PT)
+        dlam = kind in ("gradlambda", "gradlon")
         out = np.zeros((J, I))
         for j in range(J):
             for i in range(I):
                 acc = mp.mpf(0)
                 for n in range(N + 1):
-                    T = HT[n, 0, j] if kind == "gradmu" else PT[n, 0, j]
-                    if kind != "gradlambda":
-                        acc += mp.mpf(w_in[lidx(n, 0)]) * T
+                    if not dlam:
+                        acc += mp.mpf(w_in[lidx(n, 0)]) * table[n, 0, j]
                     for m in range(1, n + 1):
                         a, b = mp.mpf(w_in[lidx(n, m)]), mp.mpf(w_in[lidx(n, -m)])
-                        T = HT[n, m, j] if kind == "gradmu" else PT[n, m, j]
+                        T = table[n, m, j]
                         c, s = mp.cos(m * lam[i]), mp.sin(m * lam[i])
-                        if kind == "gradlambda":
+                        if dlam:
                             acc += 2 * m * (-a * s - b * c) * T
                         else:
                             acc += 2 * (a * c - b * s) * T
+                if kind == "gradlon":
+                    acc /= coslat[j]
                 out[j, i] = float(acc)
         return out
 
@@ -148,6 +160,7 @@ def make(N, I, J, seed):
         mu=np.array([float(x) for x in mu]), wq=np.array([float(x) for x in wq]),
         w_in=w_in, xy_in=xy_in,
         inv_val=inverse("val"), inv_gradlambda=inverse("gradlambda"), inv_gradmu=inverse("gradmu"),
+        inv_gradlon=inverse("gradlon"), inv_gradlat=inverse("gradlat"),
         fwd_val=forward("val"), fwd_divlambda=forward("divlambda"), fwd_divmu=forward("divmu"),
     )
 

@@ -58,3 +58,59 @@ def test_fortran_shim_compiles():
         r = subprocess.run([fc, "-c", src, "-module-dir", d, "-o", os.path.join(d, "m.o")],
                            capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("kmax,tmax", [(2, 2), (8, 8), (20, 20), (60, 60), (60, 40), (12, 7)])
+def test_sig_grid_clenshaw_curtis(kmax, tmax):
+    """g_Sig / g_Sig_WEIGHT (host-only, no device): Chebyshev-Lobatto levels of [-1, 0] from
+    the top, sig_i = -1/2 + cos(pi i/kmax)/2, and weights that integrate every polynomial of
+    degree <= tmax (odd part: every degree up to kmax by symmetry) exactly over [-1, 0]."""
+    import numpy as np
+    import dsht
+    sig, w = dsht.sig_grid(kmax, tmax)
+    i = np.arange(kmax + 1)
+    assert np.abs(sig - (-0.5 + 0.5 * np.cos(np.pi * i / kmax))).max() < 1e-15
+    assert sig[0] == 0.0 and sig[-1] == -1.0
+    assert np.abs(w - w[::-1]).max() < 1e-15  # symmetric about the interval's midpoint
+    for deg in range(tmax + 1):
+        exact = (0.0 - (-1.0) ** (deg + 1)) / (deg + 1)  # int_{-1}^{0} x^deg dx
+        assert abs(np.dot(w, sig ** deg) - exact) < 1e-14, (deg, np.dot(w, sig ** deg), exact)
+    if tmax == kmax:
+        # a planted mistake (ends not halved) breaks the constant's integral
+        bad = w.copy()
+        bad[0] *= 2
+        bad[-1] *= 2
+        assert abs(bad.sum() - 1.0) > 1e-8
+
+
+def test_sig_grid_rejects_what_at_initial_rejects():
+    import dsht
+    for args in [(0, 1), (5, 4), (8, 10), (8, 0)]:
+        with pytest.raises(dsht.DshtError):
+            dsht.sig_grid(*args)
+
+
+def test_host_column_wrappers_check_vector_lengths():
+    """The host-array column wrappers refuse layer vectors of the wrong length (a short one
+    would be over-read by the C entry point, a halo-sized KA one read shifted) before any
+    library call, so this runs without a device."""
+    import numpy as np
+    import dsht
+    p = object.__new__(dsht.Plan)  # argument checks only: no plan handle is needed
+    p.im, p.jm = 8, 4
+    km = 3
+    f = np.zeros((km, 4, 8))
+    half = np.zeros((km + 1, 4, 8))
+    good, ka = np.ones(km), np.ones(km + 2)
+    with pytest.raises(ValueError, match="z_CDK"):
+        p.xyr_OMG(f, f, ka)
+    with pytest.raises(ValueError, match="z_CDK"):
+        p.xyz_HydPres(f, f, good[:2], 9.8)
+    with pytest.raises(ValueError, match="z_CDK"):
+        p.xy_UVBarot(f, f, f, ka)
+    with pytest.raises(ValueError, match="z_RFDK"):
+        p.xyr_ADVFlxZ_QUICK(half, f, good, np.ones(km))
+    with pytest.raises(ValueError, match="z_FDK"):
+        p.xyz_VDiff(f, f, f, f, good, np.ones(km + 2), 1.0, 1.0)
+    with pytest.raises(ValueError, match="z_CDK"):
+        p.xyz_VDiff(f, f, f, f, ka, np.ones(km + 1), 1.0, 1.0)

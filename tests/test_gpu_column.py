@@ -304,3 +304,45 @@ def test_column_host_entries(gpu):
     assert np.array_equal(got, ref)
     with pytest.raises(dsht.DshtError):
         p.xyr_ADVFlxZ_QUICK(omg[:2], trc[:1], x["cdk"][:1], rfdk[:2])  # km = 1
+
+
+def test_column_in_place_calls_are_refused(gpu):
+    """The column kernels write their outputs while their inputs are still being read, so an
+    output that overlaps an input (out=q0 in the diffusion solve, omega over div, a flux over
+    omega) is an argument error, not a silently wrong result.  Disjoint calls still work."""
+    import torch
+    import dsht
+    import oracle
+    N, I, J, km = 21, 64, 32, 6
+    p = dsht.Plan(N, I, J, radius=6.371e6)
+    q0, rhs, kv, e3, cdk, fdk = _vdiff_case(km, J, I, 3, True)
+    dq0, drhs, dkv, de3 = (_dev(a, gpu) for a in (q0, rhs, kv, e3))
+    dc, df = _dev(cdk, gpu), _dev(fdk, gpu)
+    for bad in (dq0, drhs, dkv, de3):
+        with pytest.raises(dsht.DshtError, match="overlaps"):
+            p.col_vdiff_d(dq0, drhs, dkv, de3, dc, df, 600.0, 1.0, "D", "N", out=bad)
+    # a workspace view that starts inside q0 (shared storage)
+    big = torch.zeros(4 * km * J * I, dtype=torch.float64, device=gpu)
+    qv = big[: km * J * I].view(km, J, I)
+    qv.copy_(dq0)
+    with pytest.raises(dsht.DshtError, match="overlaps"):
+        p.col_vdiff_d(qv, drhs, dkv, de3, dc, df, 600.0, 1.0, "D", "N", work=big[km * J * I // 2:])
+    out = p.col_vdiff_d(dq0, drhs, dkv, de3, dc, df, 600.0, 1.0, "D", "N")
+    torch.cuda.synchronize()
+    ref, _ = oracle.col_vdiff(q0, rhs, kv, e3, cdk, fdk, 600.0, 1.0, "D", "N")
+    assert np.array_equal(out.cpu().numpy(), ref)
+    x = _inputs(km, J, I, seed=4)
+    ddiv, dh = _dev(x["div"], gpu), _dev(x["h"], gpu)
+    dcdk = _dev(x["cdk"], gpu)
+    alias = torch.zeros((km + 1, J, I), dtype=torch.float64, device=gpu)
+    alias[:km].copy_(ddiv)
+    with pytest.raises(dsht.DshtError, match="overlaps"):
+        p.col_omega_d(alias[:km], dh, dcdk, out=alias)
+    with pytest.raises(dsht.DshtError, match="overlaps"):
+        p.col_hydpres_d(ddiv, dh, dcdk, GRAV, out=dh)
+    omg = p.col_omega_d(ddiv, dh, dcdk)
+    rfdk = _dev(np.full(km + 1, 12.0), gpu)
+    with pytest.raises(dsht.DshtError, match="overlaps"):
+        p.col_quick_d(omg, ddiv, dcdk, rfdk, out=omg)
+    with pytest.raises(dsht.DshtError, match="overlaps"):
+        p.col_uvbarot_d(ddiv, dh, dh, dcdk, ub=ddiv[0])

@@ -128,6 +128,54 @@ def test_device_api_layouts(gpu, layout):
     assert rel_l2(got, w) < TOL
 
 
+@pytest.mark.parametrize("N,I,J", [(85, 256, 128), (170, 512, 256)])
+def test_every_kind_at_full_size(gpu, N, I, J):
+    """Every inverse kind (VAL, GRADLAMBDA, GRADMU, GRADLON, GRADLAT and the fused value +
+    GradLon + GradLat call) and every forward kind (VAL, DIVLAMBDA, DIVMU, the 𝓗 operator,
+    UV <-> vor/div) against the oracle at the benchmark grids T85 (256x128) and T170
+    (512x256), through the device entry points on 36 columns (two full 16-column tiles and a
+    ragged one), spectra in the fast "LC" layout the bench uses."""
+    import torch
+    import dsht
+    p, o = _pair(N, I, J)
+    nc = 36
+    w = red_spectrum(N, nc, seed=300 + N)
+    wt = torch.tensor(w.T.copy(), device=gpu)
+    for kind in ("VAL", "GRADLAMBDA", "GRADMU", "GRADLON", "GRADLAT"):
+        k = getattr(dsht, kind)
+        xy = torch.empty((nc, J, I), dtype=torch.float64, device=gpu)
+        p.inv_d(k, wt, xy)
+        torch.cuda.synchronize()
+        err = rel_l2(xy.cpu().numpy(), o.inv(w, k))
+        assert err < TOL, (N, kind, err)
+    outs = [torch.empty((nc, J, I), dtype=torch.float64, device=gpu) for _ in range(3)]
+    p.inv_vgrad_d(wt, *outs)
+    torch.cuda.synchronize()
+    for kind, t in zip(("VAL", "GRADLON", "GRADLAT"), outs):
+        err = rel_l2(t.cpu().numpy(), o.inv(w, getattr(dsht, kind)))
+        assert err < TOL, (N, "fused", kind, err)
+    rng = np.random.default_rng(400 + N)
+    xy = rng.standard_normal((nc, J, I)) * o.coslat[None, :, None]
+    xt = torch.tensor(xy, device=gpu)
+    for kind in ("VAL", "DIVLAMBDA", "DIVMU"):
+        k = getattr(dsht, kind)
+        out = torch.zeros((p.lm, nc), dtype=torch.float64, device=gpu)
+        p.fwd_d(k, xt, out)
+        torch.cuda.synchronize()
+        err = rel_l2(out.cpu().numpy().T, o.fwd(xy, k))
+        assert err < TOL, (N, kind, err)
+    A, B = xy[:12], xy[12:24]
+    assert rel_l2(p.w_AlphaOptr_xy(A, B), o.alpha_fwd(A, B)) < TOL
+    vor, div = p.calc_UVCosLat2VorDiv(A, B)
+    rv, rd = o.uv2vordiv(A, B)
+    assert rel_l2(vor, rv) < TOL and rel_l2(div, rd) < TOL
+    wa, wb = w[:12], w[12:24]
+    assert rel_l2(p.xy_AlphaOptr_w(wa, wb), o.alpha_inv(wa, wb)) < TOL
+    u, v = p.calc_VorDiv2UV(wa, wb)
+    ru, rv2 = o.vordiv2uv(wa, wb)
+    assert rel_l2(u, ru) < TOL and rel_l2(v, rv2) < TOL
+
+
 @pytest.mark.parametrize("chunk,nc", [(32, 80), (0, 800)])
 def test_roundtrip_t170_chunked(gpu, chunk, nc):
     """T170 (I=512, J=256) round trip over several scratch chunks: a forced 32-column chunk
@@ -225,6 +273,62 @@ def test_fortran_dropin(gpu):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "PASS" in r.stdout
+    assert "threads that made per-layer calls: 4" in r.stdout
+
+
+def test_fortran_dropin_refuses_halo_sized_vectors(gpu):
+    """A KA-sized (halo) z_CDK passed to a column wrapper stops the program with the shim's
+    shape error instead of being read as shifted layers."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(__file__)), "dennou-ogcm_amd", "spml_dropin_test")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran test program not built (amdflang missing at build time)")
+    r = subprocess.run([exe, "badsize"], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode != 0
+    assert "z_CDK has 5 entries, expected 3" in r.stdout
+    assert "BADSIZE NOT REFUSED" not in r.stdout
+
+
+@pytest.mark.parametrize("nthreads", [4, 8])
+def test_host_api_concurrent_threads(gpu, nthreads):
+    """Several host threads call dsht_xy_w / dsht_w_xy / dsht_xy_vgrad_w on their own layers at
+    once (the reference's OpenMP loops over k); every result equals the serial call bit for
+    bit."""
+    import threading
+    import dsht
+    N, I, J, K = 42, 128, 64, 24
+    p = dsht.Plan(N, I, J, kmax=K, radius=6.371e6)
+    w = red_spectrum(N, K, 77)
+    ser = [(p._inv(w[k], dsht.GRADLAT), p.xy_GradLonLat_w(w[k])) for k in range(K)]
+    ser_f = [p._fwd(ser[k][0], dsht.DIVLAMBDA) for k in range(K)]
+    out = [None] * K
+    errors = []
+
+    def worker(t):
+        try:
+            for rep in range(3):
+                for k in range(t, K, nthreads):
+                    g = p._inv(w[k], dsht.GRADLAT)
+                    vg = p.xy_GradLonLat_w(w[k])
+                    f = p._fwd(g, dsht.DIVLAMBDA)
+                    out[k] = (g, vg, f)
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    ts = [threading.Thread(target=worker, args=(t,)) for t in range(nthreads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+    for k in range(K):
+        g, vg, f = out[k]
+        assert np.array_equal(g, ser[k][0]), k
+        for a, b in zip(vg, ser[k][1]):
+            assert np.array_equal(a, b), k
+        assert np.array_equal(f, ser_f[k]), k
 
 
 @pytest.mark.parametrize("leg", [0, 1, 2])

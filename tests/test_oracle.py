@@ -102,6 +102,8 @@ def test_golden_vectors(T):
     assert rel_l2(o.inv(w)[0], g["inv_val"]) < tol
     assert rel_l2(o.inv(w, O.GRADLAMBDA)[0], g["inv_gradlambda"]) < tol
     assert rel_l2(o.inv(w, O.GRADMU)[0], g["inv_gradmu"]) < tol
+    assert rel_l2(o.inv(w, O.GRADLON)[0], g["inv_gradlon"]) < tol
+    assert rel_l2(o.inv(w, O.GRADLAT)[0], g["inv_gradlat"]) < tol
     assert rel_l2(o.fwd(xy)[0], g["fwd_val"]) < tol
     assert rel_l2(o.fwd(xy, O.DIVLAMBDA)[0], g["fwd_divlambda"]) < tol
     assert rel_l2(o.fwd(xy, O.DIVMU)[0], g["fwd_divmu"]) < tol
@@ -210,3 +212,76 @@ def test_coefficient_tables():
     assert abs(h[o.l_nm(3, 1)] - 1.0 / (12 / R ** 2 + 1e-12)) < 1e-18 * R ** 2
     v = o.coef_hvisc(1e3, 0.0, 8)
     assert abs(v[o.l_nm(1, 0)]) < 1e-20  # rigid rotation (n = 1) is not damped by viscosity
+
+
+def _gradlonlat_modes(o):
+    """Closed forms of single spherical harmonics on the oracle's grid, written in
+    elementary functions of (lambda, phi) only.  The grid field of a coefficient pair is
+    w(n,0) P_n^0 + 2 sum_m (w(n,m) cos m lambda - w(n,-m) sin m lambda) P_n^m, with P
+    normalised to int P^2 dmu = 2 and no Condon-Shortley phase.  GradLon = (1/cos phi)
+    d/dlambda and GradLat = d/dphi (the unit-sphere gradient; the reference's callers
+    divide by the radius, HBEBaroc_hspm_vfvm_mod.f90:403,416)."""
+    from math import factorial
+    lam = o.lon[None, :]
+    phi = o.lat[:, None]
+    c, s = np.cos(phi), np.sin(phi)
+    modes = []
+    # Y_1^0 = sqrt(3) sin(phi)
+    modes.append(((1, 0), 0 * lam * c, np.sqrt(3) * c * np.ones_like(lam)))
+    # Y_1^1: sqrt(6) cos(phi) cos(lambda); the imaginary part: -sqrt(6) cos(phi) sin(lambda)
+    r6 = np.sqrt(6.0)
+    modes.append(((1, 1), -r6 * np.sin(lam) * np.ones_like(c), -r6 * s * np.cos(lam)))
+    modes.append(((1, -1), -r6 * np.cos(lam) * np.ones_like(c), r6 * s * np.sin(lam)))
+    # tesseral Y_2^1: sqrt(15/2) sin(2 phi) cos(lambda)
+    k = np.sqrt(7.5)
+    modes.append(((2, 1), -2 * k * s * np.sin(lam), 2 * k * np.cos(2 * phi) * np.cos(lam)))
+    # sectoral Y_n^n: 2 c_n cos^n(phi) cos(n lambda), c_n^2 = (2n+1)!/(4^n (n!)^2)
+    for n in (3, 5):
+        cn = np.sqrt(factorial(2 * n + 1) / (4.0 ** n * factorial(n) ** 2))
+        modes.append(((n, n), -2 * n * cn * c ** (n - 1) * np.sin(n * lam),
+                      -2 * n * cn * c ** (n - 1) * s * np.cos(n * lam)))
+    return modes
+
+
+@pytest.mark.parametrize("N,I,J", [(5, 16, 8), (21, 64, 32), (42, 128, 64)])
+def test_gradlonlat_closed_forms(N, I, J):
+    """GRADLON and GRADLAT of Y_1^0, Y_1^{+-1}, a tesseral Y_2^1 and sectoral Y_3^3, Y_5^5
+    against their closed forms."""
+    o = O.Oracle(N, I, J)
+    for (n, m), glon, glat in _gradlonlat_modes(o):
+        w = np.zeros(o.L)
+        w[o.l_nm(n, m)] = 1.0
+        got_lon = o.inv(w, O.GRADLON)[0]
+        got_lat = o.inv(w, O.GRADLAT)[0]
+        scale = max(np.abs(glon).max(), np.abs(glat).max())
+        assert np.abs(got_lon - glon).max() < 1e-13 * scale, (n, m)
+        assert np.abs(got_lat - glat).max() < 1e-13 * scale, (n, m)
+
+
+@pytest.mark.parametrize("mistake", ["cos", "1/cos^2", "none"])
+def test_gradlonlat_checks_detect_planted_mistake(mistake):
+    """The closed forms and the golden vectors are not vacuous for the cos(phi) factor:
+    GradLon/GradLat formed with cos(phi) or 1/cos^2(phi) in place of 1/cos(phi) (a mistake
+    planted in the oracle's output, on the CPU) fail both."""
+    o = O.Oracle(21, 64, 32)
+    c = o.coslat[:, None]
+    fac = {"cos": c, "1/cos^2": 1.0 / c ** 2, "none": 1.0 / c}[mistake]
+    worst = 0.0
+    for (n, m), glon, glat in _gradlonlat_modes(o):
+        w = np.zeros(o.L)
+        w[o.l_nm(n, m)] = 1.0
+        lon = o.inv(w, O.GRADLAMBDA)[0] * fac
+        lat = o.inv(w, O.GRADMU)[0] * fac
+        for got, ref in ((lon, glon), (lat, glat)):
+            if np.abs(ref).max() > 0:
+                worst = max(worst, rel_l2(got, ref))
+    g = np.load(os.path.join(GOLD, "sht_T10.npz"))
+    og = O.Oracle(int(g["N"]), int(g["I"]), int(g["J"]))
+    cg = og.coslat[:, None]
+    fg = {"cos": cg, "1/cos^2": 1.0 / cg ** 2, "none": 1.0 / cg}[mistake]
+    gold = max(rel_l2(og.inv(g["w_in"], O.GRADLAMBDA)[0] * fg, g["inv_gradlon"]),
+               rel_l2(og.inv(g["w_in"], O.GRADMU)[0] * fg, g["inv_gradlat"]))
+    if mistake == "none":
+        assert worst < 1e-13 and gold < 1e-13
+    else:
+        assert worst > 0.1 and gold > 0.1, (worst, gold)
