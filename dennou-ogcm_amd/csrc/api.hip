@@ -216,17 +216,33 @@ static int run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, l
 }
 
 // inverse family: spectra -> grids
+// fixed/fixed_bytes: a caller-provided scratch (the static one of dsht_inv_noalloc_d); the
+// call then takes no workspace, runs without the chunk pipeline and cuts its columns into
+// chunks whose Fourier arrays fit the scratch
 static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, long sl, long sc,
                    double* out0, double* out1, long ldg, int ncols, const double* pro, const double* rs,
-                   hipStream_t s, double* out2 = nullptr) {
+                   hipStream_t s, double* out2 = nullptr, void* fixed = nullptr, size_t fixed_bytes = 0) {
   if (ncols <= 0) return 0;
   const int nout = inv_nout(op);
-  const int ch = chunk_cols(p, ncols);
+  int ch = chunk_cols(p, ncols);
+  if (fixed) {
+    // whole 16-column groups of every output's Fourier array within the scratch
+    const size_t per16 = (size_t)nout * fourier_bytes(p, 16);
+    const long fit = (long)(fixed_bytes / per16) * 16;
+    if (fit < 16)
+      return fail(DSHT_EARG, "the static scratch of %zu bytes cannot hold 16 columns (%zu bytes) of this plan",
+                  fixed_bytes, per16);
+    if (ch > fit) ch = (int)fit;
+  }
   const int nchunks = cdiv(ncols, ch);
   const size_t slot_bytes = nout * fourier_bytes(p, ch);
   WsHandle wl;
-  int rc = workspace_acquire(p, s, (p->pipeline && nchunks > 1 ? 2 : 1) * slot_bytes, wl);
-  if (rc) return rc;
+  int rc = 0;
+  if (fixed) {
+    wl.buf = fixed;  // no Workspace: pipe_begin leaves the chunk pipeline off
+  } else if ((rc = workspace_acquire(p, s, (p->pipeline && nchunks > 1 ? 2 : 1) * slot_bytes, wl))) {
+    return rc;
+  }
   ChunkPipe cp;
   if ((rc = pipe_begin(wl, s, p, nchunks, cp))) return rc;
   for (int k = 0, c0 = 0; c0 < ncols; k++, c0 += ch) {
@@ -286,6 +302,48 @@ static int check_spec(const void* w, long sl, long sc, int ncols, const dsht_pla
   if (sl <= 0 || sc <= 0) return fail(DSHT_EARG, "spectral strides must be positive");
   bool lc = sc == 1 && sl >= ncols, cl = sl == 1 && sc >= p->L;
   if (!(lc || cl || ncols == 1)) return fail(DSHT_EARG, "spectral strides (sl=%ld, sc=%ld) overlap", sl, sc);
+  return 0;
+}
+
+// ---------------------------------------------------------------- static scratch
+// The Fourier scratch of dsht_inv_noalloc_d: a file-scope device array, one copy per GPU,
+// held until the process ends.  Host code never passes the array's name to a kernel (on the
+// host it is only a stand-in for the symbol); it uses the device address that
+// hipGetSymbolAddress returns for the current device, looked up once per device at the
+// first dsht_create on it.
+constexpr size_t NOALLOC_SCRATCH_BYTES = DSHT_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) double2 scratch[NOALLOC_SCRATCH_BYTES / sizeof(double2)];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[MAX_DEVICES];
+static hipError_t g_scratch_err[MAX_DEVICES];
+static bool g_scratch_done[MAX_DEVICES];
+
+// with `device` current: resolve this device's copy of the scratch (once per device)
+static void scratch_init(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    ptr = nullptr;
+  }
+  g_scratch_ptr[device] = ptr;
+  g_scratch_err[device] = e;
+  g_scratch_done[device] = true;
+}
+
+static int scratch_of(int device, void** out) {
+  if (device < 0 || device >= MAX_DEVICES) return fail(DSHT_EDEVICE, "device %d out of range", device);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_done[device]) return fail(DSHT_ESTATE, "static scratch of device %d not initialised", device);
+  if (!g_scratch_ptr[device])
+    return fail(DSHT_EDEVICE, "hipGetSymbolAddress of the static scratch failed on device %d: %s", device,
+                hipGetErrorString(g_scratch_err[device]));
+  *out = g_scratch_ptr[device];
   return 0;
 }
 
@@ -423,6 +481,7 @@ int dsht_create(int nmax, int im, int jm, int kmax, double radius, dsht_plan** p
     delete p;
     return fail(DSHT_EDEVICE, "dsht_create: hipGetDevice failed");
   }
+  scratch_init(p->device);  // the plan's device is current here
   int rc = fft_plan_init(p);
   if (!rc) rc = build_tables(p);
   if (rc) {
@@ -630,6 +689,31 @@ int dsht_inv_d(dsht_plan* p, int kind, const double* w, long sl, long sc, double
   if (int rc = check_grid_ptr(xy, ldg, p)) return rc;
   if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
   return dispatch_inv(p, op, w, nullptr, sl, sc, xy, nullptr, ldg, ncols, diag, rs, (hipStream_t)stream);
+}
+
+int dsht_inv_noalloc_d(dsht_plan* p, int kind, const double* w, long sl, long sc, double* xy, long ldg, int ncols,
+                       const double* diag, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  int op;
+  const double* rs = nullptr;
+  switch (kind) {
+    case DSHT_VAL: op = I_VAL; break;
+    case DSHT_GRADLAMBDA: op = I_GRADLAMBDA; break;
+    case DSHT_GRADMU: op = I_GRADMU; break;
+    case DSHT_GRADLON: op = I_GRADLAMBDA; rs = p->d_rs_icos; break;
+    case DSHT_GRADLAT: op = I_GRADMU; rs = p->d_rs_icos; break;
+    default: return fail(DSHT_EARG, "dsht_inv_noalloc_d: kind %d is not an inverse kind", kind);
+  }
+  if (p->prec != DSHT_PREC_FP64)
+    return fail(DSHT_EUNSUPPORTED, "dsht_inv_noalloc_d runs the FP64 form only (the plan is set to form %d)", p->prec);
+  if (ncols < 0) return fail(DSHT_EARG, "ncols < 0");
+  if (ncols == 0) return 0;
+  if (int rc = check_grid_ptr(xy, ldg, p)) return rc;
+  if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
+  void* buf = nullptr;
+  if (int rc = scratch_of(p->device, &buf)) return rc;
+  return run_inv(p, op, w, nullptr, sl, sc, xy, nullptr, ldg, ncols, diag, rs, (hipStream_t)stream, nullptr, buf,
+                 NOALLOC_SCRATCH_BYTES);
 }
 
 int dsht_inv_vgrad_d(dsht_plan* p, const double* w, long sl, long sc, double* xy, double* xy_lon, double* xy_lat,

@@ -36,7 +36,7 @@ EXPORTS = [
     "dsht_get_precision", "dsht_grid_info", "dsht_sig_grid", "dsht_nm_l", "dsht_l_nm", "dsht_rn", "dsht_coef",
     "dsht_w_xy", "dsht_xy_w", "dsht_xy_vgrad_w", "dsht_w_alpha_xy", "dsht_xy_alpha_w", "dsht_uvcoslat2vordiv",
     "dsht_vordiv2uv", "dsht_w_diag_w", "dsht_xy_diag_xy", "dsht_intlonlat", "dsht_avrlonlat",
-    "dsht_ya_intlon", "dsht_ya_avrlon", "dsht_interpolate", "dsht_fwd_d", "dsht_inv_d", "dsht_inv_vgrad_d",
+    "dsht_ya_intlon", "dsht_ya_avrlon", "dsht_interpolate", "dsht_fwd_d", "dsht_inv_d", "dsht_inv_vgrad_d", "dsht_inv_noalloc_d",
     "dsht_alpha_fwd_d", "dsht_alpha_inv_d", "dsht_uv2vordiv_d", "dsht_vordiv2uv_d",
     "dsht_diag_d", "dsht_intlonlat_d", "dsht_avrlonlat_d", "dsht_ya_intlon_d",
     "dsht_interpolate_d", "dsht_coef_d", "dsht_col_omega_d", "dsht_col_hydpres_d", "dsht_col_uvbarot_d",
@@ -98,6 +98,7 @@ def load():
         "dsht_interpolate": ([vp, vp, vp, vp, vp, i], i),
         "dsht_fwd_d": ([vp, i, vp, lg, vp, lg, lg, i, vp, vp], i),
         "dsht_inv_d": ([vp, i, vp, lg, lg, vp, lg, i, vp, vp], i),
+        "dsht_inv_noalloc_d": ([vp, i, vp, lg, lg, vp, lg, i, vp, vp], i),
         "dsht_inv_vgrad_d": ([vp, vp, lg, lg, vp, vp, vp, lg, i, vp, vp], i),
         "dsht_alpha_fwd_d": ([vp, vp, vp, lg, vp, lg, lg, i, vp, vp], i),
         "dsht_alpha_inv_d": ([vp, vp, vp, lg, lg, vp, lg, i, vp], i),
@@ -433,6 +434,15 @@ class Plan:
         sl, sc = self._spec_strides(w, nc, layout)
         _check(load().dsht_inv_d(self.h, kind, _tptr(w), sl, sc, _tptr(xy), xy.stride(0), nc,
                                  _tptr(diag) if diag is not None else None, _stream(stream)))
+        return xy
+
+    def inv_noalloc_d(self, kind, w, xy, layout="LC", diag=None, stream=None):
+        """inv_d through dsht_inv_noalloc_d: no device allocation (static scratch shared by every
+        call on the GPU; FP64 form only)"""
+        nc = xy.shape[0]
+        sl, sc = self._spec_strides(w, nc, layout)
+        _check(load().dsht_inv_noalloc_d(self.h, kind, _tptr(w), sl, sc, _tptr(xy), xy.stride(0), nc,
+                                         _tptr(diag) if diag is not None else None, _stream(stream)))
         return xy
 
     def inv_vgrad_d(self, w, xy, xy_lon, xy_lat, layout="LC", diag=None, stream=None):

@@ -146,6 +146,18 @@ int dsht_fwd_d(dsht_plan* plan, int kind, const double* xy, long ldg, double* w,
                int ncols, const double* diag, void* stream);
 int dsht_inv_d(dsht_plan* plan, int kind, const double* w, long sl, long sc, double* xy, long ldg,
                int ncols, const double* diag, void* stream);
+/* No counterpart in the reference: dsht_inv_d for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as dsht_inv_d, FP64 form only (other forms:
+ * DSHT_EUNSUPPORTED).  It calls no hipMalloc/hipFree: its Fourier scratch is a static
+ * device array of DSHT_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, and
+ * columns are processed in chunks that fit it.  A plan whose 16-column Fourier block
+ * ((nmax+1)*jm*16*16 bytes) exceeds it is refused with DSHT_EARG.  Every call on one GPU
+ * uses the same scratch: calls that overlap in time on one GPU (different streams or
+ * threads) race and must be serialised by the caller.                                   */
+#define DSHT_NOALLOC_SCRATCH_BYTES (64u << 20)
+int dsht_inv_noalloc_d(dsht_plan* plan, int kind, const double* w, long sl, long sc, double* xy,
+                       long ldg, int ncols, const double* diag, void* stream);
 /* device form of dsht_xy_vgrad_w: value, GradLon and GradLat grids of each column */
 int dsht_inv_vgrad_d(dsht_plan* plan, const double* w, long sl, long sc, double* xy, double* xy_lon,
                      double* xy_lat, long ldg, int ncols, const double* diag, void* stream);
