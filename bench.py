@@ -42,8 +42,9 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 
 
-def synth_fields(torch, dev, L):
-    """Spectral state (L, 4*KM) in the fast LC layout and tendency grids (6*KM, J, I)."""
+def synth_fields(torch, dev, L, KM=KM):
+    """Spectral state (L, 4*KM) in the fast LC layout (seeded red spectrum, grid RMS per field,
+    depth-dependent T/S backgrounds)."""
     g = torch.Generator(device=dev)
     g.manual_seed(0x0D0C2026)
     n = torch.floor(torch.sqrt(torch.arange(L, device=dev, dtype=torch.float64)))
@@ -107,6 +108,24 @@ def time_steps(torch, plan, dsht, bufs, steps, warmup, min_warm_s=0.3, fused=Tru
     return wall, lat
 
 
+def latency_loop(torch, plan, dsht, bufs, nsteps):
+    """Per-step latency distribution from `nsteps` back-to-back steps, each bracketed by its own
+    event pair on the launch stream (p50/p99 over >= 1000 steps)."""
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(5):
+        run_step(plan, dsht, bufs, stream)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(nsteps)]
+    for i in range(nsteps):
+        ev[i][0].record()
+        run_step(plan, dsht, bufs, stream)
+        ev[i][1].record()
+    torch.cuda.synchronize()
+    lat = np.array([a.elapsed_time(b) for a, b in ev])
+    return {"steps": nsteps, "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
+            "max_step_ms": float(lat.max()), "mean_step_ms": float(lat.mean())}
+
+
 def kernel_profile(torch, plan, dsht, bufs, steps):
     stream = torch.cuda.current_stream().cuda_stream
     dsht.prof_reset()
@@ -125,35 +144,62 @@ def kernel_profile(torch, plan, dsht, bufs, steps):
     return out
 
 
-def cpu_baseline(seconds=12.0):
-    """FP64 CPU oracle on a bounded sample of the same workload: 60 columns (one field
-    over KM=60 layers) inverse + forward, repeated until ~`seconds` have elapsed."""
+ORACLE_BUILD = ["prebuilt oracle/libdsht_oracle.so"]
+
+
+def prepare_cpu_oracle():
+    """Compile the oracle with -O3 -march=native -fopenmp for this host and load that build
+    (before anything else loads the oracle); keeps the prebuilt library if that fails."""
+    import tempfile
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
+    try:
+        oracle.use_library(oracle.build_native(tempfile.mkdtemp(prefix="dsht_oracle_native_")))
+        ORACLE_BUILD[0] = "gcc -O3 -march=native -fopenmp on this host"
+    except Exception as e:  # noqa: BLE001 - keep the prebuilt library if the host compiler fails
+        print(f"[bench] native oracle build failed ({e}); timing the prebuilt library", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(spec, tend, reps=21, warm=2):
+    """The FP64 CPU oracle on the benchmarked step itself: the 4 prognostic fields x 60 layers
+    to grid with both derivatives (VAL, GRADLON, GRADLAT of 240 columns) and the 6 x 60
+    tendency grids back to spectral, on the same inputs as the GPU step.  The oracle is
+    compiled here with -O3 -march=native -fopenmp (OpenMP over columns) for the host that
+    runs it; the value is the median of `reps` repetitions after `warm` untimed ones.  A
+    one-thread figure comes from the same step restricted to 4 layers."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    build = ORACLE_BUILD[0]
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     oracle.set_threads(threads)
     o = oracle.Oracle(NMAX, IM, JM, RADIUS)
-    rng = np.random.default_rng(1)
-    nl = np.floor(np.sqrt(np.arange(o.L)))
-    w = rng.standard_normal((KM, o.L)) * np.sqrt(0.5 * (nl + 1.0) ** -3)
-    o.fwd(o.inv(w[:2]))  # warm
-    pairs = 0
-    t0 = time.perf_counter()
-    while True:
-        o.fwd(o.inv(w))
-        pairs += KM
-        if time.perf_counter() - t0 >= seconds:
-            break
-    dt = time.perf_counter() - t0
-    # one thread, a shorter sample of the same work
+    w = spec.T.cpu().numpy().copy()      # (4*KM, L)
+    xy = tend.cpu().numpy()              # (6*KM, J, I)
+
+    def step(wc, xc):
+        for k in (oracle.VAL, oracle.GRADLON, oracle.GRADLAT):
+            o.inv(wc, k)
+        o.fwd(xc)
+
+    for _ in range(warm):
+        step(w, xy)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        step(w, xy)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    # one thread: the same step over 4 of the 60 layers (16 inverse and 24 forward columns)
     oracle.set_threads(1)
-    p1 = 0
-    t1 = time.perf_counter()
-    while time.perf_counter() - t1 < min(3.0, seconds / 4):
-        o.fwd(o.inv(w[:4]))
-        p1 += 4
-    d1 = time.perf_counter() - t1
+    lay = [f * KM + k for f in range(N_PROG) for k in range(4)]
+    layt = [f * KM + k for f in range(N_TEND) for k in range(4)]
+    t1 = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        step(w[lay], xy[layt])
+        t1.append(time.perf_counter() - t0)
     oracle.set_threads(threads)
     model = ""
     try:
@@ -163,10 +209,13 @@ def cpu_baseline(seconds=12.0):
                 break
     except OSError:
         pass
-    return {"value": pairs / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"T170 (I=512, J=256) inverse+forward of {KM} columns, repeated {pairs // KM}x "
-                      f"in {dt:.1f} s; FP64 oracle (oracle/dsht_oracle.c), OpenMP over columns",
-            "one_thread_value": p1 / d1, "cpu_model": model, "nproc": os.cpu_count()}
+    return {"value": PAIRS_PER_STEP / med, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"the benchmarked T170L60 step ({int(PAIRS_PER_STEP)} pairs: 240 columns x VAL/GradLon/"
+                      f"GradLat inverse + 360 columns forward) on the GPU step's inputs, median of {reps} "
+                      f"repetitions ({med:.3f} s each); FP64 oracle (oracle/dsht_oracle.c, {build}), "
+                      f"OpenMP over columns",
+            "ms_per_step": 1e3 * med, "one_thread_value": (PAIRS_PER_STEP * 4 / KM) / float(np.median(t1)),
+            "cpu_model": model, "nproc": os.cpu_count()}
 
 
 def oracle_check(torch, spec, g_out, NCHK=3):
@@ -274,6 +323,36 @@ def roofline_probe(torch, plan, dsht, form, scope, bufs, reps=10):
                  "algorithmic_bytes_per_launch": by / cnt, "algorithmic_flops_per_launch": fl / cnt,
                  "traffic_source": "profiles/hbm_traffic_T170L60.md" if roof["traffic"] else None})
     return roof
+
+
+def t85_line(torch, dsht, steps, warmup, N=85, I=256, J=128, km=60):
+    """The same dynamics-step transform set at T85L60 (I=256, J=128) in FP64: throughput, the
+    FP64 Legendre contractions' achieved rate against the 78.6 TFLOP/s MFMA peak (useful
+    flops: 4 J T per pair in the even/odd form) and the FFTs' achieved HBM rate."""
+    dev = torch.device("cuda:0")
+    plan = dsht.Plan(N, I, J, kmax=km, radius=RADIUS)
+    spec = synth_fields(torch, dev, plan.lm, km)
+    g_out = [torch.empty((N_PROG * km, J, I), dtype=torch.float64, device=dev) for _ in INV_KINDS]
+    stream = torch.cuda.current_stream().cuda_stream
+    plan.inv_d(dsht.VAL, spec, g_out[0], stream=stream)
+    v = g_out[0].view(N_PROG, km, J, I)
+    tend = torch.cat([v[0] * v[2], v[1] * v[3], v[0] * v[3], v[1] * v[2], v[2] * v[2] * 1e-3, v[3] * v[3]],
+                     0).contiguous()
+    spec_t = torch.empty((plan.lm, N_TEND * km), dtype=torch.float64, device=dev)
+    bufs = (spec, g_out, tend, spec_t)
+    pairs = km * (N_PROG * len(INV_KINDS) + N_TEND) / 2
+    wall, _ = time_steps(torch, plan, dsht, bufs, steps, warmup)
+    lat = latency_loop(torch, plan, dsht, bufs, 1000)
+    prof = kernel_profile(torch, plan, dsht, bufs, 5)
+    leg = {k: {"avg_us": v["avg_us"], "TFLOPs": v["TFLOPs"], "mfma_fp64_frac": v["TFLOPs"] / FP64_PEAK_TFS,
+               "GBps": v["GBps"]} for k, v in prof.items() if k.startswith("leg")}
+    fft = {k: {"avg_us": v["avg_us"], "GBps": v["GBps"], "frac_hbm_peak": v["GBps"] / HBM_PEAK_GBS}
+           for k, v in prof.items() if k.startswith("fft")}
+    return {"workload": f"T{N}L{km} (I={I}, J={J}) dynamics-step transform set, FP64", "pairs_per_s": pairs * steps / wall,
+            "ms_per_step": 1e3 * wall / steps, "p50_step_ms": lat["p50_step_ms"], "p99_step_ms": lat["p99_step_ms"],
+            "legendre": leg, "fft": fft,
+            "mfma_note": "mfma_fp64_frac = useful FP64 flops / time / 78.6 TFLOP/s; the counter-measured MfmaUtil "
+                         "is in profiles/"}
 
 
 # ---------------------------------------------------------------- batched form
@@ -505,11 +584,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--prec", default="auto")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy 3 times (HBM counter calibration) and exit")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--latency-steps", type=int, default=1000,
+                    help="steps of the separate per-step latency loop (p50/p99) of each form")
+    ap.add_argument("--no-t85", action="store_true")
     ap.add_argument("--column-only", action="store_true",
                     help="only run the vertical column diagnostics section and print its JSON")
     args = ap.parse_args()
@@ -538,6 +619,8 @@ def main():
 
     import dsht
 
+    if not args.no_cpu:
+        prepare_cpu_oracle()
     print(f"[bench] T{NMAX} I={IM} J={JM} KM={KM}: building plan", file=sys.stderr, flush=True)
     plan = dsht.Plan(NMAX, IM, JM, kmax=KM, radius=RADIUS)
     L = plan.lm
@@ -576,7 +659,8 @@ def main():
                    zip(list(INV_KINDS) + ["forward"], outs, ref_out)}
             field_err[form] = per_field_errors(outs, ref_out)
         print(f"[bench] form {form}: timing {args.steps} steps", file=sys.stderr, flush=True)
-        wall, lat = time_steps(torch, plan, dsht, bufs, args.steps, args.warmup)
+        wall, _ = time_steps(torch, plan, dsht, bufs, args.steps, args.warmup)
+        lat = latency_loop(torch, plan, dsht, bufs, args.latency_steps)
         prof = kernel_profile(torch, plan, dsht, bufs, max(2, min(args.steps, 5)))
         # the same step with one dsht_inv_d call per inverse kind (function-per-kind API)
         wall_s, _ = time_steps(torch, plan, dsht, bufs, args.steps, args.warmup, fused=False)
@@ -584,8 +668,9 @@ def main():
         results[form] = {
             "pairs_per_s": PAIRS_PER_STEP * args.steps / wall,
             "ms_per_step": 1e3 * wall / args.steps,
-            "p50_step_ms": float(np.percentile(lat, 50)),
-            "p99_step_ms": float(np.percentile(lat, 99)),
+            "p50_step_ms": lat["p50_step_ms"],
+            "p99_step_ms": lat["p99_step_ms"],
+            "latency": lat,
             "errors": err,
             "kernels": prof,
             "separate_calls": {"pairs_per_s": PAIRS_PER_STEP * args.steps / wall_s,
@@ -603,6 +688,12 @@ def main():
     # roofline of the dominant kernel of the reported form, probed on the step's own calls
     kname = max(r["kernels"].items(), key=lambda kv: kv[1]["ms_total"])[0]
     roof = roofline_probe(torch, plan, dsht, best, kname, bufs)
+    t85 = None
+    if not args.no_t85:
+        print("[bench] T85L60 FP64 line", file=sys.stderr, flush=True)
+        t85 = t85_line(torch, dsht, args.steps, args.warmup)
+        print(f"[bench] T85: {t85['pairs_per_s']:.0f} pairs/s, {t85['ms_per_step']:.3f} ms/step", file=sys.stderr,
+              flush=True)
     print("[bench] column diagnostics", file=sys.stderr, flush=True)
     column = column_bench(torch, dsht)
     print("[bench] column: " + ", ".join(f"{k} {column[k]['avg_us']:.1f} us {column[k]['GBps']:.0f} GB/s"
@@ -622,7 +713,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.cpu_seconds)
+        cpu = cpu_baseline(spec, tend)
 
     line = {
         "metric": METRIC,
@@ -644,8 +735,9 @@ def main():
                    "pairs_per_step": PAIRS_PER_STEP, "parallelism": "single GPU"},
         "p50_step_ms": r["p50_step_ms"],
         "p99_step_ms": r["p99_step_ms"],
+        "latency_steps": args.latency_steps,
         "forms": {f: {"pairs_per_s": v["pairs_per_s"], "ms_per_step": v["ms_per_step"],
-                      "p99_step_ms": v["p99_step_ms"], "errors": v["errors"],
+                      "p50_step_ms": v["p50_step_ms"], "p99_step_ms": v["p99_step_ms"], "errors": v["errors"],
                       "separate_calls": v["separate_calls"],
                       "kernels": {kk: {"avg_us": kv["avg_us"], "GBps": kv["GBps"], "TFLOPs": kv["TFLOPs"]}
                                   for kk, kv in v["kernels"].items()}}
@@ -658,6 +750,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "column_diagnostics": column,
+        "t85_fp64": t85,
     }
     print(json.dumps(line))
 

@@ -25,6 +25,25 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+def use_library(path):
+    """Load the oracle from another build of the same source (e.g. one compiled with
+    -march=native for the host that times the CPU baseline); before any other use."""
+    global _LIB, _lib
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    _LIB = path
+
+
+def build_native(outdir):
+    """Compile dsht_oracle.c with -O3 -march=native -fopenmp into outdir (for the host it runs
+    on) and return the library path."""
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "libdsht_oracle_native.so")
+    subprocess.check_call(["gcc", "-O3", "-march=native", "-fopenmp", "-fPIC", "-std=gnu11", "-shared", "-o", out,
+                           os.path.join(_HERE, "dsht_oracle.c"), "-lm"])
+    return out
+
+
 def lib():
     global _lib
     if _lib is None:

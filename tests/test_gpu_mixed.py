@@ -142,3 +142,33 @@ def test_mixed_forward_column_tiles(gpu, form, cb):
     vor, div = p.calc_UVCosLat2VorDiv(A, B)
     rv, rd = o.uv2vordiv(A, B)
     assert rel_l2(vor, rv) < tol and rel_l2(div, rd) < tol
+
+
+@pytest.mark.parametrize("form", list(BOUND))
+def test_every_form_at_t170(gpu, form):
+    """every split form at T170 (I=512, J=256) on 40 columns of red-spectrum data
+    with a large zonal-mean background (like temperature), against the FP64 form within its
+    bound: the fused value/gradient inverse (three output groups) and the forward of grid
+    products."""
+    import torch
+    import dsht
+    N, I, J, nc = 170, 512, 256, 40
+    p = dsht.Plan(N, I, J, radius=6.371e6)
+    w = red_spectrum(N, nc, 61)
+    w[:, 0] += 15.0
+    wt = torch.tensor(w.T.copy(), device=gpu)
+    ref = [torch.empty((nc, J, I), dtype=torch.float64, device=gpu) for _ in range(3)]
+    p.inv_vgrad_d(wt, *ref)
+    prod = (ref[0] * ref[0] * 1e-2).contiguous()
+    sref = torch.zeros((p.lm, nc), dtype=torch.float64, device=gpu)
+    p.fwd_d(dsht.VAL, prod, sref)
+    p.set_precision(_prec(form))
+    got = [torch.empty_like(ref[0]) for _ in range(3)]
+    p.inv_vgrad_d(wt, *got)
+    sgot = torch.zeros_like(sref)
+    p.fwd_d(dsht.VAL, prod, sgot)
+    torch.cuda.synchronize()
+    errs = [rel_l2(a.cpu().numpy(), b.cpu().numpy()) for a, b in zip(got, ref)]
+    errs.append(rel_l2(sgot.cpu().numpy(), sref.cpu().numpy()))
+    print(form, ["%.2e" % e for e in errs])
+    assert max(errs) < BOUND[form], (form, errs)
