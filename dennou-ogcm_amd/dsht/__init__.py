@@ -45,8 +45,28 @@ EXPORTS = [
     "dsht_set_tuning", "dsht_get_tuning", "dsht_batch_create", "dsht_batch_add", "dsht_batch_finalize",
     "dsht_batch_info", "dsht_batch_ngroups", "dsht_batch_buffers", "dsht_batch_step",
     "dsht_batch_step_sequential", "dsht_batch_capture", "dsht_batch_launch", "dsht_batch_destroy",
+    "dsht_lfam3_default_config", "dsht_lfam3_create", "dsht_lfam3_destroy", "dsht_lfam3_set_rotation",
+    "dsht_lfam3_state", "dsht_lfam3_step", "dsht_lfam3_copy_run", "dsht_lfam3_units",
 ]
 TUNE_CHUNK_COLS, TUNE_LEG_CB, TUNE_FFT, TUNE_LEG, TUNE_PIPELINE, TUNE_FFT_ORDER = 0, 1, 2, 3, 4, 5
+
+
+class LfAm3Config(ctypes.Structure):
+    """dsht_lfam3_config (include/dsht.h)."""
+    _fields_ = [("km", ctypes.c_int), ("max_runs", ctypes.c_int), ("sub_cycles", ctypes.c_int),
+                ("numdiff_ord", ctypes.c_int),
+                ("dt", ctypes.c_double), ("depth", ctypes.c_double), ("grav", ctypes.c_double),
+                ("ref_dens", ctypes.c_double), ("alpha", ctypes.c_double), ("lam", ctypes.c_double),
+                ("hvisc", ctypes.c_double), ("hdiff", ctypes.c_double), ("numdiff", ctypes.c_double),
+                ("vvisc", ctypes.c_double), ("vdiff", ctypes.c_double), ("btm_frict", ctypes.c_double),
+                ("mom_sfc_bc", ctypes.c_int), ("mom_btm_bc", ctypes.c_int), ("trc_sfc_bc", ctypes.c_int),
+                ("trc_btm_bc", ctypes.c_int),
+                ("ref_temp", ctypes.c_double), ("ref_salt", ctypes.c_double), ("beta_t", ctypes.c_double),
+                ("beta_s", ctypes.c_double), ("cp0", ctypes.c_double), ("cs0", ctypes.c_double)]
+
+
+LF_U, LF_V, LF_TRC, LF_SFCPRES = range(4)
+LF_B, LF_N, LF_A = range(3)
 
 
 class DshtError(RuntimeError):
@@ -139,6 +159,14 @@ def load():
         "dsht_batch_capture": ([vp, vp], i),
         "dsht_batch_launch": ([vp, vp], i),
         "dsht_batch_destroy": ([vp], i),
+        "dsht_lfam3_default_config": ([ctypes.POINTER(LfAm3Config)], i),
+        "dsht_lfam3_create": ([vp, ctypes.POINTER(LfAm3Config), pp], i),
+        "dsht_lfam3_destroy": ([vp], i),
+        "dsht_lfam3_set_rotation": ([vp, i, d], i),
+        "dsht_lfam3_state": ([vp, i, i, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_long)], i),
+        "dsht_lfam3_step": ([vp, i, vp], i),
+        "dsht_lfam3_copy_run": ([vp, i, i, vp], i),
+        "dsht_lfam3_units": ([vp, i, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -778,3 +806,67 @@ def _tensor_from_ptr(ptr, shape):
     h.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": "<f8", "data": (ptr, False), "version": 2,
                                   "strides": None}
     return torch.as_tensor(h, device="cuda")
+
+
+def lfam3_config(**kw):
+    """dsht_lfam3_default_config with the given fields replaced (boundary types as 'D'/'N')."""
+    c = LfAm3Config()
+    _check(load().dsht_lfam3_default_config(ctypes.byref(c)))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise AttributeError(f"dsht_lfam3_config has no field {k}")
+        setattr(c, k, ord(v) if k.endswith("_bc") and isinstance(v, str) else v)
+    return c
+
+
+class LfAm3:
+    """A pool of up to max_runs independent LF-AM3 runs of one plan and km layers
+    (dsht_lfam3_*, the device form of DOGCM_TInt_LFAM3_advance_Dyn).  State views
+    (torch, float64 device): U, V (max_runs*km, J, I); TRC (2*max_runs*km, J, I) with run r's
+    temperature at rows (2r)*km.. and salinity at (2r+1)*km..; SFCPRES (max_runs, J, I)."""
+
+    def __init__(self, plan, cfg):
+        self.plan, self.cfg = plan, cfg
+        self.h = ctypes.c_void_p()
+        _check(load().dsht_lfam3_create(plan.h, ctypes.byref(cfg), ctypes.byref(self.h)))
+        self.km, self.max_runs = cfg.km, cfg.max_runs
+
+    def state(self, field, level):
+        ptr, n = ctypes.c_void_p(), ctypes.c_long()
+        _check(load().dsht_lfam3_state(self.h, int(field), int(level), ctypes.byref(ptr), ctypes.byref(n)))
+        return _tensor_from_ptr(ptr.value, (n.value, self.plan.jm, self.plan.im))
+
+    def run_view(self, field, level, run):
+        """Views of one run's slice: U/V (km, J, I); TRC (2, km, J, I); SFCPRES (J, I)."""
+        t = self.state(field, level)
+        k = self.km
+        if field == LF_TRC:
+            return t[2 * run * k:(2 * run + 2) * k].view(2, k, self.plan.jm, self.plan.im)
+        if field == LF_SFCPRES:
+            return t[run]
+        return t[run * k:(run + 1) * k]
+
+    def set_rotation(self, run, omega):
+        _check(load().dsht_lfam3_set_rotation(self.h, int(run), float(omega)))
+
+    def step(self, nruns, stream=None):
+        _check(load().dsht_lfam3_step(self.h, int(nruns), _stream(stream)))
+
+    def copy_run(self, dst, src, stream=None):
+        _check(load().dsht_lfam3_copy_run(self.h, int(dst), int(src), _stream(stream)))
+
+    def units(self, nruns):
+        f, i = ctypes.c_long(), ctypes.c_long()
+        _check(load().dsht_lfam3_units(self.h, int(nruns), ctypes.byref(f), ctypes.byref(i)))
+        return f.value, i.value
+
+    def close(self):
+        if self.h and self.h.value:
+            load().dsht_lfam3_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -308,6 +308,51 @@ int dsht_prof_enable(int on);
 int dsht_prof_reset(void);
 int dsht_prof_query(const char* name, double* total_ms, long* count, double* bytes, double* flops);
 
+/* ---------------------------------------------------------------- LF-AM3 step
+ * A device-resident LF-AM3 dynamics step (DOGCM_TInt_LFAM3_advance_Dyn,
+ * model/dogcm/src/tint/DOGCM_TInt_LFAM3_mod.f90:281-880) for up to max_runs independent runs
+ * of one plan (truncation) and km layers: rigid lid, flat bottom of depth `depth`, linear
+ * equation of state (EOS_Linear_mod.f90), no physics forcing.  Per run and step it makes the
+ * reference's 42 forward and 28 inverse transform units per layer plus 9 + 9 barotropic
+ * ones (dsht_lfam3_units), each transform over every layer of every active run in one call.
+ * State in device memory (dsht_lfam3_state): U, V (km grids per run, run r layer k at grid
+ * column r*km + k), tracers (potential temperature [K] then salinity per run: column
+ * (2r + t)*km + k), surface pressure (one grid per run), at time levels B and N; a step
+ * advances runs [0, nruns) and rotates the levels (B <- N, N <- A).  The vertical grid is
+ * the Chebyshev-Lobatto one of dsht_sig_grid(km, km, -1, 0) (km even).  A step allocates
+ * nothing itself; the plan's transform workspaces are sized by the first step on a stream.
+ * Runs are independent: a run's result does not depend on the other runs in the pool.      */
+typedef struct dsht_lfam3 dsht_lfam3;
+typedef struct dsht_lfam3_config {
+  int km, max_runs, sub_cycles, numdiff_ord;        /* SubCycleNum, NumDiffOrdH            */
+  double dt, depth, grav, ref_dens;                 /* DelTime, flat depth, Grav, RefDens  */
+  double alpha, lambda;                             /* CoriolisTermACoef, VDiffTermACoef   */
+  double hvisc, hdiff, numdiff, vvisc, vdiff, btm_frict; /* hViscCoef, hDiffCoef, NumDiffCoefH,
+                                                       vViscCoef, vDiffCoef, xy_BtmFrictCoef */
+  int mom_sfc_bc, mom_btm_bc, trc_sfc_bc, trc_btm_bc; /* 'D' or 'N'                      */
+  double ref_temp, ref_salt, beta_t, beta_s, cp0, cs0; /* EOS_Linear constants           */
+} dsht_lfam3_config;
+#define DSHT_LF_U 0
+#define DSHT_LF_V 1
+#define DSHT_LF_TRC 2
+#define DSHT_LF_SFCPRES 3
+#define DSHT_LF_B 0
+#define DSHT_LF_N 1
+#define DSHT_LF_A 2
+/* defaults: APEOGCircExp.nml (SemiImplicitScheme_nml, Constants_nml), EOS_Linear_Init     */
+int dsht_lfam3_default_config(dsht_lfam3_config* cfg);
+int dsht_lfam3_create(dsht_plan* plan, const dsht_lfam3_config* cfg, dsht_lfam3** pool);
+int dsht_lfam3_destroy(dsht_lfam3* pool);
+/* rotation rate Omega of one run (xy_Cori = 2 Omega sin(lat)); default Earth's          */
+int dsht_lfam3_set_rotation(dsht_lfam3* pool, int run, double omega);
+/* device pointer and column count of a state array (field DSHT_LF_*, level B, N or A)   */
+int dsht_lfam3_state(dsht_lfam3* pool, int field, int level, double** ptr, long* ncols);
+int dsht_lfam3_step(dsht_lfam3* pool, int nruns, void* stream);
+/* copies run src's state (levels B and N, rotation rate) into slot dst (pool compaction) */
+int dsht_lfam3_copy_run(dsht_lfam3* pool, int dst, int src, void* stream);
+/* transform units of one step of nruns runs (42 km + 9 forward, 28 km + 9 inverse each)   */
+int dsht_lfam3_units(const dsht_lfam3* pool, int nruns, long* fwd_units, long* inv_units);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,3 +114,17 @@ def test_host_column_wrappers_check_vector_lengths():
         p.xyz_VDiff(f, f, f, f, good, np.ones(km + 2), 1.0, 1.0)
     with pytest.raises(ValueError, match="z_CDK"):
         p.xyz_VDiff(f, f, f, f, ka, np.ones(km + 1), 1.0, 1.0)
+
+
+def test_lfam3_config_layout_and_defaults():
+    """dsht_lfam3_default_config fills the C struct the ctypes mirror describes (host-only):
+    the defaults of APEOGCircExp.nml and EOS_Linear_Init land in the right fields."""
+    import dsht
+    c = dsht.lfam3_config()
+    assert (c.km, c.max_runs, c.sub_cycles, c.numdiff_ord) == (20, 1, 1, 4)
+    assert (c.alpha, c.lam, c.grav, c.ref_dens) == (0.51, 0.5, 9.79764, 1027.0)
+    assert (c.hvisc, c.vvisc, c.vdiff, c.numdiff) == (3e5, 1e-3, 3e-5, 1e12)
+    assert (chr(c.mom_sfc_bc), chr(c.mom_btm_bc), chr(c.trc_sfc_bc), chr(c.trc_btm_bc)) == ("N", "D", "N", "N")
+    assert (c.ref_temp, c.ref_salt, c.beta_t, c.beta_s, c.cp0, c.cs0) == (283.0, 35.0, 1.67e-4, 0.78e-3, 3986.0, 1490.0)
+    c2 = dsht.lfam3_config(km=60, dt=900.0, mom_btm_bc="N")
+    assert c2.km == 60 and c2.dt == 900.0 and c2.mom_btm_bc == ord("N")
