@@ -623,6 +623,8 @@ def main():
                     help="steps of the separate per-step latency loop (p50/p99) of each form")
     ap.add_argument("--no-t85", action="store_true")
     ap.add_argument("--no-lfam3", action="store_true")
+    ap.add_argument("--lfam3-only", action="store_true",
+                    help="only run the LF-AM3 step section (T170L60, 1 and 4 runs) and print its JSON")
     ap.add_argument("--column-only", action="store_true",
                     help="only run the vertical column diagnostics section and print its JSON")
     args = ap.parse_args()
@@ -647,6 +649,10 @@ def main():
     if args.column_only:
         import dsht
         print(json.dumps({"column_diagnostics": column_bench(torch, dsht)}))
+        return
+    if args.lfam3_only:
+        import dsht
+        print(json.dumps({"lfam3": lfam3_bench(torch, dsht, steps=args.steps)}))
         return
 
     import dsht
