@@ -65,17 +65,25 @@ def synth_fields(torch, dev, L, KM=KM):
     return spec.contiguous()
 
 
+def form_parts(name):
+    """(Legendre precision form, FP32 grid I/O) of a bench form name such as 'bf16x6+f32grid'."""
+    base, _, io = name.partition("+")
+    return base, io == "f32grid"
+
+
 def run_step(plan, dsht, bufs, stream, fused=True):
     """One step.  fused: the three inverse kinds of each field come from one
     dsht_inv_vgrad_d call (one contraction pass for value and both gradients); otherwise
-    three dsht_inv_d calls, as the SpmlUtil function-per-kind API makes them."""
-    spec, g_out, tend, spec_t = bufs
+    three dsht_inv_d calls, as the SpmlUtil function-per-kind API makes them.  A fifth
+    element True in bufs: FP32 grids (the *_f32_d entry points)."""
+    spec, g_out, tend, spec_t = bufs[:4]
+    f32 = len(bufs) > 4 and bufs[4]
     if fused:
-        plan.inv_vgrad_d(spec, *g_out, stream=stream)
+        (plan.inv_vgrad_f32_d if f32 else plan.inv_vgrad_d)(spec, *g_out, stream=stream)
     else:
         for k, kind in enumerate(INV_KINDS):
-            plan.inv_d(getattr(dsht, kind), spec, g_out[k], stream=stream)
-    plan.fwd_d(dsht.VAL, tend, spec_t, stream=stream)
+            (plan.inv_f32_d if f32 else plan.inv_d)(getattr(dsht, kind), spec, g_out[k], stream=stream)
+    (plan.fwd_f32_d if f32 else plan.fwd_d)(dsht.VAL, tend, spec_t, stream=stream)
 
 
 def rel_l2(a, b):
@@ -286,18 +294,19 @@ def traffic_table(path=os.path.join(ROOT, "profiles", "hbm_traffic_T170L60.md"))
 def roofline_probe(torch, plan, dsht, form, scope, bufs, reps=10):
     """Average duration and algorithmic bytes/flops of `scope` on the launches of the
     step's own call (the launch shape the committed counter profile measured)."""
-    spec, g_out, tend, spec_t = bufs
+    spec, g_out, tend, spec_t = bufs[:4]
+    f32 = len(bufs) > 4 and bufs[4]
     stream = torch.cuda.current_stream().cuda_stream
-    pid = {v: k for k, v in dsht.PREC_NAMES.items()}[form]
+    pid = {v: k for k, v in dsht.PREC_NAMES.items()}[form_parts(form)[0]]
     plan.set_precision(pid)
     inv = scope.endswith("inv")
     nc = spec.shape[1] if inv else tend.shape[0]
 
     def call():
         if inv:
-            plan.inv_vgrad_d(spec, *g_out, stream=stream)
+            (plan.inv_vgrad_f32_d if f32 else plan.inv_vgrad_d)(spec, *g_out, stream=stream)
         else:
-            plan.fwd_d(dsht.VAL, tend, spec_t, stream=stream)
+            (plan.fwd_f32_d if f32 else plan.fwd_d)(dsht.VAL, tend, spec_t, stream=stream)
 
     for _ in range(3):
         call()
@@ -318,7 +327,10 @@ def roofline_probe(torch, plan, dsht, form, scope, bufs, reps=10):
     else:
         roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
     roof["traffic"] = traffic_table().get((form, scope))
-    per_call = 3 if scope == "fft_inv" else 1  # the fused inverse runs one FFT per output grid
+    # FFT launches per call of the fused inverse: three in FP64 (one per output grid), two in
+    # the split forms (the field with its longitude derivative from one read, the latitude
+    # derivative); launch-average bytes and times cover both kinds
+    per_call = (3 if form == "fp64" else 2) if scope == "fft_inv" else 1
     roof.update({"kernel": scope, "columns_per_launch": nc * reps * per_call / cnt, "avg_launch_us": 1e6 * t,
                  "algorithmic_bytes_per_launch": by / cnt, "algorithmic_flops_per_launch": fl / cnt,
                  "traffic_source": "profiles/hbm_traffic_T170L60.md" if roof["traffic"] else None})
@@ -675,20 +687,27 @@ def main():
     torch.cuda.synchronize()
 
     forms = ["fp64"] if args.prec == "fp64" else (
-        ["fp64", args.prec] if args.prec != "auto" else ["fp64", "bf16x3", "fp16x3", "bf16x6", "fp16"])
+        ["fp64", args.prec] if args.prec != "auto" else
+        ["fp64", "bf16x3", "fp16x3", "bf16x6", "fp16", "bf16x6+f32grid", "fp16x3+f32grid"])
+    # FP32 grid I/O: the same prognostic spectra, FP32 output grids, FP32 copies of the tendencies
+    g_out32 = [torch.empty((N_PROG * KM, JM, IM), dtype=torch.float32, device=dev) for _ in INV_KINDS]
+    bufs32 = (spec, g_out32, tend.float().contiguous(), spec_t, True)
     results = {}
     field_err = {}
     ref_out = None
     for form in forms:
-        pid = {v: k for k, v in dsht.PREC_NAMES.items()}[form]
+        base, f32 = form_parts(form)
+        cur = bufs32 if f32 else bufs
+        gcur = cur[1]
+        pid = {v: k for k, v in dsht.PREC_NAMES.items()}[base]
         try:
             plan.set_precision(pid)
-            run_step(plan, dsht, bufs, stream)
+            run_step(plan, dsht, cur, stream)
             torch.cuda.synchronize()
         except dsht.DshtError as e:
             print(f"[bench] form {form} unavailable: {e}", file=sys.stderr, flush=True)
             continue
-        outs = [g.clone() for g in g_out] + [spec_t.clone()]
+        outs = [g.clone().double() for g in gcur] + [spec_t.clone()]
         if ref_out is None:
             ref_out = outs
             err = {"oracle_rel_l2_inverse": oracle_check(torch, spec, g_out)}
@@ -697,12 +716,12 @@ def main():
                    zip(list(INV_KINDS) + ["forward"], outs, ref_out)}
             field_err[form] = per_field_errors(outs, ref_out)
         print(f"[bench] form {form}: timing {args.steps} steps", file=sys.stderr, flush=True)
-        wall, _ = time_steps(torch, plan, dsht, bufs, args.steps, args.warmup)
-        lat = latency_loop(torch, plan, dsht, bufs, args.latency_steps)
-        prof = kernel_profile(torch, plan, dsht, bufs, max(2, min(args.steps, 5)))
+        wall, _ = time_steps(torch, plan, dsht, cur, args.steps, args.warmup)
+        lat = latency_loop(torch, plan, dsht, cur, args.latency_steps)
+        prof = kernel_profile(torch, plan, dsht, cur, max(2, min(args.steps, 5)))
         # the same step with one dsht_inv_d call per inverse kind (function-per-kind API)
-        wall_s, _ = time_steps(torch, plan, dsht, bufs, args.steps, args.warmup, fused=False)
-        sep_err = max(rel_l2(a, b) for a, b in zip(g_out, outs[:3]))
+        wall_s, _ = time_steps(torch, plan, dsht, cur, args.steps, args.warmup, fused=False)
+        sep_err = max(rel_l2(a, b) for a, b in zip(gcur, outs[:3]))
         results[form] = {
             "pairs_per_s": PAIRS_PER_STEP * args.steps / wall,
             "ms_per_step": 1e3 * wall / args.steps,
@@ -722,10 +741,10 @@ def main():
     ok = [f for f in results if f == "fp64" or max(results[f]["errors"].values()) <= TOL_MIXED]
     best = max(ok, key=lambda f: results[f]["pairs_per_s"])
     r = results[best]
-    drift = {f: energy_drift(torch, plan, dsht, f, spec) for f in results}
+    drift = {f: energy_drift(torch, plan, dsht, f, spec) for f in results if not form_parts(f)[1]}
     # roofline of the dominant kernel of the reported form, probed on the step's own calls
     kname = max(r["kernels"].items(), key=lambda kv: kv[1]["ms_total"])[0]
-    roof = roofline_probe(torch, plan, dsht, best, kname, bufs)
+    roof = roofline_probe(torch, plan, dsht, best, kname, bufs32 if form_parts(best)[1] else bufs)
     t85 = None
     if not args.no_t85:
         print("[bench] T85L60 FP64 line", file=sys.stderr, flush=True)
@@ -771,6 +790,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if best == "fp64" else best,
+        "mixed_speedup_vs_fp64_fp64grids": (max(results[f]["pairs_per_s"] for f in ok if f != "fp64" and
+                                                 not form_parts(f)[1]) / results["fp64"]["pairs_per_s"])
+        if any(f != "fp64" and not form_parts(f)[1] for f in ok) and "fp64" in results else None,
         "data": "synthetic",
         "config": {"workload": "T170L60 dynamics-step transform set (4 fields to grid with lon/lat "
                                "derivatives, 6 tendencies to spectral); inverse kinds of a field in one "

@@ -281,6 +281,9 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
       fs = cp.aux;
       if ((rc = pipe_after(cp.evL[slot], s, fs))) return rc;
     }
+    // the field and its longitude derivative as two launches over the same G0: the FP64
+    // dual kernel (one read of G0, coefficients held in registers across both passes) ran
+    // slower at T170L60 (2 x 125 against 2 x 80 us per step's worth), the FP32 one faster
     if (out0 && (rc = launch_fft_inv(p, G0, a.gld, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
     if (dlon && (rc = launch_fft_inv(p, G0, a.gld, nc, out1 + (long)c0 * ldg, ldg, fs, p->d_rs_icos, 1))) return rc;
     if (a.G1 && (rc = launch_fft_inv(p, G1, a.gld, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
@@ -743,6 +746,68 @@ int dsht_inv_vgrad_d(dsht_plan* p, const double* w, long sl, long sc, double* xy
     return 0;
   }
   return dispatch_inv(p, I_VGRAD, w, nullptr, sl, sc, xy, xy_lon, ldg, ncols, diag, p->d_rs_icos, s, xy_lat);
+}
+
+// ---- FP32 grid I/O of the split forms: the same calls with float grids
+static int check_f32(const dsht_plan* p) {
+  if (p->prec == DSHT_PREC_FP64)
+    return fail(DSHT_EUNSUPPORTED, "FP32 grid I/O runs the split precision forms (the plan is set to FP64)");
+  if (p->fft_kind != 0 || !reg_fft_supported(p->fft.M))
+    return fail(DSHT_EUNSUPPORTED, "FP32 grid I/O needs the register FFT (DSHT_TUNE_FFT 0, I/2 with a register kernel)");
+  if (p->leg_kind == 1) return fail(DSHT_EUNSUPPORTED, "FP32 grid I/O needs DSHT_TUNE_LEG != 1");
+  return 0;
+}
+
+int dsht_fwd_f32_d(dsht_plan* p, int kind, const float* xy, long ldg, double* w, long sl, long sc, int ncols,
+                   const double* diag, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  int op = kind_fwd_op(kind);
+  if (op < 0) return fail(DSHT_EARG, "dsht_fwd_f32_d: kind %d is not a forward kind", kind);
+  if (ncols < 0) return fail(DSHT_EARG, "ncols < 0");
+  if (ncols == 0) return 0;
+  if (int rc = check_f32(p)) return rc;
+  if (int rc = check_grid_ptr(xy, ldg, p)) return rc;
+  if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
+  return mixed_run_fwd(p, op, (const double*)xy, nullptr, ldg, w, nullptr, sl, sc, ncols, diag, nullptr, 1.0,
+                       op == F_VAL ? p->d_cwv : p->d_cwd, (hipStream_t)stream, true);
+}
+
+int dsht_inv_f32_d(dsht_plan* p, int kind, const double* w, long sl, long sc, float* xy, long ldg, int ncols,
+                   const double* diag, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  int op;
+  const double* rs = nullptr;
+  switch (kind) {
+    case DSHT_VAL: op = I_VAL; break;
+    case DSHT_GRADLAMBDA: op = I_GRADLAMBDA; break;
+    case DSHT_GRADMU: op = I_GRADMU; break;
+    case DSHT_GRADLON: op = I_GRADLAMBDA; rs = p->d_rs_icos; break;
+    case DSHT_GRADLAT: op = I_GRADMU; rs = p->d_rs_icos; break;
+    default: return fail(DSHT_EARG, "dsht_inv_f32_d: kind %d is not an inverse kind", kind);
+  }
+  if (ncols < 0) return fail(DSHT_EARG, "ncols < 0");
+  if (ncols == 0) return 0;
+  if (int rc = check_f32(p)) return rc;
+  if (int rc = check_grid_ptr(xy, ldg, p)) return rc;
+  if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
+  return mixed_run_inv(p, op, w, nullptr, sl, sc, (double*)xy, nullptr, ldg, ncols, diag, rs, (hipStream_t)stream,
+                       nullptr, true);
+}
+
+int dsht_inv_vgrad_f32_d(dsht_plan* p, const double* w, long sl, long sc, float* xy, float* xy_lon, float* xy_lat,
+                         long ldg, int ncols, const double* diag, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (ncols < 0) return fail(DSHT_EARG, "ncols < 0");
+  if (!xy && !xy_lon && !xy_lat) return fail(DSHT_EARG, "dsht_inv_vgrad_f32_d: every output is NULL");
+  if (ncols == 0) return 0;
+  if (int rc = check_f32(p)) return rc;
+  float* outs[3] = {xy, xy_lon, xy_lat};
+  for (float* o : outs)
+    if (o)
+      if (int rc = check_grid_ptr(o, ldg, p)) return rc;
+  if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
+  return mixed_run_inv(p, I_VGRAD, w, nullptr, sl, sc, (double*)xy, (double*)xy_lon, ldg, ncols, diag, p->d_rs_icos,
+                       (hipStream_t)stream, (double*)xy_lat, true);
 }
 
 int dsht_alpha_fwd_d(dsht_plan* p, const double* a, const double* b, long ldg, double* w, long sl, long sc,

@@ -122,6 +122,8 @@ struct FftArgs {
   int fp32;                // FP32 arithmetic (FP32 Fourier path of the split forms)
   int order;               // row-group order of the register kernels (0 packed, 1 latitude-major)
   int deriv;               // inverse: transform i m X_m (longitude derivative) instead of X_m
+  void* xy2;               // inverse, dual form: the field into xy and its longitude derivative
+                           // (times rowscale) into xy2 from one read of the coefficients (nullable)
 };
 
 // ---------------------------------------------------------------- workspace
@@ -251,22 +253,29 @@ int launch_fft_fwd(const dsht_plan* p, const double* xy, long ldg, int ncols, do
 // rowscale: per-latitude factor on the output (nullable); deriv: transform i m X_m (the
 // longitude derivative of the field), register kernels only (fft_inv_deriv_ok)
 bool fft_inv_deriv_ok(const dsht_plan* p);
+// xy2 (dual form, register kernels only): xy receives the field and xy2 its longitude
+// derivative times rowscale, from one read of the coefficients
 int launch_fft_inv(const dsht_plan* p, const double2* G, long gld, int ncols, double* xy, long ldg,
-                   hipStream_t s, const double* rowscale = nullptr, int deriv = 0);
+                   hipStream_t s, const double* rowscale = nullptr, int deriv = 0, double* xy2 = nullptr);
 // FP32 Fourier intermediates of the split-precision forms; colmax (nullable) receives the
 // per-column max |G| as float bits
 bool reg_fft_supported(int M);
-template <typename GT>
-int reg_fft_fwd(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s);
-template <typename GT>
-int reg_fft_inv(const GT* G, const FftArgs& a, double* xy, hipStream_t s);
+template <typename GT, typename XT>
+int reg_fft_fwd(const XT* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s);
+template <typename GT, typename XT>
+int reg_fft_inv(const GT* G, const FftArgs& a, XT* xy, hipStream_t s);
 // legendre_f64d.hip: FP64 contractions as per-wave fragment streams (DSHT_TUNE_LEG = 2)
 int launch_leg_inv_direct(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t s);
 int launch_leg_fwd_direct(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t s);
 int launch_fft_fwd32(const dsht_plan* p, const double* xy, long ldg, int ncols, float2* G, long gld,
                      unsigned* colmax, hipStream_t s);
 int launch_fft_inv32(const dsht_plan* p, const float2* G, long gld, int ncols, double* xy, long ldg,
-                     hipStream_t s, const double* rowscale = nullptr, int deriv = 0);
+                     hipStream_t s, const double* rowscale = nullptr, int deriv = 0, double* xy2 = nullptr);
+// FP32 grids (the split forms' FP32 grid I/O; register kernels only)
+int launch_fft_fwd32f(const dsht_plan* p, const float* xy, long ldg, int ncols, float2* G, long gld,
+                      unsigned* colmax, hipStream_t s);
+int launch_fft_inv32f(const dsht_plan* p, const float2* G, long gld, int ncols, float* xy, long ldg,
+                      hipStream_t s, const double* rowscale = nullptr, int deriv = 0, float* xy2 = nullptr);
 // legendre_f64.hip
 int launch_leg_fwd(const dsht_plan* p, int op, const LegFwdArgs& a, hipStream_t s);
 int launch_leg_inv(const dsht_plan* p, int op, const LegInvArgs& a, hipStream_t s);
@@ -305,11 +314,12 @@ struct ProfScope {
 };
 // legendre_mixed.hip: split-precision forms
 void mixed_free(dsht_plan* p);
+// grid_f32: the grid pointers (in0/in1 forward, out0-out2 inverse) hold float
 int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, long ldg, double* out0,
                   double* out1, long sl, long sc, int ncols, const double* diag0, const double* diag1,
-                  double scale, const double* cw, hipStream_t s);
+                  double scale, const double* cw, hipStream_t s, bool grid_f32 = false);
 // outputs of I_VGRAD: out0 value, out1 GradLon, out2 GradLat (any may be NULL)
 int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, long sl, long sc,
                   double* out0, double* out1, long ldg, int ncols, const double* pro, const double* rs,
-                  hipStream_t s, double* out2 = nullptr);
+                  hipStream_t s, double* out2 = nullptr, bool grid_f32 = false);
 }  // namespace dsht

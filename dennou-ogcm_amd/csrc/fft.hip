@@ -454,6 +454,7 @@ static FftArgs fft_args(const dsht_plan* p, long ldg, int ncols, long gld) {
   a.fp32 = 0;
   a.order = p->fft_order;
   a.deriv = 0;
+  a.xy2 = nullptr;
   return a;
 }
 
@@ -488,8 +489,8 @@ static int p2_inv(const dsht_plan* p, const GT* G, const FftArgs& a, double* xy,
   return 0;
 }
 
-template <typename GT>
-static int fft_fwd_any(const dsht_plan* p, const double* xy, long ldg, int ncols, GT* G, long gld,
+template <typename GT, typename XT>
+static int fft_fwd_any(const dsht_plan* p, const XT* xy, long ldg, int ncols, GT* G, long gld,
                        unsigned* colmax, hipStream_t s) {
   FftArgs a = fft_args(p, ldg, ncols, gld);
   // forward: FP32 arithmetic for the split forms' FP32 Fourier coefficients; the register
@@ -497,73 +498,93 @@ static int fft_fwd_any(const dsht_plan* p, const double* xy, long ldg, int ncols
   // does not cost the m > 0 coefficients their relative precision
   a.fp32 = (sizeof(GT) == 8 && p->fft_kind == 0) ? 1 : 0;
   long nrows = (long)p->J * ncols;
-  ProfScope ps("fft_fwd", s, (double)nrows * (8.0 * p->I + (double)sizeof(GT) * (p->N + 1)),
+  ProfScope ps("fft_fwd", s, (double)nrows * ((double)sizeof(XT) * p->I + (double)sizeof(GT) * (p->N + 1)),
                (double)nrows * 2.5 * p->I * std::log2((double)p->I));
-  if (p->fft_kind != 1 && reg_fft_supported(a.M)) return reg_fft_fwd<GT>(xy, a, G, colmax, s);
-  switch (a.M) {
-    case 16: return p2_fwd<16>(p, xy, a, G, colmax, s);
-    case 32: return p2_fwd<32>(p, xy, a, G, colmax, s);
-    case 64: return p2_fwd<64>(p, xy, a, G, colmax, s);
-    case 128: return p2_fwd<128>(p, xy, a, G, colmax, s);
-    case 256: return p2_fwd<256>(p, xy, a, G, colmax, s);
-    case 512: return p2_fwd<512>(p, xy, a, G, colmax, s);
+  if (p->fft_kind != 1 && reg_fft_supported(a.M)) return reg_fft_fwd<GT, XT>(xy, a, G, colmax, s);
+  if constexpr (sizeof(XT) == 4) {
+    return fail(DSHT_EUNSUPPORTED, "FP32 grids need a register FFT kernel (DSHT_TUNE_FFT 0)");
+  } else {
+    switch (a.M) {
+      case 16: return p2_fwd<16>(p, xy, a, G, colmax, s);
+      case 32: return p2_fwd<32>(p, xy, a, G, colmax, s);
+      case 64: return p2_fwd<64>(p, xy, a, G, colmax, s);
+      case 128: return p2_fwd<128>(p, xy, a, G, colmax, s);
+      case 256: return p2_fwd<256>(p, xy, a, G, colmax, s);
+      case 512: return p2_fwd<512>(p, xy, a, G, colmax, s);
+    }
+    int nb = (int)cdivl(nrows, a.rows_per_block);
+    size_t lds = 2ul * a.rows_per_block * a.M * sizeof(double2);
+    hipLaunchKernelGGL(k_fft_fwd<GT>, dim3(nb), dim3(256), lds, s, xy, G, a, colmax);
+    DSHT_HIP(hipGetLastError());
+    return 0;
   }
-  int nb = (int)cdivl(nrows, a.rows_per_block);
-  size_t lds = 2ul * a.rows_per_block * a.M * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_fwd<GT>, dim3(nb), dim3(256), lds, s, xy, G, a, colmax);
-  DSHT_HIP(hipGetLastError());
-  return 0;
 }
 
 bool fft_inv_deriv_ok(const dsht_plan* p) { return p->fft_kind != 1 && reg_fft_supported(p->fft.M); }
 
-template <typename GT>
-static int fft_inv_any(const dsht_plan* p, const GT* G, long gld, int ncols, double* xy, long ldg,
-                       hipStream_t s, const double* rowscale, int deriv) {
+template <typename GT, typename XT>
+static int fft_inv_any(const dsht_plan* p, const GT* G, long gld, int ncols, XT* xy, long ldg,
+                       hipStream_t s, const double* rowscale, int deriv, XT* xy2) {
   FftArgs a = fft_args(p, ldg, ncols, gld);
   a.rowscale = rowscale;
   a.deriv = deriv;
-  if (deriv && !fft_inv_deriv_ok(p)) return fail(DSHT_EUNSUPPORTED, "derivative inverse FFT needs a register kernel");
+  a.xy2 = xy2;
+  if ((deriv || xy2) && !fft_inv_deriv_ok(p))
+    return fail(DSHT_EUNSUPPORTED, "derivative inverse FFT needs a register kernel");
   // inverse: FP32 arithmetic on the split forms' FP32 Fourier coefficients (each output point
   // is a sum of already-FP32 terms; the error stays at FP32 rounding of the output scale)
   a.fp32 = (sizeof(GT) == 8 && p->fft_kind == 0) ? 1 : 0;
   long nrows = (long)p->J * ncols;
-  ProfScope ps("fft_inv", s, (double)nrows * (8.0 * p->I + (double)sizeof(GT) * (p->N + 1)),
-               (double)nrows * 2.5 * p->I * std::log2((double)p->I));
-  if (p->fft_kind != 1 && reg_fft_supported(a.M)) return reg_fft_inv<GT>(G, a, xy, s);
-  switch (a.M) {
-    case 16: return p2_inv<16>(p, G, a, xy, s);
-    case 32: return p2_inv<32>(p, G, a, xy, s);
-    case 64: return p2_inv<64>(p, G, a, xy, s);
-    case 128: return p2_inv<128>(p, G, a, xy, s);
-    case 256: return p2_inv<256>(p, G, a, xy, s);
-    case 512: return p2_inv<512>(p, G, a, xy, s);
+  ProfScope ps("fft_inv", s, (double)nrows * ((xy2 ? 2.0 : 1.0) * sizeof(XT) * p->I + (double)sizeof(GT) * (p->N + 1)),
+               (double)nrows * (xy2 ? 2.0 : 1.0) * 2.5 * p->I * std::log2((double)p->I));
+  if (p->fft_kind != 1 && reg_fft_supported(a.M)) return reg_fft_inv<GT, XT>(G, a, xy, s);
+  if constexpr (sizeof(XT) == 4) {
+    return fail(DSHT_EUNSUPPORTED, "FP32 grids need a register FFT kernel (DSHT_TUNE_FFT 0)");
+  } else {
+    switch (a.M) {
+      case 16: return p2_inv<16>(p, G, a, xy, s);
+      case 32: return p2_inv<32>(p, G, a, xy, s);
+      case 64: return p2_inv<64>(p, G, a, xy, s);
+      case 128: return p2_inv<128>(p, G, a, xy, s);
+      case 256: return p2_inv<256>(p, G, a, xy, s);
+      case 512: return p2_inv<512>(p, G, a, xy, s);
+    }
+    int nb = (int)cdivl(nrows, a.rows_per_block);
+    size_t lds = 2ul * a.rows_per_block * a.M * sizeof(double2);
+    hipLaunchKernelGGL(k_fft_inv<GT>, dim3(nb), dim3(256), lds, s, G, xy, a);
+    DSHT_HIP(hipGetLastError());
+    return 0;
   }
-  int nb = (int)cdivl(nrows, a.rows_per_block);
-  size_t lds = 2ul * a.rows_per_block * a.M * sizeof(double2);
-  hipLaunchKernelGGL(k_fft_inv<GT>, dim3(nb), dim3(256), lds, s, G, xy, a);
-  DSHT_HIP(hipGetLastError());
-  return 0;
 }
 
 int launch_fft_fwd(const dsht_plan* p, const double* xy, long ldg, int ncols, double2* G, long gld,
                    hipStream_t s) {
-  return fft_fwd_any<double2>(p, xy, ldg, ncols, G, gld, nullptr, s);
+  return fft_fwd_any<double2, double>(p, xy, ldg, ncols, G, gld, nullptr, s);
 }
 
 int launch_fft_inv(const dsht_plan* p, const double2* G, long gld, int ncols, double* xy, long ldg,
-                   hipStream_t s, const double* rowscale, int deriv) {
-  return fft_inv_any<double2>(p, G, gld, ncols, xy, ldg, s, rowscale, deriv);
+                   hipStream_t s, const double* rowscale, int deriv, double* xy2) {
+  return fft_inv_any<double2, double>(p, G, gld, ncols, xy, ldg, s, rowscale, deriv, xy2);
 }
 
 int launch_fft_fwd32(const dsht_plan* p, const double* xy, long ldg, int ncols, float2* G, long gld,
                      unsigned* colmax, hipStream_t s) {
-  return fft_fwd_any<float2>(p, xy, ldg, ncols, G, gld, colmax, s);
+  return fft_fwd_any<float2, double>(p, xy, ldg, ncols, G, gld, colmax, s);
 }
 
 int launch_fft_inv32(const dsht_plan* p, const float2* G, long gld, int ncols, double* xy, long ldg,
-                     hipStream_t s, const double* rowscale, int deriv) {
-  return fft_inv_any<float2>(p, G, gld, ncols, xy, ldg, s, rowscale, deriv);
+                     hipStream_t s, const double* rowscale, int deriv, double* xy2) {
+  return fft_inv_any<float2, double>(p, G, gld, ncols, xy, ldg, s, rowscale, deriv, xy2);
+}
+
+int launch_fft_fwd32f(const dsht_plan* p, const float* xy, long ldg, int ncols, float2* G, long gld,
+                      unsigned* colmax, hipStream_t s) {
+  return fft_fwd_any<float2, float>(p, xy, ldg, ncols, G, gld, colmax, s);
+}
+
+int launch_fft_inv32f(const dsht_plan* p, const float2* G, long gld, int ncols, float* xy, long ldg,
+                      hipStream_t s, const double* rowscale, int deriv, float* xy2) {
+  return fft_inv_any<float2, float>(p, G, gld, ncols, xy, ldg, s, rowscale, deriv, xy2);
 }
 
 }  // namespace dsht

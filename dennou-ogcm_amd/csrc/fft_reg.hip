@@ -269,9 +269,30 @@ typedef double dvec2 __attribute__((ext_vector_type(2)));
 template <typename V>
 constexpr int WPE = sizeof(typename Cx<V>::S) == 4 ? 3 : 2;
 
-// V: arithmetic type (double2, or float2 for the FP32 Fourier path of the split forms)
-template <int R1, int R2, int NT, typename GT, typename V>
-__global__ __launch_bounds__(NT, WPE<V>) void k_fft_fwd_r(const double* __restrict__ xy, GT* __restrict__ G,
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+
+// two consecutive grid samples (one packed complex input) as FP64, non-temporal
+__device__ __forceinline__ double2 load_pair(const double* x) {
+  const dvec2 t = __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(x));
+  return make_double2(t.x, t.y);
+}
+__device__ __forceinline__ double2 load_pair(const float* x) {
+  const fvec2 t = __builtin_nontemporal_load(reinterpret_cast<const fvec2*>(x));
+  return make_double2((double)t.x, (double)t.y);
+}
+__device__ __forceinline__ void store_pair(double* x, double a, double b) {
+  const dvec2 t = {a, b};
+  __builtin_nontemporal_store(t, reinterpret_cast<dvec2*>(x));
+}
+__device__ __forceinline__ void store_pair(float* x, double a, double b) {
+  const fvec2 t = {(float)a, (float)b};
+  __builtin_nontemporal_store(t, reinterpret_cast<fvec2*>(x));
+}
+
+// V: arithmetic type (double2, or float2 for the FP32 Fourier path of the split forms);
+// XT: grid element type (double, or float for the split forms' FP32 grid I/O)
+template <int R1, int R2, int NT, typename GT, typename V, typename XT>
+__global__ __launch_bounds__(NT, WPE<V>) void k_fft_fwd_r(const XT* __restrict__ xy, GT* __restrict__ G,
                                                   FftArgs a, unsigned* __restrict__ colmax) {
   using C = RegFft<R1, R2, NT, V>;
   using S = typename Cx<V>::S;
@@ -312,18 +333,18 @@ __global__ __launch_bounds__(NT, WPE<V>) void k_fft_fwd_r(const double* __restri
   auto fetch = [&](long g) {
     int j, c;
     if (rr1 < ROWS && map_row(a, g, rr1, ROWS, j, c)) {
-      const double* xr = xy + c * a.ldg + (long)j * (2 * M);
-      const double2* x = reinterpret_cast<const double2*>(xr);
+      const XT* xr = xy + c * a.ldg + (long)j * (2 * M);
       if constexpr (REF) {
-        x0n = xr[0];
+        x0n = (double)xr[0];
 #pragma unroll
-        for (int n2 = 0; n2 < R2; n2++) {
-          const dvec2 t = __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(x + n1 + R1 * n2));
-          raw[n2] = make_double2(t.x, t.y);
-        }
-      } else {
+        for (int n2 = 0; n2 < R2; n2++) raw[n2] = load_pair(xr + 2 * (n1 + R1 * n2));
+      } else if constexpr (sizeof(XT) == 8) {
+        const double2* x = reinterpret_cast<const double2*>(xr);
 #pragma unroll
         for (int n2 = 0; n2 < R2; n2++) v[n2] = cvt<V>(x[n1 + R1 * n2]);
+      } else {
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) v[n2] = cvt<V>(load_pair(xr + 2 * (n1 + R1 * n2)));
       }
     } else {
       if constexpr (REF) {
@@ -401,10 +422,13 @@ __global__ __launch_bounds__(NT, WPE<V>) void k_fft_fwd_r(const double* __restri
   }
 }
 
-// DERIV: transform i m X_m (longitude derivative); a separate instantiation keeps the plain
-// kernel's register allocation
-template <int R1, int R2, int NT, typename GT, typename V, bool DERIV>
-__global__ __launch_bounds__(NT, WPE<V>) void k_fft_inv_r(const GT* __restrict__ G, double* __restrict__ xy,
+// MODE 0: the field; 1: its longitude derivative (transform i m X_m, times a.rowscale);
+// 2 (dual): both from one read of the coefficients -- the field into xy, then the
+// derivative, times a.rowscale, into a.xy2 -- the coefficients stay in registers between the
+// two passes.  Separate instantiations keep each form's register allocation.  XT: grid
+// element type.
+template <int R1, int R2, int NT, typename GT, typename V, int MODE, typename XT>
+__global__ __launch_bounds__(NT, WPE<V>) void k_fft_inv_r(const GT* __restrict__ G, XT* __restrict__ xy,
                                                   FftArgs a) {
   using C = RegFft<R1, R2, NT, V>;
   using S = typename Cx<V>::S;
@@ -447,10 +471,14 @@ __global__ __launch_bounds__(NT, WPE<V>) void k_fft_inv_r(const GT* __restrict__
   // prefetching holds the next group in registers through the whole pipeline: only where
   // that fits the register budget (FP32 coefficients, M <= 256)
   constexpr bool PF = sizeof(GT) == 8 && M <= 256;
+  constexpr int NPASS = MODE == 2 ? 2 : 1;
   const long gstep = gridDim.x;
   if (PF && (long)blockIdx.x < ngroups) fetch(blockIdx.x);
   for (long g0 = blockIdx.x; g0 < ngroups; g0 += gstep) {
     if (!PF) fetch(g0);
+#pragma unroll
+  for (int pass = 0; pass < NPASS; pass++) {
+    const bool deriv = MODE == 1 || (MODE == 2 && pass == 1);
     __syncthreads();  // the previous rows are out of buf
 #pragma unroll
     for (int q = 0; q < NL; q++) {
@@ -461,7 +489,7 @@ __global__ __launch_bounds__(NT, WPE<V>) void k_fft_inv_r(const GT* __restrict__
         buf[(idx % ROWS) * SR + idx / ROWS] = v;
       }
     }
-    if (PF && g0 + gstep < ngroups) fetch(g0 + gstep);
+    if (PF && pass == NPASS - 1 && g0 + gstep < ngroups) fetch(g0 + gstep);
     __syncthreads();
     // packed spectrum Z_m from the half-complex pair (X_m, X_{M-m}); X_M = 0 for m = 0
     for (int idx = tid; idx < ROWS * (M / 2 + 1); idx += C::THREADS) {
@@ -469,7 +497,7 @@ __global__ __launch_bounds__(NT, WPE<V>) void k_fft_inv_r(const GT* __restrict__
       V* X = buf + rr * SR;
       const int mc = m ? M - m : 0;
       V xm = X[m], xo = X[mc];
-      if constexpr (DERIV) {  // longitude derivative: each X_k becomes i k X_k
+      if (deriv) {  // longitude derivative: each X_k becomes i k X_k
         const S fm = (S)m, fc = (S)mc;
         xm = mk<V>(-fm * xm.y, fm * xm.x);
         xo = mk<V>(-fc * xo.y, fc * xo.x);
@@ -508,15 +536,15 @@ __global__ __launch_bounds__(NT, WPE<V>) void k_fft_inv_r(const GT* __restrict__
       dft<R1, +1>(u);
       int j, c;
       if (map_row(a, g0, rr2, ROWS, j, c)) {
-        const double s = a.rowscale ? a.rowscale[j] : 1.0;
-        double2* x = reinterpret_cast<double2*>(xy + c * a.ldg + (long)j * (2 * M));
+        // dual: the field unscaled into xy, the derivative times the row scale into xy2
+        const double s = (a.rowscale && (MODE != 2 || pass == 1)) ? a.rowscale[j] : 1.0;
+        XT* out = (MODE == 2 && pass == 1) ? reinterpret_cast<XT*>(a.xy2) : xy;
+        XT* x = out + c * a.ldg + (long)j * (2 * M);
 #pragma unroll
-        for (int k1 = 0; k1 < R1; k1++) {
-          const dvec2 t = {(double)u[k1].x * s, (double)u[k1].y * s};
-          __builtin_nontemporal_store(t, reinterpret_cast<dvec2*>(x + k2 + R2 * k1));
-        }
+        for (int k1 = 0; k1 < R1; k1++) store_pair(x + 2 * (k2 + R2 * k1), (double)u[k1].x * s, (double)u[k1].y * s);
       }
     }
+  }  // pass
   }
 }
 
@@ -546,53 +574,70 @@ static long resident_grid(K kernel, int threads, size_t lds) {
   return (long)per_cu * cu_count();
 }
 
-template <int R1, int R2, int NT, typename GT, typename V>
-static int reg_fwd_nt(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
+template <int R1, int R2, int NT, typename GT, typename V, typename XT>
+static int reg_fwd_nt(const XT* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
   using C = RegFft<R1, R2, NT, V>;
-  static const long cap = resident_grid(k_fft_fwd_r<R1, R2, NT, GT, V>, C::THREADS, C::LDS);
+  static const long cap = resident_grid(k_fft_fwd_r<R1, R2, NT, GT, V, XT>, C::THREADS, C::LDS);
   long nb = row_groups(a, C::ROWS);
   if (nb > cap) nb = cap;
-  hipLaunchKernelGGL((k_fft_fwd_r<R1, R2, NT, GT, V>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, xy, G, a,
+  hipLaunchKernelGGL((k_fft_fwd_r<R1, R2, NT, GT, V, XT>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, xy, G, a,
                      colmax);
   DSHT_HIP(hipGetLastError());
   return 0;
 }
 
-template <int R1, int R2, int NT, typename GT, typename V>
-static int reg_inv_nt(const GT* G, const FftArgs& a, double* xy, hipStream_t s) {
+template <int R1, int R2, int NT, typename GT, typename V, int MODE, typename XT>
+static int reg_inv_mode(const GT* G, const FftArgs& a, XT* xy, hipStream_t s) {
   using C = RegFft<R1, R2, NT, V>;
-  static const long cap = resident_grid(k_fft_inv_r<R1, R2, NT, GT, V, false>, C::THREADS, C::LDS);
-  static const long capd = resident_grid(k_fft_inv_r<R1, R2, NT, GT, V, true>, C::THREADS, C::LDS);
+  static const long cap = resident_grid(k_fft_inv_r<R1, R2, NT, GT, V, MODE, XT>, C::THREADS, C::LDS);
   long nb = row_groups(a, C::ROWS);
-  if (nb > (a.deriv ? capd : cap)) nb = a.deriv ? capd : cap;
-  if (a.deriv)
-    hipLaunchKernelGGL((k_fft_inv_r<R1, R2, NT, GT, V, true>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, G, xy,
-                       a);
-  else
-    hipLaunchKernelGGL((k_fft_inv_r<R1, R2, NT, GT, V, false>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, G,
-                       xy, a);
+  if (nb > cap) nb = cap;
+  hipLaunchKernelGGL((k_fft_inv_r<R1, R2, NT, GT, V, MODE, XT>), dim3((unsigned)nb), dim3(C::THREADS), C::LDS, s, G,
+                     xy, a);
   DSHT_HIP(hipGetLastError());
   return 0;
 }
 
-// workgroup size 256 threads (or 128: half the LDS per workgroup); arithmetic FP64, or
-// FP32 for the FP32 Fourier path of the split forms (a.fp32)
-template <int R1, int R2, typename GT>
-static int reg_fwd(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
-  if constexpr (sizeof(GT) == 8) {
-    if (a.fp32) return reg_fwd_nt<R1, R2, 256, GT, float2>(xy, a, G, colmax, s);
-  }
-  if (a.threads == 128 && R1 <= 128) return reg_fwd_nt<R1, R2, 128, GT, double2>(xy, a, G, colmax, s);
-  return reg_fwd_nt<R1, R2, 256, GT, double2>(xy, a, G, colmax, s);
+template <int R1, int R2, int NT, typename GT, typename V, typename XT>
+static int reg_inv_nt(const GT* G, const FftArgs& a, XT* xy, hipStream_t s) {
+  if (a.xy2) return reg_inv_mode<R1, R2, NT, GT, V, 2, XT>(G, a, xy, s);
+  if (a.deriv) return reg_inv_mode<R1, R2, NT, GT, V, 1, XT>(G, a, xy, s);
+  return reg_inv_mode<R1, R2, NT, GT, V, 0, XT>(G, a, xy, s);
 }
 
-template <int R1, int R2, typename GT>
-static int reg_inv(const GT* G, const FftArgs& a, double* xy, hipStream_t s) {
-  if constexpr (sizeof(GT) == 8) {
-    if (a.fp32) return reg_inv_nt<R1, R2, 256, GT, float2>(G, a, xy, s);
+// workgroup size 256 threads (or 128: half the LDS per workgroup); arithmetic FP64, or
+// FP32 for the FP32 Fourier path of the split forms (a.fp32); FP32 grids only with the
+// split forms' FP32 path
+template <int R1, int R2, typename GT, typename XT>
+static int reg_fwd(const XT* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
+  if constexpr (sizeof(XT) == 4) {
+    if constexpr (sizeof(GT) == 8) {
+      if (a.fp32) return reg_fwd_nt<R1, R2, 256, GT, float2, XT>(xy, a, G, colmax, s);
+    }
+    return fail(DSHT_EUNSUPPORTED, "FP32 grids need the split forms' FP32 FFT path");
+  } else {
+    if constexpr (sizeof(GT) == 8) {
+      if (a.fp32) return reg_fwd_nt<R1, R2, 256, GT, float2, XT>(xy, a, G, colmax, s);
+    }
+    if (a.threads == 128 && R1 <= 128) return reg_fwd_nt<R1, R2, 128, GT, double2, XT>(xy, a, G, colmax, s);
+    return reg_fwd_nt<R1, R2, 256, GT, double2, XT>(xy, a, G, colmax, s);
   }
-  if (a.threads == 128 && R1 <= 128) return reg_inv_nt<R1, R2, 128, GT, double2>(G, a, xy, s);
-  return reg_inv_nt<R1, R2, 256, GT, double2>(G, a, xy, s);
+}
+
+template <int R1, int R2, typename GT, typename XT>
+static int reg_inv(const GT* G, const FftArgs& a, XT* xy, hipStream_t s) {
+  if constexpr (sizeof(XT) == 4) {
+    if constexpr (sizeof(GT) == 8) {
+      if (a.fp32) return reg_inv_nt<R1, R2, 256, GT, float2, XT>(G, a, xy, s);
+    }
+    return fail(DSHT_EUNSUPPORTED, "FP32 grids need the split forms' FP32 FFT path");
+  } else {
+    if constexpr (sizeof(GT) == 8) {
+      if (a.fp32) return reg_inv_nt<R1, R2, 256, GT, float2, XT>(G, a, xy, s);
+    }
+    if (a.threads == 128 && R1 <= 128) return reg_inv_nt<R1, R2, 128, GT, double2, XT>(G, a, xy, s);
+    return reg_inv_nt<R1, R2, 256, GT, double2, XT>(G, a, xy, s);
+  }
 }
 
 // factor pairs (R1, R2) with M = R1 * R2 that have register kernels
@@ -619,27 +664,29 @@ bool reg_fft_supported(int M) {
   return false;
 }
 
-template <typename GT>
-int reg_fft_fwd(const double* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
+template <typename GT, typename XT>
+int reg_fft_fwd(const XT* xy, const FftArgs& a, GT* G, unsigned* colmax, hipStream_t s) {
 #define X(r1, r2) \
-  if (a.M == (r1) * (r2)) return reg_fwd<r1, r2, GT>(xy, a, G, colmax, s);
+  if (a.M == (r1) * (r2)) return reg_fwd<r1, r2, GT, XT>(xy, a, G, colmax, s);
   DSHT_REG_FFT_SIZES(X)
 #undef X
   return fail(DSHT_EUNSUPPORTED, "no register FFT for M = %d", a.M);
 }
 
-template <typename GT>
-int reg_fft_inv(const GT* G, const FftArgs& a, double* xy, hipStream_t s) {
+template <typename GT, typename XT>
+int reg_fft_inv(const GT* G, const FftArgs& a, XT* xy, hipStream_t s) {
 #define X(r1, r2) \
-  if (a.M == (r1) * (r2)) return reg_inv<r1, r2, GT>(G, a, xy, s);
+  if (a.M == (r1) * (r2)) return reg_inv<r1, r2, GT, XT>(G, a, xy, s);
   DSHT_REG_FFT_SIZES(X)
 #undef X
   return fail(DSHT_EUNSUPPORTED, "no register FFT for M = %d", a.M);
 }
 
-template int reg_fft_fwd<double2>(const double*, const FftArgs&, double2*, unsigned*, hipStream_t);
-template int reg_fft_fwd<float2>(const double*, const FftArgs&, float2*, unsigned*, hipStream_t);
-template int reg_fft_inv<double2>(const double2*, const FftArgs&, double*, hipStream_t);
-template int reg_fft_inv<float2>(const float2*, const FftArgs&, double*, hipStream_t);
+template int reg_fft_fwd<double2, double>(const double*, const FftArgs&, double2*, unsigned*, hipStream_t);
+template int reg_fft_fwd<float2, double>(const double*, const FftArgs&, float2*, unsigned*, hipStream_t);
+template int reg_fft_fwd<float2, float>(const float*, const FftArgs&, float2*, unsigned*, hipStream_t);
+template int reg_fft_inv<double2, double>(const double2*, const FftArgs&, double*, hipStream_t);
+template int reg_fft_inv<float2, double>(const float2*, const FftArgs&, double*, hipStream_t);
+template int reg_fft_inv<float2, float>(const float2*, const FftArgs&, float*, hipStream_t);
 
 }  // namespace dsht

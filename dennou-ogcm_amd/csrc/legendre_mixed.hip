@@ -810,7 +810,7 @@ static float weight_bound(const dsht_plan* p, bool div) {
 
 int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, long ldg, double* out0,
                   double* out1, long sl, long sc, int ncols, const double* diag0, const double* diag1,
-                  double scale, const double* cw, hipStream_t s) {
+                  double scale, const double* cw, hipStream_t s, bool grid_f32) {
   if (ncols <= 0) return 0;
   const int form = form_of(p->prec);
   if (form < 0) return fail(DSHT_EUNSUPPORTED, "precision form %d not available", p->prec);
@@ -851,8 +851,15 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
       DSHT_HIP(hipGetLastError());
     }
     const long gp = fourier_pitch(nc);
-    int rc = launch_fft_fwd32(p, in0 + (long)c0 * ldg, ldg, nc, G0, gp, scaled ? colmax : nullptr, fs);
-    if (!rc && nin > 1) rc = launch_fft_fwd32(p, in1 + (long)c0 * ldg, ldg, nc, G1, gp, scaled ? colmax + nc : nullptr, fs);
+    int rc;
+    if (grid_f32) {
+      const float *f0 = (const float*)in0, *f1 = (const float*)in1;
+      rc = launch_fft_fwd32f(p, f0 + (long)c0 * ldg, ldg, nc, G0, gp, scaled ? colmax : nullptr, fs);
+      if (!rc && nin > 1) rc = launch_fft_fwd32f(p, f1 + (long)c0 * ldg, ldg, nc, G1, gp, scaled ? colmax + nc : nullptr, fs);
+    } else {
+      rc = launch_fft_fwd32(p, in0 + (long)c0 * ldg, ldg, nc, G0, gp, scaled ? colmax : nullptr, fs);
+      if (!rc && nin > 1) rc = launch_fft_fwd32(p, in1 + (long)c0 * ldg, ldg, nc, G1, gp, scaled ? colmax + nc : nullptr, fs);
+    }
     if (rc) return rc;
     if (cp.on && (rc = pipe_after(cp.evF[slot], fs, s))) return rc;
     MxFwdArgs a;
@@ -920,7 +927,7 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
 
 int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, long sl, long sc,
                   double* out0, double* out1, long ldg, int ncols, const double* pro, const double* rs,
-                  hipStream_t s, double* out2) {
+                  hipStream_t s, double* out2, bool grid_f32) {
   if (ncols <= 0) return 0;
   const int form = form_of(p->prec);
   if (form < 0) return fail(DSHT_EUNSUPPORTED, "precision form %d not available", p->prec);
@@ -1033,10 +1040,25 @@ int mixed_run_inv(dsht_plan* p, int op, const double* in0, const double* in1, lo
       if ((rc = pipe_after(cp.evL[slot], s, fs))) return rc;
     }
     const long gp = fourier_pitch(nc);
-    if (out0 && (rc = launch_fft_inv32(p, G0, gp, nc, out0 + (long)c0 * ldg, ldg, fs))) return rc;
-    if (dlon && (rc = launch_fft_inv32(p, G0, gp, nc, out1 + (long)c0 * ldg, ldg, fs, p->d_rs_icos, 1))) return rc;
-    if (O1 && (rc = launch_fft_inv32(p, G1, gp, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
-    if (O2 && (rc = launch_fft_inv32(p, G2, gp, nc, out2 + (long)c0 * ldg, ldg, fs))) return rc;
+    // one inverse FFT per output; the field and its longitude derivative (fused value/gradient
+    // call) come from one read of G0
+    auto inv = [&](const float2* G, double* out, const double* rsc, int deriv, double* out2d) -> int {
+      if (grid_f32) {
+        float* o = (float*)out + (long)c0 * ldg;
+        float* o2 = out2d ? (float*)out2d + (long)c0 * ldg : nullptr;
+        return launch_fft_inv32f(p, G, gp, nc, o, ldg, fs, rsc, deriv, o2);
+      }
+      return launch_fft_inv32(p, G, gp, nc, out + (long)c0 * ldg, ldg, fs, rsc, deriv,
+                              out2d ? out2d + (long)c0 * ldg : nullptr);
+    };
+    if (out0 && dlon) {
+      if ((rc = inv(G0, out0, p->d_rs_icos, 0, out1))) return rc;
+    } else {
+      if (out0 && (rc = inv(G0, out0, nullptr, 0, nullptr))) return rc;
+      if (dlon && (rc = inv(G0, out1, p->d_rs_icos, 1, nullptr))) return rc;
+    }
+    if (O1 && (rc = inv(G1, out1, nullptr, 0, nullptr))) return rc;
+    if (O2 && (rc = inv(G2, out2, nullptr, 0, nullptr))) return rc;
     if (cp.on) DSHT_HIP(hipEventRecord(cp.evF[slot], fs));
   }
   return pipe_join(cp);

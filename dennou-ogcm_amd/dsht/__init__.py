@@ -37,6 +37,7 @@ EXPORTS = [
     "dsht_w_xy", "dsht_xy_w", "dsht_xy_vgrad_w", "dsht_w_alpha_xy", "dsht_xy_alpha_w", "dsht_uvcoslat2vordiv",
     "dsht_vordiv2uv", "dsht_w_diag_w", "dsht_xy_diag_xy", "dsht_intlonlat", "dsht_avrlonlat",
     "dsht_ya_intlon", "dsht_ya_avrlon", "dsht_interpolate", "dsht_fwd_d", "dsht_inv_d", "dsht_inv_vgrad_d", "dsht_inv_noalloc_d",
+    "dsht_fwd_f32_d", "dsht_inv_f32_d", "dsht_inv_vgrad_f32_d",
     "dsht_alpha_fwd_d", "dsht_alpha_inv_d", "dsht_uv2vordiv_d", "dsht_vordiv2uv_d",
     "dsht_diag_d", "dsht_intlonlat_d", "dsht_avrlonlat_d", "dsht_ya_intlon_d",
     "dsht_interpolate_d", "dsht_coef_d", "dsht_col_omega_d", "dsht_col_hydpres_d", "dsht_col_uvbarot_d",
@@ -119,6 +120,9 @@ def load():
         "dsht_fwd_d": ([vp, i, vp, lg, vp, lg, lg, i, vp, vp], i),
         "dsht_inv_d": ([vp, i, vp, lg, lg, vp, lg, i, vp, vp], i),
         "dsht_inv_noalloc_d": ([vp, i, vp, lg, lg, vp, lg, i, vp, vp], i),
+        "dsht_fwd_f32_d": ([vp, i, vp, lg, vp, lg, lg, i, vp, vp], i),
+        "dsht_inv_f32_d": ([vp, i, vp, lg, lg, vp, lg, i, vp, vp], i),
+        "dsht_inv_vgrad_f32_d": ([vp, vp, lg, lg, vp, vp, vp, lg, i, vp, vp], i),
         "dsht_inv_vgrad_d": ([vp, vp, lg, lg, vp, vp, vp, lg, i, vp, vp], i),
         "dsht_alpha_fwd_d": ([vp, vp, vp, lg, vp, lg, lg, i, vp, vp], i),
         "dsht_alpha_inv_d": ([vp, vp, vp, lg, lg, vp, lg, i, vp], i),
@@ -486,6 +490,34 @@ class Plan:
         tp = lambda t: _tptr(t) if t is not None else None
         _check(load().dsht_inv_vgrad_d(self.h, _tptr(w), sl, sc, tp(xy), tp(xy_lon), tp(xy_lat),
                                        outs[0].stride(0), nc, tp(diag), _stream(stream)))
+        return xy, xy_lon, xy_lat
+
+    # FP32 grid I/O of the split forms: grids are float32 device tensors (ncols, J, I)
+    def fwd_f32_d(self, kind, xy, w, layout="LC", diag=None, stream=None):
+        nc = xy.shape[0]
+        sl, sc = self._spec_strides(w, nc, layout)
+        _check(load().dsht_fwd_f32_d(self.h, kind, _tptr(xy), xy.stride(0), _tptr(w), sl, sc, nc,
+                                     _tptr(diag) if diag is not None else None, _stream(stream)))
+        return w
+
+    def inv_f32_d(self, kind, w, xy, layout="LC", diag=None, stream=None):
+        nc = xy.shape[0]
+        sl, sc = self._spec_strides(w, nc, layout)
+        _check(load().dsht_inv_f32_d(self.h, kind, _tptr(w), sl, sc, _tptr(xy), xy.stride(0), nc,
+                                     _tptr(diag) if diag is not None else None, _stream(stream)))
+        return xy
+
+    def inv_vgrad_f32_d(self, w, xy, xy_lon, xy_lat, layout="LC", diag=None, stream=None):
+        outs = [t for t in (xy, xy_lon, xy_lat) if t is not None]
+        if not outs:
+            raise ValueError("inv_vgrad_f32_d: every output is None")
+        nc = outs[0].shape[0]
+        if any(t.shape[0] != nc or t.stride(0) != outs[0].stride(0) for t in outs):
+            raise ValueError("inv_vgrad_f32_d: outputs differ in columns or pitch")
+        sl, sc = self._spec_strides(w, nc, layout)
+        tp = lambda t: _tptr(t) if t is not None else None
+        _check(load().dsht_inv_vgrad_f32_d(self.h, _tptr(w), sl, sc, tp(xy), tp(xy_lon), tp(xy_lat),
+                                           outs[0].stride(0), nc, tp(diag), _stream(stream)))
         return xy, xy_lon, xy_lat
 
     def alpha_fwd_d(self, a, b, w, layout="LC", diag=None, stream=None):

@@ -146,6 +146,18 @@ int dsht_fwd_d(dsht_plan* plan, int kind, const double* xy, long ldg, double* w,
                int ncols, const double* diag, void* stream);
 int dsht_inv_d(dsht_plan* plan, int kind, const double* w, long sl, long sc, double* xy, long ldg,
                int ncols, const double* diag, void* stream);
+/* FP32 grid I/O of the split precision forms (no counterpart in the reference, whose grids
+ * are real(DP)): dsht_fwd_d, dsht_inv_d and dsht_inv_vgrad_d with float grids (same layout
+ * and pitch in elements).  Spectra stay FP64; the Legendre contractions and FFTs run as in
+ * the split forms, so only the rounding of the grid values to FP32 is added (relative
+ * 6e-8).  DSHT_EUNSUPPORTED in the FP64 form and without the register FFT kernels.         */
+int dsht_fwd_f32_d(dsht_plan* plan, int kind, const float* xy, long ldg, double* w, long sl,
+                   long sc, int ncols, const double* diag, void* stream);
+int dsht_inv_f32_d(dsht_plan* plan, int kind, const double* w, long sl, long sc, float* xy,
+                   long ldg, int ncols, const double* diag, void* stream);
+int dsht_inv_vgrad_f32_d(dsht_plan* plan, const double* w, long sl, long sc, float* xy,
+                         float* xy_lon, float* xy_lat, long ldg, int ncols, const double* diag,
+                         void* stream);
 /* No counterpart in the reference: dsht_inv_d for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Same arguments and results as dsht_inv_d, FP64 form only (other forms:
