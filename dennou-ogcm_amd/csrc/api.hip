@@ -119,37 +119,6 @@ int workspace_acquire(dsht_plan* p, hipStream_t s, size_t bytes, WsHandle& out) 
   return 0;
 }
 
-int pipe_begin(WsHandle& wh, hipStream_t s, const dsht_plan* p, int nchunks, ChunkPipe& cp) {
-  cp.on = p->pipeline && nchunks > 1 && wh.ws;
-  const int device = p->device;
-  cp.s = s;
-  if (!cp.on) return 0;
-  Workspace* ws = wh.ws;
-  if (!ws->aux) {
-    DSHT_HIP(hipSetDevice(device));
-    DSHT_HIP(hipStreamCreateWithFlags(&ws->aux, hipStreamNonBlocking));
-    for (auto& e : ws->ev) DSHT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  }
-  cp.aux = ws->aux;
-  cp.evL[0] = ws->ev[0];
-  cp.evL[1] = ws->ev[1];
-  cp.evF[0] = ws->ev[2];
-  cp.evF[1] = ws->ev[3];
-  cp.evX = ws->ev[4];
-  return pipe_after(cp.evX, s, cp.aux);  // fork
-}
-
-int pipe_after(hipEvent_t e, hipStream_t from, hipStream_t to) {
-  DSHT_HIP(hipEventRecord(e, from));
-  DSHT_HIP(hipStreamWaitEvent(to, e, 0));
-  return 0;
-}
-
-int pipe_join(ChunkPipe& cp) {
-  if (!cp.on) return 0;
-  return pipe_after(cp.evX, cp.aux, cp.s);
-}
-
 static int chunk_cols(const dsht_plan* p, int ncols) { return chunk_columns(p, ncols, sizeof(double2)); }
 
 static size_t fourier_bytes(const dsht_plan* p, int ch) {
@@ -171,24 +140,15 @@ static int run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, l
   const int nchunks = cdiv(ncols, ch);
   const size_t slot_bytes = nin * fourier_bytes(p, ch);
   WsHandle wl;
-  int rc = workspace_acquire(p, s, (p->pipeline && nchunks > 1 ? 2 : 1) * slot_bytes, wl);
+  int rc = workspace_acquire(p, s, slot_bytes, wl);
   if (rc) return rc;
-  ChunkPipe cp;
-  if ((rc = pipe_begin(wl, s, p, nchunks, cp))) return rc;
-  for (int k = 0, c0 = 0; c0 < ncols; k++, c0 += ch) {
+  for (int c0 = 0; c0 < ncols; c0 += ch) {
     const int nc = ncols - c0 < ch ? ncols - c0 : ch;
-    const int slot = cp.on ? (k & 1) : 0;
-    double2* G0 = (double2*)((char*)wl.buf + slot * slot_bytes);
+    double2* G0 = (double2*)wl.buf;
     double2* G1 = G0 + (size_t)(p->N + 1) * p->J * fourier_pitch(ch);
     const long gp = fourier_pitch(nc);
-    hipStream_t fs = s;
-    if (cp.on) {
-      fs = cp.aux;
-      if (k >= 2) DSHT_HIP(hipStreamWaitEvent(fs, cp.evL[slot], 0));  // slot free again
-    }
-    if ((rc = launch_fft_fwd(p, in0 + (long)c0 * ldg, ldg, nc, G0, gp, fs))) return rc;
-    if (nin > 1 && (rc = launch_fft_fwd(p, in1 + (long)c0 * ldg, ldg, nc, G1, gp, fs))) return rc;
-    if (cp.on && (rc = pipe_after(cp.evF[slot], fs, s))) return rc;
+    if ((rc = launch_fft_fwd(p, in0 + (long)c0 * ldg, ldg, nc, G0, gp, s))) return rc;
+    if (nin > 1 && (rc = launch_fft_fwd(p, in1 + (long)c0 * ldg, ldg, nc, G1, gp, s))) return rc;
     LegFwdArgs a;
     a.G0 = G0;
     a.G1 = nin > 1 ? G1 : nullptr;
@@ -210,15 +170,14 @@ static int run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, l
     a.J2 = p->J2;
     a.J2pad = p->J2pad;
     if ((rc = launch_leg_fwd(p, op, a, s))) return rc;
-    if (cp.on) DSHT_HIP(hipEventRecord(cp.evL[slot], s));
   }
-  return pipe_join(cp);
+  return 0;
 }
 
 // inverse family: spectra -> grids
 // fixed/fixed_bytes: a caller-provided scratch (the static one of dsht_inv_noalloc_d); the
-// call then takes no workspace, runs without the chunk pipeline and cuts its columns into
-// chunks whose Fourier arrays fit the scratch
+// call then takes no workspace and cuts its columns into chunks whose Fourier arrays fit the
+// scratch
 static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, long sl, long sc,
                    double* out0, double* out1, long ldg, int ncols, const double* pro, const double* rs,
                    hipStream_t s, double* out2 = nullptr, void* fixed = nullptr, size_t fixed_bytes = 0) {
@@ -239,19 +198,15 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
   WsHandle wl;
   int rc = 0;
   if (fixed) {
-    wl.buf = fixed;  // no Workspace: pipe_begin leaves the chunk pipeline off
-  } else if ((rc = workspace_acquire(p, s, (p->pipeline && nchunks > 1 ? 2 : 1) * slot_bytes, wl))) {
+    wl.buf = fixed;
+  } else if ((rc = workspace_acquire(p, s, slot_bytes, wl))) {
     return rc;
   }
-  ChunkPipe cp;
-  if ((rc = pipe_begin(wl, s, p, nchunks, cp))) return rc;
-  for (int k = 0, c0 = 0; c0 < ncols; k++, c0 += ch) {
+  for (int c0 = 0; c0 < ncols; c0 += ch) {
     const int nc = ncols - c0 < ch ? ncols - c0 : ch;
-    const int slot = cp.on ? (k & 1) : 0;
-    double2* G0 = (double2*)((char*)wl.buf + slot * slot_bytes);
+    double2* G0 = (double2*)wl.buf;
     double2* G1 = G0 + (size_t)(p->N + 1) * p->J * fourier_pitch(ch);
     double2* G2 = G1 + (size_t)(p->N + 1) * p->J * fourier_pitch(ch);
-    if (cp.on && k >= 2) DSHT_HIP(hipStreamWaitEvent(s, cp.evF[slot], 0));  // slot free again
     LegInvArgs a;
     a.W0 = in0 + (long)c0 * sc;
     a.W1 = in1 ? in1 + (long)c0 * sc : nullptr;
@@ -277,10 +232,6 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
     a.J2pad = p->J2pad;
     if ((rc = launch_leg_inv(p, op, a, s))) return rc;
     hipStream_t fs = s;
-    if (cp.on) {
-      fs = cp.aux;
-      if ((rc = pipe_after(cp.evL[slot], s, fs))) return rc;
-    }
     // the field and its longitude derivative as two launches over the same G0: the FP64
     // dual kernel (one read of G0, coefficients held in registers across both passes) ran
     // slower at T170L60 (2 x 125 against 2 x 80 us per step's worth), the FP32 one faster
@@ -288,9 +239,8 @@ static int run_inv(dsht_plan* p, int op, const double* in0, const double* in1, l
     if (dlon && (rc = launch_fft_inv(p, G0, a.gld, nc, out1 + (long)c0 * ldg, ldg, fs, p->d_rs_icos, 1))) return rc;
     if (a.G1 && (rc = launch_fft_inv(p, G1, a.gld, nc, out1 + (long)c0 * ldg, ldg, fs))) return rc;
     if (a.G2 && (rc = launch_fft_inv(p, G2, a.gld, nc, out2 + (long)c0 * ldg, ldg, fs))) return rc;
-    if (cp.on) DSHT_HIP(hipEventRecord(cp.evF[slot], fs));
   }
-  return pipe_join(cp);
+  return 0;
 }
 
 static int check_grid_ptr(const void* x, long ldg, const dsht_plan* p) {
@@ -535,9 +485,6 @@ int dsht_destroy(dsht_plan* p) {
   for (auto& kv : p->ws) {
     if (!kv.second) continue;
     if (kv.second->buf) (void)hipFree(kv.second->buf);
-    for (auto& e : kv.second->ev)
-      if (e) (void)hipEventDestroy(e);
-    if (kv.second->aux) (void)hipStreamDestroy(kv.second->aux);
   }
   double* ds[] = {p->d_mu, p->d_wq, p->d_wl, p->d_cwv, p->d_cwd, p->d_rs_icos, p->d_rs_alpha, p->d_rs_uv,
                   p->d_pro_psi, p->d_tabFP, p->d_tabFH, p->d_tabIP, p->d_tabIH};
@@ -731,20 +678,6 @@ int dsht_inv_vgrad_d(dsht_plan* p, const double* w, long sl, long sc, double* xy
       if (int rc = check_grid_ptr(o, ldg, p)) return rc;
   if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
   hipStream_t s = (hipStream_t)stream;
-  // the first split design has no fused contraction: it runs the three transforms one after
-  // another
-  const bool fused = p->prec == DSHT_PREC_FP64 || p->leg_kind != 1;
-  if (!fused) {
-    if (xy)
-      if (int rc = dispatch_inv(p, I_VAL, w, nullptr, sl, sc, xy, nullptr, ldg, ncols, diag, nullptr, s)) return rc;
-    if (xy_lon)
-      if (int rc = dispatch_inv(p, I_GRADLAMBDA, w, nullptr, sl, sc, xy_lon, nullptr, ldg, ncols, diag, p->d_rs_icos, s))
-        return rc;
-    if (xy_lat)
-      if (int rc = dispatch_inv(p, I_GRADMU, w, nullptr, sl, sc, xy_lat, nullptr, ldg, ncols, diag, p->d_rs_icos, s))
-        return rc;
-    return 0;
-  }
   return dispatch_inv(p, I_VGRAD, w, nullptr, sl, sc, xy, xy_lon, ldg, ncols, diag, p->d_rs_icos, s, xy_lat);
 }
 
@@ -754,7 +687,6 @@ static int check_f32(const dsht_plan* p) {
     return fail(DSHT_EUNSUPPORTED, "FP32 grid I/O runs the split precision forms (the plan is set to FP64)");
   if (p->fft_kind != 0 || !reg_fft_supported(p->fft.M))
     return fail(DSHT_EUNSUPPORTED, "FP32 grid I/O needs the register FFT (DSHT_TUNE_FFT 0, I/2 with a register kernel)");
-  if (p->leg_kind == 1) return fail(DSHT_EUNSUPPORTED, "FP32 grid I/O needs DSHT_TUNE_LEG != 1");
   return 0;
 }
 

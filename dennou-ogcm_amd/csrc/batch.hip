@@ -241,10 +241,6 @@ int dsht_set_tuning(dsht_plan* p, int key, int value) {
       if (value < 0 || value > 3) return fail(DSHT_EARG, "fft kernel must be 0, 1, 2 or 3");
       p->fft_kind = value;
       return 0;
-    case DSHT_TUNE_PIPELINE:
-      if (value != 0 && value != 1) return fail(DSHT_EARG, "pipeline must be 0 or 1");
-      p->pipeline = value;
-      return 0;
     case DSHT_TUNE_LEG:
       if (value < 0 || value > 3) return fail(DSHT_EARG, "contraction kernel must be 0, 1, 2 or 3");
       p->leg_kind = value;
@@ -264,7 +260,6 @@ int dsht_get_tuning(const dsht_plan* p, int key) {
     case DSHT_TUNE_LEG_CB: return p->leg_cb;
     case DSHT_TUNE_FFT: return p->fft_kind;
     case DSHT_TUNE_LEG: return p->leg_kind;
-    case DSHT_TUNE_PIPELINE: return p->pipeline;
     case DSHT_TUNE_FFT_ORDER: return p->fft_order;
   }
   return -1;

@@ -131,10 +131,6 @@ struct Workspace {
   void* buf = nullptr;
   size_t bytes = 0;
   std::mutex mu;
-  // chunk pipeline of multi-chunk calls: a helper stream for the FFT stage and the events
-  // that order the two stages over two scratch slots (created on first use)
-  hipStream_t aux = nullptr;
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 struct HostCtx;  // per-thread host-API context (api.hip)
@@ -144,17 +140,6 @@ struct WsHandle {
   void* buf = nullptr;
   Workspace* ws = nullptr;
   std::unique_lock<std::mutex> lk;
-};
-
-// Two-stage pipeline over the column chunks of one call.  Stage L (the Legendre
-// contraction) runs on the caller's stream, stage F (the FFT) on the workspace's helper
-// stream; chunk k uses scratch slot k%2, which chunk k+2 reuses only after both stages of
-// chunk k are done with it.  fork() makes the helper stream see the caller's earlier work,
-// join() makes the caller's stream see all of the helper's.
-struct ChunkPipe {
-  bool on = false;
-  hipStream_t s = nullptr, aux = nullptr;
-  hipEvent_t evL[2] = {nullptr, nullptr}, evF[2] = {nullptr, nullptr}, evX = nullptr;
 };
 
 // ---------------------------------------------------------------- plan
@@ -194,16 +179,14 @@ struct dsht_plan {
   void* mixed = nullptr;
   int chunk_cols = 0;  // column chunk of the device path (0 = auto)
   int leg_cb = 0;      // column block of the contractions (0 = auto, 16, 32)
-  int pipeline = 0;    // 1 = two-stream chunk pipeline (FFT stage on a helper stream)
   int fft_kind = 0;    // 0 = register two-pass FFT (256-thread blocks; FP32 arithmetic in the
                        // split forms' inverse FFT), 1 = LDS Stockham, 2 = register FFT in
                        // 128-thread blocks, 3 = register FFT with FP64 arithmetic in every form
   int fft_order = 1;   // register FFT row-group order: 0 packed, 1 latitude-major (default)
   int leg_kind = 0;    // contractions.  FP64: 0 = LDS-staged inverse, fragment-stream forward;
                        // 1/3 = LDS-staged; 2 = per-wave fragment streams.
-                       // Split forms: 0 = fragment streams over packed data panels, 1 = LDS-
-                       // staged, 2 = fragment streams splitting data in place, 3 = split data
-                       // shared through LDS
+                       // Split-form inverse: 0 = per form (whole-K LDS staging for FP16X3,
+                       // packed data panels otherwise), 2 = packed panels, 3 = whole-K staging
 };
 
 namespace dsht {
@@ -239,9 +222,6 @@ int fail(int code, const char* fmt, ...);
   } while (0)
 
 // api.hip: per-stream scratch
-int pipe_begin(WsHandle& wh, hipStream_t s, const dsht_plan* p, int nchunks, ChunkPipe& cp);
-int pipe_after(hipEvent_t e, hipStream_t from, hipStream_t to);  // record on from, to waits
-int pipe_join(ChunkPipe& cp);
 int workspace_acquire(dsht_plan* p, hipStream_t s, size_t bytes, WsHandle& out);
 // tables.hip
 int build_tables(dsht_plan* p);

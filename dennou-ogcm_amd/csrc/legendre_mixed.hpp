@@ -23,9 +23,6 @@ struct MxForm {
   using T = typename std::conditional<BF, __bf16, _Float16>::type;
   using V8 = typename std::conditional<BF, bf16x8_t, f16x8_t>::type;
 };
-// product list (table term, data term)
-__device__ constexpr int PA[6] = {0, 0, 1, 0, 1, 2};
-__device__ constexpr int PB[6] = {0, 1, 0, 2, 1, 0};
 
 __device__ __forceinline__ float4_t mfma16(bf16x8_t a, bf16x8_t b, float4_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -36,26 +33,6 @@ __device__ __forceinline__ float4_t mfma16(f16x8_t a, f16x8_t b, float4_t c) {
 // 16x16x32 16-bit MFMA lane layout: A[r][k] at lane r + 16*(k/8), element k%8; B[k][c] at lane
 // c + 16*(k/8), element k%8; result element i of lane l holds D[4*(l/16) + i][l%16].
 __device__ __forceinline__ int acc_row16(int lane, int i) { return 4 * (lane >> 4) + i; }
-
-// FP16 split terms are stored pre-scaled by 2^(11 t) so the low parts of small values stay
-// in FP16's normal range (the MFMA flushes FP16 subnormals); products of total order o are
-// weighted by 2^(-11 o).  BF16 has FP32's exponent range and needs no term scaling.
-template <int FORM>
-struct TermScale {
-  static constexpr float UP = MxForm<FORM>::BF ? 1.0f : 2048.0f;
-  static constexpr float DOWN = MxForm<FORM>::BF ? 1.0f : (1.0f / 2048.0f);
-};
-
-template <int FORM>
-__device__ __forceinline__ void split_to(float x, typename MxForm<FORM>::T* out) {
-  using T = typename MxForm<FORM>::T;
-#pragma unroll
-  for (int t = 0; t < MxForm<FORM>::NT; t++) {
-    T h = (T)x;
-    out[t] = h;
-    x = (x - (float)h) * TermScale<FORM>::UP;
-  }
-}
 
 // ---------------------------------------------------------------- tables
 struct MixedTables {
@@ -112,8 +89,8 @@ struct Mx2InvJob {
   const double* pro;        // per-l prefactor of the data (nullable)
   const double* rs;         // per-latitude output scale (nullable)
   const unsigned* colmax;   // [input][column] max |pro w| (float bits); scaled forms only
-  uint16_t* pack;           // packed data panels (workspace, mx2_inv_pack_bytes)
-  bool staged;              // split the data once per block into LDS instead of packing
+  uint16_t* pack;           // packed data panels (workspace, mx2_inv_pack_bytes); unused if staged
+  bool staged;              // whole-K staged inverse: split the data once per block into LDS
   float2* G0;
   float2* G1;
   float2* G2 = nullptr;     // third output (I_VGRAD)
@@ -134,10 +111,12 @@ struct Mx2FwdJob {
   const double* diag0;
   const double* diag1;
   double scale;
-  bool shared = true;       // split the data once per block through LDS (k_mx2_fwd_sh)
 };
 
 size_t mx2_inv_pack_bytes(const MixedTables& t, int form, int op, int ncols);
+// LDS bytes per block of the whole-K staged inverse (at most MX2_STAGED_LDS_MAX to run)
+constexpr size_t MX2_STAGED_LDS_MAX = 160 * 1024;
+size_t mx2_inv_staged_lds(const dsht_plan* p, int form, int op);
 int mx2_inv(const dsht_plan* p, const MixedTables& t, int form, int op, const Mx2InvJob& j, hipStream_t s);
 int mx2_fwd(const dsht_plan* p, const MixedTables& t, int form, int op, const Mx2FwdJob& j, hipStream_t s);
 int launch_spec_colmax(const double* w, long sl, long sc, int L, int ncols, const double* pro, unsigned* out,

@@ -1,15 +1,17 @@
-// legendre_mx2.hip -- split-precision Legendre contractions with per-wave fragment streams.
+// legendre_mx2.hip -- split-precision Legendre contractions on 16x16x32 MFMA.
 //
-// Same operators and split forms as legendre_mixed.hip (the Legendre half of SPMODEL
-// w_xy / xy_w and their derivative forms, doc/description/ogcm/space_discretization.tex:
-// 199-268).  Every wave owns its output tiles and streams both MFMA operands straight from
-// memory into registers, so there is no LDS staging and no barrier in the K loop:
-//  * table fragments are pre-split and pre-swizzled (one 16-byte load per lane per term);
-//  * inverse: the spectral data are first packed by k_mx2_ipack into split-term panels in
-//    MFMA B-fragment order (prefactor, column scale and FP16 term scaling applied there),
-//    so the contraction loads them like the tables;
-//  * forward: each wave reads its Fourier fragment (north and south rows) and forms the
-//    symmetric or antisymmetric combination its row parity needs, then splits it.
+// The Legendre half of SPMODEL w_xy / xy_w and their derivative forms
+// (doc/description/ogcm/space_discretization.tex:199-268) in the split forms of
+// legendre_mixed.hip.  Table fragments are pre-split and pre-swizzled (one 16-byte load per
+// lane per term).  Two inverse kernels:
+//  * packed: k_mx2_ipack first packs the spectra into split-term panels in MFMA B-fragment
+//    order (prefactor, column scale and FP16 term scaling applied there); every wave then
+//    streams table and panel fragments from memory, with no LDS and no barrier in its K loop;
+//  * whole-K staged: a block covers one wavenumber, a column group and every latitude tile;
+//    it gathers and splits its spectra once into LDS behind one barrier, and its waves read
+//    the data fragments from there (no round trip of packed panels through memory).
+// Forward: each block stages the symmetric and antisymmetric latitude combinations of its
+// columns, split, through a double-buffered LDS panel, one K step at a time.
 // Products of one output accumulate in one MFMA chain per (table, input) group; the i*m
 // factors, signs, north/south recombination and unscaling happen once in the epilogue.
 // Blocks are placed so that all blocks of one zonal wavenumber run on one XCD (shared
@@ -267,14 +269,15 @@ __device__ __forceinline__ void inv_epilogue(const Mx2InvArgs& a,
 // block = 4 waves of one wavenumber m and one group of NCT column tiles; wave w owns the
 // MAXT latitude tiles starting at (rg*4 + w)*MAXT.  K runs over the parity-0 chunks, then
 // the parity-1 chunks; P of parity p accumulates into sym (p = 0) or anti (p = 1), H the
-// opposite; north = sym + anti, south = sym - anti.
-template <int OP, int FORM, int NCT, int MAXT, bool PACKED>
-__global__ __launch_bounds__(256) void k_mx2_inv(Mx2InvArgs a) {
+// opposite; north = sym + anti, south = sym - anti.  Table and packed data fragments stream
+// from memory (one 16-byte load per lane and term); a tile or column tile past the last one
+// repeats the last (in bounds; its results are not stored).
+template <int OP, int FORM, int NCT, int MAXT>
+__global__ __launch_bounds__(256, 2) void k_mx2_inv(Mx2InvArgs a) {
   using F = MxForm<FORM>;
   using X = Mx2<FORM>;
-  using T = typename F::T;
   using V8 = typename F::V8;
-  constexpr int NG = inv_ng(OP), NIN = InvTraits<OP>::NIN, NOUT = InvTraits<OP>::NOUT;
+  constexpr int NG = inv_ng(OP), NIN = InvTraits<OP>::NIN;
   constexpr int NTA = X::NTA, NTB = X::NTB;
   constexpr bool UP = InvTraits<OP>::USE_P, UH = InvTraits<OP>::USE_H;
   int m, sub;
@@ -289,15 +292,6 @@ __global__ __launch_bounds__(256) void k_mx2_inv(Mx2InvArgs a) {
   const long toff = a.off[m];
   const int gc0 = a.cbase[m];
   const int ct0 = cg * NCT;
-  float dsc[NCT];
-#pragma unroll
-  for (int ct = 0; ct < NCT; ct++) {
-    const int c = (ct0 + ct) * TILE + (lane & 15);
-    dsc[ct] = 1.f;
-    if constexpr (F::SCALED && !PACKED) {
-      if (c < a.ncols) dsc[ct] = colscale<NIN>(a.colmax, a.ncols, c, 1.f);
-    }
-  }
 
   float4_t acc[NG][MAXT][NCT][2][2];
 #pragma unroll
@@ -317,69 +311,27 @@ __global__ __launch_bounds__(256) void k_mx2_inv(Mx2InvArgs a) {
       V8 A[2][MAXT][NTA];
 #pragma unroll
       for (int tt = 0; tt < MAXT; tt++) {
-        const int t = t0 + tt;
+        const int t = t0 + tt < a.RT ? t0 + tt : a.RT - 1;
         const long o = pbase + ((long)t * nch + ks) * 512 + lane * 8;
 #pragma unroll
         for (int k = 0; k < NTA; k++) {
-          if constexpr (UP) A[0][tt][k] = t < a.RT ? *reinterpret_cast<const V8*>(a.tP[k] + o) : V8{};
-          if constexpr (UH) A[1][tt][k] = t < a.RT ? *reinterpret_cast<const V8*>(a.tH[k] + o) : V8{};
+          if constexpr (UP) A[0][tt][k] = *reinterpret_cast<const V8*>(a.tP[k] + o);
+          if constexpr (UH) A[1][tt][k] = *reinterpret_cast<const V8*>(a.tH[k] + o);
         }
       }
       V8 B[NIN][NCT][2][NTB];
-      if constexpr (PACKED) {
-        const long g = gc0 + (PP ? nch0 : 0) + ks;
+      const long g = gc0 + (PP ? nch0 : 0) + ks;
 #pragma unroll
-        for (int i = 0; i < NIN; i++)
+      for (int i = 0; i < NIN; i++)
 #pragma unroll
-          for (int ct = 0; ct < NCT; ct++) {
-            const int ctg = ct0 + ct;
-            const uint16_t* src = a.pack + (((g * a.CT + ctg) * NIN + i) * 2 * NTB) * 512 + lane * 8;
+        for (int ct = 0; ct < NCT; ct++) {
+          const int ctg = ct0 + ct < a.CT ? ct0 + ct : a.CT - 1;
+          const uint16_t* src = a.pack + (((g * a.CT + ctg) * NIN + i) * 2 * NTB) * 512 + lane * 8;
 #pragma unroll
-            for (int r = 0; r < 2; r++)
+          for (int r = 0; r < 2; r++)
 #pragma unroll
-              for (int t = 0; t < NTB; t++)
-                B[i][ct][r][t] = ctg < a.CT ? *reinterpret_cast<const V8*>(src + (long)(r * NTB + t) * 512) : V8{};
-          }
-      } else {
-        // split the FP64 coefficients in place: lane (c, k-group) reads 8 total wavenumbers
-        const int np = PP ? nO : nE;
-#pragma unroll
-        for (int i = 0; i < NIN; i++) {
-          const double* W = i ? a.W1 : a.W0;
-#pragma unroll
-          for (int ct = 0; ct < NCT; ct++) {
-            const int c = (ct0 + ct) * TILE + (lane & 15);
-            T tre[NTB][8], tim[NTB][8];
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-              const int k = ks * KM + 8 * (lane >> 4) + e;
-              double vre = 0.0, vim = 0.0;
-              if (k < np && c < a.ncols) {
-                const int n = m + PP + 2 * k;
-                const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
-                const double pr = a.pro ? a.pro[lre] : 1.0;
-                vre = pr * W[lre * a.sl + (long)c * a.sc];
-                if (m > 0) vim = pr * W[lim * a.sl + (long)c * a.sc];
-              }
-              T t1[NTB], t2[NTB];
-              bsplit<FORM>((float)vre * dsc[ct], t1);
-              bsplit<FORM>((float)vim * dsc[ct], t2);
-#pragma unroll
-              for (int t = 0; t < NTB; t++) {
-                tre[t][e] = t1[t];
-                tim[t][e] = t2[t];
-              }
-            }
-#pragma unroll
-            for (int t = 0; t < NTB; t++)
-#pragma unroll
-              for (int e = 0; e < 8; e++) {
-                B[i][ct][0][t][e] = tre[t][e];
-                B[i][ct][1][t][e] = tim[t][e];
-              }
-          }
+            for (int t = 0; t < NTB; t++) B[i][ct][r][t] = *reinterpret_cast<const V8*>(src + (long)(r * NTB + t) * 512);
         }
-      }
 #pragma unroll
       for (int gi = 0; gi < NG; gi++) {
         const int tb = inv_tab(OP, gi), in = inv_in(OP, gi);
@@ -403,13 +355,23 @@ __global__ __launch_bounds__(256) void k_mx2_inv(Mx2InvArgs a) {
   inv_epilogue<OP, FORM, NCT, MAXT>(a, acc, m, t0, ct0, lane);
 }
 
-// Inverse with the split data shared through LDS: per K chunk the block's threads gather
-// the FP64 coefficients of its NCT column tiles once, split them and store B fragments to a
-// double-buffered LDS panel (one barrier per chunk); each wave then reads its fragments
-// with one 16-byte LDS read per (input, column tile, re/im, term) and streams its table
-// fragments from memory one chunk ahead.  Same tiling and epilogue as k_mx2_inv.
+// Inverse with the whole K range of the block's spectra staged once in LDS.  A block covers
+// one wavenumber m, NCT column tiles and every latitude tile (wave w owns MAXT of them), so
+// each FP64 coefficient is read from memory once in the whole launch: the block's threads
+// gather the coefficients of all its K chunks (consecutive threads on consecutive columns),
+// split them into their terms and store them as B fragments in LDS behind one barrier; the
+// waves then run their K loops (parity 0 chunks, then parity 1) with table fragments from
+// memory and data fragments from LDS, and no further barrier.  Same accumulators and
+// epilogue as k_mx2_inv.
+extern __shared__ __attribute__((aligned(16))) unsigned char mx2_lds[];
+
+template <int OP, int FORM, int NCT>
+__host__ __device__ constexpr int wk_chunk_elems() {
+  return InvTraits<OP>::NIN * NCT * 2 * Mx2<FORM>::NTB * 512;
+}
+
 template <int OP, int FORM, int NCT, int MAXT>
-__global__ __launch_bounds__(256) void k_mx2_inv_lds(Mx2InvArgs a) {
+__global__ __launch_bounds__(256, 2) void k_mx2_inv_wk(Mx2InvArgs a) {
   using F = MxForm<FORM>;
   using X = Mx2<FORM>;
   using T = typename F::T;
@@ -417,10 +379,10 @@ __global__ __launch_bounds__(256) void k_mx2_inv_lds(Mx2InvArgs a) {
   constexpr int NG = inv_ng(OP), NIN = InvTraits<OP>::NIN;
   constexpr int NTA = X::NTA, NTB = X::NTB;
   constexpr bool UP = InvTraits<OP>::USE_P, UH = InvTraits<OP>::USE_H;
-  constexpr int PANEL = NIN * NCT * 2 * NTB * 512;  // elements of one chunk's B panel
-  constexpr int ITEMS = NIN * NCT * 2 * 64;         // (input, tile, re/im, lane) staging items
-  constexpr int IPT = (ITEMS + 255) / 256;          // items per thread
-  __shared__ __attribute__((aligned(16))) T sB[2][PANEL];
+  constexpr int CB = NCT * TILE;             // columns of the block
+  constexpr int IPC = NIN * CB * 4 * 2;      // staging items per chunk: (input, column, k-group, re/im)
+  constexpr int CHUNK = wk_chunk_elems<OP, FORM, NCT>();
+  T* sB = reinterpret_cast<T*>(mx2_lds);
   int m, sub;
   if (!xcd_decode(a.N, a.CG * a.RG, m, sub)) return;  // uniform over the block
   const int cg = sub % a.CG, rg = sub / a.CG;
@@ -432,65 +394,68 @@ __global__ __launch_bounds__(256) void k_mx2_inv_lds(Mx2InvArgs a) {
   const long toff = a.off[m];
   const int ct0 = cg * NCT;
 
-  // staging: item -> (input i, tile ct, re/im r, fragment lane fl); 8 coefficients each
-  double raw[IPT][8];
-  float dsc[IPT];
-#pragma unroll
-  for (int q = 0; q < IPT; q++) {
-    const int it = tid + 256 * q;
-    const int fl = it & 63, ct = (it >> 6) % NCT, i = (it >> 6) / NCT / 2;
-    const int c = (ct0 + ct) * TILE + (fl & 15);
-    dsc[q] = 1.f;
+  // ---- staging: item it -> chunk it / IPC; within the chunk (column cc, k-group kg, re/im r,
+  // input i) with cc fastest, so a wave's loads of one coefficient row are contiguous
+  {
+    const int cc = tid % CB;  // IPC and 256 are multiples of CB: a thread keeps its column
+    const int c = ct0 * TILE + cc;
+    const bool cok = c < a.ncols;
+    float dsc = 1.f;
     if constexpr (F::SCALED) {
-      if (it < ITEMS && c < a.ncols) dsc[q] = colscale<NIN>(a.colmax, a.ncols, c, 1.f);
+      if (cok) dsc = colscale<NIN>(a.colmax, a.ncols, c, 1.f);
     }
-    (void)i;
-  }
-  auto gather = [&](int ch) {
-    const int p = ch >= nch0 ? 1 : 0, ks = p ? ch - nch0 : ch, np = p ? nO : nE;
+    const long coff = (long)c * a.sc;
+    const int total = nch * IPC;
+    constexpr int U = 2;  // items per thread in flight (deeper prefetch measured no faster)
+    for (int base = tid; base < total; base += 256 * U) {
+      double v[U][8];
 #pragma unroll
-    for (int q = 0; q < IPT; q++) {
-      const int it = tid + 256 * q;
-      const int fl = it & 63, ct = (it >> 6) % NCT, r = ((it >> 6) / NCT) & 1, i = (it >> 6) / NCT / 2;
-      const int c = (ct0 + ct) * TILE + (fl & 15);
-      const double* W = i ? a.W1 : a.W0;
+      for (int u = 0; u < U; u++) {
+        const int it = base + 256 * u;
+        const int ch = it / IPC, rest = (it % IPC) / CB;
+        const int kg = rest & 3, r = (rest >> 2) & 1, i = rest >> 3;
+        const int p = ch >= nch0 ? 1 : 0, ks = p ? ch - nch0 : ch, np = p ? nO : nE;
+        const double* W = i ? a.W1 : a.W0;
 #pragma unroll
-      for (int e = 0; e < 8; e++) {
-        const int k = ks * KM + 8 * (fl >> 4) + e;
-        raw[q][e] = 0.0;
-        if (it < ITEMS && k < np && c < a.ncols && (r == 0 || m > 0)) {
-          const int n = m + p + 2 * k;
-          const long l = (long)n * (n + 1) + (r ? -m : m);
-          raw[q][e] = (a.pro ? a.pro[(long)n * (n + 1) + m] : 1.0) * W[l * a.sl + (long)c * a.sc];
+        for (int e = 0; e < 8; e++) {
+          const int k = ks * KM + 8 * kg + e;
+          v[u][e] = 0.0;
+          if (it < total && cok && k < np && (r == 0 || m > 0)) {
+            const int n = m + p + 2 * k;
+            const long l0 = (long)n * (n + 1);
+            const double pr = a.pro ? a.pro[l0 + m] : 1.0;
+            v[u][e] = pr * __builtin_nontemporal_load(W + (l0 + (r ? -m : m)) * a.sl + coff);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int it = base + 256 * u;
+        if (it >= total) continue;
+        const int ch = it / IPC, rest = (it % IPC) / CB;
+        const int kg = rest & 3, r = (rest >> 2) & 1, i = rest >> 3;
+        T tt[NTB][8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          T tmp[NTB];
+          bsplit<FORM>((float)v[u][e] * dsc, tmp);
+#pragma unroll
+          for (int t = 0; t < NTB; t++) tt[t][e] = tmp[t];
+        }
+        const int fl = (cc % TILE) + TILE * kg;
+        T* dst = sB + (long)ch * CHUNK + ((((i * NCT + cc / TILE) * 2 + r) * NTB) * 512) + fl * 8;
+#pragma unroll
+        for (int t = 0; t < NTB; t++) {
+          V8 vv;
+#pragma unroll
+          for (int e = 0; e < 8; e++) vv[e] = tt[t][e];
+          *reinterpret_cast<V8*>(dst + t * 512) = vv;
         }
       }
     }
-  };
-  auto stage = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < IPT; q++) {
-      const int it = tid + 256 * q;
-      if (it >= ITEMS) continue;
-      const int fl = it & 63, rest = it >> 6;  // rest = (i * 2 + r) * NCT + ct
-      T tt[NTB][8];
-#pragma unroll
-      for (int e = 0; e < 8; e++) {
-        T tmp[NTB];
-        bsplit<FORM>((float)raw[q][e] * dsc[q], tmp);
-#pragma unroll
-        for (int t = 0; t < NTB; t++) tt[t][e] = tmp[t];
-      }
-      const int ct = rest % NCT, r = (rest / NCT) & 1, i = rest / NCT / 2;
-      T* dst = sB[buf] + ((((i * NCT + ct) * 2 + r) * NTB) * 512) + fl * 8;
-#pragma unroll
-      for (int t = 0; t < NTB; t++) {
-        V8 v;
-#pragma unroll
-        for (int e = 0; e < 8; e++) v[e] = tt[t][e];
-        *reinterpret_cast<V8*>(dst + t * 512) = v;
-      }
-    }
-  };
+  }
+  __syncthreads();
+  if (t0 >= a.RT) return;  // no barrier follows
 
   float4_t acc[NG][MAXT][NCT][2][2];
 #pragma unroll
@@ -502,73 +467,49 @@ __global__ __launch_bounds__(256) void k_mx2_inv_lds(Mx2InvArgs a) {
 #pragma unroll
         for (int r = 0; r < 2; r++) acc[g][tt][ct][r][0] = acc[g][tt][ct][r][1] = float4_t{0, 0, 0, 0};
 
-  V8 A[2][2][MAXT][NTA];  // [slot][P/H][tile][term]
-  auto load_a = [&](int ch, int slot) {
-    const int p = ch >= nch0 ? 1 : 0, ks = p ? ch - nch0 : ch, nchp = p ? nch1 : nch0;
-    const long pbase = toff + (p ? (long)a.J2p32 * kq0 : 0);
-#pragma unroll
-    for (int tt = 0; tt < MAXT; tt++) {
-      const int t = t0 + tt;
-      const long o = pbase + ((long)t * nchp + ks) * 512 + lane * 8;
-#pragma unroll
-      for (int k = 0; k < NTA; k++) {
-        if constexpr (UP) A[slot][0][tt][k] = t < a.RT ? *reinterpret_cast<const V8*>(a.tP[k] + o) : V8{};
-        if constexpr (UH) A[slot][1][tt][k] = t < a.RT ? *reinterpret_cast<const V8*>(a.tH[k] + o) : V8{};
-      }
-    }
-  };
-  auto compute = [&](auto PAR, int slot, int buf) {
+  auto run = [&](auto PAR) {
     constexpr int PP = decltype(PAR)::value;
-    if (t0 >= a.RT) return;
-    V8 B[NIN][NCT][2][NTB];
+    const int nchp = PP ? nch1 : nch0;
+    const long pbase = toff + (PP ? (long)a.J2p32 * kq0 : 0);
+    for (int ks = 0; ks < nchp; ks++) {
+      V8 A[2][MAXT][NTA];
 #pragma unroll
-    for (int i = 0; i < NIN; i++)
+      for (int tt = 0; tt < MAXT; tt++) {
+        const int t = t0 + tt;
+        const long o = pbase + ((long)t * nchp + ks) * 512 + lane * 8;
+#pragma unroll
+        for (int k = 0; k < NTA; k++) {
+          if constexpr (UP) A[0][tt][k] = t < a.RT ? *reinterpret_cast<const V8*>(a.tP[k] + o) : V8{};
+          if constexpr (UH) A[1][tt][k] = t < a.RT ? *reinterpret_cast<const V8*>(a.tH[k] + o) : V8{};
+        }
+      }
+      const T* panel = sB + (long)(PP ? nch0 + ks : ks) * CHUNK + lane * 8;
 #pragma unroll
       for (int ct = 0; ct < NCT; ct++)
 #pragma unroll
-        for (int r = 0; r < 2; r++)
+        for (int r = 0; r < 2; r++) {
+          V8 B[NIN][NTB];
 #pragma unroll
-          for (int t = 0; t < NTB; t++)
-            B[i][ct][r][t] =
-                *reinterpret_cast<const V8*>(sB[buf] + ((((i * NCT + ct) * 2 + r) * NTB + t) * 512) + lane * 8);
+          for (int i = 0; i < NIN; i++)
 #pragma unroll
-    for (int gi = 0; gi < NG; gi++) {
-      const int tb = inv_tab(OP, gi), in = inv_in(OP, gi);
-      const int sidx = tb == 0 ? PP : 1 - PP;
+            for (int t = 0; t < NTB; t++)
+              B[i][t] = *reinterpret_cast<const V8*>(panel + (((i * NCT + ct) * 2 + r) * NTB + t) * 512);
 #pragma unroll
-      for (int tt = 0; tt < MAXT; tt++)
+          for (int gi = 0; gi < NG; gi++) {
+            const int tb = inv_tab(OP, gi), in = inv_in(OP, gi);
+            const int sidx = tb == 0 ? PP : 1 - PP;
 #pragma unroll
-        for (int ct = 0; ct < NCT; ct++)
+            for (int tt = 0; tt < MAXT; tt++)
 #pragma unroll
-          for (int r = 0; r < 2; r++)
-#pragma unroll
-            for (int pr = 0; pr < X::NP; pr++)
-              acc[gi][tt][ct][r][sidx] =
-                  mfma16(A[slot][tb][tt][X::pa(pr)], B[in][ct][r][X::pb(pr)], acc[gi][tt][ct][r][sidx]);
+              for (int pr = 0; pr < X::NP; pr++)
+                acc[gi][tt][ct][r][sidx] = mfma16(A[tb][tt][X::pa(pr)], B[in][X::pb(pr)], acc[gi][tt][ct][r][sidx]);
+          }
+        }
     }
   };
-  auto step = [&](int ch, int slot) {
-    if (ch + 1 < nch) {
-      gather(ch + 1);
-      load_a(ch + 1, slot ^ 1);
-    }
-    if (ch < nch0)
-      compute(std::integral_constant<int, 0>(), slot, ch & 1);
-    else
-      compute(std::integral_constant<int, 1>(), slot, ch & 1);
-    if (ch + 1 < nch) stage((ch + 1) & 1);
-    __syncthreads();
-  };
-
-  gather(0);
-  load_a(0, 0);
-  stage(0);
-  __syncthreads();
-  for (int ch = 0; ch < nch; ch += 2) {
-    step(ch, 0);
-    if (ch + 1 < nch) step(ch + 1, 1);
-  }
-  if (t0 < a.RT) inv_epilogue<OP, FORM, NCT, MAXT>(a, acc, m, t0, ct0, lane);
+  run(std::integral_constant<int, 0>());
+  run(std::integral_constant<int, 1>());
+  inv_epilogue<OP, FORM, NCT, MAXT>(a, acc, m, t0, ct0, lane);
 }
 
 // ---------------------------------------------------------------- forward contraction
@@ -592,161 +533,6 @@ struct Mx2FwdArgs {
   const double* minv;
   int N, J2, J2p32;
 };
-
-// block = 4 waves of one wavenumber m and one column tile; the row tiles of each parity
-// are cut into slots of MAXT tiles and the waves take slots in turn.  Rows of parity 0 pair
-// the P table with the symmetric combination G_N + G_S and H with G_N - G_S; parity 1 the
-// other way round.
-template <int OP, int FORM, int MAXT>
-__global__ __launch_bounds__(256) void k_mx2_fwd(Mx2FwdArgs a) {
-  using F = MxForm<FORM>;
-  using X = Mx2<FORM>;
-  using T = typename F::T;
-  using V8 = typename F::V8;
-  constexpr int NG = fwd_ng(OP), NIN = FwdTraits<OP>::NIN, NOUT = FwdTraits<OP>::NOUT;
-  constexpr int NTA = X::NTA, NTB = X::NTB;
-  constexpr bool UP = FwdTraits<OP>::USE_P, UH = FwdTraits<OP>::USE_H;
-  int m, ct;
-  if (!xcd_decode(a.N, a.CT, m, ct)) return;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int nE = n_par(a.N, m, 0), nO = n_par(a.N, m, 1);
-  const int tE = cdiv(nE, TILE), tO = cdiv(nO, TILE);
-  const int sE = cdiv(tE, MAXT), sO = cdiv(tO, MAXT);
-  const int KS = a.J2p32 / KM;
-  const long toff = a.off[m];
-  const int c = ct * TILE + (lane & 15);
-  const bool cok = c < a.ncols;
-  float dsc = 1.f;
-  if constexpr (F::SCALED) {
-    if (cok) dsc = colscale<NIN>(a.colmax, a.ncols, c, a.bound);
-  }
-  double u = a.minv[m] * X::DSCALE * a.scale;
-  if constexpr (F::SCALED) u /= (double)dsc;
-  const float2* Gm[2] = {a.G0 + (long)m * a.gmst, NIN > 1 ? a.G1 + (long)m * a.gmst : nullptr};
-  const double dm = (double)m;
-
-  auto slot = [&](auto PAR, int tp0) {
-    constexpr int PP = decltype(PAR)::value;
-    const int tpar = PP ? tO : tE;
-    float4_t acc[NG][MAXT][2];
-#pragma unroll
-    for (int g = 0; g < NG; g++)
-#pragma unroll
-      for (int tt = 0; tt < MAXT; tt++) acc[g][tt][0] = acc[g][tt][1] = float4_t{0, 0, 0, 0};
-    for (int ks = 0; ks < KS; ks++) {
-      V8 A[2][MAXT][NTA];
-#pragma unroll
-      for (int tt = 0; tt < MAXT; tt++) {
-        const int tp = tp0 + tt;
-        const bool ok = tp < tpar;
-        const long o = toff + ((long)(PP ? tE + tp : tp) * KS + ks) * 512 + lane * 8;
-#pragma unroll
-        for (int k = 0; k < NTA; k++) {
-          if constexpr (UP) A[0][tt][k] = ok ? *reinterpret_cast<const V8*>(a.tP[k] + o) : V8{};
-          if constexpr (UH) A[1][tt][k] = ok ? *reinterpret_cast<const V8*>(a.tH[k] + o) : V8{};
-        }
-      }
-      // data fragments: [input][q: 0 = combination for P, 1 = for H][re, im][term]
-      V8 B[NIN][2][2][NTB];
-#pragma unroll
-      for (int i = 0; i < NIN; i++) {
-        T part[2][2][NTB][8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-          const int jn = ks * KM + 8 * (lane >> 4) + e;
-          float2 gn = make_float2(0.f, 0.f), gs = make_float2(0.f, 0.f);
-          if (jn < a.J2 && cok) {
-            gn = Gm[i][(long)(a.J2 + jn) * a.gld + c];
-            gs = Gm[i][(long)(a.J2 - 1 - jn) * a.gld + c];
-          }
-          const float cw = a.cw[jn] * dsc;
-          const float symr = cw * (gn.x + gs.x), symi = cw * (gn.y + gs.y);
-          const float antr = cw * (gn.x - gs.x), anti = cw * (gn.y - gs.y);
-          // P pairs with sym at parity 0, H with anti; swapped at parity 1
-          const float vq[2][2] = {{PP ? antr : symr, PP ? anti : symi}, {PP ? symr : antr, PP ? symi : anti}};
-#pragma unroll
-          for (int q = 0; q < 2; q++)
-#pragma unroll
-            for (int r = 0; r < 2; r++) {
-              T tmp[NTB];
-              bsplit<FORM>(vq[q][r], tmp);
-#pragma unroll
-              for (int t = 0; t < NTB; t++) part[q][r][t][e] = tmp[t];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 2; q++)
-#pragma unroll
-          for (int r = 0; r < 2; r++)
-#pragma unroll
-            for (int t = 0; t < NTB; t++) {
-              V8 v;
-#pragma unroll
-              for (int e = 0; e < 8; e++) v[e] = part[q][r][t][e];
-              B[i][q][r][t] = v;
-            }
-      }
-#pragma unroll
-      for (int gi = 0; gi < NG; gi++) {
-        const int tb = fwd_tab(OP, gi), in = fwd_in(OP, gi);
-#pragma unroll
-        for (int tt = 0; tt < MAXT; tt++)
-#pragma unroll
-          for (int r = 0; r < 2; r++)
-#pragma unroll
-            for (int pr = 0; pr < X::NP; pr++)
-              acc[gi][tt][r] = mfma16(A[tb][tt][X::pa(pr)], B[in][tb][r][X::pb(pr)], acc[gi][tt][r]);
-      }
-    }
-    // ---- epilogue (FP64): combine, unscale, diagonal, scatter to SPMODEL order
-    if (!cok) return;
-#pragma unroll
-    for (int tt = 0; tt < MAXT; tt++) {
-      const int tp = tp0 + tt;
-      if (tp >= tpar) continue;
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int k = tp * TILE + acc_row16(lane, i);
-        const int n = m + PP + 2 * k;
-        if (n > a.N) continue;
-        double xr[NG], xi[NG];
-#pragma unroll
-        for (int gi = 0; gi < NG; gi++) {
-          xr[gi] = acc[gi][tt][0][i];
-          xi[gi] = acc[gi][tt][1][i];
-        }
-        double orr[NOUT], oii[NOUT];
-        if constexpr (OP == F_VAL) {
-          orr[0] = xr[0], oii[0] = xi[0];
-        } else if constexpr (OP == F_DIVLAMBDA) {  // i m P.G
-          orr[0] = -dm * xi[0], oii[0] = dm * xr[0];
-        } else if constexpr (OP == F_DIVMU) {  // -H.G
-          orr[0] = -xr[0], oii[0] = -xi[0];
-        } else if constexpr (OP == F_ALPHA) {  // i m P.A - H.B
-          orr[0] = -dm * xi[0] - xr[1], oii[0] = dm * xr[0] - xi[1];
-        } else {  // vor = i m P.V + H.U ; div = i m P.U - H.V
-          orr[0] = -dm * xi[0] + xr[1], oii[0] = dm * xr[0] + xi[1];
-          orr[1] = -dm * xi[2] - xr[3], oii[1] = dm * xr[2] - xi[3];
-        }
-        const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
-#pragma unroll
-        for (int o = 0; o < NOUT; o++) {
-          const double* dg = o == 0 ? a.diag0 : a.diag1;
-          double* W = o == 0 ? a.W0 : a.W1;
-          W[lre * a.sl + (long)c * a.sc] = orr[o] * u * (dg ? dg[lre] : 1.0);
-          if (m > 0) W[lim * a.sl + (long)c * a.sc] = oii[o] * u * (dg ? dg[lim] : 1.0);
-        }
-      }
-    }
-  };
-
-  for (int s = w; s < sE + sO; s += 4) {
-    if (s < sE)
-      slot(std::integral_constant<int, 0>(), s * MAXT);
-    else
-      slot(std::integral_constant<int, 1>(), (s - sE) * MAXT);
-  }
-}
 
 // Forward with the split data shared through LDS.  The per-wave kernel above forms and
 // splits the same data fragments once per slot and wave; here the block's 256 threads form
@@ -816,7 +602,7 @@ __global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
         }
     }
   };
-  auto stage = [&](int buf) {
+  auto stage = [&](int buf, int) {
 #pragma unroll
     for (int t = 0; t < NCT; t++)
 #pragma unroll
@@ -846,6 +632,7 @@ __global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
       }
   };
 
+
   const int nslot = sE + sO;
   for (int rbase = 0; rbase < nslot; rbase += 4 * NSW) {
     float4_t acc[NSW][NG][MAXT][NCT][2];
@@ -858,7 +645,7 @@ __global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
 #pragma unroll
           for (int t = 0; t < NCT; t++) acc[q][g][tt][t][0] = acc[q][g][tt][t][1] = float4_t{0, 0, 0, 0};
     fetch(0);
-    stage(0);
+    stage(0, 0);
     __syncthreads();
     for (int ks = 0; ks < KS; ks++) {
       const int buf = ks & 1;
@@ -902,7 +689,7 @@ __global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
             }
         }
       }
-      if (ks + 1 < KS) stage(buf ^ 1);
+      if (ks + 1 < KS) stage(buf ^ 1, ks + 1);
       __syncthreads();  // buf ^ 1 is complete; buf is restaged only after this barrier
     }
     // ---- epilogue (FP64), per slot and column tile: combine, unscale, diagonal, scatter
@@ -1000,10 +787,21 @@ __global__ __launch_bounds__(256) void k_spec_colmax2(const double* __restrict__
 // tile shapes per operator: one-group operators take 2x2 tiles per wave, the two- and
 // four-group ones fewer so that their accumulators stay within the register budget
 // (NCT: column tiles per wave; the plan's column-block knob selects 1 tile for 16, else 2)
+// LDS of the whole-K staged inverse: every K chunk of wavenumber 0 (the most) for the
+// block's column tiles
+template <int OP, int FORM>
+size_t staged_lds(const dsht_plan* p) {
+  constexpr int WNCT = (InvTraits<OP>::NIN > 1 || inv_ng(OP) == 4) ? 1 : 2;
+  const int nchmax = (kpad32(n_par(p->N, 0, 0)) + kpad32(n_par(p->N, 0, 1))) / KM;
+  return (size_t)nchmax * wk_chunk_elems<OP, FORM, WNCT>() * sizeof(typename MxForm<FORM>::T);
+}
+
 template <int OP, int NCT_>
 struct InvShape {
+  // packed inverse: two latitude tiles per wave (one row group covers T170's 8 tiles) where
+  // the accumulators allow
   static constexpr int NCT = inv_ng(OP) == 4 ? 1 : NCT_;
-  static constexpr int MAXT = inv_ng(OP) == 1 ? 2 : 1;
+  static constexpr int MAXT = (inv_ng(OP) == 1 || (inv_ng(OP) == 2 && InvTraits<OP>::NIN == 1)) ? 2 : 1;
 };
 template <int OP>
 struct FwdShape {
@@ -1049,14 +847,25 @@ int inv_launch(const dsht_plan* p, const MixedTables& t, const Mx2InvJob& j, hip
   a.RG = cdiv(a.RT, 4 * MAXT);
   const long nb = 8l * cdiv(p->N + 1, 8) * a.CG * a.RG;
   if (j.staged) {
-    hipLaunchKernelGGL((k_mx2_inv_lds<OP, FORM, NCT, MAXT>), dim3((unsigned)nb), dim3(256), 0, s, a);
-  } else if (j.pack) {
+    // whole-K staging: every latitude tile in one block where 4 waves x 2 tiles cover them
+    constexpr int WNCT = (InvTraits<OP>::NIN > 1 || inv_ng(OP) == 4) ? 1 : 2;
+    const int wmaxt = a.RT <= 4 ? 1 : 2;
+    const size_t lds = staged_lds<OP, FORM>(p);
+    a.CG = cdiv(a.CT, WNCT);
+    a.RG = cdiv(a.RT, 4 * wmaxt);
+    const long nbw = 8l * cdiv(p->N + 1, 8) * a.CG * a.RG;
+    auto k = wmaxt == 1 ? k_mx2_inv_wk<OP, FORM, WNCT, 1> : k_mx2_inv_wk<OP, FORM, WNCT, 2>;
+    if (lds > 64 * 1024) {  // above the default limit; once per kernel and process
+      static const hipError_t attr =
+          hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)MX2_STAGED_LDS_MAX);
+      if (attr != hipSuccess) return fail(DSHT_EDEVICE, "LDS size attribute: %s", hipGetErrorString(attr));
+    }
+    hipLaunchKernelGGL(k, dim3((unsigned)nbw), dim3(256), lds, s, a);
+  } else {
     const long pt = 4l * a.CT * TILE * t.nchunks * NIN;  // packing threads
     hipLaunchKernelGGL((k_mx2_ipack<FORM, NIN>), dim3((unsigned)cdivl(pt, 256)), dim3(256), 0, s, a);
     DSHT_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_mx2_inv<OP, FORM, NCT, MAXT, true>), dim3((unsigned)nb), dim3(256), 0, s, a);
-  } else {
-    hipLaunchKernelGGL((k_mx2_inv<OP, FORM, NCT, MAXT, false>), dim3((unsigned)nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_mx2_inv<OP, FORM, NCT, MAXT>), dim3((unsigned)nb), dim3(256), 0, s, a);
   }
   DSHT_HIP(hipGetLastError());
   return 0;
@@ -1090,23 +899,18 @@ int fwd_launch(const dsht_plan* p, const MixedTables& t, const Mx2FwdJob& j, hip
   a.N = p->N;
   a.J2 = p->J2;
   a.J2p32 = t.J2p32;
-  // LDS-shared form: NCT column tiles per block.  Two tiles halve the table fetches per column
-  // but double the accumulators; measured at T170 they pay only for the single-group bf16x6
-  // form (6 products per data fragment: forward 101 -> 84 us).  The column-block knob forces
-  // 1 (16) or 2 (32) tiles.
-  const int fnct = !j.shared ? 1
-                             : (p->leg_cb == 16 ? 1 : (p->leg_cb == 32 ? 2 : (fwd_ng(OP) == 1 && Mx2<FORM>::NP == 6 ? 2 : 1)));
+  // NCT column tiles per block.  Two tiles halve the table fetches per column but double the
+  // accumulators; measured at T170 they pay only for the single-group bf16x6 form (6
+  // products per data fragment: forward 101 -> 84 us).  The column-block knob forces 1 (16)
+  // or 2 (32) tiles.
+  const int fnct = p->leg_cb == 16 ? 1 : (p->leg_cb == 32 ? 2 : (fwd_ng(OP) == 1 && Mx2<FORM>::NP == 6 ? 2 : 1));
   const long nb = 8l * cdiv(p->N + 1, 8) * cdiv(a.CT, fnct);
-  if (j.shared) {
-    if (fnct == 2)
-      hipLaunchKernelGGL((k_mx2_fwd_sh<OP, FORM, FwdShape<OP>::MAXT, FwdShape<OP>::NSW, 2>), dim3((unsigned)nb), dim3(256),
-                         0, s, a);
-    else
-      hipLaunchKernelGGL((k_mx2_fwd_sh<OP, FORM, FwdShape<OP>::MAXT, FwdShape<OP>::NSW, 1>), dim3((unsigned)nb), dim3(256),
-                         0, s, a);
-  } else {
-    hipLaunchKernelGGL((k_mx2_fwd<OP, FORM, FwdShape<OP>::MAXT>), dim3((unsigned)nb), dim3(256), 0, s, a);
-  }
+  if (fnct == 2)
+    hipLaunchKernelGGL((k_mx2_fwd_sh<OP, FORM, FwdShape<OP>::MAXT, FwdShape<OP>::NSW, 2>), dim3((unsigned)nb), dim3(256), 0,
+                       s, a);
+  else
+    hipLaunchKernelGGL((k_mx2_fwd_sh<OP, FORM, FwdShape<OP>::MAXT, FwdShape<OP>::NSW, 1>), dim3((unsigned)nb), dim3(256), 0,
+                       s, a);
   DSHT_HIP(hipGetLastError());
   return 0;
 }
@@ -1142,6 +946,27 @@ int fwd_form(const dsht_plan* p, const MixedTables& t, int op, const Mx2FwdJob& 
 }
 
 }  // namespace
+
+template <int FORM>
+static size_t staged_lds_form(const dsht_plan* p, int op) {
+  switch (op) {
+    case I_VAL: return staged_lds<I_VAL, FORM>(p);
+    case I_GRADLAMBDA: return staged_lds<I_GRADLAMBDA, FORM>(p);
+    case I_GRADMU: return staged_lds<I_GRADMU, FORM>(p);
+    case I_ALPHA: return staged_lds<I_ALPHA, FORM>(p);
+    case I_VORDIV2UV: return staged_lds<I_VORDIV2UV, FORM>(p);
+    default: return staged_lds<I_VGRAD, FORM>(p);
+  }
+}
+
+size_t mx2_inv_staged_lds(const dsht_plan* p, int form, int op) {
+  switch (form) {
+    case 0: return staged_lds_form<0>(p, op);
+    case 1: return staged_lds_form<1>(p, op);
+    case 2: return staged_lds_form<2>(p, op);
+    default: return staged_lds_form<3>(p, op);
+  }
+}
 
 size_t mx2_inv_pack_bytes(const MixedTables& t, int form, int op, int ncols) {
   const int ntb = form == 1 ? 3 : (form == 2 ? 1 : (form == 3 ? 3 : 2));

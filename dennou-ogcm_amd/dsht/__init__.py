@@ -49,7 +49,7 @@ EXPORTS = [
     "dsht_lfam3_default_config", "dsht_lfam3_create", "dsht_lfam3_destroy", "dsht_lfam3_set_rotation",
     "dsht_lfam3_state", "dsht_lfam3_step", "dsht_lfam3_copy_run", "dsht_lfam3_units",
 ]
-TUNE_CHUNK_COLS, TUNE_LEG_CB, TUNE_FFT, TUNE_LEG, TUNE_PIPELINE, TUNE_FFT_ORDER = 0, 1, 2, 3, 4, 5
+TUNE_CHUNK_COLS, TUNE_LEG_CB, TUNE_FFT, TUNE_LEG, TUNE_FFT_ORDER = 0, 1, 2, 3, 5
 
 
 class LfAm3Config(ctypes.Structure):

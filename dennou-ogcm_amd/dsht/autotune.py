@@ -6,8 +6,9 @@ tendencies) under each candidate and keeps the fastest.  The candidates are:
   column chunk      TUNE_CHUNK_COLS: columns per Fourier scratch chunk (0 = whole call up to
                     512 MiB of Fourier scratch per output)
   column block      TUNE_LEG_CB: column tiles per wave / workgroup of the contractions
-  contraction       TUNE_LEG: kernel family (split forms: packed data panels or
-                    LDS-shared split data; FP64: LDS-staged or per-wave fragment streams)
+  contraction       TUNE_LEG: kernel family (split-form inverse: per form, packed data
+                    panels or whole-K LDS staging; FP64: LDS-staged or per-wave fragment
+                    streams)
 Results are cached per (nmax, im, jm, precision, ncols bucket) in a JSON file so later
 runs reuse them.
 """
@@ -35,7 +36,7 @@ def _key(plan, ncols):
 def candidates(plan, ncols):
     """Candidate knob settings as dicts {chunk, cb, leg}."""
     chunks = sorted({c for c in (64, 128, 256) if c < ncols} | {0})
-    legs = (0, 1, 2) if plan.precision == PREC_FP64 else (0, 3)
+    legs = (0, 1, 2) if plan.precision == PREC_FP64 else (0, 2, 3)
     return [{"chunk": c, "cb": cb, "leg": lg} for c in chunks for cb in (16, 32) for lg in legs]
 
 

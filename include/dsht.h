@@ -273,13 +273,10 @@ int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, d
                                   3 = register kernel, FP64 arithmetic in every form */
 #define DSHT_TUNE_LEG 3        /* Legendre contractions.  FP64: 0 = LDS-staged inverse with the
                                   fragment-stream forward (default), 1 or 3 = LDS-staged, 2 = per-wave
-                                  fragment streams.  Split forms: 0 = per-wave fragment streams
-                                  over pre-packed data panels, 1 = LDS-staged (BF16 forms
-                                  only; FP16 forms return DSHT_EUNSUPPORTED), 2 = fragment
-                                  streams splitting the data in place, 3 = split data shared
-                                  through LDS */
-#define DSHT_TUNE_PIPELINE 4   /* 1 = run each call's column chunks as a two-stream pipeline (FFT of
-                                  chunk k on a helper stream beside the contraction of chunk k+1) */
+                                  fragment streams.  Split-form inverse: 0 = per form (default:
+                                  whole-K LDS staging for FP16X3, packed data panels otherwise),
+                                  2 = packed data panels, 3 = whole-K LDS staging; 1 (a retired
+                                  design) returns DSHT_EUNSUPPORTED.  (Key 4 is retired.) */
 #define DSHT_TUNE_FFT_ORDER 5  /* row-group order of the register FFTs: 0 = packed (groups walk
                                   the columns of one latitude first and may straddle two
                                   latitudes), 1 = latitude-major (default: each group is 16

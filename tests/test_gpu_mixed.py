@@ -25,8 +25,8 @@ def _prec(name):
     return {v: k for k, v in dsht.PREC_NAMES.items()}[name]
 
 
-# contraction kernel families (dsht_set_tuning DSHT_TUNE_LEG): packed panels, LDS-staged
-# (first design), in-place split, LDS-shared split
+# contraction kernel families (dsht_set_tuning DSHT_TUNE_LEG): per form, retired (refused),
+# packed data panels, whole-K LDS staging
 @pytest.mark.parametrize("leg", [0, 1, 2, 3])
 @pytest.mark.parametrize("form", list(BOUND))
 @pytest.mark.parametrize("N,I,J", [(21, 64, 32), (42, 128, 64), (31, 96, 48)])
@@ -40,10 +40,10 @@ def test_mixed_transforms(gpu, form, N, I, J, leg):
     p.set_tuning(dsht.TUNE_LEG, leg)
     tol = BOUND[form]
     w = red_spectrum(N, 6, 3) * 1e-5  # magnitudes of a vorticity field
-    if leg == 1 and form.startswith("fp16"):  # the first split design keeps the BF16 forms only
-        with pytest.raises(dsht.DshtError, match="BF16 forms only"):
+    if leg == 1:  # a retired split-form design
+        with pytest.raises(dsht.DshtError, match="retired"):
             p._inv(w, dsht.VAL)
-        with pytest.raises(dsht.DshtError, match="BF16 forms only"):
+        with pytest.raises(dsht.DshtError, match="retired"):
             p._fwd(np.ones((1, J, I)), dsht.VAL)
         return
     for kind in ("VAL", "GRADLAMBDA", "GRADMU", "GRADLAT"):

@@ -226,29 +226,6 @@ def test_fft_kernels(gpu, N, I, J):
         assert np.array_equal(got[0], out[0][0]) and np.array_equal(got[1], out[0][1]), order
 
 
-@pytest.mark.parametrize("prec", ["fp64", "bf16x6"])
-def test_chunk_pipeline_matches_serial(gpu, prec):
-    """The two-stream chunk pipeline (DSHT_TUNE_PIPELINE) gives the serial result bit for
-    bit: same kernels, only the stream ordering differs."""
-    import torch
-    import dsht
-    N, I, J, nc = 170, 512, 256, 300
-    p = dsht.Plan(N, I, J, radius=6.371e6)
-    p.set_precision({v: k for k, v in dsht.PREC_NAMES.items()}[prec])
-    p.set_tuning(dsht.TUNE_CHUNK_COLS, 64)  # 5 chunks, the last one ragged
-    w = torch.tensor(red_spectrum(N, nc, 23).T.copy(), device=gpu)
-    out = {}
-    for pipe in (0, 1):
-        p.set_tuning(dsht.TUNE_PIPELINE, pipe)
-        xy = torch.empty((nc, J, I), dtype=torch.float64, device=gpu)
-        w2 = torch.zeros_like(w)
-        p.inv_d(dsht.GRADLAT, w, xy)
-        p.fwd_d(dsht.VAL, xy, w2)
-        torch.cuda.synchronize()
-        out[pipe] = (xy, w2)
-    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
-
-
 def test_errors(gpu):
     import dsht
     with pytest.raises(dsht.DshtError):

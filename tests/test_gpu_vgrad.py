@@ -8,8 +8,8 @@ Checks:
     multiplies the value's FP32 Fourier coefficients by i m inside the FP32 inverse FFT
     (one more FP32 rounding than the separate call, which applies i m in the FP64
     contraction epilogue);
-  * every kernel family (DSHT_TUNE_LEG), NULL outputs, both column blocks, the chunked
-    T170 path and the two-stream chunk pipeline.
+  * every kernel family (DSHT_TUNE_LEG), NULL outputs, both column blocks and the chunked
+    T170 path.
 """
 import numpy as np
 import pytest
@@ -84,8 +84,8 @@ def test_vgrad_split_forms(gpu, form, N, I, J, leg):
     p.set_tuning(dsht.TUNE_LEG, leg)
     w = red_spectrum(N, 21, 3) * 1e-5
     wt, outs = _dev(p, w, gpu)
-    if leg == 1 and form.startswith("fp16"):  # the first split design keeps the BF16 forms only
-        with pytest.raises(dsht.DshtError, match="BF16 forms only"):
+    if leg == 1:  # a retired split-form design
+        with pytest.raises(dsht.DshtError, match="retired"):
             p.inv_vgrad_d(wt, *outs)
         return
     p.inv_vgrad_d(wt, *outs)
@@ -128,9 +128,9 @@ def test_vgrad_null_outputs(gpu, prec):
         p.inv_vgrad_d(wt, None, None, None)
 
 
-@pytest.mark.parametrize("prec,pipeline", [("fp64", 0), ("fp64", 1), ("bf16x6", 0), ("bf16x6", 1), ("fp16x3", 0)])
-def test_vgrad_t170_chunked(gpu, prec, pipeline):
-    """T170 over several column chunks (64 columns per chunk forced), serial and pipelined."""
+@pytest.mark.parametrize("prec", ["fp64", "bf16x6", "fp16x3"])
+def test_vgrad_t170_chunked(gpu, prec):
+    """T170 over several column chunks (64 columns per chunk forced)."""
     import torch
     import dsht
     N, I, J, nc = 170, 512, 256, 150
@@ -138,7 +138,6 @@ def test_vgrad_t170_chunked(gpu, prec, pipeline):
     if prec != "fp64":
         p.set_precision(_prec(prec))
     p.set_tuning(dsht.TUNE_CHUNK_COLS, 64)
-    p.set_tuning(dsht.TUNE_PIPELINE, pipeline)
     w = red_spectrum(N, nc, 23)
     wt, outs = _dev(p, w, gpu)
     p.inv_vgrad_d(wt, *outs)
@@ -147,7 +146,7 @@ def test_vgrad_t170_chunked(gpu, prec, pipeline):
     sep = _separate(p, wt, outs[0])
     tol = 1e-13 if prec == "fp64" else SPLIT_SEP
     for i in range(3):
-        assert rel_l2(got[i], sep[i]) < tol, (prec, pipeline, i, rel_l2(got[i], sep[i]))
+        assert rel_l2(got[i], sep[i]) < tol, (prec, i, rel_l2(got[i], sep[i]))
     if prec == "fp64":
         return
     # the split form against the FP64 form of the same plan
