@@ -376,9 +376,12 @@ BATCH_GRIDS = {21: (72, 36), 42: (144, 72), 85: (270, 136), 170: (540, 270)}
 
 def batched_bench(torch, dsht, n_inst=64, steps=10, seq_steps=2, seed=0xB47C4, prec="bf16x3"):
     """64 independent instances of uneven truncation and layer count advanced by one
-    hipGraph-captured batched step (tile/chunk knobs from the autotuner), against the same
-    instances run as sequential per-instance calls.  Error of the split form against the
-    FP64 batched step (itself checked against the oracle in the test suite)."""
+    hipGraph-captured batched step (tile/chunk knobs from the autotuner; truncation groups are
+    parallel branches of the graph), against the same step enqueued eagerly (groups on
+    concurrent streams) and against the instances run one after another, two calls each
+    (one inverse over its 4 KM prognostic columns, one forward over its 6 KM tendency
+    columns).  Error of the split form against the FP64 batched step (itself checked against
+    the oracle in the test suite)."""
     from dsht import autotune
 
     dev = torch.device("cuda:0")
@@ -455,6 +458,7 @@ def batched_bench(torch, dsht, n_inst=64, steps=10, seq_steps=2, seed=0xB47C4, p
         "sequential_ms_per_step": 1e3 * t_seq,
         "sequential_pairs_per_s": units / 2 / t_seq,
         "speedup_vs_sequential": t_seq / t_graph,
+        "graph_vs_eager": t_eager / t_graph,
         "rel_l2_vs_fp64": err,
         "autotuned": tuning,
     }
@@ -639,6 +643,8 @@ def main():
                     help="only run the LF-AM3 step section (T170L60, 1 and 4 runs) and print its JSON")
     ap.add_argument("--column-only", action="store_true",
                     help="only run the vertical column diagnostics section and print its JSON")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only run the batched-instances section and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: the transform path has no "
@@ -665,6 +671,10 @@ def main():
     if args.lfam3_only:
         import dsht
         print(json.dumps({"lfam3": lfam3_bench(torch, dsht, steps=args.steps)}))
+        return
+    if args.batch_only:
+        import dsht
+        print(json.dumps({"batched": batched_bench(torch, dsht)}))
         return
 
     import dsht

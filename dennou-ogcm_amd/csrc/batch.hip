@@ -2,13 +2,20 @@
 // count advanced together.  The reference runs one model per process and parallelises
 // only over layers (`!$omp parallel do` around SpmlUtil calls, SpmlUtil_mod.f90:360,389);
 // here instances that share a truncation are packed side by side as columns of one set
-// of device buffers, so one contraction launch covers all of them, and the whole step
-// (every truncation group) is captured once into a hipGraph and replayed.
+// of device buffers, so one contraction launch covers all of them; the truncation groups
+// run concurrently on streams of their own (forked from and joined back to the caller's
+// stream), and the whole step is captured once into a hipGraph whose groups are parallel
+// branches.
 //
 // Per instance (KM layers) and step, the dynamics-step transform set:
 //   4 prognostic fields (vor, div, T, S) spectral -> grid with lon/lat derivatives
 //   (xy_w, xy_GradLon_w, xy_GradLat_w; one fused dsht_inv_vgrad_d call) and 6 tendency
 //   fields grid -> spectral (w_xy).
+// Layout: an instance's columns are contiguous, fields of one layer count apart: prognostic
+// field f, layer k of the instance with first layer column col0 is column NPROG col0 + f KM +
+// k of its group's spectral and grid buffers; tendency field f is column NTEND col0 + f KM +
+// k.  So an instance alone is one inverse call over NPROG KM columns and one forward call
+// over NTEND KM columns, and a group is the same two calls over all its instances.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -23,17 +30,18 @@ struct BatchInstance {
   dsht_plan* plan;
   int km;
   int group;
-  int col0;  // first layer column of this instance inside its group (per field block)
+  int col0;  // first layer column of this instance inside its group (in layers)
 };
 
 struct BatchGroup {
   dsht_plan* plan;
   int cols = 0;  // total layers of the group's instances
-  // field-major blocks: column = f * cols + (instance col0 + layer)
   double* spec = nullptr;     // (L, NPROG * cols)  LC layout
   double* grid = nullptr;     // NINV x (NPROG * cols, J, I)
   double* tend = nullptr;     // (NTEND * cols, J, I)
   double* spect = nullptr;    // (L, NTEND * cols)  LC layout
+  hipStream_t stream = nullptr;  // the group's branch (default priority)
+  hipEvent_t done = nullptr;     // recorded on the branch at the end of its step
 };
 
 }  // namespace dsht
@@ -44,6 +52,7 @@ struct dsht_batch {
   bool finalized = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipEvent_t fork = nullptr;
   size_t bytes = 0;
 };
 
@@ -108,7 +117,12 @@ int dsht_batch_finalize(dsht_batch* b) {
     DSHT_HIP(hipMemset(g.spec, 0, 8 * n_spec));
     DSHT_HIP(hipMemset(g.tend, 0, 8 * n_tend));
     b->bytes += 8 * (n_spec + n_grid + n_tend + n_spect);
+    if (b->groups.size() > 1) {
+      DSHT_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+      DSHT_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+    }
   }
+  if (b->groups.size() > 1) DSHT_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   b->finalized = true;
   return 0;
 }
@@ -126,7 +140,8 @@ int dsht_batch_info(const dsht_batch* b, int id, int* group, int* col0, int* gro
 }
 
 /* device buffers of one truncation group: spec (L, 4*cols) LC, grid 3 x (4*cols, J, I),
-   tend (6*cols, J, I), spect (L, 6*cols) LC; field f, layer column c at f*cols + c */
+   tend (6*cols, J, I), spect (L, 6*cols) LC; instance-contiguous columns (prognostic field f,
+   layer k of an instance at 4*col0 + f*km + k, tendency field f at 6*col0 + f*km + k) */
 int dsht_batch_buffers(const dsht_batch* b, int group, double** spec, double** grid, double** tend,
                        double** spect) {
   if (!b || !b->finalized) return fail(DSHT_ESTATE, "batch not finalized");
@@ -141,15 +156,24 @@ int dsht_batch_buffers(const dsht_batch* b, int group, double** spec, double** g
 
 int dsht_batch_ngroups(const dsht_batch* b) { return b ? (int)b->groups.size() : 0; }
 
-/* one step, every group, enqueued directly (wide calls over each group's columns) */
+/* one step, every group: with several groups each runs on its own stream, forked from and
+   joined back to the caller's stream (under capture: parallel branches of the graph) */
 int dsht_batch_step(dsht_batch* b, void* stream) {
   if (!b || !b->finalized) return fail(DSHT_ESTATE, "batch not finalized");
-  for (auto& g : b->groups)
-    if (int rc = batch_group_step(g, (hipStream_t)stream)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (b->groups.size() == 1) return batch_group_step(b->groups[0], s);
+  DSHT_HIP(hipEventRecord(b->fork, s));
+  for (auto& g : b->groups) {
+    DSHT_HIP(hipStreamWaitEvent(g.stream, b->fork, 0));
+    if (int rc = batch_group_step(g, g.stream)) return rc;
+    DSHT_HIP(hipEventRecord(g.done, g.stream));
+  }
+  for (auto& g : b->groups) DSHT_HIP(hipStreamWaitEvent(s, g.done, 0));
   return 0;
 }
 
-/* the unbatched baseline: each instance's step as its own calls, one after another */
+/* the unbatched baseline: each instance's step as its own two calls (its columns are
+   contiguous), one instance after another on the caller's stream */
 int dsht_batch_step_sequential(dsht_batch* b, void* stream) {
   if (!b || !b->finalized) return fail(DSHT_ESTATE, "batch not finalized");
   hipStream_t s = (hipStream_t)stream;
@@ -158,17 +182,13 @@ int dsht_batch_step_sequential(dsht_batch* b, void* stream) {
     dsht_plan* p = g.plan;
     const long IJ = (long)p->I * p->J;
     const int C = NPROG * g.cols, CT = NTEND * g.cols;
-    for (int f = 0; f < NPROG; f++) {
-      const int c = f * g.cols + in.col0;
-      double* out[NINV];
-      for (int k = 0; k < NINV; k++) out[k] = g.grid + (size_t)k * C * IJ + (size_t)c * IJ;
-      if (int rc = dsht_inv_vgrad_d(p, g.spec + c, C, 1, out[0], out[1], out[2], IJ, in.km, nullptr, s)) return rc;
-    }
-    for (int f = 0; f < NTEND; f++) {
-      const int c = f * g.cols + in.col0;
-      if (int rc = dsht_fwd_d(p, DSHT_VAL, g.tend + (size_t)c * IJ, IJ, g.spect + c, CT, 1, in.km, nullptr, s))
-        return rc;
-    }
+    const int c = NPROG * in.col0, ct = NTEND * in.col0;
+    double* out[NINV];
+    for (int k = 0; k < NINV; k++) out[k] = g.grid + (size_t)k * C * IJ + (size_t)c * IJ;
+    if (int rc = dsht_inv_vgrad_d(p, g.spec + c, C, 1, out[0], out[1], out[2], IJ, NPROG * in.km, nullptr, s))
+      return rc;
+    if (int rc = dsht_fwd_d(p, DSHT_VAL, g.tend + (size_t)ct * IJ, IJ, g.spect + ct, CT, 1, NTEND * in.km, nullptr, s))
+      return rc;
   }
   return 0;
 }
@@ -214,7 +234,10 @@ int dsht_batch_destroy(dsht_batch* b) {
   (void)hipDeviceSynchronize();
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->fork) (void)hipEventDestroy(b->fork);
   for (auto& g : b->groups) {
+    if (g.done) (void)hipEventDestroy(g.done);
+    if (g.stream) (void)hipStreamDestroy(g.stream);
     if (g.spec) (void)hipFree(g.spec);
     if (g.grid) (void)hipFree(g.grid);
     if (g.tend) (void)hipFree(g.tend);

@@ -241,12 +241,14 @@ __global__ __launch_bounds__(256) void k_col_quick(const double* __restrict__ om
 // the surface and the bottom (:836-852).  The solve is LAPACK DGTSV (Gaussian elimination
 // with partial pivoting by row interchange, then back substitution; LAPACK 3.x
 // SRC/dgtsv.f, the NRHS = 1 path), in its order of operations.  The forward sweep leaves
-// each finished row's d, du and second superdiagonal in `work` (3 km values per point,
-// pitch npts) and its right-hand side in qa; the backward sweep reads them back in
-// reverse and writes qa = q0 + x.  This round trip goes through the caches (at T170L60 the
+// each finished row's d and du in `work` (pitch npts; 3 km values per point reserved) and its
+// right-hand side in qa; the second superdiagonal is nonzero only after a row interchange,
+// so up to 64 layers a bit mask in a register records the interchanges and only those rows
+// store it (the backward sweep reads it only there).  The backward sweep reads the rows back
+// in reverse and writes qa = q0 + x.  The round trip goes through the caches (at T170L60 the
 // live rows of all points fit in the last-level cache: 132 us against 141 us non-temporal),
-// and both sweeps keep the loads of CU rows in flight together (132 -> 119 us).  A singular pivot (impossible with kv >= 0 and positive
-// thicknesses) is not reported.
+// and both sweeps keep the loads of CU rows in flight together (132 -> 119 us).  A singular
+// pivot (impossible with kv >= 0 and positive thicknesses) is not reported.
 // MOM selects calc_VDiffMOMEq (:865-975): right-hand side dt rhs[k] (no division by e3) and,
 // for a Dirichlet bottom, Ad = 1 + alpha coef (bfric + r[km-1]) with the bottom friction
 // coefficient bfric of the grid point (:963).
@@ -254,7 +256,7 @@ struct VdiffRow {
   double al, ad, au, b;
 };
 
-template <bool MOM>
+template <bool MOM, bool MASK>
 __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0, const double* __restrict__ rhs,
                                                    const double* __restrict__ kv, const double* __restrict__ e3,
                                                    long ldg, int km, const double* __restrict__ cdk,
@@ -270,6 +272,9 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
   double* wd = work + i;                   // d of finished rows
   double* wu = work + (long)km * nw + i;   // du
   double* w2 = work + 2l * km * nw + i;    // second superdiagonal
+  // MASK (km <= 64): bit j set where step j interchanged rows; the second superdiagonal is
+  // nonzero only there, so only those rows store it and the backward sweep reads only those
+  unsigned long long piv = 0ull;
   // half-level coefficient r[h] from the layers above (h-1) and below (h)
   auto rhalf = [&](int h, double kva, double e3a, double kvb, double e3b) {
     const double s = cdk[h - 1] + cdk[h];
@@ -331,7 +336,8 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
   double dl = r1.al, dn = r1.ad, dun = r1.au, bn = r1.b;
   auto eliminate = [&](int j, const VdiffRow& r2) {  // step j; r2 = row j+2
     double du2;
-    if (fabs(d) >= fabs(dl)) {
+    const bool keep = fabs(d) >= fabs(dl);  // no interchange
+    if (keep) {
       const double fact = dl / d;
       dn = dn - fact * du;
       bn = bn - fact * b;
@@ -350,7 +356,8 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
     }
     wd[(long)j * nw] = d;
     wu[(long)j * nw] = du;
-    w2[(long)j * nw] = du2;
+    if (!MASK || !keep) w2[(long)j * nw] = du2;
+    if (MASK && !keep) piv |= 1ull << j;
     qa[(long)j * ldo + i] = b;
     d = dn, du = dun, b = bn;
     dl = r2.al, dn = r2.ad, dun = r2.au, bn = r2.b;
@@ -406,7 +413,7 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
       const long r = jb - u;
       pd[u] = wd[r * nw];
       pu[u] = wu[r * nw];
-      pv[u] = w2[r * nw];
+      pv[u] = (!MASK || ((piv >> r) & 1ull)) ? w2[r * nw] : 0.0;
       pb[u] = qa[r * ldo + i];
       pq[u] = ntl(q0 + r * ldg + i);
     }
@@ -414,8 +421,8 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
     for (int u = 0; u < CU; u++) back(jb - u, pd[u], pu[u], pv[u], pb[u], pq[u]);
   }
   for (; jb >= 0; jb--)
-    back(jb, wd[(long)jb * nw], wu[(long)jb * nw], w2[(long)jb * nw], qa[(long)jb * ldo + i],
-         ntl(q0 + (long)jb * ldg + i));
+    back(jb, wd[(long)jb * nw], wu[(long)jb * nw], (!MASK || ((piv >> jb) & 1ull)) ? w2[(long)jb * nw] : 0.0,
+         qa[(long)jb * ldo + i], ntl(q0 + (long)jb * ldg + i));
 }
 
 int col_grid(int npts) { return (npts + 255) / 256; }
@@ -458,12 +465,10 @@ int launch_col_vdiff(const double* q0, const double* rhs, const double* kv, cons
                      const double* cdk, const double* fdk, double dt, double alpha, int sfc_dirichlet,
                      int btm_dirichlet, const double* bfric, double* qa, long ldo, double* work, int npts,
                      hipStream_t s) {
-  if (bfric)
-    hipLaunchKernelGGL(k_col_vdiff<true>, dim3(col_grid(npts)), dim3(256), 0, s, q0, rhs, kv, e3, ldg, km, cdk, fdk,
-                       dt, alpha, sfc_dirichlet, btm_dirichlet, bfric, qa, ldo, work, npts);
-  else
-    hipLaunchKernelGGL(k_col_vdiff<false>, dim3(col_grid(npts)), dim3(256), 0, s, q0, rhs, kv, e3, ldg, km, cdk, fdk,
-                       dt, alpha, sfc_dirichlet, btm_dirichlet, bfric, qa, ldo, work, npts);
+  auto k = bfric ? (km <= 64 ? k_col_vdiff<true, true> : k_col_vdiff<true, false>)
+                 : (km <= 64 ? k_col_vdiff<false, true> : k_col_vdiff<false, false>);
+  hipLaunchKernelGGL(k, dim3(col_grid(npts)), dim3(256), 0, s, q0, rhs, kv, e3, ldg, km, cdk, fdk, dt, alpha,
+                     sfc_dirichlet, btm_dirichlet, bfric, qa, ldo, work, npts);
   DSHT_HIP(hipGetLastError());
   return 0;
 }

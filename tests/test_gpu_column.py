@@ -202,11 +202,13 @@ def _vdiff_case(km, J, I, seed, pivot):
 
 
 @pytest.mark.parametrize("N,I,J,km", [(42, 128, 64, 20), (85, 256, 128, 60), (170, 512, 256, 60),
-                                      (21, 64, 32, 2), (21, 64, 32, 3), (21, 64, 32, 4)])
+                                      (21, 64, 32, 2), (21, 64, 32, 3), (21, 64, 32, 4), (21, 64, 32, 64),
+                                      (21, 64, 32, 70)])
 @pytest.mark.parametrize("pivot", [False, True])
 def test_vdiff_bitwise(gpu, N, I, J, km, pivot):
     """calc_VDiffTRCEq with DGTSV's elimination, bit for bit against the oracle, with and
-    without row interchanges."""
+    without row interchanges; up to 64 layers the interchanges are kept in a bit mask, above
+    (70) every row stores its second superdiagonal."""
     import torch
     import dsht
     import oracle
