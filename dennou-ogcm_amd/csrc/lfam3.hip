@@ -36,23 +36,38 @@ inline dim3 lf_grid(long n) {
 }
 
 #define LF_LOOP(n) for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < (n); g += (long)gridDim.x * blockDim.x)
+// 2-D walk over (column c, point p) of ncols columns of npts points, g = c npts + p: the column
+// is uniform over a block (grid y), so its layer and run come from scalar arithmetic, and
+// the latitude row of p is one 32-bit division
+#define LF_CP(ncols, npts)                                                                                \
+  for (long c = blockIdx.y; c < (ncols); c += gridDim.y)                                                 \
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < (npts); p += (long)gridDim.x * blockDim.x)
+#define LF_ROW(p, I) ((int)((unsigned)(p) / (unsigned)(I)))
+
+inline dim3 lf_grid2(long ncols, long npts) {
+  long bx = (npts + LF_THREADS - 1) / LF_THREADS;
+  if (bx > 4096) bx = 4096;
+  long by = ncols < 65535 ? ncols : 65535;
+  return dim3((unsigned)(bx < 1 ? 1 : bx), (unsigned)(by < 1 ? 1 : by));
+}
 
 // x*cos(phi_j), y*cos(phi_j) (the calc_UVCosLat2VorDiv inputs, HBEDiagnose_hspm_vfvm_mod.f90:278-281)
 __global__ void k_lf_uvcos(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ cs,
-                           long npts, int I, long n, double* __restrict__ ucos, double* __restrict__ vcos) {
-  LF_LOOP(n) {
-    const int j = (int)((g % npts) / I);
-    ucos[g] = u[g] * cs[j];
-    vcos[g] = v[g] * cs[j];
+                           long npts, int I, long ncols, double* __restrict__ ucos, double* __restrict__ vcos) {
+  LF_CP(ncols, npts) {
+    const long g = c * npts + p;
+    const double w = cs[LF_ROW(p, I)];
+    ucos[g] = u[g] * w;
+    vcos[g] = v[g] * w;
   }
 }
 
 // GeoPot = Grav * D * z_CK(k) (DOGCM_TInt_LFAM3_mod.f90:460)
 __global__ void k_lf_geo(double* __restrict__ geo, long npts, int K, const double* __restrict__ zck, double depth,
-                         double grav, long n) {
-  LF_LOOP(n) {
-    const int k = (int)((g / npts) % K);
-    geo[g] = grav * (depth * zck[k]);
+                         double grav, long ncols) {
+  LF_CP(ncols, npts) {
+    const int k = (int)(c % K);
+    geo[c * npts + p] = grav * (depth * zck[k]);
   }
 }
 
@@ -65,9 +80,9 @@ struct EosArgs {
   double ref_dens, ptemp_dep, press_dep, salt_dep, ref_temp, ref_salt, depth, grav;
 };
 __global__ void k_lf_eos(const double* __restrict__ ts, long npts, int K, const double* __restrict__ zck, EosArgs e,
-                         long n, double* __restrict__ dens) {
-  LF_LOOP(n) {
-    const long c = g / npts, p = g % npts;
+                         long ncols, double* __restrict__ dens) {
+  LF_CP(ncols, npts) {
+    const long g = c * npts + p;
     const int k = (int)(c % K);
     const long r = c / K;
     const double t = ts[((2 * r) * K + k) * npts + p];
@@ -91,12 +106,12 @@ struct MomArgs {
   int I, K;
   double *am, *bg, *cg;
 };
-__global__ void k_lf_mom(MomArgs a, long n) {
-  LF_LOOP(n) {
-    const long c = g / a.npts, p = g % a.npts;
+__global__ void k_lf_mom(MomArgs a, long ncols) {
+  LF_CP(ncols, a.npts) {
+    const long g = c * a.npts + p;
     const int k = (int)(c % a.K);
     const long r = c / a.K;
-    const int j = (int)(p / a.I);
+    const int j = LF_ROW(p, a.I);
     const long np = a.npts;
     const double u = a.u[g], v = a.v[g];
     const double* om = a.omg + r * (a.K + 1) * np + p;  // half level h at om[h*np]
@@ -128,11 +143,9 @@ __global__ void k_lf_mom(MomArgs a, long n) {
 // w_out[l, c] += w_in[l, c] * s  (s = 1, or 1/(RefDens R^2) for the barotropic pressure term)
 __global__ void k_lf_spec_axpy(double* __restrict__ wout, long slo, const double* __restrict__ win, long sli, int L,
                                int ncols, double s, int divide) {
-  const long n = (long)L * ncols;
-  LF_LOOP(n) {
-    const int l = (int)(g / ncols), c = (int)(g % ncols);
-    const double x = win[(long)l * sli + c];
-    wout[(long)l * slo + c] += divide ? x / s : x;
+  LF_CP(L, ncols) {
+    const double x = win[c * sli + p];
+    wout[c * slo + p] += divide ? x / s : x;
   }
 }
 
@@ -142,12 +155,12 @@ __global__ void k_lf_pred_uv(const double* __restrict__ urhs, const double* __re
                              const double* __restrict__ un, const double* __restrict__ ub,
                              const double* __restrict__ vn, const double* __restrict__ vb,
                              const double* __restrict__ two_omega, const double* __restrict__ mu, long npts, int I,
-                             int K, double half2dt, double c56dt, int implicit, long n, double* __restrict__ uo,
+                             int K, double half2dt, double c56dt, int implicit, long ncols, double* __restrict__ uo,
                              double* __restrict__ vo) {
-  LF_LOOP(n) {
-    const long c = g / npts, p = g % npts;
+  LF_CP(ncols, npts) {
+    const long g = c * npts + p;
     const long r = c / K;
-    const int j = (int)(p / I);
+    const int j = LF_ROW(p, I);
     double ur = urhs[g], vr = vrhs[g];
     if (implicit) {
       const double cif = half2dt * (two_omega[r] * mu[j]);
@@ -163,9 +176,9 @@ __global__ void k_lf_pred_uv(const double* __restrict__ urhs, const double* __re
 // x = x - xbar + xbar_ref for every layer (the barotropic part of x replaced, :520-528, 783-793)
 __global__ void k_lf_barot_shift(double* __restrict__ u, double* __restrict__ v, const double* __restrict__ ubt,
                                  const double* __restrict__ vbt, const double* __restrict__ ubr,
-                                 const double* __restrict__ vbr, long npts, int K, long n) {
-  LF_LOOP(n) {
-    const long c = g / npts, p = g % npts;
+                                 const double* __restrict__ vbr, long npts, int K, long ncols) {
+  LF_CP(ncols, npts) {
+    const long g = c * npts + p;
     const long q = (c / K) * npts + p;
     u[g] = u[g] - ubt[q] + ubr[q];
     v[g] = v[g] - vbt[q] + vbr[q];
@@ -178,15 +191,15 @@ __global__ void k_lf_barot_shift(double* __restrict__ u, double* __restrict__ v,
 __global__ void k_lf_htrc_grid(const double* __restrict__ ts, const double* __restrict__ u,
                                const double* __restrict__ v, const double* __restrict__ h,
                                const double* __restrict__ cs, const double* __restrict__ flx,
-                               const double* __restrict__ rcdk, long npts, int I, int K, long n,
+                               const double* __restrict__ rcdk, long npts, int I, int K, long ncols,
                                double* __restrict__ fu, double* __restrict__ fv, double* __restrict__ vf) {
-  LF_LOOP(n) {
-    const long c = g / npts, p = g % npts;
+  LF_CP(ncols, npts) {
+    const long g = c * npts + p;
     const int k = (int)(c % K);
     const long rt = c / K;  // 2 r + t
     const long r = rt / 2;
     const long q = (r * K + k) * npts + p;
-    const int j = (int)(p / I);
+    const int j = LF_ROW(p, I);
     const double tmp = h[q] * cs[j] * ts[g];
     fu[g] = u[q] * tmp;
     fv[g] = v[q] * tmp;
@@ -198,19 +211,18 @@ __global__ void k_lf_htrc_grid(const double* __restrict__ ts, const double* __re
 // w = AdvFilter (-w_alpha + w_vert) (HBEBaroc_hspm_vfvm_mod.f90:163-171), in place over w_alpha
 __global__ void k_lf_htrc_spec(double* __restrict__ wa, const double* __restrict__ wv, const double* __restrict__ filt,
                                int L, int ncols) {
-  const long n = (long)L * ncols;
-  LF_LOOP(n) {
-    const int l = (int)(g / ncols);
-    wa[g] = filt[l] * (-wa[g] + wv[g]);
+  LF_CP(L, ncols) {
+    const long g = c * ncols + p;
+    wa[g] = filt[c] * (-wa[g] + wv[g]);
   }
 }
 
 // predictor tracers (:535-541): ((H (2 T_N + T_B))/3 + 5/6 dt HTRC_RHS) / H
 __global__ void k_lf_pred_trc(const double* __restrict__ tn, const double* __restrict__ tb,
                               const double* __restrict__ ht, const double* __restrict__ h, long npts, int K,
-                              double c56dt, long n, double* __restrict__ to) {
-  LF_LOOP(n) {
-    const long c = g / npts, p = g % npts;
+                              double c56dt, long ncols, double* __restrict__ to) {
+  LF_CP(ncols, npts) {
+    const long g = c * npts + p;
     const long q = ((c / K) / 2 * K + c % K) * npts + p;
     const double hh = h[q] + 0.0;
     to[g] = (hh * (2.0 * tn[g] + tb[g]) / 3.0 + c56dt * ht[g]) / hh;
@@ -232,10 +244,10 @@ __global__ void k_lf_rhs_diff(const double* __restrict__ ua, const double* __res
 __global__ void k_lf_baroc_force(double* __restrict__ ubf, double* __restrict__ vbf, const double* __restrict__ ubn,
                                  const double* __restrict__ vbn, const double* __restrict__ ubb,
                                  const double* __restrict__ vbb, const double* __restrict__ two_omega,
-                                 const double* __restrict__ mu, long npts, int I, double cn, double cb, long n) {
-  LF_LOOP(n) {
-    const long r = g / npts, p = g % npts;
-    const double f = two_omega[r] * mu[(int)(p / I)];
+                                 const double* __restrict__ mu, long npts, int I, double cn, double cb, long nruns) {
+  LF_CP(nruns, npts) {
+    const long g = c * npts + p;
+    const double f = two_omega[c] * mu[LF_ROW(p, I)];
     const double coru = f * (cn * ubn[g] + cb * ubb[g]);
     const double corv = f * (cn * vbn[g] + cb * vbb[g]);
     ubf[g] = ubf[g] - corv;
@@ -248,11 +260,11 @@ __global__ void k_lf_baroc_force(double* __restrict__ ubf, double* __restrict__ 
 __global__ void k_lf_cori_step(const double* __restrict__ ur_in, const double* __restrict__ vr_in,
                                const double* __restrict__ u0, const double* __restrict__ v0,
                                const double* __restrict__ two_omega, const double* __restrict__ mu, long npts, int I,
-                               int K, double adt, double dt, int form, long n, double* __restrict__ uo,
+                               int K, double adt, double dt, int form, long ncols, double* __restrict__ uo,
                                double* __restrict__ vo) {
-  LF_LOOP(n) {
-    const long c = g / npts, p = g % npts;
-    const double cif = adt * (two_omega[c / K] * mu[(int)(p / I)]);
+  LF_CP(ncols, npts) {
+    const long g = c * npts + p;
+    const double cif = adt * (two_omega[c / K] * mu[LF_ROW(p, I)]);
     const double ur = ur_in[g], vr = vr_in[g];
     if (form == 0) {  // baroclinic: the tendency is replaced, then U_A = U_N + dt U_RHS
       const double u2 = (ur + cif * vr) / (1.0 + cif * cif);
@@ -272,12 +284,12 @@ __global__ void k_lf_barot_rhs_grid(const double* __restrict__ ub, const double*
                                     const double* __restrict__ ubf, const double* __restrict__ vbf,
                                     const double* __restrict__ sp, const double* __restrict__ spn,
                                     const double* __restrict__ two_omega, const double* __restrict__ mu,
-                                    const double* __restrict__ cs, long npts, int I, long n, double* __restrict__ am,
+                                    const double* __restrict__ cs, long npts, int I, long nruns, double* __restrict__ am,
                                     double* __restrict__ bg, double* __restrict__ spm) {
-  LF_LOOP(n) {
-    const long r = g / npts, p = g % npts;
-    const int j = (int)(p / I);
-    const double f = two_omega[r] * mu[j];
+  LF_CP(nruns, npts) {
+    const long g = c * npts + p;
+    const int j = LF_ROW(p, I);
+    const double f = two_omega[c] * mu[j];
     const double coru = f * ub[g], corv = f * vb[g];
     am[g] = -(cs[j] * (coru - vbf[g]));
     bg[g] = cs[j] * (corv + ubf[g]);
@@ -290,12 +302,11 @@ __global__ void k_lf_barot_rhs_grid(const double* __restrict__ ub, const double*
 __global__ void k_lf_barot_proj(const double* __restrict__ wdiv, long sld, const double* __restrict__ coef,
                                 const double* __restrict__ e, double ref_dens, double den, int L, int ncols,
                                 double* __restrict__ wdsp, double* __restrict__ wddiv) {
-  const long n = (long)L * ncols;
-  LF_LOOP(n) {
-    const int l = (int)(g / ncols), c = (int)(g % ncols);
-    const double wt = -wdiv[(long)l * sld + c];
-    wdsp[g] = ref_dens * coef[l] * wt / den;
-    wddiv[g] = e[l] * coef[l] * wt;
+  LF_CP(L, ncols) {
+    const long g = c * ncols + p;
+    const double wt = -wdiv[c * sld + p];
+    wdsp[g] = ref_dens * coef[c] * wt / den;
+    wddiv[g] = e[c] * coef[c] * wt;
   }
 }
 
@@ -315,10 +326,10 @@ __global__ void k_lf_final_uv(double* __restrict__ ua, double* __restrict__ va, 
                               const double* __restrict__ vbt, const double* __restrict__ uba,
                               const double* __restrict__ vba, const double* __restrict__ un,
                               const double* __restrict__ ub, const double* __restrict__ vn,
-                              const double* __restrict__ vb, const double* __restrict__ h, long npts, int K, long n,
+                              const double* __restrict__ vb, const double* __restrict__ h, long npts, int K, long ncols,
                               double* __restrict__ uo, double* __restrict__ vo, double* __restrict__ hm) {
-  LF_LOOP(n) {
-    const long c = g / npts, p = g % npts;
+  LF_CP(ncols, npts) {
+    const long g = c * npts + p;
     const long q = (c / K) * npts + p;
     const double a = ua[g] - ubt[q] + uba[q];
     const double b = va[g] - vbt[q] + vba[q];
@@ -332,9 +343,9 @@ __global__ void k_lf_final_uv(double* __restrict__ ua, double* __restrict__ va, 
 }
 
 // T_A = H_0 T_A / H_A (DOGCM_Phys_hspm_vfvm_mod.f90:641)
-__global__ void k_lf_trc_final(double* __restrict__ ta, const double* __restrict__ h, long npts, int K, long n) {
-  LF_LOOP(n) {
-    const long c = g / npts, p = g % npts;
+__global__ void k_lf_trc_final(double* __restrict__ ta, const double* __restrict__ h, long npts, int K, long ncols) {
+  LF_CP(ncols, npts) {
+    const long g = c * npts + p;
     const long q = ((c / K) / 2 * K + c % K) * npts + p;
     ta[g] = h[q] * ta[g] / h[q];
   }
@@ -412,6 +423,12 @@ void lf_free(dsht_lfam3* s) {
     hipLaunchKernelGGL(kernel, lf_grid(n), dim3(LF_THREADS), 0, s, __VA_ARGS__);         \
     DSHT_HIP(hipGetLastError());                                                         \
   } while (0)
+// kernels walking (column, point) pairs with LF_CP
+#define LF_LAUNCH2(kernel, ncols, npts, s, ...)                                          \
+  do {                                                                                   \
+    hipLaunchKernelGGL(kernel, lf_grid2(ncols, npts), dim3(LF_THREADS), 0, s, __VA_ARGS__); \
+    DSHT_HIP(hipGetLastError());                                                         \
+  } while (0)
 
 // ---- the reference's diagnostics on every active run (column kernels)
 int lf_uvbarot(dsht_lfam3* s, const double* u, const double* v, double* ub, double* vb, int nr, hipStream_t st) {
@@ -445,8 +462,7 @@ int lf_hydpres(dsht_lfam3* s, int nr, hipStream_t st) {
 // runs: vorticity into VD[0, nc), divergence into VD[nc, 2 nc)
 int lf_vordiv(dsht_lfam3* s, const double* u, const double* v, int nc, hipStream_t st) {
   dsht_plan* p = s->p;
-  const long n = (long)nc * s->np;
-  LF_LAUNCH(k_lf_uvcos, n, st, u, v, s->d_cs, s->np, s->I, n, s->ucos, s->vcos);
+  LF_LAUNCH2(k_lf_uvcos, nc, s->np, st, u, v, s->d_cs, s->np, s->I, (long)nc, s->ucos, s->vcos);
   if (int rc = dsht_uv2vordiv_d(p, s->ucos, s->vcos, s->np, s->wVD, s->wVD + nc, 2l * nc, 1, nc, nullptr, st))
     return rc;
   return dsht_inv_d(p, DSHT_VAL, s->wVD, 2l * nc, 1, s->VD, s->np, 2 * nc, nullptr, st);
@@ -490,12 +506,12 @@ int lf_mom_rhs(dsht_lfam3* s, const double* u, const double* v, const double* co
   a.am = s->Am;
   a.bg = s->Bg;
   a.cg = s->Cg;
-  LF_LAUNCH(k_lf_mom, n, st, a, n);
+  LF_LAUNCH2(k_lf_mom, nc, s->np, st, a, (long)nc);
   // (w_Vor_RHS, w_Div_RHS) = (-H(A, B), H(B, -A)) = calc_UVCosLat2VorDiv(B, -A); then
   // w_Div_RHS - w_Lapla_w(w_xy(C)) = + n(n+1) w_xy(C)
   if (int rc = dsht_uv2vordiv_d(p, s->Bg, s->Am, s->np, s->wVD, s->wVD + nc, 2l * nc, 1, nc, nullptr, st)) return rc;
   if (int rc = dsht_fwd_d(p, DSHT_VAL, s->Cg, s->np, s->wC, nc, 1, nc, s->d_nn1, st)) return rc;
-  LF_LAUNCH(k_lf_spec_axpy, (long)s->L * nc, st, s->wVD + nc, 2l * nc, s->wC, (long)nc, s->L, nc, 1.0, 0);
+  LF_LAUNCH2(k_lf_spec_axpy, s->L, nc, st, s->wVD + nc, 2l * nc, s->wC, (long)nc, s->L, nc, 1.0, 0);
   return dsht_vordiv2uv_d(p, s->wVD, s->wVD + nc, 2l * nc, 1, s->URHS, s->VRHS, s->np, nc, st);
 }
 
@@ -511,11 +527,11 @@ int lf_htrc(dsht_lfam3* s, const double* ts, const double* u, const double* v, c
                                   s->FLX + rt * (K + 1) * np, np, (int)np, st))
       return rc;
   }
-  const long n = (long)nt * np;
-  LF_LAUNCH(k_lf_htrc_grid, n, st, ts, u, v, h, s->d_cs, s->FLX, s->d_rcdk, np, s->I, s->K, n, s->FU, s->FV, s->VF);
+  LF_LAUNCH2(k_lf_htrc_grid, nt, np, st, ts, u, v, h, s->d_cs, s->FLX, s->d_rcdk, np, s->I, s->K, (long)nt, s->FU, s->FV,
+             s->VF);
   if (int rc = dsht_alpha_fwd_d(p, s->FU, s->FV, np, s->w1, nt, 1, nt, nullptr, st)) return rc;
   if (int rc = dsht_fwd_d(p, DSHT_VAL, s->VF, np, s->w2, nt, 1, nt, nullptr, st)) return rc;
-  LF_LAUNCH(k_lf_htrc_spec, (long)s->L * nt, st, s->w1, s->w2, s->d_advf, s->L, nt);
+  LF_LAUNCH2(k_lf_htrc_spec, s->L, nt, st, s->w1, s->w2, s->d_advf, s->L, nt);
   return dsht_inv_d(p, DSHT_VAL, s->w1, nt, 1, s->HT, np, nt, nullptr, st);
 }
 
@@ -706,8 +722,8 @@ int dsht_lfam3_create(dsht_plan* p, const dsht_lfam3_config* cfg, dsht_lfam3** o
     if (!rc) rc = fill(s->SP[lv], 0.0, R * np);
   }
   if (!rc) {
-    hipLaunchKernelGGL(k_lf_geo, lf_grid(NC * np), dim3(LF_THREADS), 0, st, s->Geo, np, K, s->d_zck, c.depth, c.grav,
-                       NC * np);
+    hipLaunchKernelGGL(k_lf_geo, lf_grid2(NC, np), dim3(LF_THREADS), 0, st, s->Geo, np, K, s->d_zck, c.depth, c.grav,
+                       (long)NC);
     if (hipGetLastError() != hipSuccess) rc = fail(DSHT_EDEVICE, "LF-AM3 pool: geopotential fill failed");
   }
   // lateral viscosity / diffusion factors 1/(1 - dt w_HViscCoefH), 1/(1 - dt w_HDiffCoefH)
@@ -821,19 +837,19 @@ int dsht_lfam3_step(dsht_lfam3* s, int nr, void* stream) {
   ea.ref_salt = c.ref_salt;
   ea.depth = c.depth;
   ea.grav = c.grav;
-  LF_LAUNCH(k_lf_eos, n, st, TN, np, K, s->d_zck, ea, n, s->Dens);
+  LF_LAUNCH2(k_lf_eos, nc, np, st, TN, np, K, s->d_zck, ea, (long)nc, s->Dens);
   if ((rc = lf_hydpres(s, nr, st))) return rc;
   if ((rc = lf_mom_rhs(s, UN, VN, UN, UB, VN, VB, gamma, 1.0 - gamma, PN, nc, st))) return rc;
-  LF_LAUNCH(k_lf_pred_uv, n, st, s->URHS, s->VRHS, UN, UB, VN, VB, s->d_two_omega, s->d_mu, np, I, K, 0.5 * (2.0 * dt),
-            c56dt, c.alpha > 0.0 ? 1 : 0, n, s->Upr, s->Vpr);
+  LF_LAUNCH2(k_lf_pred_uv, nc, np, st, s->URHS, s->VRHS, UN, UB, VN, VB, s->d_two_omega, s->d_mu, np, I, K,
+             0.5 * (2.0 * dt), c56dt, c.alpha > 0.0 ? 1 : 0, (long)nc, s->Upr, s->Vpr);
   if ((rc = lf_uvbarot(s, s->Upr, s->Vpr, b2[UBT], b2[VBT], nr, st))) return rc;
-  LF_LAUNCH(k_lf_barot_shift, n, st, s->Upr, s->Vpr, b2[UBT], b2[VBT], b2[UBN], b2[VBN], np, K, n);
+  LF_LAUNCH2(k_lf_barot_shift, nc, np, st, s->Upr, s->Vpr, b2[UBT], b2[VBT], b2[UBN], b2[VBN], np, K, (long)nc);
   if ((rc = lf_htrc(s, TN, UN, VN, s->H, nr, st))) return rc;
-  LF_LAUNCH(k_lf_pred_trc, ntr, st, TN, TB, s->HT, s->H, np, K, c56dt, ntr, s->TSpr);
+  LF_LAUNCH2(k_lf_pred_trc, nt, np, st, TN, TB, s->HT, s->H, np, K, c56dt, (long)nt, s->TSpr);
   // ---- corrector
   if ((rc = lf_vordiv(s, s->Upr, s->Vpr, nc, st))) return rc;
   if ((rc = lf_omega(s, s->VD + (long)nc * np, nr, st))) return rc;
-  LF_LAUNCH(k_lf_eos, n, st, s->TSpr, np, K, s->d_zck, ea, n, s->Dens);
+  LF_LAUNCH2(k_lf_eos, nc, np, st, s->TSpr, np, K, s->d_zck, ea, (long)nc, s->Dens);
   if ((rc = lf_hydpres(s, nr, st))) return rc;
   const double alpha_ = 0.5, gamma_ = 0.5;
   const double cn = gamma_ + alpha_, cb = 1.0 - alpha_ - gamma_;
@@ -851,38 +867,37 @@ int dsht_lfam3_step(dsht_lfam3* s, int nr, void* stream) {
                                (int)np, st)))
       return rc;
   }
-  LF_LAUNCH(k_lf_uvcos, n, st, UA, VA, s->d_cs, np, I, n, s->ucos, s->vcos);
+  LF_LAUNCH2(k_lf_uvcos, nc, np, st, UA, VA, s->d_cs, np, I, (long)nc, s->ucos, s->vcos);
   if ((rc = dsht_uv2vordiv_d(p, s->ucos, s->vcos, np, s->wVD, s->wVD + nc, 2l * nc, 1, nc, s->d_dmom, st))) return rc;
   if ((rc = dsht_vordiv2uv_d(p, s->wVD, s->wVD + nc, 2l * nc, 1, UA, VA, np, nc, st))) return rc;
   LF_LAUNCH(k_lf_rhs_diff, n, st, UA, UN, VA, VN, dt, n, s->URHS, s->VRHS);
   if ((rc = lf_uvbarot(s, s->URHS, s->VRHS, b2[UBF], b2[VBF], nr, st))) return rc;
-  LF_LAUNCH(k_lf_baroc_force, n2, st, b2[UBF], b2[VBF], b2[UBN], b2[VBN], b2[UBB], b2[VBB], s->d_two_omega, s->d_mu,
-            np, I, cn, cb, n2);
-  LF_LAUNCH(k_lf_cori_step, n, st, s->URHS, s->VRHS, UN, VN, s->d_two_omega, s->d_mu, np, I, K, alpha_ * dt, dt, 0, n,
-            UA, VA);
+  LF_LAUNCH2(k_lf_baroc_force, nr, np, st, b2[UBF], b2[VBF], b2[UBN], b2[VBN], b2[UBB], b2[VBB], s->d_two_omega,
+             s->d_mu, np, I, cn, cb, (long)nr);
+  LF_LAUNCH2(k_lf_cori_step, nc, np, st, s->URHS, s->VRHS, UN, VN, s->d_two_omega, s->d_mu, np, I, K, alpha_ * dt, dt, 0,
+             (long)nc, UA, VA);
   // ---- barotropic mode: one iteration with dts = dt / SubCycleNum (rigid lid)
   const double dts = dt / c.sub_cycles;
-  LF_LAUNCH(k_lf_barot_rhs_grid, n2, st, b2[UBN], b2[VBN], b2[UBF], b2[VBF], PN, PN, s->d_two_omega, s->d_mu, s->d_cs,
-            np, I, n2, b2[AM2], b2[BG2], b2[SPM]);
+  LF_LAUNCH2(k_lf_barot_rhs_grid, nr, np, st, b2[UBN], b2[VBN], b2[UBF], b2[VBF], PN, PN, s->d_two_omega, s->d_mu,
+             s->d_cs, np, I, (long)nr, b2[AM2], b2[BG2], b2[SPM]);
   if ((rc = dsht_uv2vordiv_d(p, b2[BG2], b2[AM2], np, s->wB, s->wB + nr, 2l * nr, 1, nr, nullptr, st))) return rc;
   if ((rc = dsht_fwd_d(p, DSHT_VAL, b2[SPM], np, s->wC2, nr, 1, nr, s->d_nn1, st))) return rc;
-  LF_LAUNCH(k_lf_spec_axpy, (long)L * nr, st, s->wB + nr, 2l * nr, s->wC2, (long)nr, L, nr,
-            c.ref_dens * (p->R * p->R), 1);
+  LF_LAUNCH2(k_lf_spec_axpy, L, nr, st, s->wB + nr, 2l * nr, s->wC2, (long)nr, L, nr, c.ref_dens * (p->R * p->R), 1);
   if ((rc = dsht_vordiv2uv_d(p, s->wB, s->wB + nr, 2l * nr, 1, b2[UBR], b2[VBR], np, nr, st))) return rc;
-  LF_LAUNCH(k_lf_cori_step, n2, st, b2[UBR], b2[VBR], b2[UBN], b2[VBN], s->d_two_omega, s->d_mu, np, I, 1,
-            alpha_ * dts, dts, 1, n2, b2[UBAR], b2[VBAR]);
+  LF_LAUNCH2(k_lf_cori_step, nr, np, st, b2[UBR], b2[VBR], b2[UBN], b2[VBN], s->d_two_omega, s->d_mu, np, I, 1,
+             alpha_ * dts, dts, 1, (long)nr, b2[UBAR], b2[VBAR]);
   // HBEBarot_Update_LinFreeSfc, rigid lid: project onto the divergence-free part
-  LF_LAUNCH(k_lf_uvcos, n2, st, b2[UBAR], b2[VBAR], s->d_cs, np, I, n2, b2[AM2], b2[BG2]);
+  LF_LAUNCH2(k_lf_uvcos, nr, np, st, b2[UBAR], b2[VBAR], s->d_cs, np, I, (long)nr, b2[AM2], b2[BG2]);
   if ((rc = dsht_uv2vordiv_d(p, b2[AM2], b2[BG2], np, s->wB, s->wB + nr, 2l * nr, 1, nr, nullptr, st))) return rc;
-  LF_LAUNCH(k_lf_barot_proj, (long)L * nr, st, s->wB + nr, 2l * nr, s->d_hcoef, s->d_e, c.ref_dens, alpha_ * dts, L,
-            nr, s->wDSP, s->wDDiv);
+  LF_LAUNCH2(k_lf_barot_proj, L, nr, st, s->wB + nr, 2l * nr, s->d_hcoef, s->d_e, c.ref_dens, alpha_ * dts, L, nr,
+             s->wDSP, s->wDDiv);
   if ((rc = dsht_inv_d(p, DSHT_VAL, s->wDSP, nr, 1, b2[DSP], np, nr, nullptr, st))) return rc;
   if ((rc = dsht_vordiv2uv_d(p, s->wZero, s->wDDiv, nr, 1, b2[DU], b2[DV], np, nr, st))) return rc;
   LF_LAUNCH(k_lf_add2, n2, st, b2[UBAR], b2[DU], b2[VBAR], b2[DV], PN, b2[DSP], n2, b2[UBA], b2[VBA], PA);
   // ---- full velocity, time-centred fields, tracers
   if ((rc = lf_uvbarot(s, UA, VA, b2[UBT], b2[VBT], nr, st))) return rc;
-  LF_LAUNCH(k_lf_final_uv, n, st, UA, VA, b2[UBT], b2[VBT], b2[UBA], b2[VBA], UN, UB, VN, VB, s->H, np, K, n, s->Upr,
-            s->Vpr, s->Hm);
+  LF_LAUNCH2(k_lf_final_uv, nc, np, st, UA, VA, b2[UBT], b2[VBT], b2[UBA], b2[VBA], UN, UB, VN, VB, s->H, np, K, (long)nc,
+             s->Upr, s->Vpr, s->Hm);
   if ((rc = lf_vordiv(s, s->Upr, s->Vpr, nc, st))) return rc;
   if ((rc = lf_omega(s, s->VD + (long)nc * np, nr, st))) return rc;  // from H_N
   if ((rc = lf_htrc(s, s->TSpr, s->Upr, s->Vpr, s->Hm, nr, st))) return rc;
@@ -895,7 +910,7 @@ int dsht_lfam3_step(dsht_lfam3* s, int nr, void* stream) {
   }
   if ((rc = dsht_fwd_d(p, DSHT_VAL, TA, np, s->w1, nt, 1, nt, s->d_dtrc, st))) return rc;
   if ((rc = dsht_inv_d(p, DSHT_VAL, s->w1, nt, 1, TA, np, nt, nullptr, st))) return rc;
-  LF_LAUNCH(k_lf_trc_final, ntr, st, TA, s->H, np, K, ntr);
+  LF_LAUNCH2(k_lf_trc_final, nt, np, st, TA, s->H, np, K, (long)nt);
   // ---- time levels
   LF_LAUNCH(k_lf_rotate, n, st, UB, UN, UA, n);
   LF_LAUNCH(k_lf_rotate, n, st, VB, VN, VA, n);
