@@ -563,12 +563,14 @@ def lfam3_bench(torch, dsht, steps=20, N=NMAX, km=KM):
 
 def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E, factor=1.8):
     """Poisson arrivals of independent LF-AM3 runs (truncation, KM, rotation rate drawn per
-    run), served three ways:
-      batched          continuous batching, one pool step on the GPU at a time (pools take
-                       turns), admission capped so a pool's step stays within `factor` x its
-                       one-run step time;
-      batched_uncapped the same without the latency cap;
-      one_at_a_time    the runs in arrival order, each alone.
+    run), served four ways:
+      batched            continuous batching, one pool step on the GPU at a time (pools take
+                         turns), admission capped so a pool's step stays within `factor` x
+                         its one-run step time;
+      batched_concurrent the same cap, every pool stepping on its own stream as soon as its
+                         previous step is done (pools share the GPU);
+      batched_uncapped   one pool at a time without the latency cap;
+      one_at_a_time      the runs in arrival order, each alone.
     Per-run p99 step latency is compared with the median step time of that (truncation, KM)
     run alone."""
     from dsht import serve
@@ -578,8 +580,9 @@ def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E, fac
 
     kinds = sorted({(r.nmax, r.km) for r in stream()})
     out = {}
-    for mode in ("batched", "batched_uncapped", "one_at_a_time"):
-        srv = serve.Server(latency_factor=factor if mode == "batched" else None)
+    for mode in ("batched", "batched_concurrent", "batched_uncapped", "one_at_a_time"):
+        srv = serve.Server(latency_factor=factor if mode in ("batched", "batched_concurrent") else None,
+                           serialize=mode != "batched_concurrent")
         # every mode builds (and, capped, calibrates) its pools and captures their step graphs
         # before the clock starts
         for n, km in kinds:
@@ -589,6 +592,13 @@ def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E, fac
         print(f"[bench] serving: {mode}", file=sys.stderr, flush=True)
         out[mode] = srv.run(reqs, batched=(mode != "one_at_a_time"))
         out[mode]["runs"] = reqs
+        if mode == "batched":
+            # throughput bound of batching on this mix: every run's one-run step time against
+            # its marginal cost in a full pool (t(n) = a + b n, calibrated per pool)
+            t1 = sum(srv.pools[(r.nmax, r.km)].t_ref for r in reqs)
+            tb = sum(srv.pools[(r.nmax, r.km)].t_per_run for r in reqs)
+            marginal = {f"T{n}L{km}": {"one_run_ms": pl.t_ref, "per_added_run_ms": pl.t_per_run}
+                        for (n, km), pl in srv.pools.items()}
         srv = None
         torch.cuda.synchronize()
     # each (truncation, KM) alone: the latency reference
@@ -616,10 +626,14 @@ def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E, fac
                     f"{list(serve.GRIDS)}, KM from [20, 30, 60], rotation 0.25-4x Earth; FP64; one step = "
                     f"(42 KM + 9) forward + (28 KM + 9) inverse units per run",
         "batched": b,
+        "batched_concurrent": res["batched_concurrent"],
         "batched_uncapped": res["batched_uncapped"],
         "one_at_a_time": u,
         "throughput_ratio": b["pairs_per_s"] / u["pairs_per_s"],
+        "throughput_ratio_concurrent": res["batched_concurrent"]["pairs_per_s"] / u["pairs_per_s"],
         "throughput_ratio_uncapped": res["batched_uncapped"]["pairs_per_s"] / u["pairs_per_s"],
+        "throughput_ratio_bound": t1 / tb,
+        "pool_step_model": marginal,
         "solo_step_ms": {f"T{n}L{km}": t for (n, km), t in solo.items()},
     }
 
@@ -645,6 +659,8 @@ def main():
                     help="only run the vertical column diagnostics section and print its JSON")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the batched-instances section and print its JSON")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only run the serving section and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: the transform path has no "
@@ -675,6 +691,10 @@ def main():
     if args.batch_only:
         import dsht
         print(json.dumps({"batched": batched_bench(torch, dsht)}))
+        return
+    if args.serve_only:
+        import dsht
+        print(json.dumps({"serving": serving_bench(torch, dsht)}))
         return
 
     import dsht

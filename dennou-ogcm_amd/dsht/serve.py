@@ -116,6 +116,7 @@ class Pool:
         self.bytes_per_run = 8 * (41 * km * plan.im * plan.jm + 12 * km * plan.lm)
         self.runs = []
         self.t_ref = None
+        self.t_per_run = None
         torch.cuda.synchronize(dev)
         # one full-width step sizes every workspace of the pool's stream (captures allocate
         # nothing afterwards)
@@ -139,6 +140,7 @@ class Pool:
         b = max((ts[1] - ts[0]) / max(self.cap - 1, 1), 1e-9)
         a = ts[0] - b
         self.t_ref = ts[0]
+        self.t_per_run = b  # marginal step time of one more run in a full pool
         self.max_runs = max(1, min(self.cap, int((factor * ts[0] - a) / b)))
         return self.t_ref, self.max_runs
 
