@@ -15,6 +15,11 @@ the pool's active runs, each transform of the step running over every layer of e
 one call.  Every run is computed as independent columns, so its result does not depend on
 what else is in the pool.
 
+A request may start from a restart file (`resume_from`: its state and rotation rate replace
+the seeded initial state) and may leave one when it finishes (`save_to`), in dogcm's restart
+layout (dsht/restart.py; DOGCM_IO_Restart_mod.f90), so a served run can be stopped and resumed
+later, in another pool or process.
+
 A step makes (42 KM + 9) forward and (28 KM + 9) inverse transform units per run
 (dsht_lfam3_units); pairs/s = units / 2 / s.
 """
@@ -34,10 +39,12 @@ CAPACITY = {21: 16, 42: 16, 85: 8, 170: 6}
 
 
 class Request:
-    def __init__(self, rid, nmax, km, omega_scale, nsteps, arrival, seed):
+    def __init__(self, rid, nmax, km, omega_scale, nsteps, arrival, seed, resume_from=None, save_to=None):
         self.rid, self.nmax, self.km = rid, nmax, km
         self.omega = EARTH_OMEGA * omega_scale
         self.nsteps, self.arrival, self.seed = nsteps, arrival, seed
+        self.resume_from, self.save_to = resume_from, save_to
+        self.model_time = 0.0  # seconds since the run's start (from the restart file if resumed)
         self.steps_done = 0
         self.admitted = None
         self.finished = None
@@ -173,9 +180,27 @@ class Pool:
         import torch
         req.slot = len(self.runs)
         self.runs.append(req)
+        if req.resume_from:
+            from .restart import read_restart, load_run
+            self.stream.synchronize()
+            with torch.cuda.stream(self.stream):
+                req.model_time = load_run(self.lf, req.slot, req.resume_from)
+            attrs = read_restart(req.resume_from)[3]
+            if "omega" in attrs:
+                req.omega = float(attrs["omega"])
+            else:
+                self.lf.set_rotation(req.slot, req.omega)
+            return
         with torch.cuda.stream(self.stream):
             initial_state(self, req.slot, req.seed)
         self.lf.set_rotation(req.slot, req.omega)
+
+    def save(self, req):
+        """Writes req's restart file (its state now, both time levels)."""
+        from .restart import save_run
+        self.stream.synchronize()
+        save_run(self.lf, req.slot, req.save_to, time=req.model_time + req.steps_done * self.lf.cfg.dt,
+                 omega=req.omega)
 
     def retire(self, req):
         """Frees req's slot; the pool's last run moves into it (state and rotation rate)."""
@@ -274,6 +299,8 @@ class Server:
                         if on_finish:
                             torch.cuda.current_stream().wait_stream(self.pools[key].stream)
                             on_finish(r, self.pools[key])
+                        if r.save_to:
+                            self.pools[key].save(r)
                         self.retire(r, tnow)
                         active.remove(r)
             # admission (first fit over the queue; one run at a time when unbatched)

@@ -96,3 +96,23 @@ def test_admission_by_memory_budget(gpu):
     admitted = sorted(r.admitted for r in reqs)
     assert admitted[2] > admitted[1]  # the third run waited for a slot
     assert all(r.steps_done == 3 for r in reqs)
+
+
+def test_save_and_resume_through_the_server(gpu, tmp_path):
+    """A run served for 3 steps that leaves a restart file, resumed by another server for 2
+    more steps, ends where the same run served 5 steps at once ends (relative L2 <= 1e-13);
+    the resumed request takes its rotation rate and model time from the file."""
+    from dsht import serve
+    from dsht.serve import Request, Server
+    final = {}
+    r0 = Request(0, 21, 4, 1.5, 5, 0.0, 77)
+    Server(capacity=CAP).run([r0], batched=True, on_finish=_keep(final, "full"))
+    path = str(tmp_path / "run.nc")
+    r1 = Request(1, 21, 4, 1.5, 3, 0.0, 77, save_to=path)
+    Server(capacity=CAP).run([r1], batched=True)
+    r2 = Request(2, 21, 4, 0.3, 2, 0.0, 5, resume_from=path)
+    Server(capacity=CAP).run([r2], batched=True, on_finish=_keep(final, "resumed"))
+    assert r2.model_time == 3 * serve.DT[21] and r2.omega == r1.omega
+    for a, b in zip(final[("resumed", 2)], final[("full", 0)]):
+        assert np.all(np.isfinite(a))
+        assert rel_l2(a, b) <= 1e-13, rel_l2(a, b)
