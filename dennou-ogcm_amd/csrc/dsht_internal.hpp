@@ -185,8 +185,8 @@ struct dsht_plan {
   int fft_order = 1;   // register FFT row-group order: 0 packed, 1 latitude-major (default)
   int leg_kind = 0;    // contractions.  FP64: 0 = LDS-staged inverse, fragment-stream forward;
                        // 1/3 = LDS-staged; 2 = per-wave fragment streams.
-                       // Split-form inverse: 0 = per form (whole-K LDS staging for FP16X3,
-                       // packed data panels otherwise), 2 = packed panels, 3 = whole-K staging
+                       // Split-form inverse: 0 = per form (whole-K LDS staging for BF16X6 and
+                       // FP16X3, packed data panels otherwise), 2 = packed panels, 3 = whole-K
 };
 
 namespace dsht {

@@ -323,16 +323,18 @@ int mixed_run_fwd(dsht_plan* p, int op, const double* in0, const double* in1, lo
     j.diag0 = diag0;
     j.diag1 = diag1;
     j.scale = scale;
+    j.staged = p->leg_kind == 3;
     if ((rc = mx2_fwd(p, t, form, op, j, s))) return rc;
   }
   return 0;
 }
 
-// whole-K LDS staging for the inverse: per form by default (measured at T170L60, fused
-// value/gradient call: FP16X3 76 against 91 us packed; the BF16 forms and FP16 are faster
-// packed), forced by DSHT_TUNE_LEG = 3, never where its LDS panel does not fit
+// whole-K LDS staging for the inverse: per form by default (measured in the T170L60 step with
+// FP32 grids: BF16X6 0.360 against 0.366 ms per step packed, FP16X3 0.360 against 0.377; BF16X3
+// and FP16 contractions alone are faster packed), forced by DSHT_TUNE_LEG = 3, never where its
+// LDS panel does not fit
 static bool inv_staged(const dsht_plan* p, int form, int op) {
-  const bool want = p->leg_kind == 3 || (p->leg_kind == 0 && form == 3);
+  const bool want = p->leg_kind == 3 || (p->leg_kind == 0 && (form == 1 || form == 3));
   return want && mx2_inv_staged_lds(p, form, op) <= MX2_STAGED_LDS_MAX;
 }
 

@@ -111,6 +111,7 @@ struct Mx2FwdJob {
   const double* diag0;
   const double* diag1;
   double scale;
+  bool staged = false;      // whole-K staged forward (k_mx2_fwd_wk) where its LDS panel fits
 };
 
 size_t mx2_inv_pack_bytes(const MixedTables& t, int form, int op, int ncols);

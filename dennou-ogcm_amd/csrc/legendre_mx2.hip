@@ -749,6 +749,199 @@ __global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
   }
 }
 
+// Forward with every latitude of the block's columns staged once in LDS.  A block covers one
+// wavenumber m and NCT column tiles; its threads load the north and south Fourier values of all
+// its K steps (an item is 8 latitudes of one column), form the symmetric and antisymmetric
+// combinations, split them and store them as B fragments in LDS behind one barrier.  The row
+// tiles of both parities are then dealt round robin to the waves; each wave walks (tile, K
+// step) pairs with the table fragments of the next pair in flight, reads its data fragments
+// from LDS (the combination its tile's parity needs) and runs the epilogue when a tile's K
+// loop ends, with no further barrier.  Same tables and epilogue formulas as k_mx2_fwd_sh.
+template <int OP, int FORM, int NCT>
+__host__ __device__ constexpr int fwk_panel_elems() {  // one K step: [tile][input][sym, anti][re, im][term]
+  return NCT * FwdTraits<OP>::NIN * 2 * 2 * Mx2<FORM>::NTB * 512;
+}
+
+template <int OP, int FORM, int NCT>
+__global__ __launch_bounds__(256, 2) void k_mx2_fwd_wk(Mx2FwdArgs a) {
+  using F = MxForm<FORM>;
+  using X = Mx2<FORM>;
+  using T = typename F::T;
+  using V8 = typename F::V8;
+  constexpr int NG = fwd_ng(OP), NIN = FwdTraits<OP>::NIN, NOUT = FwdTraits<OP>::NOUT;
+  constexpr int NTA = X::NTA, NTB = X::NTB;
+  constexpr bool UP = FwdTraits<OP>::USE_P, UH = FwdTraits<OP>::USE_H;
+  constexpr int CB = NCT * TILE;
+  constexpr int PANEL = fwk_panel_elems<OP, FORM, NCT>();
+  T* sB = reinterpret_cast<T*>(mx2_lds);
+  int m, cg;
+  if (!xcd_decode(a.N, cdiv(a.CT, NCT), m, cg)) return;  // uniform over the block
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int nE = n_par(a.N, m, 0), nO = n_par(a.N, m, 1);
+  const int tE = cdiv(nE, TILE), tO = cdiv(nO, TILE);
+  const int KS = a.J2p32 / KM;
+  const long toff = a.off[m];
+  const float2* Gm[2] = {a.G0 + (long)m * a.gmst, NIN > 1 ? a.G1 + (long)m * a.gmst : nullptr};
+  const double dm = (double)m;
+
+  // ---- staging: item it -> K step it / IPS; within it (column cc, k-group kg, input i), cc fastest
+  {
+    const int cc = tid % CB;  // 256 is a multiple of CB: a thread keeps its column
+    const int c = cg * CB + cc;
+    const bool cok = c < a.ncols;
+    float dsc = 1.f;
+    if constexpr (F::SCALED) {
+      if (cok) dsc = colscale<NIN>(a.colmax, a.ncols, c, a.bound);
+    }
+    constexpr int IPS = NIN * CB * 4;
+    const int total = KS * IPS;
+    for (int it = tid; it < total; it += 256) {
+      const int ks = it / IPS, rest = (it % IPS) / CB;
+      const int kg = rest & 3, i = rest >> 2;
+      const float2* g = Gm[i] + c;
+      float2 gn[8], gs[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int jn = ks * KM + 8 * kg + e;
+        gn[e] = gs[e] = make_float2(0.f, 0.f);
+        if (cok && jn < a.J2) {
+          gn[e] = g[(long)(a.J2 + jn) * a.gld];
+          gs[e] = g[(long)(a.J2 - 1 - jn) * a.gld];
+        }
+      }
+      T ts[2][2][NTB][8];  // [sym, anti][re, im][term][latitude]
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float cw = a.cw[ks * KM + 8 * kg + e] * dsc;
+        const float v[2][2] = {{cw * (gn[e].x + gs[e].x), cw * (gn[e].y + gs[e].y)},
+                               {cw * (gn[e].x - gs[e].x), cw * (gn[e].y - gs[e].y)}};
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+          for (int r = 0; r < 2; r++) {
+            T tmp[NTB];
+            bsplit<FORM>(v[q][r], tmp);
+#pragma unroll
+            for (int t = 0; t < NTB; t++) ts[q][r][t][e] = tmp[t];
+          }
+      }
+      T* dst = sB + (long)ks * PANEL + (((cc / TILE) * NIN + i) * 4 * NTB) * 512 + ((cc % TILE) + TILE * kg) * 8;
+#pragma unroll
+      for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+#pragma unroll
+          for (int t = 0; t < NTB; t++) {
+            V8 vv;
+#pragma unroll
+            for (int e = 0; e < 8; e++) vv[e] = ts[q][r][t][e];
+            *reinterpret_cast<V8*>(dst + ((q * 2 + r) * NTB + t) * 512) = vv;
+          }
+    }
+  }
+  __syncthreads();
+
+  // ---- contraction: wave w takes row tiles w, w + 4, ... (both parities, tile index over
+  // parity 0 then parity 1 as in the table); (tile, K step) pairs s = q KS + ks, q its q-th tile
+  const int ntile = tE + tO;
+  const int mytiles = w < ntile ? (ntile - w + 3) / 4 : 0;
+  const int nseq = mytiles * KS;
+  float4_t acc[NG][NCT][2];
+#pragma unroll
+  for (int g = 0; g < NG; g++)
+#pragma unroll
+    for (int t = 0; t < NCT; t++) acc[g][t][0] = acc[g][t][1] = float4_t{0, 0, 0, 0};
+  auto loadA = [&](int sq, V8 (&A)[2][NTA]) {
+    const int tile = w + 4 * (sq / KS), ks = sq % KS;
+    const long o = toff + ((long)tile * KS + ks) * 512 + lane * 8;
+#pragma unroll
+    for (int k = 0; k < NTA; k++) {
+      if constexpr (UP) A[0][k] = *reinterpret_cast<const V8*>(a.tP[k] + o);
+      if constexpr (UH) A[1][k] = *reinterpret_cast<const V8*>(a.tH[k] + o);
+    }
+  };
+  auto epilogue = [&](int tile) {
+    const int PP = tile >= tE ? 1 : 0, tp = PP ? tile - tE : tile;
+#pragma unroll
+    for (int t = 0; t < NCT; t++) {
+      const int c = (cg * NCT + t) * TILE + (lane & 15);
+      if (c >= a.ncols) continue;
+      double u = a.minv[m] * X::DSCALE * a.scale;
+      if constexpr (F::SCALED) u /= (double)colscale<NIN>(a.colmax, a.ncols, c, a.bound);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int k = tp * TILE + acc_row16(lane, i);
+        const int n = m + PP + 2 * k;
+        if (n > a.N) continue;
+        double xr[NG], xi[NG];
+#pragma unroll
+        for (int gi = 0; gi < NG; gi++) {
+          xr[gi] = acc[gi][t][0][i];
+          xi[gi] = acc[gi][t][1][i];
+        }
+        double orr[NOUT], oii[NOUT];
+        if constexpr (OP == F_VAL) {
+          orr[0] = xr[0], oii[0] = xi[0];
+        } else if constexpr (OP == F_DIVLAMBDA) {  // i m P.G
+          orr[0] = -dm * xi[0], oii[0] = dm * xr[0];
+        } else if constexpr (OP == F_DIVMU) {  // -H.G
+          orr[0] = -xr[0], oii[0] = -xi[0];
+        } else if constexpr (OP == F_ALPHA) {  // i m P.A - H.B
+          orr[0] = -dm * xi[0] - xr[1], oii[0] = dm * xr[0] - xi[1];
+        } else {  // vor = i m P.V + H.U ; div = i m P.U - H.V
+          orr[0] = -dm * xi[0] + xr[1], oii[0] = dm * xr[0] + xi[1];
+          orr[1] = -dm * xi[2] - xr[3], oii[1] = dm * xr[2] - xi[3];
+        }
+        const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
+#pragma unroll
+        for (int o = 0; o < NOUT; o++) {
+          const double* dg = o == 0 ? a.diag0 : a.diag1;
+          double* W = o == 0 ? a.W0 : a.W1;
+          __builtin_nontemporal_store(orr[o] * u * (dg ? dg[lre] : 1.0), W + lre * a.sl + (long)c * a.sc);
+          if (m > 0) __builtin_nontemporal_store(oii[o] * u * (dg ? dg[lim] : 1.0), W + lim * a.sl + (long)c * a.sc);
+        }
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+#pragma unroll
+      for (int t = 0; t < NCT; t++) acc[g][t][0] = acc[g][t][1] = float4_t{0, 0, 0, 0};
+  };
+  auto step = [&](int sq, const V8 (&A)[2][NTA]) {
+    const int tile = w + 4 * (sq / KS), ks = sq % KS;
+    const int PP = tile >= tE ? 1 : 0;
+    const T* pan = sB + (long)ks * PANEL + lane * 8;
+#pragma unroll
+    for (int gi = 0; gi < NG; gi++) {
+      const int tb = fwd_tab(OP, gi), in = fwd_in(OP, gi);
+      const int cq = tb == 0 ? PP : 1 - PP;  // 0 = sym, 1 = anti
+#pragma unroll
+      for (int t = 0; t < NCT; t++)
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+          V8 B[NTB];
+#pragma unroll
+          for (int k = 0; k < NTB; k++)
+            B[k] = *reinterpret_cast<const V8*>(pan + (((t * NIN + in) * 4 + 2 * cq + r) * NTB + k) * 512);
+#pragma unroll
+          for (int pr = 0; pr < X::NP; pr++)
+            acc[gi][t][r] = mfma16(A[tb][X::pa(pr)], B[X::pb(pr)], acc[gi][t][r]);
+        }
+    }
+    if (ks == KS - 1) epilogue(tile);
+  };
+  V8 A0[2][NTA], A1[2][NTA];
+  if (nseq > 0) loadA(0, A0);
+  for (int sq = 0; sq < nseq; sq += 2) {
+    if (sq + 1 < nseq) loadA(sq + 1, A1);
+    step(sq, A0);
+    if (sq + 1 < nseq) {
+      if (sq + 2 < nseq) loadA(sq + 2, A0);
+      step(sq + 1, A1);
+    }
+  }
+}
+
 // per-column max |pro[l] w(l, c)| (float bits), threads along columns
 __global__ __launch_bounds__(256) void k_spec_colmax2(const double* __restrict__ w, long sl, long sc, int L,
                                                       int ncols, const double* __restrict__ pro,
@@ -903,6 +1096,24 @@ int fwd_launch(const dsht_plan* p, const MixedTables& t, const Mx2FwdJob& j, hip
   // accumulators; measured at T170 they pay only for the single-group bf16x6 form (6
   // products per data fragment: forward 101 -> 84 us).  The column-block knob forces 1 (16)
   // or 2 (32) tiles.
+  if (j.staged) {  // whole-K staged forward: one column tile per block (two with leg_cb = 32)
+    const int KS = a.J2p32 / KM;
+    const bool two = p->leg_cb == 32;
+    const size_t lds = (size_t)KS * sizeof(typename MxForm<FORM>::T) *
+                       (two ? fwk_panel_elems<OP, FORM, 2>() : fwk_panel_elems<OP, FORM, 1>());
+    if (lds <= MX2_STAGED_LDS_MAX) {
+      const long nbw = 8l * cdiv(p->N + 1, 8) * cdiv(a.CT, two ? 2 : 1);
+      auto k = two ? k_mx2_fwd_wk<OP, FORM, 2> : k_mx2_fwd_wk<OP, FORM, 1>;
+      if (lds > 64 * 1024) {  // above the default limit; once per kernel and process
+        static const hipError_t attr =
+            hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)MX2_STAGED_LDS_MAX);
+        if (attr != hipSuccess) return fail(DSHT_EDEVICE, "LDS size attribute: %s", hipGetErrorString(attr));
+      }
+      hipLaunchKernelGGL(k, dim3((unsigned)nbw), dim3(256), lds, s, a);
+      DSHT_HIP(hipGetLastError());
+      return 0;
+    }
+  }
   const int fnct = p->leg_cb == 16 ? 1 : (p->leg_cb == 32 ? 2 : (fwd_ng(OP) == 1 && Mx2<FORM>::NP == 6 ? 2 : 1));
   const long nb = 8l * cdiv(p->N + 1, 8) * cdiv(a.CT, fnct);
   if (fnct == 2)

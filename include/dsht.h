@@ -274,7 +274,7 @@ int dsht_coef_d(dsht_plan* plan, int which, const double* params, int nparams, d
 #define DSHT_TUNE_LEG 3        /* Legendre contractions.  FP64: 0 = LDS-staged inverse with the
                                   fragment-stream forward (default), 1 or 3 = LDS-staged, 2 = per-wave
                                   fragment streams.  Split-form inverse: 0 = per form (default:
-                                  whole-K LDS staging for FP16X3, packed data panels otherwise),
+                                  whole-K LDS staging for BF16X6 and FP16X3, packed data panels otherwise),
                                   2 = packed data panels, 3 = whole-K LDS staging; 1 (a retired
                                   design) returns DSHT_EUNSUPPORTED.  (Key 4 is retired.) */
 #define DSHT_TUNE_FFT_ORDER 5  /* row-group order of the register FFTs: 0 = packed (groups walk
