@@ -13,7 +13,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdsht.so")
 FDIR = os.path.join(HERE, "fortran")
 
-SOURCES = ["api.hip", "tables.hip", "fft.hip", "fft_reg.hip", "legendre_f64.hip", "legendre_f64d.hip", "legendre_mixed.hip", "legendre_mx2.hip", "ops.hip", "column.hip", "batch.hip", "lfam3.hip"]
+SOURCES = ["api.hip", "tables.hip", "fft.hip", "fft_reg.hip", "legendre_f64.hip", "legendre_f64d.hip", "legendre_mixed.hip", "legendre_mx2.hip", "ops.hip", "column.hip", "batch.hip", "lfam3.hip", "redigm.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIPFLAGS = [

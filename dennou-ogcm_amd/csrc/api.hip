@@ -487,7 +487,7 @@ int dsht_destroy(dsht_plan* p) {
     if (kv.second->buf) (void)hipFree(kv.second->buf);
   }
   double* ds[] = {p->d_mu, p->d_wq, p->d_wl, p->d_cwv, p->d_cwd, p->d_rs_icos, p->d_rs_alpha, p->d_rs_uv,
-                  p->d_pro_psi, p->d_tabFP, p->d_tabFH, p->d_tabIP, p->d_tabIH};
+                  p->d_pro_psi, p->d_lat_cs, p->d_tabFP, p->d_tabFH, p->d_tabIP, p->d_tabIH};
   for (double* d : ds)
     if (d) (void)hipFree(d);
   if (p->d_offF) (void)hipFree(p->d_offF);
@@ -1253,6 +1253,154 @@ int dsht_col_vdiff(dsht_plan* p, const double* q0, const double* rhs, const doub
                                 (long)IJ, dw, lw, c->s);
   if (rc) return rc;
   return d2h_sync(c, qa, dout, IJ * km);
+}
+
+// ---------------------------------------------------------------- Redi/GM mixing
+// (LPhys_RediGM_hspm_vfvm_AddMixingTerm, LPhys_RediGM_hspm_vfvm_mod.f90:218-372)
+int dsht_redigm_default_config(dsht_redigm_config* c) {
+  if (!c) return fail(DSHT_EARG, "NULL config");
+  c->kappa_redi = 800.0;
+  c->kappa_gm = 800.0;
+  c->slope_max = 4e-3;
+  c->sd = 1e-3;
+  c->interior_taper = DSHT_TAPER_DM95;
+  c->pbl_taper = DSHT_TAPER_LDD95;
+  c->omega = 2.0 * 3.1415926535897932 / (60.0 * 60.0 * 23.9345);
+  c->ref_dens = 1027.0;
+  c->grav = 9.79764;
+  return 0;
+}
+
+static int redigm_params(const dsht_redigm_config* c, RediParams* r) {
+  if (!c) return fail(DSHT_EARG, "NULL config");
+  if (c->interior_taper != DSHT_TAPER_DM95 && c->interior_taper != DSHT_TAPER_GKW91)
+    return fail(DSHT_EARG, "interior tapering must be DSHT_TAPER_DM95 or DSHT_TAPER_GKW91");
+  if (c->pbl_taper != DSHT_TAPER_LDD95)
+    return fail(DSHT_EUNSUPPORTED,
+                "boundary-layer tapering must be DSHT_TAPER_LDD95 (with the linear form the reference "
+                "multiplies the slopes by an undefined factor)");
+  r->kappa_redi = c->kappa_redi;
+  r->kappa_gm = c->kappa_gm;
+  r->slope_max = c->slope_max;
+  r->sd = c->sd;
+  r->omega = c->omega;
+  r->ref_dens = c->ref_dens;
+  r->grav = c->grav;
+  r->gkw91 = c->interior_taper == DSHT_TAPER_GKW91;
+  return 0;
+}
+
+// checks shared by the column part and the whole tendency; ldgr: pitch of the gradients
+static int redigm_check(dsht_plan* p, const dsht_redigm_config* cfg, const double* ptemp, const double* salt,
+                        const double* h, const double* z, long ldg, int km, const double* topo, const double* cdk,
+                        const double* rfdk, RediParams* r) {
+  if (int rc = check_col(p, km, cdk, ldg)) return rc;
+  if (km < 2) return fail(DSHT_EARG, "the Redi/GM mixing needs km >= 2");
+  if (!ptemp || !salt || !h || !z || !topo || !rfdk) return fail(DSHT_EARG, "pointer is NULL");
+  return redigm_params(cfg, r);
+}
+
+int dsht_col_redigm_d(dsht_plan* p, const dsht_redigm_config* cfg, const double* ptemp, const double* salt,
+                      const double* h, const double* z, long ldg, int km, const double* topo, const double* cdk,
+                      const double* rfdk, const double* grad_lon, const double* grad_lat, double* flx_lon,
+                      double* flx_lat, double* vdiv, long ldo, void* stream) {
+  RediParams r;
+  if (int rc = redigm_check(p, cfg, ptemp, salt, h, z, ldg, km, topo, cdk, rfdk, &r)) return rc;
+  if (!grad_lon || !grad_lat || !flx_lon || !flx_lat || !vdiv) return fail(DSHT_EARG, "pointer is NULL");
+  if (ldo < (long)p->I * p->J) return fail(DSHT_EARG, "output column pitch %ld must be >= I*J", ldo);
+  const long IJ = (long)p->I * p->J;
+  const long nl = (km - 1) * ldg + IJ, n2 = (2l * km - 1) * ldg + IJ, no = (2l * km - 1) * ldo + IJ;
+  const Span ins[] = {span_of(ptemp, nl),   span_of(salt, nl),     span_of(h, nl),   span_of(z, nl),
+                      span_of(topo, IJ),    span_of(cdk, km),      span_of(rfdk, km + 1),
+                      span_of(grad_lon, n2), span_of(grad_lat, n2)};
+  static const char* const names[] = {"ptemp", "salt", "h", "z", "topo", "cdk", "rfdk", "grad_lon", "grad_lat"};
+  const Span outs[] = {span_of(flx_lon, no), span_of(flx_lat, no), span_of(vdiv, no)};
+  static const char* const onames[] = {"flx_lon", "flx_lat", "vdiv"};
+  for (int i = 0; i < 3; i++) {
+    if (int rc = check_disjoint(onames[i], outs[i], ins, names, 9)) return rc;
+    for (int k = i + 1; k < 3; k++)
+      if (overlaps(outs[i], outs[k])) return fail(DSHT_EARG, "the outputs %s and %s overlap", onames[i], onames[k]);
+  }
+  return launch_redigm_col(p, r, ptemp, salt, h, z, ldg, km, topo, cdk, rfdk, grad_lon, grad_lat, ldg, flx_lon,
+                           flx_lat, vdiv, ldo, (hipStream_t)stream);
+}
+
+long dsht_redigm_work_size(const dsht_plan* p, int km) {
+  if (check_plan(p)) return -1;
+  if (km < 2) {
+    fail(DSHT_EARG, "the Redi/GM mixing needs km >= 2");
+    return -1;
+  }
+  return 10l * km * p->I * p->J + 2l * km * p->L;
+}
+
+int dsht_redigm_d(dsht_plan* p, const dsht_redigm_config* cfg, const double* ptemp, const double* salt,
+                  const double* h, const double* z, long ldg, int km, const double* topo, const double* cdk,
+                  const double* rfdk, double* ptemp_rhs, double* salt_rhs, long ldo, double* work, long lwork,
+                  void* stream) {
+  RediParams r;
+  if (int rc = redigm_check(p, cfg, ptemp, salt, h, z, ldg, km, topo, cdk, rfdk, &r)) return rc;
+  if (!ptemp_rhs || !salt_rhs || !work) return fail(DSHT_EARG, "pointer is NULL");
+  if (ldo < (long)p->I * p->J) return fail(DSHT_EARG, "output column pitch %ld must be >= I*J", ldo);
+  const long need = dsht_redigm_work_size(p, km);
+  if (lwork < need) return fail(DSHT_EARG, "workspace of %ld doubles is below %ld", lwork, need);
+  if (((uintptr_t)work) % 16) return fail(DSHT_EARG, "the workspace must be 16-byte aligned");
+  const long IJ = (long)p->I * p->J, nl = (km - 1) * ldg + IJ, no = (km - 1) * ldo + IJ;
+  const Span ins[] = {span_of(ptemp, nl), span_of(salt, nl),  span_of(h, nl),       span_of(z, nl),
+                      span_of(topo, IJ),  span_of(cdk, km),   span_of(rfdk, km + 1)};
+  static const char* const names[] = {"ptemp", "salt", "h", "z", "topo", "cdk", "rfdk"};
+  const Span out_t = span_of(ptemp_rhs, no), out_s = span_of(salt_rhs, no), out_w = span_of(work, lwork);
+  if (int rc = check_disjoint("ptemp_rhs", out_t, ins, names, 7)) return rc;
+  if (int rc = check_disjoint("salt_rhs", out_s, ins, names, 7)) return rc;
+  if (int rc = check_disjoint("workspace", out_w, ins, names, 7)) return rc;
+  if (overlaps(out_t, out_s) || overlaps(out_t, out_w) || overlaps(out_s, out_w))
+    return fail(DSHT_EARG, "ptemp_rhs, salt_rhs and the workspace must not overlap");
+  hipStream_t s = (hipStream_t)stream;
+  const long n2 = 2l * km;
+  double* glon = work;  // 2 km grids each, pitch I*J (temperature layers, then salinity)
+  double* glat = glon + n2 * IJ;
+  double* flon = glat + n2 * IJ;
+  double* flat = flon + n2 * IJ;
+  double* vdv = flat + n2 * IJ;
+  double* w = vdv + n2 * IJ;  // 2 km spectra, coefficient-major (sl = 2 km)
+  int rc;
+  // calc_GradTRC (:431-442): w_xy of each layer, xy_GradLon_w and xy_GradLat_w
+  if ((rc = dsht_fwd_d(p, DSHT_VAL, ptemp, ldg, w, n2, 1, km, nullptr, s))) return rc;
+  if ((rc = dsht_fwd_d(p, DSHT_VAL, salt, ldg, w + km, n2, 1, km, nullptr, s))) return rc;
+  if ((rc = dsht_inv_vgrad_d(p, w, n2, 1, nullptr, glon, glat, IJ, (int)n2, nullptr, s))) return rc;
+  if ((rc = launch_redigm_col(p, r, ptemp, salt, h, z, ldg, km, topo, cdk, rfdk, glon, glat, IJ, flon, flat, vdv,
+                              IJ, s)))
+    return rc;
+  // append_Redi_GM_RHS (:340-346): xy_w(w_AlphaOptr_xy(FLon cos, FLat cos)) into glon
+  if ((rc = dsht_alpha_fwd_d(p, flon, flat, IJ, w, n2, 1, (int)n2, nullptr, s))) return rc;
+  if ((rc = dsht_inv_d(p, DSHT_VAL, w, n2, 1, glon, IJ, (int)n2, nullptr, s))) return rc;
+  return launch_redigm_rhs(p, ptemp_rhs, salt_rhs, ldo, glon, vdv, IJ, km, s);
+}
+
+int dsht_redigm(dsht_plan* p, const dsht_redigm_config* cfg, const double* ptemp, const double* salt,
+                const double* h, const double* z, int km, const double* topo, const double* cdk, const double* rfdk,
+                double* ptemp_rhs, double* salt_rhs) {
+  HOST_PROLOGUE(p, km);
+  if (!ptemp || !salt || !h || !z || !topo || !cdk || !rfdk || !ptemp_rhs || !salt_rhs)
+    return fail(DSHT_EARG, "NULL array");
+  const long lw = dsht_redigm_work_size(p, km);
+  if (lw < 0) return DSHT_EARG;
+  double *din, *drhs, *dk, *dw;
+  const double* in[] = {ptemp, salt, h, z, topo};
+  const size_t nin[] = {IJ * km, IJ * km, IJ * km, IJ * km, IJ};
+  const double* rin[] = {ptemp_rhs, salt_rhs};
+  const size_t nr[] = {IJ * km, IJ * km};
+  const double* kin[] = {cdk, rfdk};
+  const size_t nk[] = {(size_t)km, (size_t)km + 1};
+  int rc;
+  if ((rc = col_stage(c, 0, in, nin, 5, &din)) || (rc = col_stage(c, 1, rin, nr, 2, &drhs)) ||
+      (rc = col_stage(c, 2, kin, nk, 2, &dk)) || (rc = ctx_buf(c, 3, (size_t)lw, &dw)))
+    return rc;
+  if ((rc = dsht_redigm_d(p, cfg, din, din + IJ * km, din + 2 * IJ * km, din + 3 * IJ * km, (long)IJ, km,
+                          din + 4 * IJ * km, dk, dk + km, drhs, drhs + IJ * km, (long)IJ, dw, lw, c->s)))
+    return rc;
+  DSHT_HIP(hipMemcpyAsync(ptemp_rhs, drhs, sizeof(double) * IJ * km, hipMemcpyDeviceToHost, c->s));
+  return d2h_sync(c, salt_rhs, drhs + IJ * km, IJ * km);
 }
 
 int dsht_coef(const dsht_plan* pc, int which, const double* params, int nparams, double* out) {

@@ -162,6 +162,7 @@ struct dsht_plan {
   double* d_rs_alpha = nullptr;   // 1/(R cos^2 phi)  (J)
   double* d_rs_uv = nullptr;      // 1/(R cos phi)    (J)
   double* d_pro_psi = nullptr;    // R^2 * LaplaInv = -R^2/(n(n+1)), 0 at n=0 (L)
+  double* d_lat_cs = nullptr;     // cos(lat_j) (J), then |sin(lat_j)| (J), of the host lat
   // Legendre tables (FP64), swizzled into MFMA fragment order
   double* d_tabFP = nullptr;
   double* d_tabFH = nullptr;
@@ -275,6 +276,17 @@ int launch_col_vdiff(const double* q0, const double* rhs, const double* kv, cons
                      const double* cdk, const double* fdk, double dt, double alpha, int sfc_dirichlet,
                      int btm_dirichlet, const double* bfric, double* qa, long ldo, double* work, int npts,
                      hipStream_t s);
+// redigm.hip: the grid-space part of the Redi/GM mixing and the tracer update
+struct RediParams {
+  double kappa_redi, kappa_gm, slope_max, sd, omega, ref_dens, grav;
+  int gkw91;  // interior tapering GKW91 (else DM95)
+};
+int launch_redigm_col(const dsht_plan* p, const RediParams& c, const double* pt, const double* sa, const double* h,
+                      const double* z, long ldg, int km, const double* topo, const double* cdk, const double* rfdk,
+                      const double* glon, const double* glat, long ldgr, double* flon, double* flat, double* vdiv,
+                      long ldo, hipStream_t s);
+int launch_redigm_rhs(const dsht_plan* p, double* rt, double* rs, long ldo, const double* hdiv, const double* vdiv,
+                      long ldw, int km, hipStream_t s);
 int launch_lonlat_sum(const dsht_plan* p, const double* xy, long ldg, int ncols, double* out,
                       double factor, hipStream_t s);
 int launch_lon_sum(const dsht_plan* p, const double* xy, long ldg, int ncols, double* out,

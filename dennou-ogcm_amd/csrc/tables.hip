@@ -202,6 +202,15 @@ int build_tables(dsht_plan* p) {
   DSHT_HIP(hipMemcpy(p->wq.data(), p->d_wq, sizeof(double) * J, hipMemcpyDeviceToHost));
   for (int j = 0; j < J; j++) p->lat[j] = std::asin(p->mu[j]);
   for (int k = 0; k < p->I; k++) p->lon[k] = 2.0 * M_PI * k / p->I;
+  // xy_CosLat = cos(xy_Lat) (SpmlUtil_mod.f90:232) and |sin(xy_Lat)| as the reference
+  // evaluates them on its latitudes
+  std::vector<double> cs(2 * (size_t)J);
+  for (int j = 0; j < J; j++) {
+    cs[j] = std::cos(p->lat[j]);
+    cs[J + j] = std::fabs(std::sin(p->lat[j]));
+  }
+  if ((rc = alloc(&p->d_lat_cs, 2 * (size_t)J))) return rc;
+  DSHT_HIP(hipMemcpy(p->d_lat_cs, cs.data(), sizeof(double) * 2 * J, hipMemcpyHostToDevice));
   return 0;
 }
 

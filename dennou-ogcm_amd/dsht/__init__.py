@@ -48,6 +48,7 @@ EXPORTS = [
     "dsht_batch_step_sequential", "dsht_batch_capture", "dsht_batch_launch", "dsht_batch_destroy",
     "dsht_lfam3_default_config", "dsht_lfam3_create", "dsht_lfam3_destroy", "dsht_lfam3_set_rotation",
     "dsht_lfam3_state", "dsht_lfam3_step", "dsht_lfam3_copy_run", "dsht_lfam3_units",
+    "dsht_redigm_default_config", "dsht_col_redigm_d", "dsht_redigm_work_size", "dsht_redigm_d", "dsht_redigm",
 ]
 TUNE_CHUNK_COLS, TUNE_LEG_CB, TUNE_FFT, TUNE_LEG, TUNE_FFT_ORDER = 0, 1, 2, 3, 5
 
@@ -65,6 +66,15 @@ class LfAm3Config(ctypes.Structure):
                 ("ref_temp", ctypes.c_double), ("ref_salt", ctypes.c_double), ("beta_t", ctypes.c_double),
                 ("beta_s", ctypes.c_double), ("cp0", ctypes.c_double), ("cs0", ctypes.c_double)]
 
+
+class RediGMConfig(ctypes.Structure):
+    """dsht_redigm_config (include/dsht.h)."""
+    _fields_ = [("kappa_redi", ctypes.c_double), ("kappa_gm", ctypes.c_double), ("slope_max", ctypes.c_double),
+                ("sd", ctypes.c_double), ("interior_taper", ctypes.c_int), ("pbl_taper", ctypes.c_int),
+                ("omega", ctypes.c_double), ("ref_dens", ctypes.c_double), ("grav", ctypes.c_double)]
+
+
+TAPER_GKW91, TAPER_DM95, TAPER_LDD95 = 1, 2, 3
 
 LF_U, LF_V, LF_TRC, LF_SFCPRES = range(4)
 LF_B, LF_N, LF_A = range(3)
@@ -171,6 +181,13 @@ def load():
         "dsht_lfam3_step": ([vp, i, vp], i),
         "dsht_lfam3_copy_run": ([vp, i, i, vp], i),
         "dsht_lfam3_units": ([vp, i, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)], i),
+        "dsht_redigm_default_config": ([ctypes.POINTER(RediGMConfig)], i),
+        "dsht_col_redigm_d": ([vp, ctypes.POINTER(RediGMConfig), vp, vp, vp, vp, lg, i, vp, vp, vp, vp, vp, vp, vp,
+                               vp, lg, vp], i),
+        "dsht_redigm_work_size": ([vp, i], lg),
+        "dsht_redigm_d": ([vp, ctypes.POINTER(RediGMConfig), vp, vp, vp, vp, lg, i, vp, vp, vp, vp, vp, lg, vp, lg,
+                           vp], i),
+        "dsht_redigm": ([vp, ctypes.POINTER(RediGMConfig), vp, vp, vp, vp, i, vp, vp, vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -746,6 +763,84 @@ class Plan:
                                      _ptr(bf) if bf is not None else None, _ptr(out)))
         return out
 
+    # Redi/GM isopycnal mixing (LPhys_RediGM_hspm_vfvm_AddMixingTerm,
+    # LPhys_RediGM_hspm_vfvm_mod.f90:218-372)
+    def _redigm_in(self, ptemp, salt, h, z, topo, cdk, rfdk):
+        import torch
+        km, ldg = self._col_in(cdk, ptemp, salt, h, z)
+        if km < 2:
+            raise ValueError("the Redi/GM mixing needs km >= 2")
+        if tuple(topo.shape) != (self.jm, self.im) or not topo.is_contiguous() or topo.dtype != torch.float64 \
+                or not topo.is_cuda:
+            raise ValueError(f"topo must be a contiguous float64 device grid ({self.jm}, {self.im})")
+        if rfdk.dtype != torch.float64 or not rfdk.is_cuda or rfdk.dim() != 1 or rfdk.numel() != km + 1 \
+                or rfdk.stride(0) != 1:
+            raise ValueError(f"rfdk must be a contiguous float64 device vector of {km + 1} values")
+        return km, ldg
+
+    def col_redigm_d(self, ptemp, salt, h, z, topo, cdk, rfdk, grad_lon, grad_lat, cfg=None, stream=None):
+        """The grid-space part of the Redi/GM mixing: from (km, J, I) ptemp [K], salt, h, z and
+        the (2km, J, I) GradLon / GradLat grids of w_xy(ptemp) then w_xy(salt) (not divided by
+        the radius), returns (flx_lon, flx_lat, vdiv), each (2km, J, I): the AlphaOptr inputs
+        (FLonRedi + FLonGM) cos(lat), (FLatRedi + FLatGM) cos(lat) and the vertical flux
+        divergence of each tracer layer."""
+        import torch
+        km, ldg = self._redigm_in(ptemp, salt, h, z, topo, cdk, rfdk)
+        for g in (grad_lon, grad_lat):
+            if tuple(g.shape) != (2 * km, self.jm, self.im) or g.stride(0) != ldg or g.stride(2) != 1 \
+                    or g.stride(1) != self.im or g.dtype != torch.float64 or not g.is_cuda:
+                raise ValueError(f"gradients must be float64 device grids (2km, {self.jm}, {self.im}) with the "
+                                 "layer pitch of the fields")
+        outs = [torch.empty((2 * km, self.jm, self.im), dtype=torch.float64, device=ptemp.device) for _ in range(3)]
+        cfg = redigm_config() if cfg is None else cfg
+        _check(load().dsht_col_redigm_d(self.h, ctypes.byref(cfg), _tptr(ptemp), _tptr(salt), _tptr(h), _tptr(z),
+                                        ldg, km, _tptr(topo), _tptr(cdk), _tptr(rfdk), _tptr(grad_lon),
+                                        _tptr(grad_lat), _tptr(outs[0]), _tptr(outs[1]), _tptr(outs[2]),
+                                        outs[0].stride(0), _stream(stream)))
+        return tuple(outs)
+
+    def redigm_d(self, ptemp, salt, h, z, topo, cdk, rfdk, ptemp_rhs, salt_rhs, cfg=None, work=None, stream=None):
+        """LPhys_RediGM_hspm_vfvm_AddMixingTerm on device grids: adds the Redi/GM tendency to
+        the (km, J, I) ptemp_rhs and salt_rhs in place and returns them.  work: optional float64
+        device scratch of >= dsht_redigm_work_size doubles."""
+        import torch
+        km, ldg = self._redigm_in(ptemp, salt, h, z, topo, cdk, rfdk)
+        ldo = ptemp_rhs.stride(0)
+        for r in (ptemp_rhs, salt_rhs):
+            if tuple(r.shape) != (km, self.jm, self.im) or r.stride(0) != ldo or r.stride(2) != 1 \
+                    or r.stride(1) != self.im or r.dtype != torch.float64 or not r.is_cuda:
+                raise ValueError(f"tendencies must be float64 device grids (km, {self.jm}, {self.im}) of one pitch")
+        need = load().dsht_redigm_work_size(self.h, km)
+        if need < 0:
+            _check(-1)
+        if work is None:
+            work = torch.empty(need, dtype=torch.float64, device=ptemp.device)
+            if stream is not None:
+                work.record_stream(torch.cuda.ExternalStream(stream))
+        elif work.dtype != torch.float64 or not work.is_cuda or not work.is_contiguous():
+            raise ValueError("work must be a contiguous float64 device tensor")
+        cfg = redigm_config() if cfg is None else cfg
+        _check(load().dsht_redigm_d(self.h, ctypes.byref(cfg), _tptr(ptemp), _tptr(salt), _tptr(h), _tptr(z), ldg,
+                                    km, _tptr(topo), _tptr(cdk), _tptr(rfdk), _tptr(ptemp_rhs), _tptr(salt_rhs),
+                                    ldo, _tptr(work), work.numel(), _stream(stream)))
+        return ptemp_rhs, salt_rhs
+
+    def RediGM_AddMixingTerm(self, xyz_PTemp_RHS, xyz_Salt_RHS, xyz_PTemp, xyz_Salt, xyz_H, xyz_Z, xy_Topo,
+                             z_CDK, z_RFDK, cfg=None):
+        """LPhys_RediGM_hspm_vfvm_AddMixingTerm on host arrays ((km, J, I) fields, layer 0 on
+        top; z_RFDK the km+1 half levels): returns the updated (PTemp_RHS, Salt_RHS)."""
+        (pr, sr, pt, sa, h, z), km = self._col_host(xyz_PTemp_RHS, xyz_Salt_RHS, xyz_PTemp, xyz_Salt, xyz_H, xyz_Z)
+        topo = np.ascontiguousarray(xy_Topo, dtype=np.float64)
+        if topo.shape != (self.jm, self.im):
+            raise ValueError(f"xy_Topo must be ({self.jm}, {self.im})")
+        cdk = self._col_vec("z_CDK", z_CDK, km)
+        rfdk = self._col_vec("z_RFDK", z_RFDK, km + 1)
+        pr, sr = pr.copy(), sr.copy()
+        cfg = redigm_config() if cfg is None else cfg
+        _check(load().dsht_redigm(self.h, ctypes.byref(cfg), _ptr(pt), _ptr(sa), _ptr(h), _ptr(z), km, _ptr(topo),
+                                  _ptr(cdk), _ptr(rfdk), _ptr(pr), _ptr(sr)))
+        return pr, sr
+
     def set_tuning(self, key, value):
         _check(load().dsht_set_tuning(self.h, int(key), int(value)))
 
@@ -838,6 +933,19 @@ def _tensor_from_ptr(ptr, shape):
     h.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": "<f8", "data": (ptr, False), "version": 2,
                                   "strides": None}
     return torch.as_tensor(h, device="cuda")
+
+
+def redigm_config(**kw):
+    """dsht_redigm_default_config with the given fields replaced (tapering names 'DM95',
+    'GKW91', 'LDD95' or the TAPER_* numbers)."""
+    c = RediGMConfig()
+    _check(load().dsht_redigm_default_config(ctypes.byref(c)))
+    names = {"GKW91": TAPER_GKW91, "DM95": TAPER_DM95, "LDD95": TAPER_LDD95}
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise AttributeError(f"dsht_redigm_config has no field {k}")
+        setattr(c, k, names[v] if isinstance(v, str) else v)
+    return c
 
 
 def lfam3_config(**kw):

@@ -362,6 +362,64 @@ int dsht_lfam3_copy_run(dsht_lfam3* pool, int dst, int src, void* stream);
 /* transform units of one step of nruns runs (42 km + 9 forward, 28 km + 9 inverse each)   */
 int dsht_lfam3_units(const dsht_lfam3* pool, int nruns, long* fwd_units, long* inv_units);
 
+/* ---------------------------------------------------------------- Redi/GM mixing
+ * Isopycnal (Redi) diffusion and Gent-McWilliams skew flux of potential temperature and
+ * salinity: LPhys_RediGM_hspm_vfvm_AddMixingTerm (model/dogcm/src/physics/lphys/
+ * LPhys_RediGM_hspm_vfvm_mod.f90:218-372).  Slopes from the JM95 expansion coefficients
+ * (EOSDriver_alpha_beta, EOS_JM95_mod.f90:272-332) of theta = ptemp [K], S [psu] at
+ * p = -ref_dens*grav*z; interior slope tapering DM95 or GKW91 times the LDD95 boundary
+ * tapering (LPhys_RediGMHelper_mod.f90:175-431); Redi and skew fluxes
+ * (calc_IsopycDiffFlux :460-556, calc_SkewFlux :560-679); then, per tracer,
+ * rhs += xy_w(w_AlphaOptr_xy(FLon cos, FLat cos)) and rhs += (FSig above - FSig below)
+ * RCDK / H (:308-370).  The plan's radius is RPlanet.  The 'linear' boundary tapering
+ * (LinearSlopeTapering) is not supported: with it the reference multiplies by an
+ * undefined factor (prepare_SlopeTapering :219-233).                                     */
+#define DSHT_TAPER_GKW91 1
+#define DSHT_TAPER_DM95 2
+#define DSHT_TAPER_LDD95 3
+typedef struct dsht_redigm_config {
+  double kappa_redi, kappa_gm;  /* DiffCoefRedi, DiffCoefGM [m2/s]                          */
+  double slope_max, sd;         /* SlopeMax, Sd (DM95)                                      */
+  int interior_taper;           /* DSHT_TAPER_DM95 or DSHT_TAPER_GKW91                      */
+  int pbl_taper;                /* DSHT_TAPER_LDD95                                         */
+  double omega;                 /* Omega [1/s] (LDD95's eddy displacement c/(2 Omega|sin|)) */
+  double ref_dens, grav;        /* RefDens, Grav                                            */
+} dsht_redigm_config;
+/* defaults: RediGM_nml of APEOGCircExp.nml (800, 800, 4e-3, 1e-3, DM95, LDD95), Earth's
+ * Omega (DOGCM_Admin_Constants_mod.f90:158), RefDens 1027, Grav 9.79764                    */
+int dsht_redigm_default_config(dsht_redigm_config* cfg);
+/* The grid-space part on device grids, given the horizontal gradients: ptemp, salt, h
+ * (xyz_H), z (xyz_Z, negative below the surface): km layers each (pitch ldg, layer 0 on top);
+ * topo: one I x J grid (xy_Topo, the positive bottom depth); cdk (km) and rfdk (km+1,
+ * z_RFDK(KS-1:KE)) device vectors; grad_lon, grad_lat: 2*km grids (pitch ldg), the
+ * xy_GradLon_w / xy_GradLat_w of w_xy(ptemp) (layers 0..km-1) then of w_xy(salt), not
+ * divided by RPlanet.  Writes 2*km grids each (pitch ldo; temperature layers, then
+ * salinity): flx_lon = (FLonRedi + FLonGM) cos(lat), flx_lat likewise (the AlphaOptr
+ * inputs) and vdiv = (FSig above - FSig below) RCDK / H.  Outputs must not overlap inputs.
+ * km >= 2.                                                                                */
+int dsht_col_redigm_d(dsht_plan* plan, const dsht_redigm_config* cfg, const double* ptemp,
+                      const double* salt, const double* h, const double* z, long ldg, int km,
+                      const double* topo, const double* cdk, const double* rfdk,
+                      const double* grad_lon, const double* grad_lat, double* flx_lon,
+                      double* flx_lat, double* vdiv, long ldo, void* stream);
+/* device doubles of work that dsht_redigm_d needs for km layers                            */
+long dsht_redigm_work_size(const dsht_plan* plan, int km);
+/* The whole tendency on device grids: the forward transform of ptemp and salt, their
+ * GradLon and GradLat (one fused inverse), dsht_col_redigm_d, the AlphaOptr forward of the
+ * fluxes and the inverse of the divergence; then ptemp_rhs[k] = (ptemp_rhs[k] + hdiv[k]) +
+ * vdiv[k], salt_rhs likewise (km grids each, pitch ldo; they may not overlap the inputs).
+ * The transforms run in the plan's precision form.  work: lwork >= dsht_redigm_work_size
+ * device doubles (contents undefined on return).                                          */
+int dsht_redigm_d(dsht_plan* plan, const dsht_redigm_config* cfg, const double* ptemp,
+                  const double* salt, const double* h, const double* z, long ldg, int km,
+                  const double* topo, const double* cdk, const double* rfdk, double* ptemp_rhs,
+                  double* salt_rhs, long ldo, double* work, long lwork, void* stream);
+/* host-array form, the reference's calling form: xyz(IM, JM, KS:KE) fields, xy_Topo, host
+ * vectors cdk and rfdk; ptemp_rhs and salt_rhs are updated in place                        */
+int dsht_redigm(dsht_plan* plan, const dsht_redigm_config* cfg, const double* ptemp,
+                const double* salt, const double* h, const double* z, int km, const double* topo,
+                const double* cdk, const double* rfdk, double* ptemp_rhs, double* salt_rhs);
+
 #ifdef __cplusplus
 }
 #endif
