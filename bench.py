@@ -561,6 +561,123 @@ def lfam3_bench(torch, dsht, steps=20, N=NMAX, km=KM):
     return out
 
 
+def _redigm_fields(torch, dsht, plan, km, seed):
+    """A stratified ocean on the plan's grid: potential temperature [K], salinity, thickness
+    factor, layer depths, bottom depth (3-5 km), layer thicknesses and reciprocal half-level
+    spacings of the Chebyshev-Lobatto sigma levels."""
+    import numpy as np
+    fk, _ = dsht.sig_grid(km, km)
+    cdk = fk[:-1] - fk[1:]
+    ck = np.empty(km + 2)
+    ck[1:-1] = 0.5 * (fk[:-1] + fk[1:])
+    ck[0], ck[-1] = fk[0] + 0.5 * cdk[0], fk[-1] - 0.5 * cdk[-1]
+    rfdk = 1.0 / (ck[:-1] - ck[1:])
+    J, I = plan.jm, plan.im
+    lon = plan.x_Lon[None, :]
+    mu = np.sin(plan.y_Lat)[:, None]
+    topo = 3000.0 + 2000.0 * (0.5 + 0.5 * np.cos(2 * lon) * (1 - mu ** 2))
+    z = topo[None] * ck[1:-1, None, None]
+    rng = np.random.default_rng(seed)
+    pt = 275.0 + 20.0 * np.exp(z / 800.0) + 6.0 * (mu ** 2) * np.cos(lon)[None] * np.exp(z / 1500.0) \
+        + 0.02 * rng.standard_normal(z.shape)
+    sa = 34.5 + 0.5 * np.exp(z / 1000.0) + 0.3 * mu * np.sin(3 * lon)[None] * np.exp(z / 2000.0)
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device="cuda")
+    return {"pt": t(pt), "sa": t(sa), "h": t(np.broadcast_to(topo, z.shape)), "z": t(z), "topo": t(topo),
+            "cdk": t(cdk), "rfdk": t(rfdk)}, (pt, sa, np.broadcast_to(topo, z.shape).copy(), z, topo, cdk, rfdk)
+
+
+def redigm_bench(torch, dsht, reps=10, km=KM, cpu=True):
+    """Redi/GM isopycnal mixing of both tracers (LPhys_RediGM_hspm_vfvm_AddMixingTerm) on
+    T170L60 device grids, FP64 and BF16X6 transforms.  One call: the forward transform of the
+    2 km tracer layers, their GradLon and GradLat (one fused inverse), the column kernel
+    (slopes, tapering, Redi and skew fluxes), the AlphaOptr forward of the fluxes and the
+    inverse of the divergence, and the accumulation into both right-hand sides.  Transform
+    units per call: 2 km + 4 km forward (AlphaOptr takes two grids), 4 km + 2 km inverse.
+    The column kernel's roofline: its algorithmic bytes (8 input and 6 output layer fields
+    once, plus the bottom depth) over its average duration (library timers, HIP events on
+    its stream).  The CPU leg times the oracle's restatement at T42L20 on this host."""
+    out = {"workload": f"Redi/GM tendency of ptemp and salt, T{NMAX}L{km} (I={IM}, J={JM}), "
+                       "DM95 x LDD95 tapering, kappa 800 m2/s",
+           "units_per_call": {"forward": 6 * km, "inverse": 6 * km}}
+    plan = dsht.Plan(NMAX, IM, JM, kmax=km, radius=RADIUS)
+    d, _ = _redigm_fields(torch, dsht, plan, km, seed=0x4ED1)
+    rt = torch.zeros((km, JM, IM), dtype=torch.float64, device="cuda")
+    rs = torch.zeros_like(rt)
+    work = torch.empty(dsht.load().dsht_redigm_work_size(plan.h, km), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    args = (d["pt"], d["sa"], d["h"], d["z"], d["topo"], d["cdk"], d["rfdk"], rt, rs)
+    for form, pid in (("fp64", dsht.PREC_FP64), ("bf16x6", dsht.PREC_BF16X6)):
+        plan.set_precision(pid)
+        for _ in range(2):
+            plan.redigm_d(*args, work=work, stream=stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            plan.redigm_d(*args, work=work, stream=stream)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        dsht.prof_reset()
+        dsht.prof_enable(True)
+        for _ in range(reps):
+            plan.redigm_d(*args, work=work, stream=stream)
+        torch.cuda.synchronize()
+        dsht.prof_enable(False)
+        kern = {}
+        for name in ("redigm_col", "redigm_rhs", "leg_inv", "leg_fwd", "fft_inv", "fft_fwd"):
+            kms, cnt, by, _ = dsht.prof_query(name)
+            if cnt:
+                us = 1e3 * kms / cnt
+                kern[name] = {"avg_us": us, "launches_per_call": cnt / reps, "bytes_per_launch": by / cnt,
+                              "GBps": by / cnt / (us * 1e-6) / 1e9}
+        col = kern.get("redigm_col", {})
+        out[form] = {"ms_per_call": ms, "pairs_per_s": 6 * km / (ms * 1e-3), "kernels": kern,
+                     "finite": bool(torch.isfinite(rt).all() and torch.isfinite(rs).all()),
+                     "roofline_col": {"bound": "hbm", "achieved": col.get("GBps"), "peak": HBM_PEAK_GBS,
+                                      "unit": "GB/s", "frac": (col.get("GBps") or 0) / HBM_PEAK_GBS}}
+    plan.set_precision(dsht.PREC_FP64)
+    if cpu:
+        out["cpu_baseline"] = redigm_cpu(torch, dsht)
+    return out
+
+
+def redigm_cpu(torch, dsht, N=42, I=128, J=64, km=20, reps=3):
+    """The oracle's Redi/GM restatement (numpy column part, one thread, around the C oracle's
+    transforms, OpenMP over columns) timed at T42L20 on this host, and the device call at the
+    same size."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import redigm as RG
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    plan = dsht.Plan(N, I, J, kmax=km, radius=RADIUS)
+    d, (pt, sa, h, z, topo, cdk, rfdk) = _redigm_fields(torch, dsht, plan, km, seed=0x4ED2)
+    o = O.Oracle(N, I, J, RADIUS)
+    cfg = RG.Config()
+    RG.add_mixing_term(o, pt, sa, h, z, topo, cdk, rfdk, cfg, lat=plan.y_Lat)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ref = RG.add_mixing_term(o, pt, sa, h, z, topo, cdk, rfdk, cfg, lat=plan.y_Lat)
+    cpu_s = (time.perf_counter() - t0) / reps
+    rt = torch.zeros((km, J, I), dtype=torch.float64, device="cuda")
+    rs = torch.zeros_like(rt)
+    args = (d["pt"], d["sa"], d["h"], d["z"], d["topo"], d["cdk"], d["rfdk"], rt, rs)
+    plan.redigm_d(*args)
+    torch.cuda.synchronize()
+    err = max(float(np.linalg.norm(g.cpu().numpy() - r) / np.linalg.norm(r)) for g, r in zip((rt, rs), ref))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        rt.zero_()
+        rs.zero_()
+        plan.redigm_d(*args)
+    e1.record()
+    e1.synchronize()
+    return {"sample": f"T{N}L{km} (I={I}, J={J}) Redi/GM tendency, {reps} calls", "cpu_ms_per_call": 1e3 * cpu_s,
+            "gpu_ms_per_call": e0.elapsed_time(e1) / 20, "cores": threads, "kind": "port",
+            "rel_l2_gpu_vs_oracle": err}
+
+
 def serving_bench(torch, dsht, n_req=32, rate=20.0, nsteps=200, seed=0x5E4E, factor=1.8):
     """Poisson arrivals of independent LF-AM3 runs (truncation, KM, rotation rate drawn per
     run), served four ways:
@@ -661,6 +778,9 @@ def main():
                     help="only run the batched-instances section and print its JSON")
     ap.add_argument("--serve-only", action="store_true",
                     help="only run the serving section and print its JSON")
+    ap.add_argument("--redigm-only", action="store_true",
+                    help="only run the Redi/GM mixing section and print its JSON")
+    ap.add_argument("--no-redigm", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: the transform path has no "
@@ -695,6 +815,10 @@ def main():
     if args.serve_only:
         import dsht
         print(json.dumps({"serving": serving_bench(torch, dsht)}))
+        return
+    if args.redigm_only:
+        import dsht
+        print(json.dumps({"redigm": redigm_bench(torch, dsht, cpu=not args.no_cpu)}))
         return
 
     import dsht
@@ -785,7 +909,12 @@ def main():
     column = column_bench(torch, dsht)
     print("[bench] column: " + ", ".join(f"{k} {column[k]['avg_us']:.1f} us {column[k]['GBps']:.0f} GB/s"
                                          for k in ("omega", "hydpres", "uvbarot", "quick", "vdiff")), file=sys.stderr, flush=True)
-    batched = serving = lf = None
+    batched = serving = lf = rg = None
+    if not args.no_redigm:
+        print("[bench] Redi/GM mixing", file=sys.stderr, flush=True)
+        rg = redigm_bench(torch, dsht, cpu=not args.no_cpu)
+        print("[bench] Redi/GM: " + ", ".join(f"{f} {rg[f]['ms_per_call']:.2f} ms/call" for f in ("fp64", "bf16x6")),
+              file=sys.stderr, flush=True)
     if not args.no_lfam3:
         print("[bench] LF-AM3 step", file=sys.stderr, flush=True)
         lf = lfam3_bench(torch, dsht)
@@ -848,6 +977,7 @@ def main():
         "column_diagnostics": column,
         "t85_fp64": t85,
         "lfam3": lf,
+        "redigm": rg,
     }
     print(json.dumps(line))
 

@@ -1296,6 +1296,7 @@ static int redigm_check(dsht_plan* p, const dsht_redigm_config* cfg, const doubl
                         const double* rfdk, RediParams* r) {
   if (int rc = check_col(p, km, cdk, ldg)) return rc;
   if (km < 2) return fail(DSHT_EARG, "the Redi/GM mixing needs km >= 2");
+  if (km > 1024) return fail(DSHT_EARG, "the Redi/GM mixing supports at most 1024 layers");
   if (!ptemp || !salt || !h || !z || !topo || !rfdk) return fail(DSHT_EARG, "pointer is NULL");
   return redigm_params(cfg, r);
 }

@@ -97,77 +97,125 @@ __device__ inline void layer_slopes(const RediArgs& a, int k, double tm, double 
   L.slat = fac * (al * L.ga[0] - be * L.ga[1]);
 }
 
-__global__ __launch_bounds__(256) void k_redigm_col(RediArgs a) {
+// the loads of one layer: h, z and the four gradients of layer k, and ptemp/salt of the next
+// layer in the walk's direction (k+1 down, k-1 up; unused past the end)
+struct Pre {
+  double h, z, g0, a0, g1, a1, tn, sn;
+};
+
+constexpr int RING = 2;  // layers whose loads are in flight ahead of the one being computed
+
+constexpr int REDI_LDS_LAYERS = 1024;  // layers whose half-level weights the block keeps in LDS
+
+__global__ __launch_bounds__(256, 2) void k_redigm_col(RediArgs a) {
+  // per layer k: m1 = cdk[k]/(cdk[k-1]+cdk[k]), m2 = cdk[k-1]/(cdk[k-1]+cdk[k]) (r_m1, r_m2 of
+  // the half level on top of layer k, calc_IsopycDiffFlux :491-494) and z_RCDK = 1/cdk[k]
+  // (DOGCM_Admin_Grid_mod.f90:315), each the same IEEE division as in the loop
+  __shared__ double wm1[REDI_LDS_LAYERS], wm2[REDI_LDS_LAYERS], wrc[REDI_LDS_LAYERS];
+  const int km = a.km;
+  for (int k = threadIdx.x; k < km; k += 256) {
+    const double c1 = a.cdk[k];
+    wrc[k] = 1.0 / c1;
+    if (k > 0) {
+      const double c0 = a.cdk[k - 1];
+      wm1[k] = c1 / (c0 + c1);
+      wm2[k] = c0 / (c0 + c1);
+    }
+  }
+  __syncthreads();
   const long q = (long)blockIdx.x * 256 + threadIdx.x;
   if (q >= a.npts) return;
-  const int km = a.km;
   const int j = (int)(q / a.I);
   const double cs = a.lat_cs[j], asn = a.lat_cs[a.J + j];
   const double topo = a.topo[q];
   const long TS = (long)km * a.ldgr;  // salinity block of the gradient inputs
-  // loads of layer k (its T and S at k-1, k, k+1 as given)
-  auto load = [&](int k, double tm, double tc, double tp, double sm, double sc, double sp, Lay& L, double& z) {
+  // dir = +1: the next layer is k+1 (downward walk); -1: k-1 (upward)
+  auto fetch = [&](int k, int dir, Pre& f) {
     const long g = (long)k * a.ldg + q, gr = (long)k * a.ldgr + q;
-    L.h = a.h[g];
-    z = a.z[g];
-    L.gl[0] = a.glon[gr] / a.R;
-    L.ga[0] = a.glat[gr] / a.R;
-    L.gl[1] = a.glon[TS + gr] / a.R;
-    L.ga[1] = a.glat[TS + gr] / a.R;
-    layer_slopes(a, k, tm, tc, tp, sm, sc, sp, z, L);
+    f.h = a.h[g];
+    f.z = a.z[g];
+    f.g0 = a.glon[gr];
+    f.a0 = a.glat[gr];
+    f.g1 = a.glon[TS + gr];
+    f.a1 = a.glat[TS + gr];
+    const int kn = k + dir;
+    const bool in = kn >= 0 && kn < km;
+    f.tn = in ? a.pt[g + dir * a.ldg] : 0.0;
+    f.sn = in ? a.sa[g + dir * a.ldg] : 0.0;
   };
-  auto T = [&](int k) { return a.pt[(long)k * a.ldg + q]; };
-  auto S = [&](int k) { return a.sa[(long)k * a.ldg + q]; };
+  // layer k's slopes from its loads and ptemp/salt at k-1, k, k+1
+  auto slopes_of = [&](int k, const Pre& f, double tm, double tc, double tp, double sm, double sc, double sp,
+                       Lay& L) {
+    L.h = f.h;
+    L.gl[0] = f.g0 / a.R;
+    L.ga[0] = f.a0 / a.R;
+    L.gl[1] = f.g1 / a.R;
+    L.ga[1] = f.a1 / a.R;
+    layer_slopes(a, k, tm, tc, tp, sm, sc, sp, f.z, L);
+  };
 
-  // ---- pass 1: the LDD95 scans (TaperingFunc_LDD95 :392-428)
+  // ---- pass 1: the LDD95 scans (TaperingFunc_LDD95 :392-428), each stopping at its first hit
   double disph = 2.0 / (a.two_omega * asn);
   disph = fmin(fmax(15e3, disph), 100e3);
   const double dzmax = a.smax * disph;
   int ks = -1, kb = km;
   double tmps = 0.0, tmpb = 0.0;
-  {
+  for (int dir = 1; dir >= -1; dir -= 2) {
+    const int k0 = dir > 0 ? 0 : km - 1;
+    Pre ring[RING];
+#pragma unroll
+    for (int u = 0; u < RING; u++)
+      if (u < km) fetch(k0 + dir * u, dir, ring[u]);
+    double xc = a.pt[(long)k0 * a.ldg + q], yc = a.sa[(long)k0 * a.ldg + q];
+    double xb = xc, yb = yc;  // the layer behind in the walk (k0 itself at the start)
     double mx = 0.0;
-    for (int k = 0; k < km; k++) {
-      Lay L;
-      double z;
-      const int km1 = k > 0 ? k - 1 : k, kp1 = k + 1 < km ? k + 1 : k;
-      load(k, T(km1), T(k), T(kp1), S(km1), S(k), S(kp1), L, z);
-      const double sab = sqrt(L.slon * L.slon + L.slat * L.slat);
-      if (sab > mx) mx = sab;
-      if ((0.0 - z) > dzmax) {
-        ks = k;
-        tmps = fmin(mx, a.smax) * disph;
-        break;
-      }
-    }
-    mx = 0.0;
-    for (int k = km - 1; k >= 0; k--) {
-      Lay L;
-      double z;
-      const int km1 = k > 0 ? k - 1 : k, kp1 = k + 1 < km ? k + 1 : k;
-      load(k, T(km1), T(k), T(kp1), S(km1), S(k), S(kp1), L, z);
-      const double sab = sqrt(L.slon * L.slon + L.slat * L.slat);
-      if (sab > mx) mx = sab;
-      if ((z - (-topo)) > dzmax) {
-        kb = k;
-        tmpb = fmin(mx, a.smax) * disph;
-        break;
+    bool done = false;
+    for (int i0 = 0; i0 < km && !done; i0 += RING) {
+#pragma unroll
+      for (int u = 0; u < RING; u++) {
+        const int i = i0 + u;
+        if (done || i >= km) continue;
+        const int k = k0 + dir * i;
+        const Pre c = ring[u];
+        if (i + RING < km) fetch(k0 + dir * (i + RING), dir, ring[u]);
+        const bool end = i + 1 == km;
+        const double xn = end ? xc : c.tn, yn = end ? yc : c.sn;
+        Lay L;
+        if (dir > 0)
+          slopes_of(k, c, xb, xc, xn, yb, yc, yn, L);
+        else
+          slopes_of(k, c, xn, xc, xb, yn, yc, yb, L);
+        const double sab = sqrt(L.slon * L.slon + L.slat * L.slat);
+        if (sab > mx) mx = sab;
+        const bool hit = dir > 0 ? (0.0 - c.z) > dzmax : (c.z - (-topo)) > dzmax;
+        if (hit) {
+          if (dir > 0) {
+            ks = k;
+            tmps = fmin(mx, a.smax) * disph;
+          } else {
+            kb = k;
+            tmpb = fmin(mx, a.smax) * disph;
+          }
+          done = true;
+        }
+        xb = xc, yb = yc, xc = xn, yc = yn;
       }
     }
   }
 
   // ---- pass 2: every layer, the half level above it, and the divergence of the layer above
   const double kr = a.kr, kg = a.kg;
-  Lay P;                                 // layer k-1
-  double pl[2] = {0.0, 0.0}, pa[2] = {0.0, 0.0};   // psi of layer k-1 (GM)
-  double fs_up[2] = {0.0, 0.0};          // FSig on the half level above layer k-1
-  double tc = T(0), sc = S(0), tm = tc, sm = sc;
-  for (int k = 0; k < km; k++) {
+  Lay P;                                          // layer k-1
+  double pl[2] = {0.0, 0.0}, pa[2] = {0.0, 0.0};  // psi of layer k-1 (GM)
+  double fs_up[2] = {0.0, 0.0};                   // FSig on the half level above layer k-1
+  double tc = a.pt[q], sc = a.sa[q], tm = tc, sm = sc;
+  const long oS = (long)km * a.ldo;
+  auto body = [&](int k, const Pre& f) {
     const bool last = k + 1 == km;
-    const double tp = last ? tc : T(k + 1), sp = last ? sc : S(k + 1);
+    const double tp = last ? tc : f.tn, sp = last ? sc : f.sn;
     Lay C;
-    double z;
-    load(k, tm, tc, tp, sm, sc, sp, C, z);
+    slopes_of(k, f, tm, tc, tp, sm, sc, sp, C);
+    const double z = f.z;
     // taper (prepare_SlopeTapering: T1 T2)
     const double mag2 = C.slon * C.slon + C.slat * C.slat;
     const double t1 =
@@ -175,11 +223,12 @@ __global__ __launch_bounds__(256) void k_redigm_col(RediArgs a) {
     double r = 1.0;
     if (k <= ks) r = fmin((0.0 - z) / (tmps + 1e-13), 1.0);
     if (k >= kb) r = fmin((z - (-topo)) / (tmpb + 1e-13), 1.0);
-    const double t2 = 0.5 * (1.0 + sin(PI_REF * (r - 0.5)));
+    // r = 1 gives 0.5 (1 + sin(pi/2)) = 1 exactly; sin only where the ratio is below 1
+    const double t2 = r == 1.0 ? 1.0 : 0.5 * (1.0 + sin(PI_REF * (r - 0.5)));
     C.tap = t1 * t2;
     const long o = (long)k * a.ldo + q;
-    const long oS = (long)km * a.ldo;
     double cl[2], ca[2];
+#pragma unroll
     for (int x = 0; x < 2; x++) {
       const double dz = (0.5 * (C.da[x] + C.db[x])) / C.h;
       const double flr = kr * (C.gl[x] + (C.tap * C.slon) * dz);
@@ -192,12 +241,12 @@ __global__ __launch_bounds__(256) void k_redigm_col(RediArgs a) {
       a.flat[x * oS + o] = (far_ + fag) * cs;
     }
     if (k > 0) {  // the half level between layers k-1 (up) and k (lo)
-      const double c0 = a.cdk[k - 1], c1 = a.cdk[k];
-      const double m1 = c1 / (c0 + c1), m2 = c0 / (c0 + c1);
+      const double m1 = wm1[k], m2 = wm2[k];
       const double sl = (m1 * P.tap) * P.slon + (m2 * C.tap) * C.slon;
       const double sa_ = (m1 * P.tap) * P.slat + (m2 * C.tap) * C.slat;
       const double hden = m1 * P.h + m2 * C.h;
-      const double rc = 1.0 / c0;
+      const double rc = wrc[k - 1];
+#pragma unroll
       for (int x = 0; x < 2; x++) {
         const double glon = m1 * P.gl[x] + m2 * C.gl[x];
         const double glat = m1 * P.ga[x] + m2 * C.ga[x];
@@ -213,11 +262,28 @@ __global__ __launch_bounds__(256) void k_redigm_col(RediArgs a) {
     P = C;
     pl[0] = cl[0], pl[1] = cl[1], pa[0] = ca[0], pa[1] = ca[1];
     tm = tc, sm = sc, tc = tp, sc = sp;
+  };
+  {
+    Pre ring[RING];
+#pragma unroll
+    for (int u = 0; u < RING; u++)
+      if (u < km) fetch(u, 1, ring[u]);
+    for (int k0 = 0; k0 < km; k0 += RING) {
+#pragma unroll
+      for (int u = 0; u < RING; u++) {
+        const int k = k0 + u;
+        if (k < km) {
+          const Pre c = ring[u];
+          if (k + RING < km) fetch(k + RING, 1, ring[u]);
+          body(k, c);
+        }
+      }
+    }
   }
   {  // the bottom layer: FSig below it is zero
-    const double rc = 1.0 / a.cdk[km - 1];
+    const double rc = wrc[km - 1];
     for (int x = 0; x < 2; x++)
-      a.vdiv[x * (long)km * a.ldo + (long)(km - 1) * a.ldo + q] = ((fs_up[x] - 0.0) * rc) / P.h;
+      a.vdiv[x * oS + (long)(km - 1) * a.ldo + q] = ((fs_up[x] - 0.0) * rc) / P.h;
   }
 }
 

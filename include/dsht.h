@@ -396,7 +396,7 @@ int dsht_redigm_default_config(dsht_redigm_config* cfg);
  * divided by RPlanet.  Writes 2*km grids each (pitch ldo; temperature layers, then
  * salinity): flx_lon = (FLonRedi + FLonGM) cos(lat), flx_lat likewise (the AlphaOptr
  * inputs) and vdiv = (FSig above - FSig below) RCDK / H.  Outputs must not overlap inputs.
- * km >= 2.                                                                                */
+ * 2 <= km <= 1024.                                                                        */
 int dsht_col_redigm_d(dsht_plan* plan, const dsht_redigm_config* cfg, const double* ptemp,
                       const double* salt, const double* h, const double* z, long ldg, int km,
                       const double* topo, const double* cdk, const double* rfdk,
