@@ -26,6 +26,7 @@ namespace dsht {
 namespace {
 
 constexpr int CU = 8;  // layers whose loads are in flight together
+constexpr int VDIFF_LR = 10;  // finished rows of the vertical solve kept in LDS per thread (60 KB)
 
 // every grid value is read or written once per call: non-temporal
 __device__ __forceinline__ double ntl(const double* x) { return __builtin_nontemporal_load(x); }
@@ -240,15 +241,16 @@ __global__ __launch_bounds__(256) void k_col_quick(const double* __restrict__ om
 // (:825-830), and Dirichlet ('D', the boundary r doubled) or Neumann ('N', dropped) rows at
 // the surface and the bottom (:836-852).  The solve is LAPACK DGTSV (Gaussian elimination
 // with partial pivoting by row interchange, then back substitution; LAPACK 3.x
-// SRC/dgtsv.f, the NRHS = 1 path), in its order of operations.  The forward sweep leaves
-// each finished row's d and du in `work` (pitch npts; 3 km values per point reserved) and its
-// right-hand side in qa; the second superdiagonal is nonzero only after a row interchange,
-// so up to 64 layers a bit mask in a register records the interchanges and only those rows
-// store it (the backward sweep reads it only there).  The backward sweep reads the rows back
-// in reverse and writes qa = q0 + x.  The round trip goes through the caches (at T170L60 the
-// live rows of all points fit in the last-level cache: 132 us against 141 us non-temporal),
-// and both sweeps keep the loads of CU rows in flight together (132 -> 119 us).  A singular
-// pivot (impossible with kv >= 0 and positive thicknesses) is not reported.
+// SRC/dgtsv.f, the NRHS = 1 path), in its order of operations.  Each finished row of the
+// forward sweep (d, du and its right-hand side) waits for the backward sweep in one of three
+// places: the last RR rows in registers (slots fixed by unrolled loops), the LR rows before
+// them in LDS (row-major over the block's threads), the rest in `work` (d, du; pitch npts;
+// 3 km values per point reserved) and qa (the right-hand side).  The grid has one thread per
+// point and few waves per SIMD, so the registers cost no occupancy.  The second superdiagonal
+// is nonzero only after a row interchange, so up to 64 layers a bit mask in a register
+// records the interchanges and only those rows store it.  The backward sweep writes
+// qa = q0 + x.  Both sweeps keep the loads of CU rows in flight together.  A singular pivot
+// (impossible with kv >= 0 and positive thicknesses) is not reported.
 // MOM selects calc_VDiffMOMEq (:865-975): right-hand side dt rhs[k] (no division by e3) and,
 // for a Dirichlet bottom, Ad = 1 + alpha coef (bfric + r[km-1]) with the bottom friction
 // coefficient bfric of the grid point (:963).
@@ -256,7 +258,7 @@ struct VdiffRow {
   double al, ad, au, b;
 };
 
-template <bool MOM, bool MASK>
+template <bool MOM, bool MASK, int RR, int LR>
 __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0, const double* __restrict__ rhs,
                                                    const double* __restrict__ kv, const double* __restrict__ e3,
                                                    long ldg, int km, const double* __restrict__ cdk,
@@ -334,7 +336,18 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
   double d = r0.ad, du = r0.au, b = r0.b;
   VdiffRow r1 = next_row(1);
   double dl = r1.al, dn = r1.ad, dun = r1.au, bn = r1.b;
-  auto eliminate = [&](int j, const VdiffRow& r2) {  // step j; r2 = row j+2
+  // RR > 0: the finished rows of the last RR steps before the final one (j = J1 .. km-3) stay
+  // in registers (d, du, b); the LR rows before them (j = J0 .. J1-1) in LDS, row-major over
+  // the block's threads; only the rows above J0 make the workspace round trip.  The rare
+  // second superdiagonal of an interchange goes to the workspace in every tier.
+  const int J1 = km - 2 - RR;
+  const int J0 = J1 - LR > 0 ? J1 - LR : 0;
+  double rd[RR > 0 ? RR : 1], ru[RR > 0 ? RR : 1], rb[RR > 0 ? RR : 1];
+  __shared__ double sd[LR > 0 ? LR : 1][256], su[LR > 0 ? LR : 1][256], sb[LR > 0 ? LR : 1][256];
+  const int tx = threadIdx.x;
+  // step j with row j+2 = r2; the finished row j goes to register slot reg (reg >= 0), to
+  // LDS (J0 <= j < J1) or to the workspace
+  auto eliminate = [&](int j, const VdiffRow& r2, int reg) {
     double du2;
     const bool keep = fabs(d) >= fabs(dl);  // no interchange
     if (keep) {
@@ -354,16 +367,26 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
       b = bn;
       bn = tb - fact * bn;
     }
-    wd[(long)j * nw] = d;
-    wu[(long)j * nw] = du;
+    if (RR > 0 && reg >= 0) {
+      rd[reg] = d;
+      ru[reg] = du;
+      rb[reg] = b;
+    } else if (LR > 0 && j >= J0) {
+      sd[j - J0][tx] = d;
+      su[j - J0][tx] = du;
+      sb[j - J0][tx] = b;
+    } else {
+      wd[(long)j * nw] = d;
+      wu[(long)j * nw] = du;
+      qa[(long)j * ldo + i] = b;
+    }
     if (!MASK || !keep) w2[(long)j * nw] = du2;
     if (MASK && !keep) piv |= 1ull << j;
-    qa[(long)j * ldo + i] = b;
     d = dn, du = dun, b = bn;
     dl = r2.al, dn = r2.ad, dun = r2.au, bn = r2.b;
   };
   int j = 0;
-  for (; j + CU - 1 <= km - 3; j += CU) {  // rows j+2 .. j+CU+1 loaded together
+  for (; j + CU - 1 <= J1 - 1; j += CU) {  // rows j+2 .. j+CU+1 loaded together
     double pr[CU], pk[CU], pe[CU];
 #pragma unroll
     for (int u = 0; u < CU; u++) {
@@ -373,9 +396,27 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
       pe[u] = ntl(e3 + (long)kn * ldg + i);
     }
 #pragma unroll
-    for (int u = 0; u < CU; u++) eliminate(j + u, build_row(j + u + 2, pr[u], pk[u], pe[u]));
+    for (int u = 0; u < CU; u++) eliminate(j + u, build_row(j + u + 2, pr[u], pk[u], pe[u]), -1);
   }
-  for (; j <= km - 3; j++) eliminate(j, next_row(j + 2));
+  for (; j < J1; j++) eliminate(j, next_row(j + 2), -1);
+  if constexpr (RR > 0) {  // the register rows, unrolled so the slots are registers
+#pragma unroll
+    for (int u0 = 0; u0 < RR; u0 += CU) {
+      double pr[CU], pk[CU], pe[CU];
+#pragma unroll
+      for (int v = 0; v < CU; v++) {
+        if (u0 + v < RR) {
+          const int k = J1 + u0 + v + 2, kn = k + 1 < km ? k + 1 : km - 1;
+          pr[v] = ntl(rhs + (long)k * ldg + i);
+          pk[v] = ntl(kv + (long)kn * ldg + i);
+          pe[v] = ntl(e3 + (long)kn * ldg + i);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < CU; v++)
+        if (u0 + v < RR) eliminate(J1 + u0 + v, build_row(J1 + u0 + v + 2, pr[v], pk[v], pe[v]), u0 + v);
+    }
+  }
   {  // last elimination step, rows km-2 and km-1
     if (fabs(d) >= fabs(dl)) {
       const double fact = dl / d;
@@ -405,7 +446,44 @@ __global__ __launch_bounds__(256) void k_col_vdiff(const double* __restrict__ q0
     xb = xa;
     xa = x;
   };
-  int jb = km - 3;
+  if constexpr (RR > 0) {  // the register rows, in reverse
+#pragma unroll
+    for (int u0 = RR - 1; u0 >= 0; u0 -= CU) {
+      double pv[CU], pq[CU];
+#pragma unroll
+      for (int v = 0; v < CU; v++) {
+        if (u0 - v >= 0) {
+          const long r = J1 + u0 - v;
+          pv[v] = (!MASK || ((piv >> r) & 1ull)) ? w2[r * nw] : 0.0;
+          pq[v] = ntl(q0 + r * ldg + i);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < CU; v++)
+        if (u0 - v >= 0) back(J1 + u0 - v, rd[u0 - v], ru[u0 - v], pv[v], rb[u0 - v], pq[v]);
+    }
+  }
+  if constexpr (LR > 0) {  // the LDS rows, in reverse
+    int jl = J1 - 1;
+    for (; jl - (CU - 1) >= J0; jl -= CU) {
+      double pv[CU], pq[CU];
+#pragma unroll
+      for (int u = 0; u < CU; u++) {
+        const long r = jl - u;
+        pv[u] = (!MASK || ((piv >> r) & 1ull)) ? w2[r * nw] : 0.0;
+        pq[u] = ntl(q0 + r * ldg + i);
+      }
+#pragma unroll
+      for (int u = 0; u < CU; u++) {
+        const int r = jl - u;
+        back(r, sd[r - J0][tx], su[r - J0][tx], pv[u], sb[r - J0][tx], pq[u]);
+      }
+    }
+    for (; jl >= J0; jl--)
+      back(jl, sd[jl - J0][tx], su[jl - J0][tx], (!MASK || ((piv >> jl) & 1ull)) ? w2[(long)jl * nw] : 0.0,
+           sb[jl - J0][tx], ntl(q0 + (long)jl * ldg + i));
+  }
+  int jb = (LR > 0 ? J0 : J1) - 1;
   for (; jb - (CU - 1) >= 0; jb -= CU) {  // rows jb .. jb-CU+1 loaded together
     double pd[CU], pu[CU], pv[CU], pb[CU], pq[CU];
 #pragma unroll
@@ -465,8 +543,13 @@ int launch_col_vdiff(const double* q0, const double* rhs, const double* kv, cons
                      const double* cdk, const double* fdk, double dt, double alpha, int sfc_dirichlet,
                      int btm_dirichlet, const double* bfric, double* qa, long ldo, double* work, int npts,
                      hipStream_t s) {
-  auto k = bfric ? (km <= 64 ? k_col_vdiff<true, true> : k_col_vdiff<true, false>)
-                 : (km <= 64 ? k_col_vdiff<false, true> : k_col_vdiff<false, false>);
+  // the last finished rows stay in registers (32 with the interchange mask, 24 without) and
+  // the VDIFF_LR before them in LDS when the column is long enough for the register rows
+  constexpr int RM = 32, RN = 24;
+  auto k = bfric ? (km <= 64 ? (km >= RM + 2 ? k_col_vdiff<true, true, RM, VDIFF_LR> : k_col_vdiff<true, true, 0, 0>)
+                             : k_col_vdiff<true, false, RN, VDIFF_LR>)
+                 : (km <= 64 ? (km >= RM + 2 ? k_col_vdiff<false, true, RM, VDIFF_LR> : k_col_vdiff<false, true, 0, 0>)
+                             : k_col_vdiff<false, false, RN, VDIFF_LR>);
   hipLaunchKernelGGL(k, dim3(col_grid(npts)), dim3(256), 0, s, q0, rhs, kv, e3, ldg, km, cdk, fdk, dt, alpha,
                      sfc_dirichlet, btm_dirichlet, bfric, qa, ldo, work, npts);
   DSHT_HIP(hipGetLastError());

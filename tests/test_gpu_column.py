@@ -202,13 +202,17 @@ def _vdiff_case(km, J, I, seed, pivot):
 
 
 @pytest.mark.parametrize("N,I,J,km", [(42, 128, 64, 20), (85, 256, 128, 60), (170, 512, 256, 60),
-                                      (21, 64, 32, 2), (21, 64, 32, 3), (21, 64, 32, 4), (21, 64, 32, 64),
-                                      (21, 64, 32, 70)])
+                                      (21, 64, 32, 2), (21, 64, 32, 3), (21, 64, 32, 4), (21, 64, 32, 33),
+                                      (21, 64, 32, 34), (21, 64, 32, 36), (21, 64, 32, 40), (21, 64, 32, 44),
+                                      (21, 64, 32, 64), (21, 64, 32, 70), (21, 64, 32, 100)])
 @pytest.mark.parametrize("pivot", [False, True])
 def test_vdiff_bitwise(gpu, N, I, J, km, pivot):
     """calc_VDiffTRCEq with DGTSV's elimination, bit for bit against the oracle, with and
     without row interchanges; up to 64 layers the interchanges are kept in a bit mask, above
-    (70) every row stores its second superdiagonal."""
+    (70, 100) every row stores its second superdiagonal.  The layer counts cover where the
+    finished rows wait: all in the workspace (km < 34), registers and part of the LDS rows
+    only (34-43), registers, all LDS rows and the workspace (44 and up; 24 register rows
+    above 64 layers)."""
     import torch
     import dsht
     import oracle
