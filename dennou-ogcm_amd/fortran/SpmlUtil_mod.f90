@@ -25,6 +25,13 @@ module SpmlUtil_mod
   integer(c_int), parameter :: DSHT_GRADLAMBDA = 3, DSHT_GRADMU = 4, DSHT_GRADLON = 5, DSHT_GRADLAT = 6
   integer(c_int), parameter :: DSHT_COEF_LAPLA = 0, DSHT_COEF_LAPLAINV = 1
 
+  ! dsht_redigm_config (include/dsht.h): RediGM_nml parameters and the constants it uses
+  type, bind(C) :: redigm_config_t
+    real(c_double) :: kappa_redi, kappa_gm, slope_max, sd
+    integer(c_int) :: interior_taper, pbl_taper
+    real(c_double) :: omega, ref_dens, grav
+  end type redigm_config_t
+
   interface
     integer(c_int) function c_create(nmax, im, jm, kmax, radius, plan) bind(C, name="dsht_create")
       import :: c_int, c_double, c_ptr
@@ -188,6 +195,19 @@ module SpmlUtil_mod
       type(c_ptr), value :: bfric
       real(c_double) :: qa(*)
     end function
+    integer(c_int) function c_redigm_default_config(cfg) bind(C, name="dsht_redigm_default_config")
+      import :: c_int, redigm_config_t
+      type(redigm_config_t) :: cfg
+    end function
+    integer(c_int) function c_redigm(plan, cfg, pt, sa, h, z, nk, topo, cdk, rfdk, pt_rhs, sa_rhs) &
+         bind(C, name="dsht_redigm")
+      import :: c_int, c_ptr, c_double, redigm_config_t
+      type(c_ptr), value :: plan
+      type(redigm_config_t), intent(in) :: cfg
+      integer(c_int), value :: nk
+      real(c_double), intent(in) :: pt(*), sa(*), h(*), z(*), topo(*), cdk(*), rfdk(*)
+      real(c_double) :: pt_rhs(*), sa_rhs(*)
+    end function
   end interface
 
   ! public procedures (SpmlUtil_mod.f90:71-135, horizontal part)
@@ -206,6 +226,8 @@ module SpmlUtil_mod
   ! extensions: the grid-space column kernels of the dynamics and physics modules
   ! (HBEDiagnose_OMG / _HydPres / _UVBarot, calc_ADVFlxZ_QUICK, calc_VDiffTRCEq / MOMEq)
   public :: calc_OMG_xyr, calc_HydPres_xyz, calc_UVBarot_xy, calc_ADVFlxZ_QUICK_xyr, calc_VDiff_xyz
+  ! extension: the Redi/GM isopycnal mixing of LPhys_RediGM_hspm_vfvm_AddMixingTerm
+  public :: calc_RediGM_xyz
 
   interface calc_VorDiv2UV
     module procedure calc_VorDiv2UV_w_xy
@@ -648,5 +670,52 @@ contains
                          alpha, int(ichar(SeaSfcBCType), c_int), int(ichar(SeaBtmBCType), c_int), bf, xyz_qA), &
              'calc_VDiff_xyz')
   end subroutine calc_VDiff_xyz
+
+  ! LPhys_RediGM_hspm_vfvm_AddMixingTerm (LPhys_RediGM_hspm_vfvm_mod.f90:218-372): adds the
+  ! Redi and GM tendencies of potential temperature and salinity to xyz_PTemp_RHS and
+  ! xyz_Salt_RHS.  Layers KS:KE of the fields, xy_Topo, z_CDK(KS:KE) and z_RFDK(KS-1:KE).
+  ! Parameters default to RediGM_nml's defaults of APEOGCircExp.nml (DiffCoefRedi = DiffCoefGM
+  ! = 800, InteriorTaperingName = 'DM95', PBLTaperingName = 'LDD95', SlopeMax = 4e-3,
+  ! Sd = 1e-3) and Earth's Omega, RefDens 1027, Grav 9.79764.
+  subroutine calc_RediGM_xyz(xyz_PTemp_RHS, xyz_Salt_RHS, xyz_PTemp, xyz_Salt, xyz_H, xyz_Z, xy_Topo, &
+                             z_CDK, z_RFDK, DiffCoefRedi, DiffCoefGM, SlopeMax, Sd, InteriorTaperingName, &
+                             Omega, RefDens, Grav)
+    real(DP), intent(inout) :: xyz_PTemp_RHS(:,:,:), xyz_Salt_RHS(:,:,:)
+    real(DP), intent(in) :: xyz_PTemp(:,:,:), xyz_Salt(:,:,:), xyz_H(:,:,:), xyz_Z(:,:,:)
+    real(DP), intent(in) :: xy_Topo(0:im-1, jm), z_CDK(:), z_RFDK(:)
+    real(DP), intent(in), optional :: DiffCoefRedi, DiffCoefGM, SlopeMax, Sd, Omega, RefDens, Grav
+    character(*), intent(in), optional :: InteriorTaperingName
+    type(redigm_config_t) :: cfg
+    integer :: nk
+    nk = size(xyz_PTemp, 3)
+    call col_shape('calc_RediGM_xyz', nk, xyz_PTemp, nk)
+    call col_shape('calc_RediGM_xyz', nk, xyz_Salt, nk)
+    call col_shape('calc_RediGM_xyz', nk, xyz_H, nk)
+    call col_shape('calc_RediGM_xyz', nk, xyz_Z, nk)
+    call col_shape('calc_RediGM_xyz', nk, xyz_PTemp_RHS, nk)
+    call col_shape('calc_RediGM_xyz', nk, xyz_Salt_RHS, nk)
+    call col_vec('calc_RediGM_xyz', 'z_CDK', z_CDK, nk)
+    call col_vec('calc_RediGM_xyz', 'z_RFDK', z_RFDK, nk + 1)
+    call chk(c_redigm_default_config(cfg), 'calc_RediGM_xyz')
+    if (present(DiffCoefRedi)) cfg%kappa_redi = DiffCoefRedi
+    if (present(DiffCoefGM)) cfg%kappa_gm = DiffCoefGM
+    if (present(SlopeMax)) cfg%slope_max = SlopeMax
+    if (present(Sd)) cfg%sd = Sd
+    if (present(Omega)) cfg%omega = Omega
+    if (present(RefDens)) cfg%ref_dens = RefDens
+    if (present(Grav)) cfg%grav = Grav
+    if (present(InteriorTaperingName)) then
+      select case (trim(InteriorTaperingName))
+      case ('DM95')
+        cfg%interior_taper = 2
+      case ('GKW91')
+        cfg%interior_taper = 1
+      case default
+        call arg_error('calc_RediGM_xyz', 'InteriorTaperingName must be DM95 or GKW91')
+      end select
+    end if
+    call chk(c_redigm(plan, cfg, xyz_PTemp, xyz_Salt, xyz_H, xyz_Z, int(nk, c_int), xy_Topo, z_CDK, z_RFDK, &
+                      xyz_PTemp_RHS, xyz_Salt_RHS), 'calc_RediGM_xyz')
+  end subroutine calc_RediGM_xyz
 
 end module SpmlUtil_mod

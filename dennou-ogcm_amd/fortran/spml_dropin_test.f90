@@ -148,6 +148,7 @@ program spml_dropin_test
   call check('per-layer calls', err, 1d-12)
 
   call column_checks()
+  call redigm_checks()
 
   if (command_argument_count() > 0) then
     call get_command_argument(1, arg)
@@ -226,6 +227,62 @@ contains
     e1 = maxval(abs(qa(:, :, 1:NK-1) - (T0 + DT * 1d-4)))
     call check('calc_VDiff_xyz momentum form (bottom friction)', max(e1, e2), 1d-14)
   end subroutine column_checks
+
+  ! the Redi/GM mixing through calc_RediGM_xyz: a flat-bottomed, stratified ocean with a
+  ! tilted temperature field gets a nonzero tendency whose tracer content is zero (the
+  ! horizontal divergence has no global mean and the vertical flux divergence telescopes);
+  ! a horizontally uniform ocean has no isopycnal slope and gets no tendency
+  subroutine redigm_checks()
+    integer, parameter :: NK = 6
+    real(DP), parameter :: DEPTH = 4000d0
+    real(DP) :: pt(0:IM-1, JM, NK), sa(0:IM-1, JM, NK), h(0:IM-1, JM, NK), z(0:IM-1, JM, NK)
+    real(DP) :: prhs(0:IM-1, JM, NK), srhs(0:IM-1, JM, NK), topo(0:IM-1, JM)
+    real(DP) :: fk(0:NK), ck(0:NK+1), cdk(NK), rfdk(NK+1), content, scale
+    integer :: kk, i, j
+    do kk = 0, NK
+      fk(kk) = -0.5d0 + 0.5d0 * cos(acos(-1d0) * kk / NK)
+    end do
+    do kk = 1, NK
+      cdk(kk) = fk(kk - 1) - fk(kk)
+      ck(kk) = 0.5d0 * (fk(kk - 1) + fk(kk))
+    end do
+    ck(0) = fk(0) + 0.5d0 * cdk(1)
+    ck(NK + 1) = fk(NK) - 0.5d0 * cdk(NK)
+    do kk = 1, NK + 1
+      rfdk(kk) = 1d0 / (ck(kk - 1) - ck(kk))
+    end do
+    topo = DEPTH
+    do kk = 1, NK
+      h(:, :, kk) = DEPTH
+      z(:, :, kk) = DEPTH * ck(kk)
+      do j = 1, JM
+        do i = 0, IM - 1
+          pt(i, j, kk) = 275d0 + 20d0 * exp(z(i, j, kk) / 800d0) &
+               & + 3d0 * cos(y_Lat(j))**2 * cos(x_Lon(i)) * exp(z(i, j, kk) / 1500d0)
+          sa(i, j, kk) = 34.5d0 + 0.5d0 * exp(z(i, j, kk) / 1000d0) + 0.2d0 * sin(y_Lat(j)) * exp(z(i, j, kk) / 2000d0)
+        end do
+      end do
+    end do
+    prhs = 0d0
+    srhs = 0d0
+    call calc_RediGM_xyz(prhs, srhs, pt, sa, h, z, topo, cdk, rfdk)
+    content = 0d0
+    scale = 0d0
+    do kk = 1, NK
+      content = content + cdk(kk) * AvrLonLat_xy(prhs(:, :, kk))
+      scale = scale + cdk(kk) * AvrLonLat_xy(abs(prhs(:, :, kk)))
+    end do
+    call check('calc_RediGM_xyz (nonzero tendency)', merge(0d0, 1d0, scale > 0d0), 0d0)
+    call check('calc_RediGM_xyz (tracer content conserved)', abs(content) / scale, 1d-11)
+    do kk = 1, NK
+      pt(:, :, kk) = 275d0 + 20d0 * exp(z(:, :, kk) / 800d0)
+      sa(:, :, kk) = 35d0
+    end do
+    prhs = 0d0
+    srhs = 0d0
+    call calc_RediGM_xyz(prhs, srhs, pt, sa, h, z, topo, cdk, rfdk, InteriorTaperingName='GKW91')
+    call check('calc_RediGM_xyz (no slope, no tendency)', max(maxval(abs(prhs)), maxval(abs(srhs))), 1d-20)
+  end subroutine redigm_checks
 
   ! a halo-sized (KA = nk + 2) layer-thickness vector: the shim must refuse it
   subroutine bad_size()
