@@ -803,6 +803,18 @@ int dsht_diag_d(dsht_plan* p, const double* d, const double* w, double* out, lon
   return launch_diag(d, w, out, sl, sc, p->L, ncols, (hipStream_t)stream);
 }
 
+int dsht_spec_energy_d(dsht_plan* p, int kind, const double* w, long sl, long sc, int ncols, double* nma, double* na,
+                       void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (kind != DSHT_SPEC_ENERGY && kind != DSHT_SPEC_ENSTROPHY)
+    return fail(DSHT_EARG, "dsht_spec_energy_d: kind must be DSHT_SPEC_ENERGY or DSHT_SPEC_ENSTROPHY");
+  if (ncols < 0) return fail(DSHT_EARG, "ncols < 0");
+  if (!nma && !na) return fail(DSHT_EARG, "dsht_spec_energy_d: both outputs are NULL");
+  if (ncols == 0) return 0;
+  if (int rc = check_spec(w, sl, sc, ncols, p)) return rc;
+  return launch_spec_energy(p, kind == DSHT_SPEC_ENSTROPHY, w, sl, sc, ncols, nma, na, (hipStream_t)stream);
+}
+
 int dsht_intlonlat_d(dsht_plan* p, const double* xy, long ldg, double* out, int ncols, void* stream) {
   if (int rc = check_plan(p)) return rc;
   if (ncols < 0 || !out) return fail(DSHT_EARG, "bad arguments");
@@ -1402,6 +1414,21 @@ int dsht_redigm(dsht_plan* p, const dsht_redigm_config* cfg, const double* ptemp
     return rc;
   DSHT_HIP(hipMemcpyAsync(ptemp_rhs, drhs, sizeof(double) * IJ * km, hipMemcpyDeviceToHost, c->s));
   return d2h_sync(c, salt_rhs, drhs + IJ * km, IJ * km);
+}
+
+int dsht_spec_energy(dsht_plan* p, int kind, const double* wa, int ncols, double* nma, double* na) {
+  HOST_PROLOGUE(p, ncols);
+  if (!wa || (!nma && !na)) return fail(DSHT_EARG, "NULL array");
+  const size_t n1 = (size_t)p->N + 1, nm = n1 * (2 * (size_t)p->N + 1) * ncols, nn = n1 * ncols;
+  double *dw, *dout;
+  int rc;
+  if ((rc = ctx_buf(c, 1, LL * ncols, &dw)) || (rc = ctx_buf(c, 0, nm + nn, &dout))) return rc;
+  if ((rc = h2d(c, dw, wa, LL * ncols))) return rc;
+  if ((rc = dsht_spec_energy_d(p, kind, dw, 1, (long)LL, ncols, nma ? dout : nullptr, na ? dout + nm : nullptr,
+                               c->s)))
+    return rc;
+  if (nma && na) DSHT_HIP(hipMemcpyAsync(nma, dout, sizeof(double) * nm, hipMemcpyDeviceToHost, c->s));
+  return nma && !na ? d2h_sync(c, nma, dout, nm) : d2h_sync(c, na, dout + nm, nn);
 }
 
 int dsht_coef(const dsht_plan* pc, int which, const double* params, int nparams, double* out) {

@@ -287,6 +287,8 @@ int launch_redigm_col(const dsht_plan* p, const RediParams& c, const double* pt,
                       long ldo, hipStream_t s);
 int launch_redigm_rhs(const dsht_plan* p, double* rt, double* rs, long ldo, const double* hdiv, const double* vdiv,
                       long ldw, int km, hipStream_t s);
+int launch_spec_energy(const dsht_plan* p, int ens, const double* w, long sl, long sc, int ncols, double* nma,
+                       double* na, hipStream_t s);
 int launch_lonlat_sum(const dsht_plan* p, const double* xy, long ldg, int ncols, double* out,
                       double factor, hipStream_t s);
 int launch_lon_sum(const dsht_plan* p, const double* xy, long ldg, int ncols, double* out,

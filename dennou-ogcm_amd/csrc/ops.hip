@@ -7,6 +7,7 @@
 //     (DOGCM_Phys_hspm_vfvm_mod.f90:640, 741-742)
 //   sums: IntLonLat_xy, AvrLonLat_xy, ya_IntLon_xya (SpmlUtil_mod.f90:123-124)
 //   Interpolate_w (SpmlUtil_mod.f90:127)
+//   energy and enstrophy spectra of a streamfunction (SpmlUtil_mod.f90:131-132)
 #include <cmath>
 
 #include "dsht_internal.hpp"
@@ -36,6 +37,45 @@ int launch_diag(const double* d, const double* w, double* out, long sl, long sc,
   long n = (long)L * ncols;
   if (n == 0) return 0;
   hipLaunchKernelGGL(k_diag, dim3((unsigned)cdivl(n, 256)), dim3(256), 0, s, d, w, out, sl, sc, L, ncols);
+  DSHT_HIP(hipGetLastError());
+  return 0;
+}
+
+// Energy and enstrophy spectra of a streamfunction (nma_/na_EnergyFromStreamfunc_wa and
+// nma_/na_EnstrophyFromStreamfunc_wa, cascaded from SPMODEL at SpmlUtil_mod.f90:131-132).
+// With q = sum s_n^m P_n^m e^{i m lambda} (P normalised to 2) the global mean of q^2 is
+// sum_n [w(n,0)^2 + 2 sum_{m>0} (w(n,m)^2 + w(n,-m)^2)], so on the unit sphere
+//   E(n,+-m) = (1/2) n(n+1) c_m w(n,+-m)^2,   Z(n,+-m) = (1/2) n^2 (n+1)^2 c_m w(n,+-m)^2,
+// c_0 = 1, c_m = 2 otherwise; summed over (n, m) they are the global means of |grad psi|^2/2
+// and (lapla psi)^2/2.  nma(n, m, c) at n + (N+1) ((m+N) + (2N+1) c), zero where |m| > n;
+// na(n, c) = sum over m = -n..n (ascending) at n + (N+1) c.  One thread per (n, column).
+__global__ void k_spec_energy(const double* __restrict__ w, long sl, long sc, int N, int ncols, int ens,
+                              double* __restrict__ nma, double* __restrict__ na) {
+#pragma clang fp contract(off)
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)(N + 1) * ncols) return;
+  const int n = (int)(e % (N + 1)), c = (int)(e / (N + 1));
+  const double nn1 = (double)n * (double)(n + 1);
+  const double f = ens ? 0.5 * nn1 * nn1 : 0.5 * nn1;
+  double sum = 0.0;
+  for (int m = -N; m <= N; m++) {
+    double v = 0.0;
+    if (m >= -n && m <= n) {
+      const double x = w[((long)n * (n + 1) + m) * sl + (long)c * sc];
+      v = ((x * x) * (m == 0 ? 1.0 : 2.0)) * f;
+      sum += v;
+    }
+    if (nma) nma[n + (long)(N + 1) * ((m + N) + (long)(2 * N + 1) * c)] = v;
+  }
+  if (na) na[n + (long)(N + 1) * c] = sum;
+}
+
+int launch_spec_energy(const dsht_plan* p, int ens, const double* w, long sl, long sc, int ncols, double* nma,
+                       double* na, hipStream_t s) {
+  const long n = (long)(p->N + 1) * ncols;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_spec_energy, dim3((unsigned)cdivl(n, 256)), dim3(256), 0, s, w, sl, sc, p->N, ncols, ens, nma,
+                     na);
   DSHT_HIP(hipGetLastError());
   return 0;
 }

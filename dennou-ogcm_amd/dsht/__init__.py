@@ -49,7 +49,9 @@ EXPORTS = [
     "dsht_lfam3_default_config", "dsht_lfam3_create", "dsht_lfam3_destroy", "dsht_lfam3_set_rotation",
     "dsht_lfam3_state", "dsht_lfam3_step", "dsht_lfam3_copy_run", "dsht_lfam3_units",
     "dsht_redigm_default_config", "dsht_col_redigm_d", "dsht_redigm_work_size", "dsht_redigm_d", "dsht_redigm",
+    "dsht_spec_energy", "dsht_spec_energy_d",
 ]
+SPEC_ENERGY, SPEC_ENSTROPHY = 0, 1
 TUNE_CHUNK_COLS, TUNE_LEG_CB, TUNE_FFT, TUNE_LEG, TUNE_FFT_ORDER = 0, 1, 2, 3, 5
 
 
@@ -181,6 +183,8 @@ def load():
         "dsht_lfam3_step": ([vp, i, vp], i),
         "dsht_lfam3_copy_run": ([vp, i, i, vp], i),
         "dsht_lfam3_units": ([vp, i, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)], i),
+        "dsht_spec_energy": ([vp, i, vp, i, vp, vp], i),
+        "dsht_spec_energy_d": ([vp, i, vp, lg, lg, i, vp, vp, vp], i),
         "dsht_redigm_default_config": ([ctypes.POINTER(RediGMConfig)], i),
         "dsht_col_redigm_d": ([vp, ctypes.POINTER(RediGMConfig), vp, vp, vp, vp, lg, i, vp, vp, vp, vp, vp, vp, vp,
                                vp, lg, vp], i),
@@ -762,6 +766,30 @@ class Plan:
                                      float(dt), float(alpha), ord(sfc_bc), ord(btm_bc),
                                      _ptr(bf) if bf is not None else None, _ptr(out)))
         return out
+
+    # energy and enstrophy spectra of streamfunctions (nma_/na_EnergyFromStreamfunc_wa,
+    # nma_/na_EnstrophyFromStreamfunc_wa, SpmlUtil_mod.f90:131-132)
+    def spec_energy(self, wa, kind=SPEC_ENERGY):
+        """(nma, na) of host spectra wa (ncols, L): nma (ncols, 2N+1, N+1) with nma[c, m+N, n]
+        (zero where |m| > n), na (ncols, N+1) their sums over m; unit sphere."""
+        w = _f64(np.asarray(wa, dtype=np.float64).reshape(-1, self.lm))
+        nc = w.shape[0]
+        nma = np.zeros((nc, 2 * self.nmax + 1, self.nmax + 1))
+        na = np.zeros((nc, self.nmax + 1))
+        _check(load().dsht_spec_energy(self.h, int(kind), _ptr(w), nc, _ptr(nma), _ptr(na)))
+        return nma, na
+
+    def nma_EnergyFromStreamfunc_wa(self, wa):
+        return self.spec_energy(wa, SPEC_ENERGY)[0]
+
+    def na_EnergyFromStreamfunc_wa(self, wa):
+        return self.spec_energy(wa, SPEC_ENERGY)[1]
+
+    def nma_EnstrophyFromStreamfunc_wa(self, wa):
+        return self.spec_energy(wa, SPEC_ENSTROPHY)[0]
+
+    def na_EnstrophyFromStreamfunc_wa(self, wa):
+        return self.spec_energy(wa, SPEC_ENSTROPHY)[1]
 
     # Redi/GM isopycnal mixing (LPhys_RediGM_hspm_vfvm_AddMixingTerm,
     # LPhys_RediGM_hspm_vfvm_mod.f90:218-372)

@@ -195,6 +195,12 @@ module SpmlUtil_mod
       type(c_ptr), value :: bfric
       real(c_double) :: qa(*)
     end function
+    integer(c_int) function c_spec_energy(plan, kind, wa, ncols, nma, na) bind(C, name="dsht_spec_energy")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan, nma, na
+      integer(c_int), value :: kind, ncols
+      real(c_double), intent(in) :: wa(*)
+    end function
     integer(c_int) function c_redigm_default_config(cfg) bind(C, name="dsht_redigm_default_config")
       import :: c_int, redigm_config_t
       type(redigm_config_t) :: cfg
@@ -221,6 +227,8 @@ module SpmlUtil_mod
   public :: l_nm, nm_l
   public :: IntLonLat_xy, ya_IntLon_xya, AvrLonLat_xy, ya_AvrLon_xya
   public :: a_Interpolate_wa, Interpolate_w
+  public :: nma_EnergyFromStreamfunc_wa, na_EnergyFromStreamfunc_wa
+  public :: nma_EnstrophyFromStreamfunc_wa, na_EnstrophyFromStreamfunc_wa
   ! extension without a counterpart in the reference module (see calc_GradLonLat_xya_wa)
   public :: calc_GradLonLat_xya_wa
   ! extensions: the grid-space column kernels of the dynamics and physics modules
@@ -670,6 +678,58 @@ contains
                          alpha, int(ichar(SeaSfcBCType), c_int), int(ichar(SeaBtmBCType), c_int), bf, xyz_qA), &
              'calc_VDiff_xyz')
   end subroutine calc_VDiff_xyz
+
+  ! Energy and enstrophy spectra of streamfunctions (cascaded from SPMODEL,
+  ! SpmlUtil_mod.f90:131-132), on the unit sphere: E(n,+-m) = 1/2 n(n+1) c_m w(n,+-m)**2 and
+  ! the enstrophy with (n(n+1))**2, c_0 = 1, c_m = 2; the sums over (n, m) are the global means
+  ! of |grad psi|**2/2 and (lapla psi)**2/2.  nma(0:nm, -nm:nm, :) is zero where |m| > n.
+  function nma_EnergyFromStreamfunc_wa(wa_StrFunc) result(nma)
+    real(DP), intent(in) :: wa_StrFunc(:,:)
+    real(DP) :: nma(0:nm, -nm:nm, size(wa_StrFunc, 2))
+    call spec_nma(0, wa_StrFunc, nma, 'nma_EnergyFromStreamfunc_wa')
+  end function nma_EnergyFromStreamfunc_wa
+
+  function nma_EnstrophyFromStreamfunc_wa(wa_StrFunc) result(nma)
+    real(DP), intent(in) :: wa_StrFunc(:,:)
+    real(DP) :: nma(0:nm, -nm:nm, size(wa_StrFunc, 2))
+    call spec_nma(1, wa_StrFunc, nma, 'nma_EnstrophyFromStreamfunc_wa')
+  end function nma_EnstrophyFromStreamfunc_wa
+
+  function na_EnergyFromStreamfunc_wa(wa_StrFunc) result(na)
+    real(DP), intent(in) :: wa_StrFunc(:,:)
+    real(DP) :: na(0:nm, size(wa_StrFunc, 2))
+    call spec_na(0, wa_StrFunc, na, 'na_EnergyFromStreamfunc_wa')
+  end function na_EnergyFromStreamfunc_wa
+
+  function na_EnstrophyFromStreamfunc_wa(wa_StrFunc) result(na)
+    real(DP), intent(in) :: wa_StrFunc(:,:)
+    real(DP) :: na(0:nm, size(wa_StrFunc, 2))
+    call spec_na(1, wa_StrFunc, na, 'na_EnstrophyFromStreamfunc_wa')
+  end function na_EnstrophyFromStreamfunc_wa
+
+  subroutine spec_nma(kind, wa, out, where)
+    integer, intent(in) :: kind
+    real(DP), intent(in) :: wa(:,:)
+    real(DP), intent(out) :: out(:,:,:)
+    character(*), intent(in) :: where
+    real(DP), allocatable, target :: buf(:,:,:)
+    if (size(wa, 1) /= lm) call arg_error(where, 'spectral arrays must have lm rows')
+    allocate(buf(0:nm, -nm:nm, size(wa, 2)))
+    call chk(c_spec_energy(plan, int(kind, c_int), wa, int(size(wa, 2), c_int), c_loc(buf), c_null_ptr), where)
+    out = buf
+  end subroutine spec_nma
+
+  subroutine spec_na(kind, wa, out, where)
+    integer, intent(in) :: kind
+    real(DP), intent(in) :: wa(:,:)
+    real(DP), intent(out) :: out(:,:)
+    character(*), intent(in) :: where
+    real(DP), allocatable, target :: buf(:,:)
+    if (size(wa, 1) /= lm) call arg_error(where, 'spectral arrays must have lm rows')
+    allocate(buf(0:nm, size(wa, 2)))
+    call chk(c_spec_energy(plan, int(kind, c_int), wa, int(size(wa, 2), c_int), c_null_ptr, c_loc(buf)), where)
+    out = buf
+  end subroutine spec_na
 
   ! LPhys_RediGM_hspm_vfvm_AddMixingTerm (LPhys_RediGM_hspm_vfvm_mod.f90:218-372): adds the
   ! Redi and GM tendencies of potential temperature and salinity to xyz_PTemp_RHS and

@@ -149,6 +149,7 @@ program spml_dropin_test
 
   call column_checks()
   call redigm_checks()
+  call spectrum_checks()
 
   if (command_argument_count() > 0) then
     call get_command_argument(1, arg)
@@ -283,6 +284,33 @@ contains
     call calc_RediGM_xyz(prhs, srhs, pt, sa, h, z, topo, cdk, rfdk, InteriorTaperingName='GKW91')
     call check('calc_RediGM_xyz (no slope, no tendency)', max(maxval(abs(prhs)), maxval(abs(srhs))), 1d-20)
   end subroutine redigm_checks
+
+  ! energy and enstrophy spectra: for psi = a P_1^0 + b P_3^2 cos(2 lambda) the spectra hold
+  ! a**2 at (1, 0) and 1/2 * 12 * 2 * (b/2)**2 at (3, 2) (s_3^2 = b/2), and their sums equal the
+  ! global means of |grad psi|**2/2 and (lapla psi)**2/2 on the grid
+  subroutine spectrum_checks()
+    real(DP) :: ws((NM + 1)**2, 1), en(0:NM, -NM:NM, 1), ena(0:NM, 1), zn(0:NM, -NM:NM, 1), zna(0:NM, 1)
+    real(DP) :: glon(0:IM-1, JM), glat(0:IM-1, JM), zeta(0:IM-1, JM), e_grid, z_grid
+    real(DP), parameter :: a = 0.7d0, b = -0.4d0
+    ws = 0d0
+    ws(l_nm(1, 0), 1) = a
+    ws(l_nm(3, 2), 1) = b / 2d0
+    en = nma_EnergyFromStreamfunc_wa(ws)
+    ena = na_EnergyFromStreamfunc_wa(ws)
+    zn = nma_EnstrophyFromStreamfunc_wa(ws)
+    zna = na_EnstrophyFromStreamfunc_wa(ws)
+    call check('nma_EnergyFromStreamfunc_wa (1,0) and (3,2)', &
+         max(abs(en(1, 0, 1) - a**2), abs(en(3, 2, 1) - 12d0 * (b / 2d0)**2), &
+             abs(sum(en) - en(1, 0, 1) - en(3, 2, 1))), 1d-15)
+    call check('na_ = sum over m of nma_', maxval(abs(ena(:, 1) - sum(en(:, :, 1), 2))), 1d-15)
+    glon = xy_GradLon_w(ws(:, 1))
+    glat = xy_GradLat_w(ws(:, 1))
+    e_grid = AvrLonLat_xy(0.5d0 * (glon**2 + glat**2))
+    call check('energy spectrum sums to the mean of |grad psi|**2/2', abs(sum(ena) - e_grid) / e_grid, 1d-13)
+    zeta = xy_w(w_Lapla_w(ws(:, 1)))
+    z_grid = AvrLonLat_xy(0.5d0 * zeta**2)
+    call check('enstrophy spectrum sums to the mean of (lapla psi)**2/2', abs(sum(zna) - z_grid) / z_grid, 1d-13)
+  end subroutine spectrum_checks
 
   ! a halo-sized (KA = nk + 2) layer-thickness vector: the shim must refuse it
   subroutine bad_size()

@@ -138,6 +138,19 @@ int dsht_ya_avrlon(dsht_plan* plan, const double* xy, double* out, int ncols);
 int dsht_interpolate(dsht_plan* plan, const double* w, const double* lon, const double* lat,
                      double* out, int npts);
 
+/* Energy and enstrophy spectra of streamfunctions: nma_EnergyFromStreamfunc_wa,
+ * na_EnergyFromStreamfunc_wa, nma_EnstrophyFromStreamfunc_wa, na_EnstrophyFromStreamfunc_wa
+ * (cascaded from SPMODEL, SpmlUtil_mod.f90:131-132).  On the unit sphere, for the
+ * normalisation above: E(n,+-m) = (1/2) n(n+1) c_m w(n,+-m)^2 and Z(n,+-m) =
+ * (1/2) n^2 (n+1)^2 c_m w(n,+-m)^2 with c_0 = 1, c_m = 2 (m != 0), so that their sum over
+ * (n, m) is the global mean of |grad psi|^2 / 2, resp. (lapla psi)^2 / 2 (divide by R^2, R^4
+ * for a sphere of radius R).  nma(0:nmax, -nmax:nmax, ncols), Fortran order, zero where
+ * |m| > n; na(0:nmax, ncols) = the sums over m.  Either output may be NULL.  The reference's
+ * SPMODEL source is absent: this normalisation is fixed by the Parseval identity above.     */
+#define DSHT_SPEC_ENERGY 0
+#define DSHT_SPEC_ENSTROPHY 1
+int dsht_spec_energy(dsht_plan* plan, int kind, const double* wa, int ncols, double* nma, double* na);
+
 /* ---------------------------------------------------------------- device API
  * Device pointers, stream-ordered on `stream` (a hipStream_t; NULL = default).
  * diag: optional device array of L multipliers fused into the spectral side
@@ -184,6 +197,9 @@ int dsht_vordiv2uv_d(dsht_plan* plan, const double* vor, const double* div, long
                      double* u, double* v, long ldg, int ncols, void* stream);
 int dsht_diag_d(dsht_plan* plan, const double* d, const double* w, double* out, long sl, long sc,
                 int ncols, void* stream);
+/* device form of dsht_spec_energy; nma, na device memory                                  */
+int dsht_spec_energy_d(dsht_plan* plan, int kind, const double* w, long sl, long sc, int ncols,
+                       double* nma, double* na, void* stream);
 /* FP64 global sums; out is device memory */
 int dsht_intlonlat_d(dsht_plan* plan, const double* xy, long ldg, double* out, int ncols,
                      void* stream);

@@ -358,3 +358,26 @@ def col_vdiff(q0, rhs, kv, e3, cdk, fdk, dt, alpha, sfc_bc="D", btm_bc="N", btm_
         sw = lib().orc_col_vdiff_mom(_p(q0), _p(rhs), _p(kv), _p(e3), km, _p(cdk), _p(fdk), float(dt),
                                      float(alpha), ord(sfc_bc), ord(btm_bc), _p(bf), _p(qa), npts)
     return qa, int(sw)
+
+
+def spec_energy(w, nmax, enstrophy=False):
+    """nma_/na_EnergyFromStreamfunc_wa and the enstrophy forms (SPMODEL, cascaded at
+    SpmlUtil_mod.f90:131-132) on the unit sphere for this build's normalisation: E(n, +-m) =
+    ((w^2) c_m) (n(n+1)/2), c_0 = 1, c_m = 2; the enstrophy with (n(n+1))^2 / 2.  w: (ncols, L).
+    Returns nma (ncols, 2N+1, N+1) [c, m+N, n] (zero where |m| > n) and na (ncols, N+1), the
+    sums over m = -n..n in ascending order."""
+    w = np.asarray(w, dtype=np.float64).reshape(-1, (nmax + 1) ** 2)
+    nc = w.shape[0]
+    nma = np.zeros((nc, 2 * nmax + 1, nmax + 1))
+    na = np.zeros((nc, nmax + 1))
+    for n in range(nmax + 1):
+        nn1 = float(n) * float(n + 1)
+        f = 0.5 * nn1 * nn1 if enstrophy else 0.5 * nn1
+        acc = np.zeros(nc)
+        for m in range(-n, n + 1):
+            x = w[:, n * (n + 1) + m]
+            v = ((x * x) * (1.0 if m == 0 else 2.0)) * f
+            nma[:, m + nmax, n] = v
+            acc = acc + v
+        na[:, n] = acc
+    return nma, na
