@@ -793,12 +793,23 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     if args.copy_calibration:
+        # three kernels of known traffic for the HBM counters: a 256 MiB device copy (reads and
+        # writes 256 MiB), a sum over 256 MiB (reads only) and a 256 MiB fill (writes only)
         a = torch.ones(32 << 20, dtype=torch.float64, device=dev)  # 256 MiB
         b = torch.empty_like(a)
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8, "launches": 3}))
+        s = torch.empty((), dtype=torch.float64, device=dev)
+        for _ in range(3):
+            torch.sum(a, dim=0, out=s)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.fill_(2.0)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8,
+                          "sum_bytes_read": a.numel() * 8, "fill_bytes_written": b.numel() * 8, "launches": 3,
+                          "sum": float(s)}))
         return
     if args.column_only:
         import dsht
