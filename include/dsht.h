@@ -425,7 +425,8 @@ long dsht_redigm_work_size(const dsht_plan* plan, int km);
  * fluxes and the inverse of the divergence; then ptemp_rhs[k] = (ptemp_rhs[k] + hdiv[k]) +
  * vdiv[k], salt_rhs likewise (km grids each, pitch ldo; they may not overlap the inputs).
  * The transforms run in the plan's precision form.  work: lwork >= dsht_redigm_work_size
- * device doubles (contents undefined on return).                                          */
+ * device doubles (contents undefined on return).  After one call on a stream the call
+ * allocates nothing and can be captured in a hipGraph on that stream.                    */
 int dsht_redigm_d(dsht_plan* plan, const dsht_redigm_config* cfg, const double* ptemp,
                   const double* salt, const double* h, const double* z, long ldg, int km,
                   const double* topo, const double* cdk, const double* rfdk, double* ptemp_rhs,

@@ -19,6 +19,7 @@ the oracle's restatement of LPhys_RediGM_hspm_vfvm_AddMixingTerm
   its mean carries it multiplied by that ratio and by n, so the tendency is good to ~1e-4
   (relative L2 <= 3e-4, max-norm <= 2e-3).
 * At T170L60 the whole tendency conserves the tracer content of a flat-bottomed ocean.
+* The call captured in a hipGraph replays to the eager result bit for bit.
 * Argument errors: linear PBL tapering, km < 2, overlapping outputs, a short workspace.
 """
 import math
@@ -241,3 +242,41 @@ def test_redigm_argument_errors(gpu):
     with pytest.raises(ValueError, match="km >= 2"):
         p.redigm_d(one["pt"], one["sa"], one["h"], one["z"], one["topo"], one["cdk"], one["rfdk"], rt[:1], rs[:1])
     torch.cuda.synchronize()
+
+
+def test_redigm_graph_replays(gpu):
+    """dsht_redigm_d captured in a hipGraph after one eager call on the capture stream (the
+    plan's transform workspace of that stream is then sized; the call allocates nothing else
+    and records no memset): every replay, with eager work and synchronisations between the
+    replays, equals the eager result bit for bit."""
+    import torch
+    import dsht
+    N, I, J, km = 42, 128, 64, 20
+    p = dsht.Plan(N, I, J, radius=R_EARTH)
+    x = _fields(p, km, seed=29, tilt=2.0)
+    d = _dev(x, gpu)
+    s = torch.cuda.Stream()
+    rt = torch.zeros((km, J, I), dtype=torch.float64, device=gpu)
+    rs = torch.zeros_like(rt)
+    work = torch.empty(dsht.load().dsht_redigm_work_size(p.h, km), dtype=torch.float64, device=gpu)
+    args = (d["pt"], d["sa"], d["h"], d["z"], d["topo"], d["cdk"], d["rfdk"], rt, rs)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        p.redigm_d(*args, work=work, stream=s.cuda_stream)
+    s.synchronize()
+    ref_t, ref_s = rt.clone(), rs.clone()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            rt.zero_()
+            rs.zero_()
+            p.redigm_d(*args, work=work, stream=s.cuda_stream)
+    for rep in range(3):
+        rt.fill_(float("nan"))
+        rs.fill_(float("nan"))
+        torch.cuda.synchronize()
+        (d["pt"] * 2.0).sum().item()  # eager work between the replays
+        with torch.cuda.stream(s):
+            g.replay()
+        s.synchronize()
+        assert torch.equal(rt, ref_t) and torch.equal(rs, ref_s), rep
