@@ -582,7 +582,8 @@ def _redigm_fields(torch, dsht, plan, km, seed):
         + 0.02 * rng.standard_normal(z.shape)
     sa = 34.5 + 0.5 * np.exp(z / 1000.0) + 0.3 * mu * np.sin(3 * lon)[None] * np.exp(z / 2000.0)
     t = lambda a: torch.tensor(np.ascontiguousarray(a), device="cuda")
-    return {"pt": t(pt), "sa": t(sa), "h": t(np.broadcast_to(topo, z.shape)), "z": t(z), "topo": t(topo),
+    trc = t(np.concatenate([pt, sa]))  # both tracers in one array, as the model's xyza_TRC
+    return {"pt": trc[:km], "sa": trc[km:], "h": t(np.broadcast_to(topo, z.shape)), "z": t(z), "topo": t(topo),
             "cdk": t(cdk), "rfdk": t(rfdk)}, (pt, sa, np.broadcast_to(topo, z.shape).copy(), z, topo, cdk, rfdk)
 
 

@@ -1378,8 +1378,12 @@ int dsht_redigm_d(dsht_plan* p, const dsht_redigm_config* cfg, const double* pte
   double* w = vdv + n2 * IJ;  // 2 km spectra, coefficient-major (sl = 2 km)
   int rc;
   // calc_GradTRC (:431-442): w_xy of each layer, xy_GradLon_w and xy_GradLat_w
-  if ((rc = dsht_fwd_d(p, DSHT_VAL, ptemp, ldg, w, n2, 1, km, nullptr, s))) return rc;
-  if ((rc = dsht_fwd_d(p, DSHT_VAL, salt, ldg, w + km, n2, 1, km, nullptr, s))) return rc;
+  if (salt == ptemp + (long)km * ldg) {  // both tracers in one array (xyza_TRC): one call
+    if ((rc = dsht_fwd_d(p, DSHT_VAL, ptemp, ldg, w, n2, 1, (int)n2, nullptr, s))) return rc;
+  } else {
+    if ((rc = dsht_fwd_d(p, DSHT_VAL, ptemp, ldg, w, n2, 1, km, nullptr, s))) return rc;
+    if ((rc = dsht_fwd_d(p, DSHT_VAL, salt, ldg, w + km, n2, 1, km, nullptr, s))) return rc;
+  }
   if ((rc = dsht_inv_vgrad_d(p, w, n2, 1, nullptr, glon, glat, IJ, (int)n2, nullptr, s))) return rc;
   if ((rc = launch_redigm_col(p, r, ptemp, salt, h, z, ldg, km, topo, cdk, rfdk, glon, glat, IJ, flon, flat, vdv,
                               IJ, s)))

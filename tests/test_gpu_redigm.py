@@ -161,6 +161,27 @@ def _full_case(gpu, N, I, J, km, seed, prec=None):
     return p, x, rhs0, got, ref, cfg
 
 
+def test_redigm_tracers_in_one_array(gpu):
+    """Temperature and salinity as the two halves of one array (the model's xyza_TRC): the
+    library makes one forward call over both; the result equals the separate-array call."""
+    import torch
+    import dsht
+    N, I, J, km = 42, 128, 64, 20
+    p = dsht.Plan(N, I, J, radius=R_EARTH)
+    x = _fields(p, km, seed=31, tilt=2.0)
+    d = _dev(x, gpu)
+    trc = torch.cat([d["pt"], d["sa"]]).contiguous()
+    outs = []
+    for pt, sa in ((d["pt"], d["sa"]), (trc[:km], trc[km:])):
+        rt = torch.zeros((km, J, I), dtype=torch.float64, device=gpu)
+        rs = torch.zeros_like(rt)
+        p.redigm_d(pt, sa, d["h"], d["z"], d["topo"], d["cdk"], d["rfdk"], rt, rs)
+        outs.append((rt, rs))
+    torch.cuda.synchronize()
+    for a, b in zip(outs[0], outs[1]):
+        assert torch.equal(a, b)
+
+
 @pytest.mark.parametrize("N,I,J,km", [(21, 64, 32, 8), (42, 128, 64, 20)])
 def test_redigm_matches_oracle(gpu, N, I, J, km):
     from conftest import rel_l2
