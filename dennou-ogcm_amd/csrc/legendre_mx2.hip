@@ -408,7 +408,13 @@ __global__ __launch_bounds__(256, 2) void k_mx2_inv_wk(Mx2InvArgs a) {
     const int total = nch * IPC;
     constexpr int U = 2;  // items per thread in flight (deeper prefetch measured no faster)
     for (int base = tid; base < total; base += 256 * U) {
-      double v[U][8];
+      // All loads of a round are issued before any is used: they are unconditional (a
+      // padding element reads element 0, which always exists) and the predicate is applied
+      // afterwards.  A load whose value is used inside the branch that guards it is waited
+      // for inside that branch, which serializes the round trips.
+      double v[U][8], pr[U][8];
+      bool ok[U][8];
+      long lp[U][8];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int it = base + 256 * u;
@@ -419,15 +425,24 @@ __global__ __launch_bounds__(256, 2) void k_mx2_inv_wk(Mx2InvArgs a) {
 #pragma unroll
         for (int e = 0; e < 8; e++) {
           const int k = ks * KM + 8 * kg + e;
-          v[u][e] = 0.0;
-          if (it < total && cok && k < np && (r == 0 || m > 0)) {
-            const int n = m + p + 2 * k;
-            const long l0 = (long)n * (n + 1);
-            const double pr = a.pro ? a.pro[l0 + m] : 1.0;
-            v[u][e] = pr * __builtin_nontemporal_load(W + (l0 + (r ? -m : m)) * a.sl + coff);
-          }
+          const int n = m + p + 2 * k;
+          const long l0 = (long)n * (n + 1);
+          ok[u][e] = it < total && cok && k < np && (r == 0 || m > 0);
+          lp[u][e] = ok[u][e] ? l0 + m : 0;
+          v[u][e] = __builtin_nontemporal_load(W + (ok[u][e] ? (l0 + (r ? -m : m)) * a.sl + coff : 0));
+          pr[u][e] = 1.0;
         }
       }
+      if (a.pro) {
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+          for (int e = 0; e < 8; e++) pr[u][e] = a.pro[lp[u][e]];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[u][e] = ok[u][e] ? pr[u][e] * v[u][e] : 0.0;
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int it = base + 256 * u;
@@ -543,7 +558,7 @@ struct Mx2FwdArgs {
 // the combination its parity needs (P with sym at parity 0, with anti at parity 1; H the
 // opposite).  Same tables, slots and epilogue as k_mx2_fwd.
 template <int OP, int FORM, int MAXT, int NSW, int NCT>
-__global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
+__global__ __launch_bounds__(256, fwd_ng(OP) == 1 ? 2 : 1) void k_mx2_fwd_sh(Mx2FwdArgs a) {
   using F = MxForm<FORM>;
   using X = Mx2<FORM>;
   using T = typename F::T;
@@ -582,24 +597,29 @@ __global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
   }
 
   // staging is split in two so that the next K step's loads are in flight during this
-  // step's MFMAs: fetch() loads the north/south values, stage() combines, splits and stores
+  // step's MFMAs: fetch() loads the north/south values, stage() combines, splits and stores.
+  // fetch() loads unconditionally (a latitude past the last one or a column past the last
+  // one reads latitude 0 of column 0) and stage() zeroes those values: with no load inside
+  // a branch, a wait for the table fragments does not also wait for these loads.
   float2 gN[NCT][NIN][2], gS[NCT][NIN][2];
   float gw[2];
+  bool gok[NCT][2];
   auto fetch = [&](int ks) {
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int jn = ks * KM + 2 * jp + h;
       gw[h] = a.cw[jn];
 #pragma unroll
-      for (int t = 0; t < NCT; t++)
+      for (int t = 0; t < NCT; t++) {
+        gok[t][h] = jn < a.J2 && csok[t];
+        const long oN = gok[t][h] ? (long)(a.J2 + jn) * a.gld + cs[t] : 0;
+        const long oS = gok[t][h] ? (long)(a.J2 - 1 - jn) * a.gld + cs[t] : 0;
 #pragma unroll
         for (int i = 0; i < NIN; i++) {
-          gN[t][i][h] = gS[t][i][h] = make_float2(0.f, 0.f);
-          if (jn < a.J2 && csok[t]) {
-            gN[t][i][h] = Gm[i][(long)(a.J2 + jn) * a.gld + cs[t]];
-            gS[t][i][h] = Gm[i][(long)(a.J2 - 1 - jn) * a.gld + cs[t]];
-          }
+          gN[t][i][h] = Gm[i][oN];
+          gS[t][i][h] = Gm[i][oS];
         }
+      }
     }
   };
   auto stage = [&](int buf, int) {
@@ -610,7 +630,8 @@ __global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
         float v[2][4];  // [latitude][sym re, sym im, anti re, anti im]
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-          const float2 gn = gN[t][i][h], gs = gS[t][i][h];
+          const float2 z = make_float2(0.f, 0.f);
+          const float2 gn = gok[t][h] ? gN[t][i][h] : z, gs = gok[t][h] ? gS[t][i][h] : z;
           const float cw = gw[h] * sdsc[t];
           v[h][0] = cw * (gn.x + gs.x);
           v[h][1] = cw * (gn.y + gs.y);
@@ -647,27 +668,37 @@ __global__ __launch_bounds__(256) void k_mx2_fwd_sh(Mx2FwdArgs a) {
     fetch(0);
     stage(0, 0);
     __syncthreads();
+    // one-group operators request the next K step's values after the first slot's table
+    // fragments (below); the multi-group ones, whose accumulators leave no registers for
+    // both, request them first
+    constexpr bool LATE = NG == 1;
     for (int ks = 0; ks < KS; ks++) {
       const int buf = ks & 1;
-      if (ks + 1 < KS) fetch(ks + 1);
+      if (!LATE && ks + 1 < KS) fetch(ks + 1);
 #pragma unroll
       for (int q = 0; q < NSW; q++) {
         const int sidx = rbase + w + 4 * q;
-        if (sidx >= nslot) continue;  // uniform over the wave
+        const bool act = sidx < nslot;  // uniform over the wave
         const int PP = sidx >= sE ? 1 : 0;
         const int tp0 = (PP ? sidx - sE : sidx) * MAXT, tpar = PP ? tO : tE;
         V8 A[2][MAXT][NTA];
+        if (act) {
 #pragma unroll
-        for (int tt = 0; tt < MAXT; tt++) {
-          const int tp = tp0 + tt;
-          const bool ok = tp < tpar;
-          const long o = toff + ((long)(PP ? tE + tp : tp) * KS + ks) * 512 + lane * 8;
+          for (int tt = 0; tt < MAXT; tt++) {
+            // a tile past the slot's last one repeats the last (its results are not stored)
+            const int tp = tp0 + tt < tpar ? tp0 + tt : tpar - 1;
+            const long o = toff + ((long)(PP ? tE + tp : tp) * KS + ks) * 512 + lane * 8;
 #pragma unroll
-          for (int k = 0; k < NTA; k++) {
-            if constexpr (UP) A[0][tt][k] = ok ? *reinterpret_cast<const V8*>(a.tP[k] + o) : V8{};
-            if constexpr (UH) A[1][tt][k] = ok ? *reinterpret_cast<const V8*>(a.tH[k] + o) : V8{};
+            for (int k = 0; k < NTA; k++) {
+              if constexpr (UP) A[0][tt][k] = *reinterpret_cast<const V8*>(a.tP[k] + o);
+              if constexpr (UH) A[1][tt][k] = *reinterpret_cast<const V8*>(a.tH[k] + o);
+            }
           }
         }
+        // requested after the first slot's table fragments, that slot's products wait for
+        // its tables only
+        if (LATE && q == 0 && ks + 1 < KS) fetch(ks + 1);
+        if (!act) continue;
 #pragma unroll
         for (int gi = 0; gi < NG; gi++) {
           const int tb = fwd_tab(OP, gi), in = fwd_in(OP, gi);
