@@ -310,33 +310,41 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_inv(LegInvArgs a) {
 #pragma unroll
           for (int r = 0; r < 2; r++) acc[o][t][ct][r][0] = acc[o][t][ct][r][1] = double4_t{0, 0, 0, 0};
 
-    double bre[NIN][EPT], bim[NIN][EPT];
+    double bre[NIN][EPT], bim[NIN][EPT], bpr[EPT];
+    bool bok[EPT];
     double aP0[MAXT][NKS], aH0[MAXT][NKS], aP1[MAXT][NKS], aH1[MAXT][NKS];
     auto chunk_pos = [&](int ch, int& p, int& k0) {
       p = ch < nch0 ? 0 : 1;
       k0 = (p ? ch - nch0 : ch) * KC;
     };
+    // The next chunk's loads are issued before this chunk's products and used after them,
+    // so none of them may be used inside the branch that guards it (such a load is waited
+    // for there): load_b loads unconditionally (a padding element reads element 0) and
+    // stage() applies the predicate and the prefactor; load_a clamps its index (padding
+    // tiles and K steps are never multiplied or stored).
     auto load_b = [&](int ch) {
       int p, k0;
       chunk_pos(ch, p, k0);
       const int np = p ? nO : nE;
 #pragma unroll
-      for (int i = 0; i < NIN; i++)
+      for (int q = 0; q < EPT; q++) {
+        const int e = tid + NT * q;
+        const int kk = e / CB, cc = e % CB;
+        const int k = k0 + kk, c = c0 + cc;
+        const int n = m + p + 2 * k;
+        bok[q] = e < KC * CB && k < np && c < a.ncols;
+        const long l0 = bok[q] ? (long)n * (n + 1) : 0;
+        const long lre = bok[q] ? l0 + m : 0, lim = bok[q] ? l0 - m : 0;
+        const long co = bok[q] ? (long)c * a.sc : 0;
+        bpr[q] = 1.0;
+        if (a.pro) bpr[q] = a.pro[lre];
 #pragma unroll
-        for (int q = 0; q < EPT; q++) {
-          const int e = tid + NT * q;
-          const int kk = e / CB, cc = e % CB;
-          const int k = k0 + kk, c = c0 + cc;
-          bre[i][q] = bim[i][q] = 0.0;
-          if (e < KC * CB && k < np && c < a.ncols) {
-            const int n = m + p + 2 * k;
-            const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
-            const double pr = a.pro ? a.pro[lre] : 1.0;
-            // each block reads its spectra once: non-temporal
-            bre[i][q] = pr * __builtin_nontemporal_load(Win[i] + lre * a.sl + (long)c * a.sc);
-            if (m > 0) bim[i][q] = pr * __builtin_nontemporal_load(Win[i] + lim * a.sl + (long)c * a.sc);
-          }
+        for (int i = 0; i < NIN; i++) {
+          // each block reads its spectra once: non-temporal
+          bre[i][q] = __builtin_nontemporal_load(Win[i] + lre * a.sl + co);
+          bim[i][q] = __builtin_nontemporal_load(Win[i] + lim * a.sl + co);
         }
+      }
     };
     auto load_a = [&](int ch, double (&xP)[MAXT][NKS], double (&xH)[MAXT][NKS]) {
       int p, k0;
@@ -349,10 +357,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_inv(LegInvArgs a) {
 #pragma unroll
         for (int ks = 0; ks < NKS; ks++) {
           const int kg = k0 / 4 + ks;
-          const long io = pbase + (long)t * (kp / 4) * 64 + (long)kg * 64 + lane;
           const bool ok = t < RT && kg < kp / 4;
-          xP[tt][ks] = (UP && ok) ? tabP[io] : 0.0;
-          xH[tt][ks] = (UH && ok) ? tabH[io] : 0.0;
+          const long io = ok ? pbase + (long)t * (kp / 4) * 64 + (long)kg * 64 + lane : lane;
+          xP[tt][ks] = UP ? tabP[io] : 0.0;
+          xH[tt][ks] = UH ? tabH[io] : 0.0;
         }
       }
     };
@@ -364,8 +372,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_leg_inv(LegInvArgs a) {
           const int e = tid + NT * q;
           if (e < KC * CB) {
             const int kk = e / CB, cc = e % CB;
-            sB[i][0][kk * PITCH + cc] = bre[i][q];
-            sB[i][1][kk * PITCH + cc] = bim[i][q];
+            sB[i][0][kk * PITCH + cc] = bok[q] ? bpr[q] * bre[i][q] : 0.0;
+            sB[i][1][kk * PITCH + cc] = bok[q] && m > 0 ? bpr[q] * bim[i][q] : 0.0;
           }
         }
     };

@@ -94,13 +94,16 @@ __device__ __forceinline__ bool xcd_decode(int N, int cblk, int& m, int& sub) {
   return m <= N;
 }
 
+__device__ __forceinline__ float colscale_of(float mx, float bound) {
+  mx *= bound;
+  return mx > 0.f ? exp2f(-ceilf(log2f(mx))) : 1.f;
+}
 template <int NIN>
 __device__ __forceinline__ float colscale(const unsigned* colmax, int ncols, int c, float bound) {
   float mx = 0.f;
 #pragma unroll
   for (int i = 0; i < NIN; i++) mx = fmaxf(mx, __uint_as_float(colmax[(long)i * ncols + c]));
-  mx *= bound;
-  return mx > 0.f ? exp2f(-ceilf(log2f(mx))) : 1.f;
+  return colscale_of(mx, bound);
 }
 
 struct Mx2InvArgs {
@@ -154,19 +157,31 @@ __global__ __launch_bounds__(256) void k_mx2_ipack(Mx2InvArgs a) {
   if constexpr (F::SCALED) {
     if (c < a.ncols) dsc = colscale<NIN>(a.colmax, a.ncols, c, 1.f);
   }
-  double vre[8], vim[8];
+  // all 16 loads are issued before any is used: unconditional (a padding element reads
+  // element 0) with the predicate and the prefactor applied afterwards
+  double vre[8], vim[8], pr[8];
+  bool ok[8];
+  long lp[8];
 #pragma unroll
   for (int e = 0; e < 8; e++) {
     const int k = ks * KM + 8 * q + e;
-    vre[e] = vim[e] = 0.0;
-    if (k < np && c < a.ncols) {
-      const int n = m + p + 2 * k;
-      const long lre = (long)n * (n + 1) + m, lim = (long)n * (n + 1) - m;
-      const double pr = a.pro ? a.pro[lre] : 1.0;
-      // the spectra are read once here: non-temporal
-      vre[e] = pr * __builtin_nontemporal_load(W + lre * a.sl + (long)c * a.sc);
-      if (m > 0) vim[e] = pr * __builtin_nontemporal_load(W + lim * a.sl + (long)c * a.sc);
-    }
+    const int n = m + p + 2 * k;
+    ok[e] = k < np && c < a.ncols;
+    const long l0 = ok[e] ? (long)n * (n + 1) : 0, co = ok[e] ? (long)c * a.sc : 0;
+    lp[e] = ok[e] ? l0 + m : 0;
+    // the spectra are read once here: non-temporal
+    vre[e] = __builtin_nontemporal_load(W + (ok[e] ? l0 + m : 0) * a.sl + co);
+    vim[e] = __builtin_nontemporal_load(W + (ok[e] ? l0 - m : 0) * a.sl + co);
+    pr[e] = 1.0;
+  }
+  if (a.pro) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) pr[e] = a.pro[lp[e]];
+  }
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    vre[e] = ok[e] ? pr[e] * vre[e] : 0.0;
+    vim[e] = ok[e] && m > 0 ? pr[e] * vim[e] : 0.0;
   }
   T tre[NTB][8], tim[NTB][8];
 #pragma unroll
@@ -205,6 +220,41 @@ __device__ __forceinline__ void inv_epilogue(const Mx2InvArgs& a,
   constexpr int NG = inv_ng(OP), NIN = InvTraits<OP>::NIN, NOUT = InvTraits<OP>::NOUT;
   const double mi = a.minv[m] * X::DSCALE;
   const double dm = (double)m;
+  // The lane's row scales (north and south) and column scales are loaded before any is used,
+  // so that they cost one round trip rather than one per row (an out-of-range row or column
+  // reads entry 0; its results are not stored).
+  double rsN[MAXT][4], rsS[MAXT][4];
+#pragma unroll
+  for (int tt = 0; tt < MAXT; tt++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int jr = (t0 + tt) * TILE + acc_row16(lane, i);
+      const bool ok = jr < a.J2;
+      rsN[tt][i] = rsS[tt][i] = 1.0;
+      if (a.rs) {
+        rsN[tt][i] = a.rs[ok ? a.J2 + jr : 0];
+        rsS[tt][i] = a.rs[ok ? a.J2 - 1 - jr : 0];
+      }
+    }
+  double us[NCT];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ct++) us[ct] = mi;
+  if constexpr (F::SCALED) {
+    unsigned cm[NCT][NIN];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      const int c = (ct0 + ct) * TILE + (lane & 15);
+#pragma unroll
+      for (int i = 0; i < NIN; i++) cm[ct][i] = a.colmax[(long)i * a.ncols + (c < a.ncols ? c : 0)];
+    }
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      float mx = 0.f;
+#pragma unroll
+      for (int i = 0; i < NIN; i++) mx = fmaxf(mx, __uint_as_float(cm[ct][i]));
+      us[ct] = mi / (double)colscale_of(mx, 1.f);
+    }
+  }
 #pragma unroll
   for (int tt = 0; tt < MAXT; tt++) {
     const int t = t0 + tt;
@@ -213,8 +263,7 @@ __device__ __forceinline__ void inv_epilogue(const Mx2InvArgs& a,
     for (int ct = 0; ct < NCT; ct++) {
       const int c = (ct0 + ct) * TILE + (lane & 15);
       if (c >= a.ncols) continue;
-      double u = mi;
-      if constexpr (F::SCALED) u = mi / (double)colscale<NIN>(a.colmax, a.ncols, c, 1.f);
+      const double u = us[ct];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int jr = t * TILE + acc_row16(lane, i);
@@ -249,7 +298,7 @@ __device__ __forceinline__ void inv_epilogue(const Mx2InvArgs& a,
           onr[1] = -dm * ni[0] + nr[3], oni[1] = dm * nr[0] + ni[3];
           osr[1] = -dm * si[0] + sr[3], osi[1] = dm * sr[0] + si[3];
         }
-        const double sN = u * (a.rs ? a.rs[jN] : 1.0), sS = u * (a.rs ? a.rs[jS] : 1.0);
+        const double sN = u * rsN[tt][i], sS = u * rsS[tt][i];
 #pragma unroll
         for (int o = 0; o < NOUT; o++) {
           float2* Gb = o == 0 ? a.G0 : (o == 1 ? a.G1 : a.G2);
