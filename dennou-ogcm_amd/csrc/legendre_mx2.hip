@@ -1060,11 +1060,16 @@ __global__ __launch_bounds__(256) void k_spec_colmax2(const double* __restrict__
 // tile shapes per operator: one-group operators take 2x2 tiles per wave, the two- and
 // four-group ones fewer so that their accumulators stay within the register budget
 // (NCT: column tiles per wave; the plan's column-block knob selects 1 tile for 16, else 2)
+// column tiles of a whole-K staged inverse block: one for two-input and four-group
+// operators, else the requested count (two by default, one with the 16-column knob)
+template <int OP>
+constexpr int wk_nct(int req) {
+  return (InvTraits<OP>::NIN > 1 || inv_ng(OP) == 4) ? 1 : req;
+}
 // LDS of the whole-K staged inverse: every K chunk of wavenumber 0 (the most) for the
 // block's column tiles
-template <int OP, int FORM>
+template <int OP, int FORM, int WNCT = wk_nct<OP>(2)>
 size_t staged_lds(const dsht_plan* p) {
-  constexpr int WNCT = (InvTraits<OP>::NIN > 1 || inv_ng(OP) == 4) ? 1 : 2;
   const int nchmax = (kpad32(n_par(p->N, 0, 0)) + kpad32(n_par(p->N, 0, 1))) / KM;
   return (size_t)nchmax * wk_chunk_elems<OP, FORM, WNCT>() * sizeof(typename MxForm<FORM>::T);
 }
@@ -1121,9 +1126,9 @@ int inv_launch(const dsht_plan* p, const MixedTables& t, const Mx2InvJob& j, hip
   const long nb = 8l * cdiv(p->N + 1, 8) * a.CG * a.RG;
   if (j.staged) {
     // whole-K staging: every latitude tile in one block where 4 waves x 2 tiles cover them
-    constexpr int WNCT = (InvTraits<OP>::NIN > 1 || inv_ng(OP) == 4) ? 1 : 2;
+    constexpr int WNCT = wk_nct<OP>(NCT_);
     const int wmaxt = a.RT <= 4 ? 1 : 2;
-    const size_t lds = staged_lds<OP, FORM>(p);
+    const size_t lds = staged_lds<OP, FORM, WNCT>(p);
     a.CG = cdiv(a.CT, WNCT);
     a.RG = cdiv(a.RT, 4 * wmaxt);
     const long nbw = 8l * cdiv(p->N + 1, 8) * a.CG * a.RG;
