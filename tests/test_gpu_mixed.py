@@ -144,6 +144,38 @@ def test_mixed_forward_column_tiles(gpu, form, cb):
     assert rel_l2(vor, rv) < tol and rel_l2(div, rd) < tol
 
 
+@pytest.mark.parametrize("leg", [0, 2, 3])
+@pytest.mark.parametrize("cb", [16, 32])
+@pytest.mark.parametrize("form", list(BOUND))
+def test_mixed_inverse_column_tiles(gpu, form, cb, leg):
+    """The inverse with one (16) and two (32) column tiles per block, in every family (per
+    form, packed, whole-K staged), over 21 columns (the last block partly empty) at an odd
+    truncation (the last wavenumber has no odd-parity row): each kind alone against the oracle,
+    and the fused value/gradient pass against the separate kinds of the same form."""
+    import torch
+    import dsht
+    import oracle
+    N, I, J, R = 31, 96, 48, 6.371e6
+    p = dsht.Plan(N, I, J, radius=R)
+    o = oracle.Oracle(N, I, J, R)
+    p.set_precision(_prec(form))
+    p.set_tuning(dsht.TUNE_LEG_CB, cb)
+    p.set_tuning(dsht.TUNE_LEG, leg)
+    tol = BOUND[form]
+    w = red_spectrum(N, 21, 17) * 1e-5
+    sep = {}
+    for kind in ("VAL", "GRADLON", "GRADLAT"):
+        got = p._inv(w, getattr(dsht, kind))
+        assert rel_l2(got, o.inv(w, getattr(dsht, kind))) < tol, (form, cb, leg, kind)
+        sep[kind] = got
+    wt = torch.tensor(w.T.copy(), device=gpu)
+    g = [torch.empty((21, J, I), dtype=torch.float64, device=gpu) for _ in range(3)]
+    p.inv_vgrad_d(wt, *g)
+    torch.cuda.synchronize()
+    for kind, t in zip(("VAL", "GRADLON", "GRADLAT"), g):
+        assert rel_l2(t.cpu().numpy(), sep[kind].reshape(21, J, I)) < tol, (form, cb, leg, kind)
+
+
 @pytest.mark.parametrize("form", list(BOUND))
 def test_every_form_at_t170(gpu, form):
     """every split form at T170 (I=512, J=256) on 40 columns of red-spectrum data
