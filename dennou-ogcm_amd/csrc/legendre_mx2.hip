@@ -606,7 +606,13 @@ struct Mx2FwdArgs {
 // the accumulators of up to NSW slots (slot s = round*4*NSW + w + 4*i) and reads, per slot,
 // the combination its parity needs (P with sym at parity 0, with anti at parity 1; H the
 // opposite).  Same tables, slots and epilogue as k_mx2_fwd.
-template <int OP, int FORM, int MAXT, int NSW, int NCT>
+//
+// KSC > 0: the number of K steps (J2p32 / 32) as a constant, so that the K loop unrolls into
+// straight-line code.  Then one-group operators, in a round where no wave has a second slot,
+// request their slot's table fragments of the next K step right after issuing this step's
+// products: the next step's products find them loaded, and the compiler's waits (exact in
+// straight-line code) leave the younger Fourier loads in flight.
+template <int OP, int FORM, int MAXT, int NSW, int NCT, int KSC = 0>
 __global__ __launch_bounds__(256, fwd_ng(OP) == 1 ? 2 : 1) void k_mx2_fwd_sh(Mx2FwdArgs a) {
   using F = MxForm<FORM>;
   using X = Mx2<FORM>;
@@ -624,7 +630,7 @@ __global__ __launch_bounds__(256, fwd_ng(OP) == 1 ? 2 : 1) void k_mx2_fwd_sh(Mx2
   const int nE = n_par(a.N, m, 0), nO = n_par(a.N, m, 1);
   const int tE = cdiv(nE, TILE), tO = cdiv(nO, TILE);
   const int sE = cdiv(tE, MAXT), sO = cdiv(tO, MAXT);
-  const int KS = a.J2p32 / KM;
+  const int KS = KSC ? KSC : a.J2p32 / KM;
   const long toff = a.off[m];
   const float2* Gm[2] = {a.G0 + (long)m * a.gmst, NIN > 1 ? a.G1 + (long)m * a.gmst : nullptr};
   const double dm = (double)m;
@@ -721,6 +727,25 @@ __global__ __launch_bounds__(256, fwd_ng(OP) == 1 ? 2 : 1) void k_mx2_fwd_sh(Mx2
     // fragments (below); the multi-group ones, whose accumulators leave no registers for
     // both, request them first
     constexpr bool LATE = NG == 1;
+    const bool pre = LATE && KSC > 0 && nslot - rbase <= 4;  // uniform over the block
+    auto load_tables = [&](int ks, int sidx, V8 (&A)[2][MAXT][NTA]) {
+      const int PP = sidx >= sE ? 1 : 0;
+      const int tp0 = (PP ? sidx - sE : sidx) * MAXT, tpar = PP ? tO : tE;
+#pragma unroll
+      for (int tt = 0; tt < MAXT; tt++) {
+        // a tile past the slot's last one repeats the last (its results are not stored)
+        const int tp = tp0 + tt < tpar ? tp0 + tt : tpar - 1;
+        const long o = toff + ((long)(PP ? tE + tp : tp) * KS + ks) * 512 + lane * 8;
+#pragma unroll
+        for (int k = 0; k < NTA; k++) {
+          if constexpr (UP) A[0][tt][k] = *reinterpret_cast<const V8*>(a.tP[k] + o);
+          if constexpr (UH) A[1][tt][k] = *reinterpret_cast<const V8*>(a.tH[k] + o);
+        }
+      }
+    };
+    V8 A0k[2][MAXT][NTA];  // slot 0's fragments kept across K steps (KSC > 0)
+    if (pre && rbase + w < nslot) load_tables(0, rbase + w, A0k);
+#pragma unroll(KSC ? KSC : 1)
     for (int ks = 0; ks < KS; ks++) {
       const int buf = ks & 1;
       if (!LATE && ks + 1 < KS) fetch(ks + 1);
@@ -729,44 +754,45 @@ __global__ __launch_bounds__(256, fwd_ng(OP) == 1 ? 2 : 1) void k_mx2_fwd_sh(Mx2
         const int sidx = rbase + w + 4 * q;
         const bool act = sidx < nslot;  // uniform over the wave
         const int PP = sidx >= sE ? 1 : 0;
-        const int tp0 = (PP ? sidx - sE : sidx) * MAXT, tpar = PP ? tO : tE;
-        V8 A[2][MAXT][NTA];
-        if (act) {
+        auto products = [&](const V8 (&A)[2][MAXT][NTA]) {
 #pragma unroll
-          for (int tt = 0; tt < MAXT; tt++) {
-            // a tile past the slot's last one repeats the last (its results are not stored)
-            const int tp = tp0 + tt < tpar ? tp0 + tt : tpar - 1;
-            const long o = toff + ((long)(PP ? tE + tp : tp) * KS + ks) * 512 + lane * 8;
+          for (int gi = 0; gi < NG; gi++) {
+            const int tb = fwd_tab(OP, gi), in = fwd_in(OP, gi);
+            const int cq = tb == 0 ? PP : 1 - PP;  // 0 = sym, 1 = anti
 #pragma unroll
-            for (int k = 0; k < NTA; k++) {
-              if constexpr (UP) A[0][tt][k] = *reinterpret_cast<const V8*>(a.tP[k] + o);
-              if constexpr (UH) A[1][tt][k] = *reinterpret_cast<const V8*>(a.tH[k] + o);
-            }
+            for (int t = 0; t < NCT; t++)
+#pragma unroll
+              for (int r = 0; r < 2; r++) {
+                V8 B[NTB];
+#pragma unroll
+                for (int k = 0; k < NTB; k++)
+                  B[k] = *reinterpret_cast<const V8*>(sB[buf] + t * PANEL + ((in * 4 + 2 * cq + r) * NTB + k) * FRAG +
+                                                      lane * 8);
+#pragma unroll
+                for (int tt = 0; tt < MAXT; tt++)
+#pragma unroll
+                  for (int pr = 0; pr < X::NP; pr++)
+                    acc[q][gi][tt][t][r] = mfma16(A[tb][tt][X::pa(pr)], B[X::pb(pr)], acc[q][gi][tt][t][r]);
+              }
           }
-        }
-        // requested after the first slot's table fragments, that slot's products wait for
-        // its tables only
-        if (LATE && q == 0 && ks + 1 < KS) fetch(ks + 1);
-        if (!act) continue;
-#pragma unroll
-        for (int gi = 0; gi < NG; gi++) {
-          const int tb = fwd_tab(OP, gi), in = fwd_in(OP, gi);
-          const int cq = tb == 0 ? PP : 1 - PP;  // 0 = sym, 1 = anti
-#pragma unroll
-          for (int t = 0; t < NCT; t++)
-#pragma unroll
-            for (int r = 0; r < 2; r++) {
-              V8 B[NTB];
-#pragma unroll
-              for (int k = 0; k < NTB; k++)
-                B[k] = *reinterpret_cast<const V8*>(sB[buf] + t * PANEL + ((in * 4 + 2 * cq + r) * NTB + k) * FRAG +
-                                                    lane * 8);
-#pragma unroll
-              for (int tt = 0; tt < MAXT; tt++)
-#pragma unroll
-                for (int pr = 0; pr < X::NP; pr++)
-                  acc[q][gi][tt][t][r] = mfma16(A[tb][tt][X::pa(pr)], B[X::pb(pr)], acc[q][gi][tt][t][r]);
-            }
+        };
+        if constexpr (KSC > 0) {
+          // slot 0's fragments live across K steps; with pre they were requested a step ahead
+          V8 A1[2][MAXT][NTA];
+          auto& A = q == 0 ? A0k : A1;
+          if (act && !(q == 0 && pre)) load_tables(ks, sidx, A);
+          // requested after the first slot's table fragments, that slot's products wait for
+          // its tables only
+          if (LATE && q == 0 && ks + 1 < KS) fetch(ks + 1);
+          if (!act) continue;
+          products(A);
+          if (q == 0 && pre && ks + 1 < KS) load_tables(ks + 1, sidx, A0k);
+        } else {
+          V8 A[2][MAXT][NTA];
+          if (act) load_tables(ks, sidx, A);
+          if (LATE && q == 0 && ks + 1 < KS) fetch(ks + 1);
+          if (!act) continue;
+          products(A);
         }
       }
       if (ks + 1 < KS) stage(buf ^ 1, ks + 1);
@@ -1201,9 +1227,17 @@ int fwd_launch(const dsht_plan* p, const MixedTables& t, const Mx2FwdJob& j, hip
   }
   const int fnct = p->leg_cb == 16 ? 1 : (p->leg_cb == 32 ? 2 : (fwd_ng(OP) == 1 && Mx2<FORM>::NP == 6 ? 2 : 1));
   const long nb = 8l * cdiv(p->N + 1, 8) * cdiv(a.CT, fnct);
-  if (fnct == 2)
+  // four K steps (a 256-latitude grid such as T170's): the unrolled form
+  const bool ks4 = fwd_ng(OP) == 1 && a.J2p32 / KM == 4;
+  if (fnct == 2 && ks4)
+    hipLaunchKernelGGL((k_mx2_fwd_sh<OP, FORM, FwdShape<OP>::MAXT, FwdShape<OP>::NSW, 2, fwd_ng(OP) == 1 ? 4 : 0>),
+                       dim3((unsigned)nb), dim3(256), 0, s, a);
+  else if (fnct == 2)
     hipLaunchKernelGGL((k_mx2_fwd_sh<OP, FORM, FwdShape<OP>::MAXT, FwdShape<OP>::NSW, 2>), dim3((unsigned)nb), dim3(256), 0,
                        s, a);
+  else if (ks4)
+    hipLaunchKernelGGL((k_mx2_fwd_sh<OP, FORM, FwdShape<OP>::MAXT, FwdShape<OP>::NSW, 1, fwd_ng(OP) == 1 ? 4 : 0>),
+                       dim3((unsigned)nb), dim3(256), 0, s, a);
   else
     hipLaunchKernelGGL((k_mx2_fwd_sh<OP, FORM, FwdShape<OP>::MAXT, FwdShape<OP>::NSW, 1>), dim3((unsigned)nb), dim3(256), 0,
                        s, a);
