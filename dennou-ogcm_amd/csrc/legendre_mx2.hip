@@ -598,14 +598,13 @@ struct Mx2FwdArgs {
   int N, J2, J2p32;
 };
 
-// Forward with the split data shared through LDS.  The per-wave kernel above forms and
-// splits the same data fragments once per slot and wave; here the block's 256 threads form
-// the symmetric and antisymmetric combinations of one 32-latitude K step once (two latitudes
-// of one column each), split them and store both combinations' B fragments to a
-// double-buffered LDS panel (one barrier per K step).  K is the outer loop: each wave keeps
-// the accumulators of up to NSW slots (slot s = round*4*NSW + w + 4*i) and reads, per slot,
-// the combination its parity needs (P with sym at parity 0, with anti at parity 1; H the
-// opposite).  Same tables, slots and epilogue as k_mx2_fwd.
+// Forward with the split data shared through LDS.  The block's 256 threads form the symmetric
+// and antisymmetric combinations of one 32-latitude K step once (two latitudes of one column
+// each), split them and store both combinations' B fragments to a double-buffered LDS panel
+// (one barrier per K step).  K is the outer loop: each wave keeps the accumulators of up to
+// NSW slots (slot s = round*4*NSW + w + 4*i; a slot is MAXT 16-row tiles of one parity) and
+// reads, per slot, the combination its parity needs (P with sym at parity 0, with anti at
+// parity 1; H the opposite).
 //
 // KSC > 0: the number of K steps (J2p32 / 32) as a constant, so that the K loop unrolls into
 // straight-line code.  Then one-group operators, in a round where no wave has a second slot,
