@@ -144,6 +144,28 @@ def test_mixed_forward_column_tiles(gpu, form, cb):
     assert rel_l2(vor, rv) < tol and rel_l2(div, rd) < tol
 
 
+@pytest.mark.parametrize("cb", [16, 32])
+@pytest.mark.parametrize("form", list(BOUND))
+def test_mixed_forward_four_k_steps(gpu, form, cb):
+    """The forward at J = 256 (four 32-latitude K steps: the unrolled kernel whose one-slot
+    rounds request the next step's table fragments early) with one (16) and two (32) column
+    tiles per block, over 21 columns (the last block partly empty), every forward kind against
+    the oracle (rows of both parities, rounds of one and of two slots per wave)."""
+    import dsht
+    import oracle
+    N, I, J, R = 170, 512, 256, 6.371e6
+    p = dsht.Plan(N, I, J, radius=R)
+    o = oracle.Oracle(N, I, J, R)
+    p.set_precision(_prec(form))
+    p.set_tuning(dsht.TUNE_LEG_CB, cb)
+    tol = {"bf16x6": 1e-6, "fp16x3": 3e-6, "bf16x3": 6e-5, "fp16": 4e-3}[form]
+    rng = np.random.default_rng(23)
+    xy = rng.standard_normal((21, J, I)) * o.coslat[None, :, None]
+    for kind in ("VAL", "DIVLAMBDA", "DIVMU"):
+        k = getattr(dsht, kind)
+        assert rel_l2(p._fwd(xy, k), o.fwd(xy, k)) < tol, (form, cb, kind)
+
+
 @pytest.mark.parametrize("leg", [0, 2, 3])
 @pytest.mark.parametrize("cb", [16, 32])
 @pytest.mark.parametrize("form", list(BOUND))
