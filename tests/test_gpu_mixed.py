@@ -200,7 +200,7 @@ def test_mixed_inverse_column_tiles(gpu, form, cb, leg):
 
 @pytest.mark.parametrize("form", list(BOUND))
 def test_every_form_at_t170(gpu, form):
-    """every split form at T170 (I=512, J=256) on 40 columns of red-spectrum data
+    """The T170L60 step's grid: every split form at T170 (I=512, J=256) on 40 columns of red-spectrum data
     with a large zonal-mean background (like temperature), against the FP64 form within its
     bound: the fused value/gradient inverse (three output groups) and the forward of grid
     products."""
