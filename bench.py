@@ -784,7 +784,7 @@ def main():
     ap.add_argument("--no-redigm", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only: the transform path has no "
+        print(json.dumps({"error": "bench.py measures one GPU only: the transform path has no "
                                    "multi-GPU form here; layers, fields and instances are independent columns, so "
                                    "it would shard by column blocks, one process per GPU", "n_gpus": args.gpus}))
         sys.exit(2)

@@ -1,7 +1,7 @@
 /*
  * dsht_oracle.c -- FP64 CPU restatement of dogcm's spherical-harmonic transform path.
  *
- * TEST INFRASTRUCTURE ONLY (parity checker and timed CPU baseline; see dsht_oracle.h).
+ * TEST INFRASTRUCTURE ONLY (parity oracle and timed CPU baseline; see dsht_oracle.h).
  *
  * The arithmetic of the reference path lives in SPMODEL >= 0.8.0 (wa_module) and
  * ISPACK 1.0.4 (SNPACK/SJPACK Legendre transforms, FTPACK FFT), neither of which is

@@ -1,6 +1,6 @@
 """CPU restatement of dogcm's LF-AM3 dynamics step on the oracle's transforms.
 
-TEST INFRASTRUCTURE ONLY (imported by tests/ and bench.py's cpu_baseline leg as the checker
+TEST INFRASTRUCTURE ONLY (imported by tests/ and bench.py's cpu_baseline leg as the oracle
 of the device LF-AM3 step); the product path never imports it.
 
 It restates DOGCM_TInt_LFAM3_advance_Dyn (model/dogcm/src/tint/DOGCM_TInt_LFAM3_mod.f90:

@@ -1,7 +1,7 @@
 """ctypes wrapper of the FP64 CPU oracle (oracle/dsht_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg, as the checker.  The product path never imports this module.
+cpu_baseline leg, as the parity oracle.  The product path never imports this module.
 
 Array conventions (numpy, C order):
   grid      (ncols, J, I)   -- SpmlUtil xy(0:I-1,1:J) per column, longitude fastest
@@ -362,7 +362,7 @@ def col_vdiff(q0, rhs, kv, e3, cdk, fdk, dt, alpha, sfc_bc="D", btm_bc="N", btm_
 
 def spec_energy(w, nmax, enstrophy=False):
     """nma_/na_EnergyFromStreamfunc_wa and the enstrophy forms (SPMODEL, cascaded at
-    SpmlUtil_mod.f90:131-132) on the unit sphere for this build's normalisation: E(n, +-m) =
+    SpmlUtil_mod.f90:131-132) on the unit sphere for this library's normalisation: E(n, +-m) =
     ((w^2) c_m) (n(n+1)/2), c_0 = 1, c_m = 2; the enstrophy with (n(n+1))^2 / 2.  w: (ncols, L).
     Returns nma (ncols, 2N+1, N+1) [c, m+N, n] (zero where |m| > n) and na (ncols, N+1), the
     sums over m = -n..n in ascending order."""

@@ -1,6 +1,6 @@
 """CPU restatement of dogcm's Redi/GM isopycnal mixing tendency on the oracle's transforms.
 
-TEST INFRASTRUCTURE ONLY (imported by tests/ and bench.py's cpu_baseline leg as the checker
+TEST INFRASTRUCTURE ONLY (imported by tests/ and bench.py's cpu_baseline leg as the oracle
 of the device Redi/GM path); the product path never imports it.
 
 It restates LPhys_RediGM_hspm_vfvm_AddMixingTerm (model/dogcm/src/physics/lphys/

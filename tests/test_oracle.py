@@ -1,4 +1,4 @@
-"""Pins the FP64 CPU oracle (oracle/dsht_oracle.c) before it is trusted as the checker.
+"""Pins the FP64 CPU oracle (oracle/dsht_oracle.c) before it is trusted as the parity oracle.
 
 Each check is chosen so that a plausible mistake (dropped term, wrong sign, transposed
 index, wrong normalisation or phase) fails at least one of them:

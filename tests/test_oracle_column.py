@@ -1,5 +1,5 @@
 """Pins the oracle's vertical column diagnostics (oracle/dsht_oracle.c: orc_col_*), the
-checker of HBEDiagnose_OMG, _HydPres and _UVBarot
+oracle of HBEDiagnose_OMG, _HydPres and _UVBarot
 (model/dogcm/src/dynamics/hspm_vfvm/HBEDiagnose_hspm_vfvm_mod.f90:90-143, 221-255, 290-325)
 and of the QUICK vertical advective flux (HBEBaroc_hspm_vfvm_mod.f90:233-317, pinned by the
 polynomial closed forms further down).

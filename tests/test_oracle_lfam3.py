@@ -1,5 +1,5 @@
 """Pins the CPU restatement of the LF-AM3 dynamics step (oracle/lfam3.py) before it is trusted
-as the checker of the device step.
+as the oracle of the device step.
 
 Known answers and invariants, each chosen so that a plausible mistake fails one of them:
   - Rossby-Haurwitz wave: a small Y_4^3 streamfunction on a resting, unstratified ocean
