@@ -280,6 +280,14 @@ __device__ __forceinline__ double2 load_pair(const float* x) {
   const fvec2 t = __builtin_nontemporal_load(reinterpret_cast<const fvec2*>(x));
   return make_double2((double)t.x, (double)t.y);
 }
+// the same pair in the grid's own type, converted where it is used: a conversion next to the
+// load would wait for the load there
+__device__ __forceinline__ dvec2 load_raw(const double* x) {
+  return __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(x));
+}
+__device__ __forceinline__ fvec2 load_raw(const float* x) {
+  return __builtin_nontemporal_load(reinterpret_cast<const fvec2*>(x));
+}
 __device__ __forceinline__ void store_pair(double* x, double a, double b) {
   const dvec2 t = {a, b};
   __builtin_nontemporal_store(t, reinterpret_cast<dvec2*>(x));
@@ -324,21 +332,27 @@ __global__ __launch_bounds__(NT, WPE<V>) void k_fft_fwd_r(const XT* __restrict__
   const int rs = tid % ROWS;
   // pass-1 inputs straight from global memory; the next group's rows are loaded into the
   // same registers once the current group's pass-1 output is in LDS
-  // FP64 arithmetic loads straight into v; FP32 arithmetic (REF) keeps the raw FP64 samples
-  // and converts them at the top of the next group, so the loads stay in flight across the
-  // barrier (the conversion does not wait on them right after issue)
+  // FP64 arithmetic loads straight into v; FP32 arithmetic (REF) keeps the samples as loaded,
+  // in the grid's type, and converts them at the top of the next group, so the loads stay in
+  // flight across the barrier.  REF loads are issued by every thread: a thread whose row is
+  // past the last column reads the grid's first row, which is transformed but never stored
+  // (rows do not mix in either pass or in the store).
   V v[R2];
-  double2 raw[REF ? R2 : 1];
-  double x0 = 0.0, x0n = 0.0;  // reference sample of the thread's pass-1 row (REF): current, next
+  using RV = std::conditional_t<sizeof(XT) == 4, fvec2, dvec2>;
+  RV raw[REF ? R2 : 1];
+  double x0 = 0.0;  // reference sample of the thread's pass-1 row (REF)
+  XT x0n = 0;       // the next group's, as loaded
   auto fetch = [&](long g) {
-    int j, c;
-    if (rr1 < ROWS && map_row(a, g, rr1, ROWS, j, c)) {
-      const XT* xr = xy + c * a.ldg + (long)j * (2 * M);
-      if constexpr (REF) {
-        x0n = (double)xr[0];
+    int j = 0, c = 0;
+    if constexpr (REF) {
+      const bool ok = rr1 < ROWS && map_row(a, g, rr1, ROWS, j, c);
+      const XT* xr = xy + (ok ? c * a.ldg + (long)j * (2 * M) : 0L);
+      x0n = xr[0];
 #pragma unroll
-        for (int n2 = 0; n2 < R2; n2++) raw[n2] = load_pair(xr + 2 * (n1 + R1 * n2));
-      } else if constexpr (sizeof(XT) == 8) {
+      for (int n2 = 0; n2 < R2; n2++) raw[n2] = load_raw(xr + 2 * (n1 + R1 * n2));
+    } else if (rr1 < ROWS && map_row(a, g, rr1, ROWS, j, c)) {
+      const XT* xr = xy + c * a.ldg + (long)j * (2 * M);
+      if constexpr (sizeof(XT) == 8) {
         const double2* x = reinterpret_cast<const double2*>(xr);
 #pragma unroll
         for (int n2 = 0; n2 < R2; n2++) v[n2] = cvt<V>(x[n1 + R1 * n2]);
@@ -347,23 +361,17 @@ __global__ __launch_bounds__(NT, WPE<V>) void k_fft_fwd_r(const XT* __restrict__
         for (int n2 = 0; n2 < R2; n2++) v[n2] = cvt<V>(load_pair(xr + 2 * (n1 + R1 * n2)));
       }
     } else {
-      if constexpr (REF) {
-        x0n = 0.0;
 #pragma unroll
-        for (int n2 = 0; n2 < R2; n2++) raw[n2] = make_double2(0.0, 0.0);
-      } else {
-#pragma unroll
-        for (int n2 = 0; n2 < R2; n2++) v[n2] = mk<V>(0, 0);
-      }
+      for (int n2 = 0; n2 < R2; n2++) v[n2] = mk<V>(0, 0);
     }
   };
   const long gstep = gridDim.x;
   if ((long)blockIdx.x < ngroups) fetch(blockIdx.x);
   for (long g0 = blockIdx.x; g0 < ngroups; g0 += gstep) {
     if constexpr (REF) {
-      x0 = x0n;
+      x0 = (double)x0n;
 #pragma unroll
-      for (int n2 = 0; n2 < R2; n2++) v[n2] = mk<V>(raw[n2].x - x0, raw[n2].y - x0);
+      for (int n2 = 0; n2 < R2; n2++) v[n2] = mk<V>((double)raw[n2].x - x0, (double)raw[n2].y - x0);
     }
     dft<R2, -1>(v);
     if (colmax && tid < ROWS) smax[tid] = 0u;
